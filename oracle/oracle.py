@@ -1,0 +1,360 @@
+"""ctypes wrapper over the FP64 CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product package never does.  See wc_oracle.h for what each
+function restates and how it is pinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_math.so")
+GHOST = 4
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
+
+
+class Grid(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("dx", C.c_double), ("nx", C.c_int * 3),
+                ("nb", C.c_int * 3), ("nnode", C.c_int64), ("nbucket", C.c_int64)]
+
+
+class SceneStruct(C.Structure):
+    _fields_ = [
+        ("g", Grid),
+        ("rho", C.c_double), ("flip", C.c_double), ("friction", C.c_double),
+        ("radius", C.c_double), ("pmass", C.c_double), ("pvol", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("solid_projection", C.c_int),
+        ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
+        ("np", C.c_int64),
+        ("x", _dp), ("v", _dp), ("C", _dp),
+        ("act", _u8p), ("lphi", _dp), ("sphi", _dp),
+        ("w", _dp * 3), ("mass", _dp * 3), ("vel", _dp * 3), ("vol", _dp * 3),
+        ("rhs", _dp * 3), ("invC", _dp * 3), ("ustar", _dp * 3),
+        ("psi_fs", _dp * 3), ("unew", _dp * 3), ("valid", _u8p * 3),
+        ("dof", _i32p), ("pressure", _dp),
+        ("ndof", C.c_int64), ("rowstart", _u32p), ("colidx", _u32p),
+        ("val", _dp), ("b", _dp), ("p", _dp), ("nnz_cap", C.c_int64),
+        ("last_residual", C.c_double), ("last_iters", C.c_int), ("last_ok", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so (and the reference harness where the tree exists)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if os.path.isdir("/root/reference/libWetCloth") and (force or not os.path.exists(REF_LIB_PATH)):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.wco_quad_kernel.restype = C.c_double
+        L.wco_quad_kernel.argtypes = [C.c_double]
+        L.wco_fraction_inside2.restype = C.c_double
+        L.wco_fraction_inside2.argtypes = [C.c_double] * 2
+        L.wco_fraction_inside4.restype = C.c_double
+        L.wco_fraction_inside4.argtypes = [C.c_double] * 4
+        L.wco_box_container_phi.restype = C.c_double
+        L.wco_box_container_phi.argtypes = [_dp, _dp, _dp]
+        L.wco_hash3.restype = C.c_uint64
+        L.wco_hash3.argtypes = [C.c_uint64] * 3
+        L.wco_uniform.restype = C.c_double
+        L.wco_uniform.argtypes = [C.c_uint64]
+        L.wco_grid_init.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int, C.c_double]
+        L.wco_node_index.restype = C.c_int64
+        L.wco_node_index.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int]
+        L.wco_seed_block.restype = C.c_int64
+        L.wco_seed_block.argtypes = [C.POINTER(Grid), _dp, _dp, C.c_uint64, _dp, C.c_int64]
+        L.wco_seed_velocity.argtypes = [C.POINTER(Grid), C.c_uint64, C.c_double,
+                                        C.c_int64, _dp, _dp]
+        L.wco_scene_create.restype = C.POINTER(SceneStruct)
+        L.wco_scene_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
+                                       C.c_double]
+        L.wco_scene_destroy.argtypes = [C.POINTER(SceneStruct)]
+        for nm in ("wco_activate", "wco_update_liquid_phi", "wco_update_solid_phi",
+                   "wco_update_solid_weights", "wco_p2g"):
+            getattr(L, nm).argtypes = [C.POINTER(SceneStruct)]
+        L.wco_step_velocity.argtypes = [C.POINTER(SceneStruct), C.c_double]
+        L.wco_assemble.restype = C.c_int64
+        L.wco_assemble.argtypes = [C.POINTER(SceneStruct), C.c_double]
+        L.wco_solve_pressure.restype = C.c_int
+        L.wco_solve_pressure.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_int]
+        L.wco_apply_pressure.argtypes = [C.POINTER(SceneStruct), C.c_double]
+        L.wco_g2p_advect.argtypes = [C.POINTER(SceneStruct), C.c_double]
+        L.wco_substep.restype = C.c_int
+        L.wco_substep.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_double, C.c_int]
+        L.wco_spmv.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
+        L.wco_mic0_factor.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double,
+                                      C.c_double, _dp, _dp]
+        L.wco_solve_lower.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp]
+        L.wco_solve_lower_transpose.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
+        L.wco_dot.restype = C.c_double
+        L.wco_dot.argtypes = [C.c_int64, _dp, _dp]
+        L.wco_abs_max.restype = C.c_double
+        L.wco_abs_max.argtypes = [C.c_int64, _dp]
+        L.wco_pcg_solve.restype = C.c_int
+        L.wco_pcg_solve.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
+                                    C.c_int, C.c_double, C.c_double, _dp,
+                                    C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's MathUtilities/BLASWrapper compiled from its sources, or None."""
+    if not os.path.exists(REF_LIB_PATH):
+        return None
+    L = C.CDLL(REF_LIB_PATH)
+    for nm, n in (("ref_quad_kernel", 1), ("ref_N_kernel2", 3), ("ref_fraction_inside2", 2),
+                  ("ref_fraction_inside4", 4), ("ref_lerp", 3), ("ref_clamp", 3)):
+        f = getattr(L, nm)
+        f.restype = C.c_double
+        f.argtypes = [C.c_double] * n
+    L.ref_inverse_D_coeff.restype = C.c_double
+    L.ref_inverse_D_coeff.argtypes = [C.c_double, C.c_int]
+    L.ref_blas_dot.restype = C.c_double
+    L.ref_blas_dot.argtypes = [_dp, _dp, C.c_long]
+    L.ref_blas_abs_max.restype = C.c_double
+    L.ref_blas_abs_max.argtypes = [_dp, C.c_long]
+    return L
+
+
+def ptr(a: np.ndarray, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def grid(n, dx) -> Grid:
+    g = Grid()
+    lib().wco_grid_init(C.byref(g), int(n[0]), int(n[1]), int(n[2]), float(dx))
+    return g
+
+
+def node_index(g: Grid, i, j, k) -> int:
+    return lib().wco_node_index(C.byref(g), i, j, k)
+
+
+def node_index_array(g: Grid, i, j, k):
+    """Vectorised blocked node index (same formula as wco_node_index)."""
+    i = np.asarray(i, dtype=np.int64)
+    j = np.asarray(j, dtype=np.int64)
+    k = np.asarray(k, dtype=np.int64)
+    b = ((k >> 2) * g.nb[1] + (j >> 2)) * g.nb[0] + (i >> 2)
+    return (b << 6) | ((k & 3) << 4) | ((j & 3) << 2) | (i & 3)
+
+
+def seed_block(g: Grid, lo, hi, seed: int) -> np.ndarray:
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    n = lib().wco_seed_block(C.byref(g), ptr(lo), ptr(hi), C.c_uint64(seed), None, 0)
+    x = np.zeros(3 * n, dtype=np.float64)
+    lib().wco_seed_block(C.byref(g), ptr(lo), ptr(hi), C.c_uint64(seed), ptr(x), n)
+    return x.reshape(3, n)
+
+
+def seed_velocity(g: Grid, seed: int, sigma: float, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[1]
+    v = np.zeros((3, n), dtype=np.float64)
+    lib().wco_seed_velocity(C.byref(g), C.c_uint64(seed), float(sigma), n, ptr(x), ptr(v))
+    return v
+
+
+class OracleScene:
+    """FP64 CPU scene: one reference substep per :meth:`substep` call."""
+
+    def __init__(self, n, dx, x, v=None, Cmat=None, rho=1000.0, flip=0.99,
+                 gravity=(0.0, -9.81, 0.0), pvol=None, radius=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        npart = x.shape[1]
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        Cm = (np.zeros((9, npart)) if Cmat is None
+              else np.ascontiguousarray(Cmat, dtype=np.float64))
+        pvol = dx ** 3 / 8.0 if pvol is None else pvol
+        pmass = rho * pvol
+        radius = (3.0 * pvol / (4.0 * np.pi)) ** (1.0 / 3.0) if radius is None else radius
+        self.n = tuple(int(a) for a in n)
+        self.dx = float(dx)
+        self.np = npart
+        self.s = lib().wco_scene_create(self.n[0], self.n[1], self.n[2], self.dx, rho, flip,
+                                        gravity[0], gravity[1], gravity[2], npart,
+                                        ptr(x), ptr(v), ptr(Cm), pmass, pvol, radius)
+
+    def __del__(self):
+        if getattr(self, "s", None) is not None and _lib is not None:
+            _lib.wco_scene_destroy(self.s)
+            self.s = None
+
+    @property
+    def st(self) -> SceneStruct:
+        return self.s.contents
+
+    @property
+    def grid(self) -> Grid:
+        return self.st.g
+
+    def _arr(self, p, n, dtype=np.float64):
+        if n == 0:
+            return np.zeros(0, dtype=dtype)
+        return np.ctypeslib.as_array(p, shape=(n,)).view(dtype).copy()
+
+    def field(self, name: str, axis: int | None = None) -> np.ndarray:
+        st = self.st
+        nn = st.g.nnode
+        if name in ("w", "mass", "vel", "vol", "rhs", "invC", "ustar", "psi_fs", "unew"):
+            return self._arr(getattr(st, name)[axis], nn)
+        if name == "valid":
+            return self._arr(st.valid[axis], nn, np.uint8)
+        if name in ("lphi", "sphi", "pressure"):
+            return self._arr(getattr(st, name), nn)
+        if name == "dof":
+            return self._arr(st.dof, nn, np.int32)
+        if name == "act":
+            return self._arr(st.act, st.g.nbucket, np.uint8)
+        raise KeyError(name)
+
+    def particles(self):
+        st = self.st
+        n = self.np
+        x = self._arr(st.x, 3 * n).reshape(3, n)
+        v = self._arr(st.v, 3 * n).reshape(3, n)
+        Cm = self._arr(st.C, 9 * n).reshape(9, n)
+        return x, v, Cm
+
+    def system(self):
+        st = self.st
+        n = st.ndof
+        rs = self._arr(st.rowstart, n + 1, np.uint32)
+        nnz = int(rs[-1]) if n > 0 else 0
+        col = self._arr(st.colidx, nnz, np.uint32)
+        val = self._arr(st.val, nnz)
+        b = self._arr(st.b, n)
+        p = self._arr(st.p, n)
+        return rs, col, val, b, p
+
+    def set_solid_projection(self, on: bool):
+        self.st.solid_projection = 1 if on else 0
+
+    # reference phases
+    def activate(self): lib().wco_activate(self.s)
+    def update_liquid_phi(self): lib().wco_update_liquid_phi(self.s)
+    def update_solid_phi(self): lib().wco_update_solid_phi(self.s)
+    def update_solid_weights(self): lib().wco_update_solid_weights(self.s)
+    def p2g(self): lib().wco_p2g(self.s)
+    def step_velocity(self, dt): lib().wco_step_velocity(self.s, dt)
+    def assemble(self, dt): return lib().wco_assemble(self.s, dt)
+    def solve_pressure(self, tol, maxit): return lib().wco_solve_pressure(self.s, tol, maxit)
+    def apply_pressure(self, dt): lib().wco_apply_pressure(self.s, dt)
+    def g2p_advect(self, dt): lib().wco_g2p_advect(self.s, dt)
+
+    def substep(self, dt, tol=1e-8, maxit=1000):
+        return lib().wco_substep(self.s, dt, tol, maxit)
+
+    def pre_solve(self, dt):
+        """Phases up to and including assembly."""
+        self.activate()
+        self.update_liquid_phi()
+        self.update_solid_phi()
+        self.update_solid_weights()
+        self.p2g()
+        self.step_velocity(dt)
+        return self.assemble(dt)
+
+    @property
+    def last_solve(self):
+        st = self.st
+        return bool(st.last_ok), int(st.last_iters), float(st.last_residual)
+
+
+# ---------------------------------------------------------------- CSR helpers
+
+def csr_args(rs, col, val):
+    rs = np.ascontiguousarray(rs, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    return rs, col, val
+
+
+def spmv(rs, col, val, x):
+    rs, col, val = csr_args(rs, col, val)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(len(rs) - 1)
+    lib().wco_spmv(len(rs) - 1, ptr(rs, _u32p), ptr(col, _u32p), ptr(val), ptr(x), ptr(y))
+    return y
+
+
+def mic0(rs, col, val, tau=0.97, mdr=0.25):
+    rs, col, val = csr_args(rs, col, val)
+    n = len(rs) - 1
+    invdiag = np.zeros(n)
+    lv = np.zeros(max(len(val), 1))
+    lib().wco_mic0_factor(n, ptr(rs, _u32p), ptr(col, _u32p), ptr(val), tau, mdr,
+                          ptr(invdiag), ptr(lv))
+    return invdiag, lv[: len(val)]
+
+
+def apply_precond(rs, col, invdiag, lv, r):
+    rs, col, _ = csr_args(rs, col, lv)
+    n = len(rs) - 1
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    lv = np.ascontiguousarray(lv, dtype=np.float64)
+    invdiag = np.ascontiguousarray(invdiag, dtype=np.float64)
+    z = np.zeros(n)
+    lib().wco_solve_lower(n, ptr(rs, _u32p), ptr(col, _u32p), ptr(invdiag), ptr(lv),
+                          ptr(r), ptr(z))
+    lib().wco_solve_lower_transpose(n, ptr(rs, _u32p), ptr(col, _u32p), ptr(invdiag),
+                                    ptr(lv), ptr(z))
+    return z
+
+
+def solve_lower(rs, col, invdiag, lv, r):
+    rs, col, _ = csr_args(rs, col, lv)
+    n = len(rs) - 1
+    z = np.zeros(n)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    lib().wco_solve_lower(n, ptr(rs, _u32p), ptr(col, _u32p),
+                          ptr(np.ascontiguousarray(invdiag, dtype=np.float64)),
+                          ptr(np.ascontiguousarray(lv, dtype=np.float64)), ptr(r), ptr(z))
+    return z
+
+
+def pcg(rs, col, val, b, tol=1e-8, maxit=500, tau=0.97, mdr=0.25):
+    rs, col, val = csr_args(rs, col, val)
+    n = len(rs) - 1
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n)
+    res = C.c_double(0)
+    it = C.c_int(0)
+    ok = lib().wco_pcg_solve(n, ptr(rs, _u32p), ptr(col, _u32p), ptr(val), ptr(b), ptr(x),
+                             tol, maxit, tau, mdr, C.byref(res), C.byref(it))
+    return bool(ok), x, it.value, res.value
+
+
+# ------------------------------------------------------------ scene configs
+
+def dam_break(n: int, block_hi=(0.4, 0.6, 1.0), seed: int = 0, sigma: float = 0.0):
+    """Particles of the dam-break block at n^3 (dx = 1/n): (grid, x, v)."""
+    dx = 1.0 / n
+    g = grid((n, n, n), dx)
+    x = seed_block(g, (0.0, 0.0, 0.0), block_hi, seed)
+    v = seed_velocity(g, seed, sigma, x) if sigma > 0 else np.zeros_like(x)
+    return g, x, v

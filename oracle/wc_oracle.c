@@ -1,0 +1,1162 @@
+/*
+ * FP64 CPU oracle for the libWetCloth liquid substep -- TEST INFRASTRUCTURE.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this; the product library never does.  See wc_oracle.h for the parity
+ * status of each function.  Reference paths are relative to libWetCloth/Core.
+ *
+ * Build: gcc -O2 -ffp-contract=off (no fused multiply-add: the reference's
+ * x86-64 SSE2 build has none, and the HIP kernels are built the same way).
+ */
+#include "wc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define THETA_CRITERION 0.01 /* Pressure.cpp:20 */
+
+/* ------------------------------------------------------------------ grid */
+
+void wco_grid_init(wco_grid* g, int ni, int nj, int nk, double dx) {
+  g->n[0] = ni;
+  g->n[1] = nj;
+  g->n[2] = nk;
+  g->dx = dx;
+  for (int d = 0; d < 3; ++d) {
+    /* one bucket of ghost cells below, the interior, then two more buckets
+     * (ghost cells above plus the far faces of the last cells) */
+    g->nb[d] = (g->n[d] + WCO_BUCKET - 1) / WCO_BUCKET + 3;
+    g->nx[d] = g->nb[d] * WCO_BUCKET;
+  }
+  g->nbucket = (int64_t)g->nb[0] * g->nb[1] * g->nb[2];
+  g->nnode = g->nbucket * 64;
+}
+
+int64_t wco_bucket_index(const wco_grid* g, int bi, int bj, int bk) {
+  return ((int64_t)bk * g->nb[1] + bj) * g->nb[0] + bi;
+}
+
+int64_t wco_node_index(const wco_grid* g, int i, int j, int k) {
+  return (wco_bucket_index(g, i >> 2, j >> 2, k >> 2) << 6) |
+         ((int64_t)(k & 3) << 4) | ((j & 3) << 2) | (i & 3);
+}
+
+static int in_range(const wco_grid* g, int i, int j, int k) {
+  return i >= 0 && j >= 0 && k >= 0 && i < g->nx[0] && j < g->nx[1] &&
+         k < g->nx[2];
+}
+
+static int node_active(const wco_scene* s, int i, int j, int k) {
+  if (!in_range(&s->g, i, j, k)) return 0;
+  return s->act[wco_bucket_index(&s->g, i >> 2, j >> 2, k >> 2)] != 0;
+}
+
+/* world coordinate of dense node index i on an axis, shifted by off cells */
+static double node_pos(const wco_grid* g, int i, double off) {
+  return ((double)(i - WCO_GHOST) + off) * g->dx;
+}
+
+/* ----------------------------------------------------------- math kernels */
+
+/* MathUtilities.h:416-424 */
+double wco_quad_kernel(double x) {
+  const double ax = fabs(x);
+  if (ax < 0.5) return 0.75 - ax * ax;
+  if (ax < 1.5) return 0.5 * (1.5 - ax) * (1.5 - ax);
+  return 0.0;
+}
+
+/* MathUtilities.h:286-293 */
+double wco_fraction_inside2(double phi_left, double phi_right) {
+  if (phi_left < 0 && phi_right < 0) return 1;
+  if (phi_left < 0 && phi_right >= 0) return phi_left / (phi_left - phi_right);
+  if (phi_left >= 0 && phi_right < 0) return phi_right / (phi_right - phi_left);
+  return 0;
+}
+
+static void rotate4(double* a) {
+  const double t = a[0];
+  a[0] = a[1];
+  a[1] = a[2];
+  a[2] = a[3];
+  a[3] = t;
+}
+
+/* MathUtilities.h:301-375: area fraction of a square with corner values
+ * (bl, br, tl, tr) that lies where phi < 0; corners are walked in the cyclic
+ * order bl, br, tr, tl. */
+double wco_fraction_inside4(double bl, double br, double tl, double tr) {
+  const int inside = (bl < 0) + (tl < 0) + (br < 0) + (tr < 0);
+  double c[4] = {bl, br, tr, tl};
+  if (inside == 4) return 1.0;
+  if (inside == 3) {
+    while (c[0] < 0) rotate4(c);
+    const double s0 = 1.0 - wco_fraction_inside2(c[0], c[3]);
+    const double s1 = 1.0 - wco_fraction_inside2(c[0], c[1]);
+    return 1.0 - 0.5 * s0 * s1;
+  }
+  if (inside == 2) {
+    while (c[0] >= 0 || !(c[1] < 0 || c[2] < 0)) rotate4(c);
+    if (c[1] < 0) {
+      const double sl = wco_fraction_inside2(c[0], c[3]);
+      const double sr = wco_fraction_inside2(c[1], c[2]);
+      return 0.5 * (sl + sr);
+    }
+    const double mid = 0.25 * (c[0] + c[1] + c[2] + c[3]);
+    if (mid < 0) {
+      double area = 0.0;
+      const double s1 = 1.0 - wco_fraction_inside2(c[0], c[3]);
+      const double s3 = 1.0 - wco_fraction_inside2(c[2], c[3]);
+      area += 0.5 * s1 * s3;
+      const double s2 = 1 - wco_fraction_inside2(c[2], c[1]);
+      const double s0 = 1 - wco_fraction_inside2(c[0], c[1]);
+      area += 0.5 * s0 * s2;
+      return 1.0 - area;
+    } else {
+      double area = 0.0;
+      const double s0 = wco_fraction_inside2(c[0], c[1]);
+      const double s1 = wco_fraction_inside2(c[0], c[3]);
+      area += 0.5 * s0 * s1;
+      const double s2 = wco_fraction_inside2(c[2], c[1]);
+      const double s3 = wco_fraction_inside2(c[2], c[3]);
+      area += 0.5 * s2 * s3;
+      return area;
+    }
+  }
+  if (inside == 1) {
+    while (c[0] >= 0) rotate4(c);
+    const double s0 = wco_fraction_inside2(c[0], c[3]);
+    const double s1 = wco_fraction_inside2(c[0], c[1]);
+    return 0.5 * s0 * s1;
+  }
+  return 0;
+}
+
+/* DistanceFields.cpp:68-78 box_phi with zero corner radius, negated for a
+ * container (inside=1 gives sign -1, DistanceFields.cpp:486; identity
+ * rotation, DistanceFields.cpp:601-609): positive inside the box. */
+double wco_box_container_phi(const double pos[3], const double lo[3],
+                             const double hi[3]) {
+  double dd[3], da[3];
+  for (int d = 0; d < 3; ++d) {
+    const double c = 0.5 * (lo[d] + hi[d]);
+    const double e = 0.5 * (hi[d] - lo[d]);
+    dd[d] = fabs(pos[d] - c) - e;
+    da[d] = dd[d] > 0.0 ? dd[d] : 0.0;
+  }
+  double m = dd[0] > dd[1] ? dd[0] : dd[1];
+  m = m > dd[2] ? m : dd[2];
+  if (m > 0.0) m = 0.0;
+  const double phi = m + sqrt(da[0] * da[0] + da[1] * da[1] + da[2] * da[2]);
+  return -phi;
+}
+
+/* ---------------------------------------------------------------- seeding */
+
+static uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint64_t wco_hash3(uint64_t seed, uint64_t a, uint64_t b) {
+  return mix64(mix64(mix64(seed) ^ a) ^ b);
+}
+
+double wco_uniform(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+
+static void block_cells(const wco_grid* g, const double lo[3],
+                        const double hi[3], int c0[3], int cn[3]) {
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = (int)floor(lo[d] / g->dx);
+    const int c1 = (int)ceil(hi[d] / g->dx);
+    cn[d] = c1 > c0[d] ? c1 - c0[d] : 0;
+  }
+}
+
+int64_t wco_seed_block(const wco_grid* g, const double lo[3], const double hi[3],
+                       uint64_t seed, double* x, int64_t cap) {
+  int c0[3], cn[3];
+  block_cells(g, lo, hi, c0, cn);
+  int64_t count = 0;
+  const double dx = g->dx;
+  for (int kk = 0; kk < cn[2]; ++kk)
+    for (int jj = 0; jj < cn[1]; ++jj)
+      for (int ii = 0; ii < cn[0]; ++ii)
+        for (int sub = 0; sub < 8; ++sub) {
+          const uint64_t id =
+              ((((uint64_t)kk * cn[1] + jj) * cn[0] + ii) << 3) | sub;
+          const int cc[3] = {c0[0] + ii, c0[1] + jj, c0[2] + kk};
+          const int sc[3] = {sub & 1, (sub >> 1) & 1, (sub >> 2) & 1};
+          double p[3];
+          int keep = 1;
+          for (int d = 0; d < 3; ++d) {
+            const double u = wco_uniform(wco_hash3(seed, id, (uint64_t)d));
+            p[d] = ((double)cc[d] + ((double)sc[d] + u) * 0.5) * dx;
+            if (!(p[d] >= lo[d] && p[d] < hi[d])) keep = 0;
+          }
+          if (!keep) continue;
+          if (x && count < cap) {
+            for (int d = 0; d < 3; ++d) x[(int64_t)d * cap + count] = p[d];
+          }
+          ++count;
+        }
+  return count;
+}
+
+/* Irwin-Hall: the sum of 12 uniforms minus 6 has mean 0 and variance 1. */
+static double noise_cell(uint64_t seed, int i, int j, int k, int comp) {
+  const uint64_t key = ((uint64_t)(k + (1 << 20)) << 42) |
+                       ((uint64_t)(j + (1 << 20)) << 21) |
+                       (uint64_t)(i + (1 << 20));
+  double s = 0.0;
+  for (int t = 0; t < 12; ++t)
+    s += wco_uniform(wco_hash3(seed ^ 0xA076D1F0C3B5E927ull, key,
+                               (uint64_t)(comp * 16 + t)));
+  return s - 6.0;
+}
+
+/* Stencil of three nodes along one axis for a grid offset by off cells
+ * (TwoDScene.cpp:3700-3713): base node and the clamp that keeps it inside
+ * the dense grid. */
+static int stencil_base(const wco_grid* g, int axis, double x, double off) {
+  const double t = x / g->dx + ((double)WCO_GHOST - off);
+  const double fl = floor(t);
+  const double fr = t - fl;
+  int b = (int)fl + (fr > 0.5 ? 0 : -1);
+  if (b < 0) b = 0;
+  if (b > g->nx[axis] - 3) b = g->nx[axis] - 3;
+  return b;
+}
+
+static void stencil_weights(const wco_grid* g, double x, int base, double off,
+                            double w[3]) {
+  for (int a = 0; a < 3; ++a)
+    w[a] = wco_quad_kernel((x - node_pos(g, base + a, off)) / g->dx);
+}
+
+void wco_seed_velocity(const wco_grid* g, uint64_t seed, double sigma,
+                       int64_t np, const double* x, double* v) {
+  /* filtered field on world cells [-1, n] stored with offset 1 */
+  const int m[3] = {g->n[0] + 2, g->n[1] + 2, g->n[2] + 2};
+  const int q[3] = {g->n[0] + 4, g->n[1] + 4, g->n[2] + 4};
+  const int64_t mm = (int64_t)m[0] * m[1] * m[2];
+  const int64_t qq = (int64_t)q[0] * q[1] * q[2];
+  double* raw = (double*)malloc(sizeof(double) * 3 * qq);
+  double* fil = (double*)malloc(sizeof(double) * 3 * mm);
+  const double scale = sigma / sqrt(27.0);
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < q[2]; ++k)
+      for (int j = 0; j < q[1]; ++j)
+        for (int i = 0; i < q[0]; ++i)
+          raw[c * qq + ((int64_t)k * q[1] + j) * q[0] + i] =
+              noise_cell(seed, i - 2, j - 2, k - 2, c);
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < m[2]; ++k)
+      for (int j = 0; j < m[1]; ++j)
+        for (int i = 0; i < m[0]; ++i) {
+          double s = 0.0;
+          for (int dk = 0; dk < 3; ++dk)
+            for (int dj = 0; dj < 3; ++dj)
+              for (int di = 0; di < 3; ++di)
+                s += raw[c * qq + ((int64_t)(k + dk) * q[1] + (j + dj)) * q[0] +
+                         (i + di)];
+          fil[c * mm + ((int64_t)k * m[1] + j) * m[0] + i] = s * scale;
+        }
+  for (int64_t p = 0; p < np; ++p) {
+    const double px[3] = {x[p], x[np + p], x[2 * np + p]};
+    int base[3];
+    double w[3][3];
+    for (int d = 0; d < 3; ++d) {
+      base[d] = stencil_base(g, d, px[d], 0.5);
+      stencil_weights(g, px[d], base[d], 0.5, w[d]);
+    }
+    for (int c = 0; c < 3; ++c) {
+      double acc = 0.0;
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            /* dense cell -> world cell -> filtered-array index */
+            int wi = base[0] + c0 - WCO_GHOST + 1;
+            int wj = base[1] + c1 - WCO_GHOST + 1;
+            int wk = base[2] + c2 - WCO_GHOST + 1;
+            double f = 0.0;
+            if (wi >= 0 && wj >= 0 && wk >= 0 && wi < m[0] && wj < m[1] &&
+                wk < m[2])
+              f = fil[c * mm + ((int64_t)wk * m[1] + wj) * m[0] + wi];
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            acc += ww * f;
+          }
+      v[(int64_t)c * np + p] = acc;
+    }
+  }
+  free(raw);
+  free(fil);
+}
+
+/* ------------------------------------------------------------------ scene */
+
+static void* xcalloc(size_t n, size_t sz) {
+  void* p = calloc(n ? n : 1, sz);
+  if (!p) abort();
+  return p;
+}
+
+wco_scene* wco_scene_create(int ni, int nj, int nk, double dx, double rho,
+                            double flip, double gx, double gy, double gz,
+                            int64_t np, const double* x, const double* v,
+                            const double* C, double pmass, double pvol,
+                            double radius) {
+  wco_scene* s = (wco_scene*)xcalloc(1, sizeof(wco_scene));
+  wco_grid_init(&s->g, ni, nj, nk, dx);
+  s->rho = rho;
+  s->flip = flip;
+  s->friction = 1.0;
+  s->radius = radius;
+  s->pmass = pmass;
+  s->pvol = pvol;
+  s->gravity[0] = gx;
+  s->gravity[1] = gy;
+  s->gravity[2] = gz;
+  s->solid_projection = 1;
+  for (int d = 0; d < 3; ++d) {
+    s->box_lo[d] = 0.0;
+    s->box_hi[d] = (double)s->g.n[d] * dx;
+  }
+  s->np = np;
+  s->x = (double*)xcalloc(3 * np, sizeof(double));
+  s->v = (double*)xcalloc(3 * np, sizeof(double));
+  s->C = (double*)xcalloc(9 * np, sizeof(double));
+  if (x) memcpy(s->x, x, sizeof(double) * 3 * np);
+  if (v) memcpy(s->v, v, sizeof(double) * 3 * np);
+  if (C) memcpy(s->C, C, sizeof(double) * 9 * np);
+  const int64_t nn = s->g.nnode;
+  s->act = (uint8_t*)xcalloc(s->g.nbucket, 1);
+  s->lphi = (double*)xcalloc(nn, sizeof(double));
+  s->sphi = (double*)xcalloc(nn, sizeof(double));
+  for (int d = 0; d < 3; ++d) {
+    s->w[d] = (double*)xcalloc(nn, sizeof(double));
+    s->mass[d] = (double*)xcalloc(nn, sizeof(double));
+    s->vel[d] = (double*)xcalloc(nn, sizeof(double));
+    s->vol[d] = (double*)xcalloc(nn, sizeof(double));
+    s->rhs[d] = (double*)xcalloc(nn, sizeof(double));
+    s->invC[d] = (double*)xcalloc(nn, sizeof(double));
+    s->ustar[d] = (double*)xcalloc(nn, sizeof(double));
+    s->psi_fs[d] = (double*)xcalloc(nn, sizeof(double));
+    s->unew[d] = (double*)xcalloc(nn, sizeof(double));
+    s->valid[d] = (uint8_t*)xcalloc(nn, 1);
+  }
+  s->dof = (int32_t*)xcalloc(nn, sizeof(int32_t));
+  s->pressure = (double*)xcalloc(nn, sizeof(double));
+  return s;
+}
+
+static void free_system(wco_scene* s) {
+  free(s->rowstart);
+  free(s->colidx);
+  free(s->val);
+  free(s->b);
+  free(s->p);
+  s->rowstart = NULL;
+  s->colidx = NULL;
+  s->val = NULL;
+  s->b = NULL;
+  s->p = NULL;
+}
+
+void wco_scene_destroy(wco_scene* s) {
+  if (!s) return;
+  free(s->x);
+  free(s->v);
+  free(s->C);
+  free(s->act);
+  free(s->lphi);
+  free(s->sphi);
+  for (int d = 0; d < 3; ++d) {
+    free(s->w[d]);
+    free(s->mass[d]);
+    free(s->vel[d]);
+    free(s->vol[d]);
+    free(s->rhs[d]);
+    free(s->invC[d]);
+    free(s->ustar[d]);
+    free(s->psi_fs[d]);
+    free(s->unew[d]);
+    free(s->valid[d]);
+  }
+  free(s->dof);
+  free(s->pressure);
+  free_system(s);
+  free(s);
+}
+
+/* Bucket activation: buckets holding any stencil node of a liquid particle on
+ * the u, v, w, corner or centre grids (findNodes, TwoDScene.cpp:3683-3752,
+ * called from resampleNodes :5076-5085), then one ring of neighbours
+ * (expandFluidNodesMarked(1), TwoDScene.cpp:5021-5064). */
+void wco_activate(wco_scene* s) {
+  const wco_grid* g = &s->g;
+  uint8_t* touched = (uint8_t*)xcalloc(g->nbucket, 1);
+  static const double offs[5][3] = {{0.0, 0.5, 0.5},
+                                    {0.5, 0.0, 0.5},
+                                    {0.5, 0.5, 0.0},
+                                    {0.0, 0.0, 0.0},
+                                    {0.5, 0.5, 0.5}};
+  for (int64_t p = 0; p < s->np; ++p) {
+    const double px[3] = {s->x[p], s->x[s->np + p], s->x[2 * s->np + p]};
+    for (int gi = 0; gi < 5; ++gi) {
+      int blo[3], bhi[3];
+      for (int d = 0; d < 3; ++d) {
+        const int b = stencil_base(g, d, px[d], offs[gi][d]);
+        blo[d] = b >> 2;
+        bhi[d] = (b + 2) >> 2;
+      }
+      for (int bk = blo[2]; bk <= bhi[2]; ++bk)
+        for (int bj = blo[1]; bj <= bhi[1]; ++bj)
+          for (int bi = blo[0]; bi <= bhi[0]; ++bi)
+            touched[wco_bucket_index(g, bi, bj, bk)] = 1;
+    }
+  }
+  for (int bk = 0; bk < g->nb[2]; ++bk)
+    for (int bj = 0; bj < g->nb[1]; ++bj)
+      for (int bi = 0; bi < g->nb[0]; ++bi) {
+        int a = 0;
+        for (int t = -1; t <= 1 && !a; ++t)
+          for (int u = -1; u <= 1 && !a; ++u)
+            for (int r = -1; r <= 1 && !a; ++r) {
+              const int x = bi + r, y = bj + u, z = bk + t;
+              if (x < 0 || y < 0 || z < 0 || x >= g->nb[0] || y >= g->nb[1] ||
+                  z >= g->nb[2])
+                continue;
+              if (touched[wco_bucket_index(g, x, y, z)]) a = 1;
+            }
+        s->act[wco_bucket_index(g, bi, bj, bk)] = (uint8_t)a;
+      }
+  free(touched);
+}
+
+static double cell_solid_phi(const wco_scene* s, int i, int j, int k) {
+  const double p[3] = {node_pos(&s->g, i, 0.5), node_pos(&s->g, j, 0.5),
+                       node_pos(&s->g, k, 0.5)};
+  return wco_box_container_phi(p, s->box_lo, s->box_hi);
+}
+
+/* TwoDScene.cpp:3286-3373 updateLiquidPhi (no surface tension/viscosity) */
+void wco_update_liquid_phi(wco_scene* s) {
+  const wco_grid* g = &s->g;
+  const double dx = g->dx;
+  const double bucket_size = WCO_BUCKET * dx;
+  for (int64_t c = 0; c < g->nnode; ++c) s->lphi[c] = 3.0 * bucket_size;
+  const double r0 = dx * 0.883644 > s->radius ? dx * 0.883644 : s->radius;
+  for (int64_t p = 0; p < s->np; ++p) {
+    const double px[3] = {s->x[p], s->x[s->np + p], s->x[2 * s->np + p]};
+    int base[3];
+    for (int d = 0; d < 3; ++d) base[d] = stencil_base(g, d, px[d], 0.5);
+    for (int c2 = 0; c2 < 3; ++c2)
+      for (int c1 = 0; c1 < 3; ++c1)
+        for (int c0 = 0; c0 < 3; ++c0) {
+          const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+          if (!node_active(s, i, j, k)) continue;
+          const double d0 = px[0] - node_pos(g, i, 0.5);
+          const double d1 = px[1] - node_pos(g, j, 0.5);
+          const double d2 = px[2] - node_pos(g, k, 0.5);
+          const double phi = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - r0;
+          const int64_t n = wco_node_index(g, i, j, k);
+          if (phi < s->lphi[n]) s->lphi[n] = phi;
+        }
+  }
+  for (int k = 0; k < g->nx[2]; ++k)
+    for (int j = 0; j < g->nx[1]; ++j)
+      for (int i = 0; i < g->nx[0]; ++i) {
+        if (!node_active(s, i, j, k)) continue;
+        if (cell_solid_phi(s, i, j, k) < 0.0)
+          s->lphi[wco_node_index(g, i, j, k)] = -0.5 * dx;
+      }
+}
+
+/* TwoDScene.cpp:7940-7970 updateSolidPhi: corner phi of the container in
+ * activated buckets (the container is static: solid face velocity 0). */
+void wco_update_solid_phi(wco_scene* s) {
+  const wco_grid* g = &s->g;
+  for (int k = 0; k < g->nx[2]; ++k)
+    for (int j = 0; j < g->nx[1]; ++j)
+      for (int i = 0; i < g->nx[0]; ++i) {
+        const int64_t n = wco_node_index(g, i, j, k);
+        if (!node_active(s, i, j, k)) {
+          s->sphi[n] = 0.0;
+          continue;
+        }
+        const double p[3] = {node_pos(g, i, 0.0), node_pos(g, j, 0.0),
+                             node_pos(g, k, 0.0)};
+        s->sphi[n] = wco_box_container_phi(p, s->box_lo, s->box_hi);
+      }
+}
+
+/* corner phi as seen by updateSolidWeights: 0.5 dx where the corner's bucket
+ * is not activated (TwoDScene.cpp:1899-1911) */
+static double corner_phi_w(const wco_scene* s, int i, int j, int k) {
+  if (!node_active(s, i, j, k)) return 0.5 * s->g.dx;
+  return s->sphi[wco_node_index(&s->g, i, j, k)];
+}
+
+/* TwoDScene.cpp:1871-1962 updateSolidWeights with the corner order of
+ * connectSolidPhiNodes (TwoDScene.cpp:4447-4590). */
+void wco_update_solid_weights(wco_scene* s) {
+  const wco_grid* g = &s->g;
+  for (int k = 0; k < g->nx[2]; ++k)
+    for (int j = 0; j < g->nx[1]; ++j)
+      for (int i = 0; i < g->nx[0]; ++i) {
+        const int64_t n = wco_node_index(g, i, j, k);
+        if (!node_active(s, i, j, k)) {
+          s->w[0][n] = s->w[1][n] = s->w[2][n] = 0.0;
+          continue;
+        }
+        double f;
+        f = wco_fraction_inside4(
+            corner_phi_w(s, i, j, k), corner_phi_w(s, i, j + 1, k),
+            corner_phi_w(s, i, j, k + 1), corner_phi_w(s, i, j + 1, k + 1));
+        f = 1.0 - f;
+        s->w[0][n] = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+        f = wco_fraction_inside4(
+            corner_phi_w(s, i, j, k), corner_phi_w(s, i, j, k + 1),
+            corner_phi_w(s, i + 1, j, k), corner_phi_w(s, i + 1, j, k + 1));
+        f = 1.0 - f;
+        s->w[1][n] = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+        f = wco_fraction_inside4(
+            corner_phi_w(s, i, j, k), corner_phi_w(s, i, j + 1, k),
+            corner_phi_w(s, i + 1, j, k), corner_phi_w(s, i + 1, j + 1, k));
+        f = 1.0 - f;
+        s->w[2][n] = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+      }
+}
+
+/* face-grid offsets (in cells) of the u, v, w grids: TwoDScene.cpp:5076-5081 */
+static const double face_off[3][3] = {
+    {0.0, 0.5, 0.5}, {0.5, 0.0, 0.5}, {0.5, 0.5, 0.0}};
+
+/* TwoDScene.cpp:6609-6919 mapParticleNodesAPIC, fluid branch (:6700-6713,
+ * :6706-6712): P = sum fm w (v_d + fB.row(d).(x_n - x_p)), M = sum fm w,
+ * V = sum fvol w; u = P/M where M > 1e-20. */
+void wco_p2g(wco_scene* s) {
+  const wco_grid* g = &s->g;
+  const int64_t np = s->np;
+  double* mom[3];
+  for (int d = 0; d < 3; ++d) {
+    memset(s->mass[d], 0, sizeof(double) * g->nnode);
+    memset(s->vel[d], 0, sizeof(double) * g->nnode);
+    memset(s->vol[d], 0, sizeof(double) * g->nnode);
+    mom[d] = (double*)xcalloc(g->nnode, sizeof(double));
+  }
+  const double fm = s->pmass, fvol = s->pvol;
+  for (int64_t p = 0; p < np; ++p) {
+    const double px[3] = {s->x[p], s->x[np + p], s->x[2 * np + p]};
+    for (int d = 0; d < 3; ++d) {
+      int base[3];
+      double w[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], face_off[d][a]);
+        stencil_weights(g, px[a], base[a], face_off[d][a], w[a]);
+      }
+      const double vd = s->v[(int64_t)d * np + p];
+      const double b0 = s->C[(int64_t)(d * 3 + 0) * np + p];
+      const double b1 = s->C[(int64_t)(d * 3 + 1) * np + p];
+      const double b2 = s->C[(int64_t)(d * 3 + 2) * np + p];
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            if (!node_active(s, i, j, k)) continue;
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            const double e0 = node_pos(g, i, face_off[d][0]) - px[0];
+            const double e1 = node_pos(g, j, face_off[d][1]) - px[1];
+            const double e2 = node_pos(g, k, face_off[d][2]) - px[2];
+            const double vel = vd + (b0 * e0 + b1 * e1 + b2 * e2);
+            const int64_t n = wco_node_index(g, i, j, k);
+            mom[d][n] += vel * fm * ww;
+            s->mass[d][n] += fm * ww;
+            s->vol[d][n] += fvol * ww;
+          }
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    for (int64_t n = 0; n < g->nnode; ++n)
+      if (s->mass[d][n] > 1e-20) s->vel[d][n] = mom[d][n] / s->mass[d][n];
+    free(mom[d]);
+  }
+}
+
+/* LinearizedImplicitEuler.cpp:224-337 stepVelocity for liquid without
+ * solids (psi = 0, no elasto mass, drag coefficient 0 by TwoDScene.cpp:4047):
+ *   rhs = 0 - M g (-dt) + M u          (LIE:548-659, SimpleGravityForce:26-54;
+ *                                        the pore-pressure term adds 0)
+ *   C = M + Q*0 with Q = 1 (fallback, LIE:950-955); invC = 1/C or 1 (LIE:1122)
+ *   u* = rhs/M if M > 1e-20 else rhs   (LIE:128-175)
+ *   psi_fs = ((1-psi) invC + psi invCs P) M = invC M (LIE:1052-1107) */
+void wco_step_velocity(wco_scene* s, double dt) {
+  const wco_grid* g = &s->g;
+  for (int d = 0; d < 3; ++d) {
+    const double gd = s->gravity[d];
+    for (int64_t n = 0; n < g->nnode; ++n) {
+      if (!s->act[n >> 6]) {
+        s->rhs[d][n] = s->invC[d][n] = s->ustar[d][n] = s->psi_fs[d][n] = 0.0;
+        continue;
+      }
+      const double M = s->mass[d][n];
+      double rhs = 0.0;
+      rhs -= M * gd * (-dt);
+      rhs += M * s->vel[d][n];
+      s->rhs[d][n] = rhs;
+      const double Q = 1.0, Ms = 0.0;
+      const double Cc = M + Q * Ms;
+      const double invC = Cc > 1e-20 ? 1.0 / Cc : 1.0;
+      s->invC[d][n] = invC;
+      s->ustar[d][n] = M > 1e-20 ? rhs / M : rhs;
+      const double psi = 0.0, invCs = 0.0;
+      const double P = M > 1e-20 ? 0.0 / M : 1.0;
+      s->psi_fs[d][n] = ((1.0 - psi) * invC + psi * invCs * P) * M;
+    }
+  }
+}
+
+/* ------------------------------------------------------- pressure system */
+
+typedef struct {
+  uint32_t col;
+  double val;
+} entry;
+
+static void insert_sorted(entry* row, int* len, uint32_t col, double v) {
+  /* SparseMatrix::add_to_element (SparseMatrix.h:88-103) */
+  int k = 0;
+  for (; k < *len; ++k) {
+    if (row[k].col == col) {
+      row[k].val += v;
+      return;
+    }
+    if (row[k].col > col) break;
+  }
+  for (int t = *len; t > k; --t) row[t] = row[t - 1];
+  row[k].col = col;
+  row[k].val = v;
+  ++*len;
+}
+
+static int cell_exists(const wco_scene* s, int i, int j, int k) {
+  return node_active(s, i, j, k);
+}
+
+/* Pressure.cpp:285-791 solveNodePressure: DOF classification (:348-392),
+ * bucket-major numbering (:394-439, partial sum over buckets in bucket
+ * order), then the 7-point rows in face order left, right, bottom, top,
+ * near, far (:444-791). */
+int64_t wco_assemble(wco_scene* s, double dt) {
+  const wco_grid* g = &s->g;
+  const double dx = g->dx;
+  const double coeff = dt / (dx * dx);
+  free_system(s);
+  int64_t ndof = 0;
+  for (int64_t n = 0; n < g->nnode; ++n) {
+    s->dof[n] = -1;
+    s->pressure[n] = 0.0;
+  }
+  /* storage order is bucket-major, so a running count is the partial sum */
+  for (int64_t bucket = 0; bucket < g->nbucket; ++bucket) {
+    if (!s->act[bucket]) continue;
+    const int bk = (int)(bucket / ((int64_t)g->nb[0] * g->nb[1]));
+    const int bj = (int)((bucket / g->nb[0]) % g->nb[1]);
+    const int bi = (int)(bucket % g->nb[0]);
+    for (int l = 0; l < 64; ++l) {
+      const int i = bi * 4 + (l & 3), j = bj * 4 + ((l >> 2) & 3),
+                k = bk * 4 + (l >> 4);
+      const int64_t n = (bucket << 6) | l;
+      if (!(s->lphi[n] < 0)) continue;
+      /* faces: x at (i),(i+1); y at (j),(j+1); z at (k),(k+1) */
+      const int fi[6][3] = {{i, j, k},     {i + 1, j, k}, {i, j, k},
+                            {i, j + 1, k}, {i, j, k},     {i, j, k + 1}};
+      int open = 0;
+      for (int f = 0; f < 6 && !open; ++f) {
+        const int* q = fi[f];
+        if (!node_active(s, q[0], q[1], q[2])) continue;
+        if (s->w[f >> 1][wco_node_index(g, q[0], q[1], q[2])] > 0) open = 1;
+      }
+      if (open) s->dof[n] = (int32_t)ndof++;
+    }
+  }
+  s->ndof = ndof;
+  s->rowstart = (uint32_t*)xcalloc(ndof + 1, sizeof(uint32_t));
+  s->colidx = (uint32_t*)xcalloc(7 * ndof, sizeof(uint32_t));
+  s->val = (double*)xcalloc(7 * ndof, sizeof(double));
+  s->b = (double*)xcalloc(ndof, sizeof(double));
+  s->p = (double*)xcalloc(ndof, sizeof(double));
+  uint32_t nnz = 0;
+  for (int64_t bucket = 0; bucket < g->nbucket; ++bucket) {
+    if (!s->act[bucket]) continue;
+    const int bk = (int)(bucket / ((int64_t)g->nb[0] * g->nb[1]));
+    const int bj = (int)((bucket / g->nb[0]) % g->nb[1]);
+    const int bi = (int)(bucket % g->nb[0]);
+    for (int l = 0; l < 64; ++l) {
+      const int64_t n = (bucket << 6) | l;
+      const int32_t me = s->dof[n];
+      if (me < 0) continue;
+      const int i = bi * 4 + (l & 3), j = bj * 4 + ((l >> 2) & 3),
+                k = bk * 4 + (l >> 4);
+      const double center_phi = s->lphi[n];
+      entry row[7];
+      int len = 0;
+      double rhs = 0.0;
+      /* face f: axis f>>1; low side (f even) adds, high side subtracts */
+      for (int f = 0; f < 6; ++f) {
+        const int axis = f >> 1, hi = f & 1;
+        int q[3] = {i, j, k};
+        q[axis] += hi;
+        if (!node_active(s, q[0], q[1], q[2])) continue;
+        const int64_t fn = wco_node_index(g, q[0], q[1], q[2]);
+        const double w = s->w[axis][fn];
+        const double uf = s->ustar[axis][fn], pfs = s->psi_fs[axis][fn];
+        const double us = 0.0, psf = 0.0, usolid = 0.0;
+        const double flux = ((uf * pfs + us * psf) * w + usolid * (1.0 - w)) / dx;
+        if (hi)
+          rhs -= flux;
+        else
+          rhs += flux;
+        if (w > 0.0) {
+          const double psi = 0.0, Vs = 0.0, Q = 1.0, invCs = 0.0;
+          const double M = s->mass[axis][fn];
+          const double P = M > 1e-20 ? 0.0 / M : 1.0;
+          const double Vf = s->vol[axis][fn];
+          double term = w * coeff;
+          term *= (1.0 - psi) * s->invC[axis][fn] * (Vf + Vs * Q) +
+                  psi * invCs * (Vs + Vf * P);
+          int nb[3] = {i, j, k};
+          nb[axis] += hi ? 1 : -1;
+          if (!cell_exists(s, nb[0], nb[1], nb[2])) continue;
+          const int64_t nn = wco_node_index(g, nb[0], nb[1], nb[2]);
+          const double nphi = s->lphi[nn];
+          if (nphi < 0.0) {
+            const int32_t other = s->dof[nn];
+            if (other >= 0) {
+              insert_sorted(row, &len, (uint32_t)me, term);
+              insert_sorted(row, &len, (uint32_t)other, -term);
+            }
+          } else {
+            double theta = wco_fraction_inside2(center_phi, nphi);
+            if (theta < THETA_CRITERION) theta = THETA_CRITERION;
+            insert_sorted(row, &len, (uint32_t)me, term / theta);
+          }
+        }
+      }
+      s->b[me] = rhs;
+      s->rowstart[me] = nnz;
+      for (int t = 0; t < len; ++t) {
+        s->colidx[nnz] = row[t].col;
+        s->val[nnz] = row[t].val;
+        ++nnz;
+      }
+    }
+  }
+  s->rowstart[ndof] = nnz;
+  s->nnz_cap = 7 * ndof;
+  return ndof;
+}
+
+/* ------------------------------------------------------------ PCGSolver */
+
+/* SparseMatrix.h:298-322 multiply */
+void wco_spmv(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+              const double* val, const double* x, double* y) {
+  for (int64_t i = 0; i < n; ++i) {
+    double r = 0;
+    for (uint32_t e = rowstart[i]; e < rowstart[i + 1]; ++e) r += val[e] * x[col[e]];
+    y[i] = r;
+  }
+}
+
+/* first entry of row i strictly above the diagonal */
+static uint32_t upper_start(const uint32_t* rowstart, const uint32_t* col,
+                            int64_t i) {
+  uint32_t e = rowstart[i];
+  while (e < rowstart[i + 1] && col[e] <= (uint32_t)i) ++e;
+  return e;
+}
+
+/* PCGSolver.h:97-205 factor_modified_incomplete_cholesky0.  The factor's
+ * column k is row k's strictly-upper part, so lv is indexed like val. */
+void wco_mic0_factor(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                     const double* val, double tau, double mdr,
+                     double* invdiag, double* lv) {
+  double* adiag = (double*)xcalloc(n, sizeof(double));
+  uint32_t* cs = (uint32_t*)xcalloc(n + 1, sizeof(uint32_t));
+  for (int64_t i = 0; i < n; ++i) {
+    invdiag[i] = 0;
+    cs[i] = upper_start(rowstart, col, i);
+    for (uint32_t e = rowstart[i]; e < rowstart[i + 1]; ++e) {
+      if (col[e] > (uint32_t)i)
+        lv[e] = val[e];
+      else if (col[e] == (uint32_t)i)
+        invdiag[i] = adiag[i] = val[e];
+    }
+  }
+  for (int64_t k = 0; k < n; ++k) {
+    if (adiag[k] == 0) continue;
+    if (invdiag[k] < mdr * adiag[k])
+      invdiag[k] = 1 / sqrt(adiag[k]);
+    else
+      invdiag[k] = 1 / sqrt(invdiag[k]);
+    const uint32_t c0 = cs[k], c1 = rowstart[k + 1];
+    for (uint32_t p = c0; p < c1; ++p) lv[p] *= invdiag[k];
+    for (uint32_t p = c0; p < c1; ++p) {
+      const uint32_t j = col[p];
+      const double multiplier = lv[p];
+      double missing = 0;
+      uint32_t a = c0;
+      uint32_t b = rowstart[j];
+      while (a < c1 && col[a] < j) {
+        while (b < rowstart[j + 1]) {
+          if (col[b] < col[a])
+            ++b;
+          else if (col[b] == col[a])
+            break;
+          else {
+            missing += lv[a];
+            break;
+          }
+        }
+        ++a;
+      }
+      if (a < c1 && col[a] == j) invdiag[j] -= multiplier * lv[a];
+      ++a;
+      b = cs[j];
+      const uint32_t bend = rowstart[j + 1];
+      while (a < c1 && b < bend) {
+        if (col[b] < col[a])
+          ++b;
+        else if (col[b] == col[a]) {
+          lv[b] -= multiplier * lv[a];
+          ++a;
+          ++b;
+        } else {
+          missing += lv[a];
+          ++a;
+        }
+      }
+      while (a < c1) {
+        missing += lv[a];
+        ++a;
+      }
+      invdiag[j] -= tau * multiplier * missing;
+    }
+  }
+  free(adiag);
+  free(cs);
+}
+
+/* PCGSolver.h:211-223 solve_lower (column-oriented forward substitution) */
+void wco_solve_lower(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                     const double* invdiag, const double* lv,
+                     const double* rhs, double* out) {
+  memcpy(out, rhs, sizeof(double) * n);
+  for (int64_t i = 0; i < n; ++i) {
+    out[i] *= invdiag[i];
+    for (uint32_t e = upper_start(rowstart, col, i); e < rowstart[i + 1]; ++e)
+      out[col[e]] -= lv[e] * out[i];
+  }
+}
+
+/* PCGSolver.h:226-239 solve_lower_transpose_in_place */
+void wco_solve_lower_transpose(int64_t n, const uint32_t* rowstart,
+                               const uint32_t* col, const double* invdiag,
+                               const double* lv, double* x) {
+  for (int64_t i = n - 1; i >= 0; --i) {
+    for (uint32_t e = upper_start(rowstart, col, i); e < rowstart[i + 1]; ++e)
+      x[i] -= lv[e] * x[col[e]];
+    x[i] *= invdiag[i];
+  }
+}
+
+/* BLASWrapper.h:24-29 (sequential sum; Eigen's dot is SIMD-blocked) */
+double wco_dot(int64_t n, const double* a, const double* b) {
+  double s = 0;
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* BLASWrapper.h:33-50 */
+double wco_abs_max(int64_t n, const double* a) {
+  double m = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (fabs(a[i]) > m) m = fabs(a[i]);
+  return m;
+}
+
+/* PCGSolver.h:260-308 PCGSolver::solve */
+int wco_pcg_solve(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                  const double* val, const double* rhs, double* x, double tol,
+                  int maxit, double tau, double mdr, double* residual_out,
+                  int* iters_out) {
+  if (tol < 1e-30) tol = 1e-30; /* PCGSolver.h:253 */
+  for (int64_t i = 0; i < n; ++i) x[i] = 0;
+  if (n == 0) {
+    *residual_out = 0;
+    *iters_out = 0;
+    return 1;
+  }
+  double* r = (double*)xcalloc(n, sizeof(double));
+  double* z = (double*)xcalloc(n, sizeof(double));
+  double* sv = (double*)xcalloc(n, sizeof(double));
+  double* invdiag = (double*)xcalloc(n, sizeof(double));
+  double* lv = (double*)xcalloc(rowstart[n] ? rowstart[n] : 1, sizeof(double));
+  memcpy(r, rhs, sizeof(double) * n);
+  int ok = 0, it = 0;
+  double res = wco_abs_max(n, r);
+  *residual_out = res;
+  if (res < 1e-30) {
+    *iters_out = 0;
+    ok = 1;
+    goto done;
+  }
+  {
+    const double tolr = tol * res;
+    wco_mic0_factor(n, rowstart, col, val, tau, mdr, invdiag, lv);
+    wco_solve_lower(n, rowstart, col, invdiag, lv, r, z);
+    wco_solve_lower_transpose(n, rowstart, col, invdiag, lv, z);
+    double rho = wco_dot(n, z, r);
+    if (rho == 0 || rho != rho) {
+      *iters_out = 0;
+      ok = 0;
+      goto done;
+    }
+    memcpy(sv, z, sizeof(double) * n);
+    for (it = 0; it < maxit; ++it) {
+      wco_spmv(n, rowstart, col, val, sv, z);
+      const double alpha = rho / wco_dot(n, sv, z);
+      for (int64_t i = 0; i < n; ++i) x[i] += sv[i] * alpha;
+      for (int64_t i = 0; i < n; ++i) r[i] += z[i] * -alpha;
+      res = wco_abs_max(n, r);
+      *residual_out = res;
+      if (res <= tolr) {
+        *iters_out = it + 1;
+        ok = 1;
+        goto done;
+      }
+      wco_solve_lower(n, rowstart, col, invdiag, lv, r, z);
+      wco_solve_lower_transpose(n, rowstart, col, invdiag, lv, z);
+      const double rho_new = wco_dot(n, z, r);
+      const double beta = rho_new / rho;
+      for (int64_t i = 0; i < n; ++i) z[i] += sv[i] * beta;
+      double* t = sv;
+      sv = z;
+      z = t;
+      rho = rho_new;
+    }
+    *iters_out = it;
+    ok = 0;
+  }
+done:
+  free(r);
+  free(z);
+  free(sv);
+  free(invdiag);
+  free(lv);
+  return ok;
+}
+
+int wco_solve_pressure(wco_scene* s, double tol, int maxit) {
+  double res = 0;
+  int it = 0;
+  const int ok = wco_pcg_solve(s->ndof, s->rowstart, s->colidx, s->val, s->b,
+                               s->p, tol, maxit, 0.97, 0.25, &res, &it);
+  s->last_residual = res;
+  s->last_iters = it;
+  s->last_ok = ok;
+  for (int64_t n = 0; n < s->g.nnode; ++n)
+    if (s->dof[n] >= 0) s->pressure[n] = s->p[s->dof[n]];
+  return ok;
+}
+
+/* LIE:2435-2504 applyPressureDragFluid (drag_by_future_solid = false):
+ * addSolidDragRHS adds Q*Ms*us = 0 (LIE:689-717); applyPressureGradsFluidRHS
+ * (Pressure.cpp:1137-1314); performInvLocalSolve u = rhs*invC (LIE:65-104);
+ * extrapolate x3 (Pressure.cpp:1826-1903); then acceptVelocity (LIE:2176) and
+ * constrainLiquidVelocity (TwoDScene.cpp:1827-1866). */
+void wco_apply_pressure(wco_scene* s, double dt) {
+  const wco_grid* g = &s->g;
+  const double dx = g->dx;
+  for (int d = 0; d < 3; ++d) {
+    for (int k = 0; k < g->nx[2]; ++k)
+      for (int j = 0; j < g->nx[1]; ++j)
+        for (int i = 0; i < g->nx[0]; ++i) {
+          const int64_t n = wco_node_index(g, i, j, k);
+          if (!s->act[n >> 6]) {
+            s->unew[d][n] = 0.0;
+            s->valid[d][n] = 0;
+            continue;
+          }
+          const double Q = 1.0, Ms = 0.0, us = 0.0, Vs = 0.0;
+          double rhs = s->rhs[d][n];
+          rhs += Q * Ms * us;
+          uint8_t valid = 0;
+          if (s->w[d][n] > 0.) {
+            int l[3] = {i, j, k};
+            l[d] -= 1;
+            const int lex = cell_exists(s, l[0], l[1], l[2]);
+            const int rex = 1; /* the right cell shares the face's bucket */
+            const int64_t ln = lex ? wco_node_index(g, l[0], l[1], l[2]) : -1;
+            const double left_phi = lex ? s->lphi[ln] : 0.0;
+            const double right_phi = rex ? s->lphi[n] : 0.0;
+            if (left_phi < 0.0 || right_phi < 0.0) {
+              double theta = 1.;
+              if (right_phi >= 0.0 || left_phi >= 0.0) {
+                theta = wco_fraction_inside2(left_phi, right_phi);
+                if (theta < THETA_CRITERION) theta = THETA_CRITERION;
+              }
+              double pg = 0;
+              if (lex) pg -= s->pressure[ln];
+              if (rex) pg += s->pressure[n];
+              pg /= (dx * theta);
+              const double term = pg * dt * (s->vol[d][n] + Vs * Q);
+              rhs -= term;
+              valid = 1;
+            }
+          }
+          s->unew[d][n] = rhs * s->invC[d][n];
+          s->valid[d][n] = valid;
+        }
+    /* two extrapolation layers, each reading copies */
+    double* ucopy = (double*)xcalloc(g->nnode, sizeof(double));
+    uint8_t* vcopy = (uint8_t*)xcalloc(g->nnode, 1);
+    static const int off[6][3] = {{-1, 0, 0}, {1, 0, 0},  {0, -1, 0},
+                                  {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
+    for (int layer = 0; layer < 2; ++layer) {
+      memcpy(ucopy, s->unew[d], sizeof(double) * g->nnode);
+      memcpy(vcopy, s->valid[d], g->nnode);
+      for (int k = 0; k < g->nx[2]; ++k)
+        for (int j = 0; j < g->nx[1]; ++j)
+          for (int i = 0; i < g->nx[0]; ++i) {
+            const int64_t n = wco_node_index(g, i, j, k);
+            if (!s->act[n >> 6] || vcopy[n]) continue;
+            double sum = 0;
+            int count = 0;
+            for (int r = 0; r < 6; ++r) {
+              const int a = i + off[r][0], b = j + off[r][1], c = k + off[r][2];
+              if (!node_active(s, a, b, c)) continue;
+              const int64_t m = wco_node_index(g, a, b, c);
+              if (!vcopy[m]) continue;
+              sum += ucopy[m];
+              ++count;
+            }
+            if (count > 0) {
+              s->unew[d][n] = sum / (double)count;
+              s->valid[d][n] = 1;
+            }
+          }
+    }
+    free(ucopy);
+    free(vcopy);
+    /* constrainLiquidVelocity: lerp toward the (static) solid velocity */
+    for (int64_t n = 0; n < g->nnode; ++n) {
+      if (!s->act[n >> 6]) continue;
+      if (s->w[d][n] == 0.0) {
+        const double fr = s->friction, usolid = 0.0;
+        s->unew[d][n] = (1 - fr) * s->unew[d][n] + fr * usolid;
+      }
+    }
+  }
+}
+
+/* TwoDScene.cpp:6936-7090 mapNodeParticlesAPIC (fluid branch), then
+ * SceneStepper.cpp:17-41 advectScene and TwoDScene.cpp:1776-1822
+ * solidProjection (weights of the corner grid from the pre-advection
+ * position, gradient from the advected one). */
+void wco_g2p_advect(wco_scene* s, double dt) {
+  const wco_grid* g = &s->g;
+  const int64_t np = s->np;
+  const double dx = g->dx;
+  const double invD = 4.0 / (dx * dx); /* MathUtilities.h:728-737 */
+  for (int64_t p = 0; p < np; ++p) {
+    const double px[3] = {s->x[p], s->x[np + p], s->x[2 * np + p]};
+    double fv[3] = {0, 0, 0};
+    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+      int base[3];
+      double w[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], face_off[d][a]);
+        stencil_weights(g, px[a], base[a], face_off[d][a], w[a]);
+      }
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            if (!node_active(s, i, j, k)) continue;
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            const double fnv = s->unew[d][wco_node_index(g, i, j, k)];
+            fv[d] += fnv * ww;
+            const double e0 = node_pos(g, i, face_off[d][0]) - px[0];
+            const double e1 = node_pos(g, j, face_off[d][1]) - px[1];
+            const double e2 = node_pos(g, k, face_off[d][2]) - px[2];
+            B[d * 3 + 0] += fnv * ww * e0 * invD;
+            B[d * 3 + 1] += fnv * ww * e1 * invD;
+            B[d * 3 + 2] += fnv * ww * e2 * invD;
+          }
+    }
+    for (int t = 0; t < 9; ++t) s->C[(int64_t)t * np + p] = B[t] * s->flip;
+    double nx[3];
+    for (int d = 0; d < 3; ++d) {
+      s->v[(int64_t)d * np + p] = fv[d];
+      nx[d] = px[d] + fv[d] * dt;
+    }
+    if (s->solid_projection) {
+      int base[3];
+      double w[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], 0.0);
+        stencil_weights(g, px[a], base[a], 0.0, w[a]);
+      }
+      double phi_ori = 0.0, gr[3] = {0, 0, 0};
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            const double phi = node_active(s, i, j, k)
+                                   ? s->sphi[wco_node_index(g, i, j, k)]
+                                   : 3.0 * dx;
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            phi_ori += phi * ww;
+            gr[0] += phi * invD * ww * (node_pos(g, i, 0.0) - nx[0]);
+            gr[1] += phi * invD * ww * (node_pos(g, j, 0.0) - nx[1]);
+            gr[2] += phi * invD * ww * (node_pos(g, k, 0.0) - nx[2]);
+          }
+      const double gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+      if (gn > 1e-20) {
+        gr[0] /= gn;
+        gr[1] /= gn;
+        gr[2] /= gn;
+      }
+      const double dp[3] = {fv[0] * dt, fv[1] * dt, fv[2] * dt};
+      const double phi_now = phi_ori + (gr[0] * dp[0] + gr[1] * dp[1] + gr[2] * dp[2]);
+      if (phi_now < 0.0) {
+        nx[0] -= phi_now * gr[0];
+        nx[1] -= phi_now * gr[1];
+        nx[2] -= phi_now * gr[2];
+      }
+    }
+    for (int d = 0; d < 3; ++d) s->x[(int64_t)d * np + p] = nx[d];
+  }
+}
+
+/* Core/WetClothCore.cpp:77-300 substep order for liquid-only scenes with
+ * correctionStrength = 0 and no splitting/merging (parity settings). */
+int wco_substep(wco_scene* s, double dt, double tol, int maxit) {
+  wco_activate(s);               /* rebucketize + resampleNodes :115-117 */
+  wco_update_liquid_phi(s);      /* :134 */
+  wco_update_solid_phi(s);       /* :146 */
+  wco_update_solid_weights(s);   /* :150 */
+  wco_p2g(s);                    /* :162 */
+  wco_step_velocity(s, dt);      /* :179 */
+  wco_assemble(s, dt);           /* :191 projectFine -> solveNodePressure */
+  const int ok = wco_solve_pressure(s, tol, maxit);
+  wco_apply_pressure(s, dt);     /* :224, :230, :239 */
+  wco_g2p_advect(s, dt);         /* :248, :258, :262 */
+  return ok;
+}

@@ -1,0 +1,168 @@
+/*
+ * FP64 CPU oracle for the libWetCloth liquid substep (TEST INFRASTRUCTURE).
+ *
+ * This is a plain-C restatement of the reference's per-substep liquid path,
+ * used only as the checker: by tests/, by __graft_entry__.smoke() and by the
+ * cpu_baseline leg of bench.py.  It is never linked into, loaded by or called
+ * from the product library (libwetcloth_amd).
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference root, libWetCloth/Core/...).  The grid is the reference's
+ * sparse bucketed MAC grid restated as a dense grid in the same 4x4x4 bucket
+ * order; bucket activation is emulated so the pressure DOF set and numbering
+ * are the reference's.
+ *
+ * Parity status (see the test that pins each function):
+ *   wco_quad_kernel, wco_fraction_inside2/4   pinned bitwise against the
+ *       reference's MathUtilities.h compiled from its own sources (oracle/_ref)
+ *       and against textbook closed forms.
+ *   wco_mic0_factor, wco_solve_lower(_transpose)  pinned: tau=0 tridiagonal
+ *       == exact Cholesky, tau=1 row-sum preservation, brute-force dense IC(0).
+ *   wco_pcg_solve       pinned against scipy.sparse.linalg.spsolve.
+ *   wco_p2g / wco_g2p_advect  pinned by mass/momentum conservation and affine
+ *       reproduction; P2G momentum brute-forced in numpy on tiny scenes.
+ *   wco_assemble        pinned by symmetry, M-matrix sign pattern, zero row
+ *       sums of interior liquid rows and the textbook 7-point Laplacian.
+ *   wco_apply_pressure  pinned by the reference's own divergence diagnostic
+ *       (discrete divergence <= tol after projection).
+ *   wco_update_liquid_phi / activation / extrapolation: pinned by hand cases
+ *       on tiny grids (tests/test_oracle_grid.py).
+ *   BLAS dot: a sequential sum; the reference uses Eigen's SIMD-blocked dot,
+ *       so dot products match to rounding only (parity by tolerance).
+ */
+#ifndef WC_ORACLE_H
+#define WC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Dense padded MAC grid.  World cell w in [0, n) lies in [w*dx, (w+1)*dx];
+ * dense index i = w + WCO_GHOST.  Nodes are stored in 4x4x4 buckets,
+ * bucket-major (x fastest), then local k*16 + j*4 + i -- the reference's
+ * bucket/local numbering (Core/Sorter.h:66-68, Core/TwoDScene.cpp:3754-3761). */
+#define WCO_GHOST 4
+#define WCO_BUCKET 4
+
+typedef struct {
+  int n[3];     /* interior cells per axis */
+  double dx;
+  int nx[3];    /* dense nodes per axis (multiple of 4) */
+  int nb[3];    /* buckets per axis */
+  int64_t nnode;
+  int64_t nbucket;
+} wco_grid;
+
+void wco_grid_init(wco_grid* g, int ni, int nj, int nk, double dx);
+int64_t wco_node_index(const wco_grid* g, int i, int j, int k);
+int64_t wco_bucket_index(const wco_grid* g, int bi, int bj, int bk);
+
+/* ---- math kernels (Core/MathUtilities.h) ---- */
+double wco_quad_kernel(double x);
+double wco_fraction_inside2(double phi_left, double phi_right);
+double wco_fraction_inside4(double bl, double br, double tl, double tr);
+double wco_box_container_phi(const double pos[3], const double lo[3],
+                             const double hi[3]);
+
+/* ---- synthetic scene seeding ---- */
+uint64_t wco_hash3(uint64_t seed, uint64_t a, uint64_t b);
+double wco_uniform(uint64_t h);
+/* Dam-break block [lo,hi] at 8 particles per cell, jittered in 2x2x2
+ * sub-cells.  Returns the number of particles; writes x (3*cap, SoA planes of
+ * length cap) only when x != NULL and the count fits cap. */
+int64_t wco_seed_block(const wco_grid* g, const double lo[3], const double hi[3],
+                       uint64_t seed, double* x, int64_t cap);
+/* Smoothed Gaussian initial velocity (Irwin-Hall N(0,1) per cell, 3x3x3 box
+ * filter, rescaled to sigma, B2-interpolated from cell centres). */
+void wco_seed_velocity(const wco_grid* g, uint64_t seed, double sigma,
+                       int64_t np, const double* x, double* v);
+
+/* ---- scene ---- */
+typedef struct {
+  wco_grid g;
+  double rho, flip, friction, radius, pmass, pvol;
+  double gravity[3];
+  int solid_projection;
+  double box_lo[3], box_hi[3];  /* container (solid outside) */
+
+  int64_t np;
+  double *x, *v, *C;  /* SoA planes: x[d*np+p], C[(r*3+c)*np+p] = fB(r,c) */
+
+  /* grid state, all of length g.nnode unless noted */
+  uint8_t* act;        /* per bucket */
+  double* lphi;        /* liquid phi at cell centres */
+  double* sphi;        /* solid phi at cell corners (activated buckets) */
+  double* w[3];        /* face solid weights */
+  double* mass[3];     /* P2G face fluid mass */
+  double* vel[3];      /* P2G face fluid velocity */
+  double* vol[3];      /* P2G face fluid volume */
+  double* rhs[3];      /* M u + M g dt, then minus pressure term */
+  double* invC[3];
+  double* ustar[3];    /* predicted velocity (m_node_v_fluid_plus) */
+  double* psi_fs[3];
+  double* unew[3];     /* final face velocity after projection */
+  uint8_t* valid[3];
+  int32_t* dof;        /* cell -> pressure DOF or -1 */
+  double* pressure;    /* per cell */
+
+  /* assembled system (CSR, rows sorted) */
+  int64_t ndof;
+  uint32_t* rowstart;
+  uint32_t* colidx;
+  double* val;
+  double* b;
+  double* p;  /* solution per DOF */
+  int64_t nnz_cap;
+
+  double last_residual;
+  int last_iters;
+  int last_ok;
+} wco_scene;
+
+wco_scene* wco_scene_create(int ni, int nj, int nk, double dx, double rho,
+                            double flip, double gx, double gy, double gz,
+                            int64_t np, const double* x, const double* v,
+                            const double* C, double pmass, double pvol,
+                            double radius);
+void wco_scene_destroy(wco_scene* s);
+
+void wco_activate(wco_scene* s);
+void wco_update_liquid_phi(wco_scene* s);
+void wco_update_solid_phi(wco_scene* s);
+void wco_update_solid_weights(wco_scene* s);
+void wco_p2g(wco_scene* s);
+void wco_step_velocity(wco_scene* s, double dt);
+int64_t wco_assemble(wco_scene* s, double dt);
+int wco_solve_pressure(wco_scene* s, double tol, int maxit);
+void wco_apply_pressure(wco_scene* s, double dt);
+void wco_g2p_advect(wco_scene* s, double dt);
+/* one substep in the reference order (Core/WetClothCore.cpp:77-300) */
+int wco_substep(wco_scene* s, double dt, double tol, int maxit);
+
+/* ---- PCGSolver restatement on a sorted CSR matrix ---- */
+void wco_spmv(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+              const double* val, const double* x, double* y);
+/* lv[e] holds L(col[e], row) for upper entries e (col[e] > row). */
+void wco_mic0_factor(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                     const double* val, double tau, double mdr,
+                     double* invdiag, double* lv);
+void wco_solve_lower(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                     const double* invdiag, const double* lv,
+                     const double* rhs, double* out);
+void wco_solve_lower_transpose(int64_t n, const uint32_t* rowstart,
+                               const uint32_t* col, const double* invdiag,
+                               const double* lv, double* x);
+double wco_dot(int64_t n, const double* a, const double* b);
+double wco_abs_max(int64_t n, const double* a);
+int wco_pcg_solve(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                  const double* val, const double* rhs, double* x, double tol,
+                  int maxit, double tau, double mdr, double* residual_out,
+                  int* iters_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
