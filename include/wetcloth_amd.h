@@ -1,0 +1,170 @@
+/*
+ * wetcloth_amd -- C ABI of the MI355X (gfx950) liquid-substep library.
+ *
+ * Plain pointers and sizes only.  Host pointers everywhere: the context owns
+ * all device memory and keeps particles and grids resident in HBM between
+ * calls; only residual/iteration counts and what the caller explicitly
+ * downloads cross the host/device boundary.  One HIP stream per context; a
+ * context is not thread-safe (like the reference's TwoDScene/PCGSolver).
+ *
+ * Status: 0 = OK, 1 = PCG not converged (PCGSolver::solve returned false),
+ * < 0 = error (wc_last_error has the message).  Nothing throws across the ABI.
+ *
+ * Reference paths below are relative to libWetCloth/Core in the reference.
+ */
+#ifndef WETCLOTH_AMD_H
+#define WETCLOTH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t wc_status;
+#define WC_OK 0
+#define WC_NOT_CONVERGED 1
+#define WC_ERR_ARG (-1)
+#define WC_ERR_HIP (-2)
+#define WC_ERR_STATE (-3)
+#define WC_ERR_NOMEM (-4)
+
+typedef struct wc_ctx wc_ctx;
+
+/* arithmetic of the substep */
+#define WC_FP64 0  /* the reference's precision (scalar = double, MathDefs.h:24) */
+#define WC_MIXED 1 /* FP32 state/transfers/PCG vectors, FP64 reductions + refinement */
+
+/* Scene description: the LiquidInfo / bucket fields the liquid path reads
+ * (App/TwoDSceneXMLParser.cpp:1849-1904, TwoDScene::setBucketInfo
+ * TwoDScene.cpp:1614).  The container is the box [0, n*dx] per axis with
+ * solid outside (a DFT_BOX solid with inside=1, DistanceFields.cpp:68-78). */
+typedef struct wc_desc {
+  int32_t ni, nj, nk;               /* container cells per axis */
+  double dx;                        /* cell size = bucket size / 4 */
+  double liquid_density;            /* kg/m^3 */
+  double flip_coeff;                /* LiquidInfo::flip_coeff (default 0.99) */
+  double liquid_boundary_friction;  /* LiquidInfo (default 1) */
+  double gravity[3];                /* SimpleGravityForce */
+  double particle_mass;             /* uniform m_fluid_m of every particle */
+  double particle_volume;           /* uniform m_fluid_vol */
+  double particle_radius;           /* uniform m_radius */
+  int32_t solid_projection;         /* run solidProjection after advection */
+  int32_t precision;                /* WC_FP64 or WC_MIXED */
+  int64_t max_particles;            /* device capacity reserved at create */
+} wc_desc;
+
+/* Context lifetime. */
+wc_status wc_create(const wc_desc* desc, wc_ctx** out);
+void wc_destroy(wc_ctx* ctx);
+const char* wc_last_error(const wc_ctx* ctx);
+
+/* Particle state.  Replaces the TwoDScene members the transfers read and
+ * write (TwoDScene.h:932-960): m_x (AoS, 4 doubles per particle),
+ * m_fluid_v (AoS 4) and m_fB (Eigen 3N x 3, column-major).  Any pointer may
+ * be NULL (that quantity is zeroed on set / skipped on get). */
+wc_status wc_set_particles(wc_ctx* ctx, int64_t n, const double* x4,
+                           const double* fluid_v4, const double* fB);
+wc_status wc_get_particles(wc_ctx* ctx, double* x4, double* fluid_v4,
+                           double* fB);
+/* Same, structure-of-arrays: x[d*n+p], v[d*n+p], C[(r*3+c)*n+p] = fB(r,c). */
+wc_status wc_set_particles_soa(wc_ctx* ctx, int64_t n, const double* x,
+                               const double* v, const double* C);
+wc_status wc_get_particles_soa(wc_ctx* ctx, double* x, double* v, double* C);
+wc_status wc_num_particles(const wc_ctx* ctx, int64_t* n);
+/* Synthetic dam-break block [lo, hi) at 8 particles per cell, jittered in
+ * 2x2x2 sub-cells, optional smoothed Gaussian velocity of std sigma.  No
+ * reference counterpart (the reference samples liquid from distance fields,
+ * TwoDScene.cpp:5527-5608); seeded on the device so large scenes need no
+ * host staging. */
+wc_status wc_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3],
+                        uint64_t seed, double sigma, int64_t* n_out);
+
+/* ---- the substep, phase by phase, in Core/WetClothCore.cpp:77-300 order ---- */
+
+/* updateParticleBoundingBox + rebucketizeParticles (TwoDScene.cpp:1588-1708):
+ * sort particles by bucket on the device. */
+wc_status wc_rebucketize(wc_ctx* ctx);
+/* resampleNodes bucket activation (TwoDScene.cpp:5069-5120), updateLiquidPhi
+ * (:3286), updateSolidPhi (:7940), updateSolidWeights (:1871) and
+ * TwoDScene::mapParticleNodesAPIC (TwoDScene.h:487, TwoDScene.cpp:6609). */
+wc_status wc_map_particle_nodes(wc_ctx* ctx);
+/* LinearizedImplicitEuler::stepVelocity (LinearizedImplicitEuler.cpp:224). */
+wc_status wc_step_velocity(wc_ctx* ctx, double dt);
+/* LinearizedImplicitEuler::projectFine -> pressure::solveNodePressure
+ * (Pressure.cpp:285-830) with the MIC(0) PCGSolver<double>::solve
+ * (PCGSolver/PCGSolver.h:260) at the solve call site (Pressure.cpp:797). */
+wc_status wc_project(wc_ctx* ctx, double dt, double tol, int32_t maxit,
+                     double* residual_out, int32_t* iters_out);
+/* applyPressureDragFluid (LinearizedImplicitEuler.cpp:2435), acceptVelocity
+ * (:2176) and TwoDScene::constrainLiquidVelocity (TwoDScene.cpp:1827). */
+wc_status wc_apply_pressure(wc_ctx* ctx, double dt);
+/* TwoDScene::mapNodeParticlesAPIC (TwoDScene.h:493, TwoDScene.cpp:6936) fused
+ * with SceneStepper::advectScene (SceneStepper.cpp:17) and
+ * TwoDScene::solidProjection (TwoDScene.cpp:1776). */
+wc_status wc_map_node_particles(wc_ctx* ctx, double dt);
+/* All of the above for one substep (WetClothCore::stepSystem's loop body). */
+wc_status wc_step(wc_ctx* ctx, double dt, double tol, int32_t maxit,
+                  double* residual_out, int32_t* iters_out);
+
+/* ---- diagnostics (downloads for parity checks) ---- */
+#define WC_FIELD_ACT 0       /* uint8 per bucket: activated */
+#define WC_FIELD_LPHI 1      /* double per node: liquid phi at cell centres */
+#define WC_FIELD_W 2         /* double per node and axis: face solid weight */
+#define WC_FIELD_MASS 3      /* double per node and axis: P2G fluid mass */
+#define WC_FIELD_VEL 4       /* double per node and axis: P2G fluid velocity */
+#define WC_FIELD_VOL 5       /* double per node and axis: P2G fluid volume */
+#define WC_FIELD_RHS 6       /* double per node and axis: M u + M g dt */
+#define WC_FIELD_USTAR 7     /* double per node and axis: predicted velocity */
+#define WC_FIELD_UNEW 8      /* double per node and axis: projected velocity */
+#define WC_FIELD_DOF 9       /* int32 per node: pressure DOF or -1 */
+#define WC_FIELD_PRESSURE 10 /* double per node */
+#define WC_FIELD_VALID 11    /* uint8 per node and axis */
+wc_status wc_grid_info(const wc_ctx* ctx, int32_t nx[3], int64_t* nnode,
+                       int64_t* nbucket);
+wc_status wc_download_field(wc_ctx* ctx, int32_t field, int32_t axis,
+                            void* host);
+/* The assembled pressure system in the reference's DOF order as a compact
+ * CSR with sorted rows (robertbridson::FixedSparseMatrix, SparseMatrix.h:222)
+ * plus rhs and solution.  Call with NULL arrays to query ndof and nnz. */
+wc_status wc_download_system(wc_ctx* ctx, int64_t* ndof, int64_t* nnz,
+                             uint32_t* rowstart, uint32_t* col, double* val,
+                             double* rhs, double* solution);
+/* Wall time of the last wc_step's phases on the context's stream (HIP
+ * events), in ms: rebucketize, P2G, grid update, assemble, solve, apply,
+ * G2P -- the reference's timing_buffer phases (App/ParticleSimulation.cpp:19). */
+wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
+
+/* ---- robertbridson::PCGSolver<double> on a host CSR matrix ---- */
+
+/* Literal drop-in for PCGSolver<double>::solve(matrix, rhs, result,
+ * residual_out, iterations_out) (PCGSolver/PCGSolver.h:260-308) after
+ * set_solver_parameters(tol, maxit) (:249).  The CSR is
+ * FixedSparseMatrix::construct_from_matrix's layout (SparseMatrix.h:251):
+ * rows with sorted column indices; the matrix must be symmetric.  Like
+ * form_preconditioner (:322-324) the factor always uses tau = 0.97 and
+ * min-diagonal ratio 0.25. */
+wc_status wc_pcg_solve_csr(int64_t n, const uint32_t* rowstart,
+                           const uint32_t* col, const double* val,
+                           const double* rhs, double* result, double tol,
+                           int32_t maxit, double* residual_out,
+                           int32_t* iters_out);
+/* factor_modified_incomplete_cholesky0 (PCGSolver.h:97-205).  lv is
+ * indexed like val: lv[e] = L(col[e], row) for entries above the diagonal. */
+wc_status wc_mic0_factor_csr(int64_t n, const uint32_t* rowstart,
+                             const uint32_t* col, const double* val,
+                             double tau, double mdr, double* invdiag,
+                             double* lv);
+/* apply_preconditioner: solve_lower then solve_lower_transpose_in_place
+ * (PCGSolver.h:211-239, :326-329). */
+wc_status wc_mic0_apply_csr(int64_t n, const uint32_t* rowstart,
+                            const uint32_t* col, const double* invdiag,
+                            const double* lv, const double* r, double* z);
+/* multiply(FixedSparseMatrix, x, result) (SparseMatrix.h:298-322). */
+wc_status wc_spmv_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                      const double* val, const double* x, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WETCLOTH_AMD_H */

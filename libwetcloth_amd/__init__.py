@@ -1,0 +1,329 @@
+"""libwetcloth_amd: MI355X (gfx950) liquid substep of libWetCloth.
+
+Python binding of the C ABI in include/wetcloth_amd.h (libwcgpu.so, built
+in-tree by libwetcloth_amd/build.py).  The binding never falls back to a CPU
+implementation: if the shared library is missing or no GPU is present the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwcgpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "wetcloth_amd.h")
+
+WC_OK = 0
+WC_NOT_CONVERGED = 1
+WC_FP64 = 0
+WC_MIXED = 1
+
+FIELD = {
+    "act": 0, "lphi": 1, "w": 2, "mass": 3, "vel": 4, "vol": 5, "rhs": 6,
+    "ustar": 7, "unew": 8, "dof": 9, "pressure": 10, "valid": 11,
+}
+_FIELD_DTYPE = {"act": np.uint8, "dof": np.int32, "valid": np.uint8}
+
+PHASES = ("rebucketize", "p2g", "step_velocity", "assemble", "solve", "apply_pressure", "g2p")
+
+_dp = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class WcDesc(C.Structure):
+    _fields_ = [
+        ("ni", C.c_int32), ("nj", C.c_int32), ("nk", C.c_int32),
+        ("dx", C.c_double),
+        ("liquid_density", C.c_double),
+        ("flip_coeff", C.c_double),
+        ("liquid_boundary_friction", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("particle_mass", C.c_double),
+        ("particle_volume", C.c_double),
+        ("particle_radius", C.c_double),
+        ("solid_projection", C.c_int32),
+        ("precision", C.c_int32),
+        ("max_particles", C.c_int64),
+    ]
+
+
+class WetclothError(RuntimeError):
+    pass
+
+
+class NotConverged(WetclothError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libwcgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise WetclothError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                                "or python libwetcloth_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.wc_create.argtypes = [C.POINTER(WcDesc), C.POINTER(vp)]
+        L.wc_destroy.argtypes = [vp]
+        L.wc_destroy.restype = None
+        L.wc_last_error.argtypes = [vp]
+        L.wc_last_error.restype = C.c_char_p
+        for nm in ("wc_set_particles", "wc_set_particles_soa"):
+            getattr(L, nm).argtypes = [vp, C.c_int64, _dp, _dp, _dp]
+        for nm in ("wc_get_particles", "wc_get_particles_soa"):
+            getattr(L, nm).argtypes = [vp, _dp, _dp, _dp]
+        L.wc_num_particles.argtypes = [vp, _i64p]
+        L.wc_seed_block.argtypes = [vp, _dp, _dp, C.c_uint64, C.c_double, _i64p]
+        L.wc_rebucketize.argtypes = [vp]
+        L.wc_map_particle_nodes.argtypes = [vp]
+        L.wc_step_velocity.argtypes = [vp, C.c_double]
+        L.wc_project.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _i32p]
+        L.wc_apply_pressure.argtypes = [vp, C.c_double]
+        L.wc_map_node_particles.argtypes = [vp, C.c_double]
+        L.wc_step.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _i32p]
+        L.wc_grid_info.argtypes = [vp, _i32p, _i64p, _i64p]
+        L.wc_download_field.argtypes = [vp, C.c_int32, C.c_int32, vp]
+        L.wc_download_system.argtypes = [vp, _i64p, _i64p, _u32p, _u32p, _dp, _dp, _dp]
+        L.wc_phase_times.argtypes = [vp, _dp]
+        L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
+                                       C.c_int32, _dp, _i32p]
+        L.wc_mic0_factor_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double, C.c_double,
+                                         _dp, _dp]
+        L.wc_mic0_apply_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp]
+        L.wc_spmv_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    """Entry points declared in include/wetcloth_amd.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:wc_status|void|const char\*)\s+(wc_\w+)\s*\(",
+                                 txt, flags=re.M)))
+
+
+def _ptr(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _check(st, ctx=None, allow_nc=False):
+    if st == WC_OK or (allow_nc and st == WC_NOT_CONVERGED):
+        return st
+    msg = lib().wc_last_error(ctx).decode()
+    if st == WC_NOT_CONVERGED:
+        raise NotConverged(msg or "PCG did not converge")
+    raise WetclothError(f"status {st}: {msg}")
+
+
+class Scene:
+    """A liquid scene resident on the GPU, stepped phase by phase in the
+    reference's substep order (Core/WetClothCore.cpp:77-300)."""
+
+    def __init__(self, n, dx, rho=1000.0, flip=0.99, gravity=(0.0, -9.81, 0.0),
+                 pvol=None, radius=None, solid_projection=True, precision=WC_FP64,
+                 friction=1.0):
+        n = tuple(int(a) for a in n)
+        pvol = dx ** 3 / 8.0 if pvol is None else pvol
+        radius = (3.0 * pvol / (4.0 * np.pi)) ** (1.0 / 3.0) if radius is None else radius
+        d = WcDesc()
+        d.ni, d.nj, d.nk = n
+        d.dx = dx
+        d.liquid_density = rho
+        d.flip_coeff = flip
+        d.liquid_boundary_friction = friction
+        for i in range(3):
+            d.gravity[i] = gravity[i]
+        d.particle_mass = rho * pvol
+        d.particle_volume = pvol
+        d.particle_radius = radius
+        d.solid_projection = 1 if solid_projection else 0
+        d.precision = precision
+        d.max_particles = 0
+        self.desc = d
+        self.n = n
+        self.dx = dx
+        h = C.c_void_p()
+        _check(lib().wc_create(C.byref(d), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().wc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- particles ----
+    def set_particles(self, x, v=None, Cm=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = x.shape[1]
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        Cm = None if Cm is None else np.ascontiguousarray(Cm, dtype=np.float64)
+        _check(lib().wc_set_particles_soa(self.h, n, _ptr(x), _ptr(v), _ptr(Cm)), self.h)
+
+    def set_particles_reference_layout(self, x4, v4=None, fB=None):
+        n = len(x4) // 4
+        _check(lib().wc_set_particles(self.h, n, _ptr(x4), _ptr(v4), _ptr(fB)), self.h)
+
+    def num_particles(self) -> int:
+        n = C.c_int64()
+        _check(lib().wc_num_particles(self.h, C.byref(n)), self.h)
+        return n.value
+
+    def particles(self):
+        n = self.num_particles()
+        x = np.zeros((3, n))
+        v = np.zeros((3, n))
+        Cm = np.zeros((9, n))
+        _check(lib().wc_get_particles_soa(self.h, _ptr(x), _ptr(v), _ptr(Cm)), self.h)
+        return x, v, Cm
+
+    def particles_reference_layout(self):
+        n = self.num_particles()
+        x4 = np.zeros(4 * n)
+        v4 = np.zeros(4 * n)
+        fB = np.zeros(9 * n)
+        _check(lib().wc_get_particles(self.h, _ptr(x4), _ptr(v4), _ptr(fB)), self.h)
+        return x4, v4, fB
+
+    def seed_block(self, lo, hi, seed=0, sigma=0.0) -> int:
+        lo = np.asarray(lo, dtype=np.float64)
+        hi = np.asarray(hi, dtype=np.float64)
+        n = C.c_int64()
+        _check(lib().wc_seed_block(self.h, _ptr(lo), _ptr(hi), C.c_uint64(seed), sigma,
+                                   C.byref(n)), self.h)
+        return n.value
+
+    # ---- substep phases (reference names) ----
+    def rebucketizeParticles(self):
+        _check(lib().wc_rebucketize(self.h), self.h)
+
+    def mapParticleNodesAPIC(self):
+        _check(lib().wc_map_particle_nodes(self.h), self.h)
+
+    def stepVelocity(self, dt):
+        _check(lib().wc_step_velocity(self.h, dt), self.h)
+
+    def projectFine(self, dt, tol=1e-8, maxit=1000):
+        res = C.c_double()
+        it = C.c_int32()
+        st = lib().wc_project(self.h, dt, tol, maxit, C.byref(res), C.byref(it))
+        _check(st, self.h, allow_nc=True)
+        return st == WC_OK, it.value, res.value
+
+    def applyPressureDragFluid(self, dt):
+        _check(lib().wc_apply_pressure(self.h, dt), self.h)
+
+    def mapNodeParticlesAPIC(self, dt):
+        """G2P fused with advectScene and solidProjection."""
+        _check(lib().wc_map_node_particles(self.h, dt), self.h)
+
+    def step(self, dt, tol=1e-8, maxit=1000):
+        res = C.c_double()
+        it = C.c_int32()
+        st = lib().wc_step(self.h, dt, tol, maxit, C.byref(res), C.byref(it))
+        _check(st, self.h, allow_nc=True)
+        return st == WC_OK, it.value, res.value
+
+    # ---- diagnostics ----
+    def grid_info(self):
+        nx = (C.c_int32 * 3)()
+        nn = C.c_int64()
+        nb = C.c_int64()
+        _check(lib().wc_grid_info(self.h, nx, C.byref(nn), C.byref(nb)), self.h)
+        return tuple(nx), nn.value, nb.value
+
+    def field(self, name, axis=0):
+        _, nn, nb = self.grid_info()
+        dt = _FIELD_DTYPE.get(name, np.float64)
+        out = np.zeros(nb if name == "act" else nn, dtype=dt)
+        _check(lib().wc_download_field(self.h, FIELD[name], axis,
+                                       out.ctypes.data_as(C.c_void_p)), self.h)
+        return out
+
+    def system(self):
+        n = C.c_int64()
+        nnz = C.c_int64()
+        L = lib()
+        _check(L.wc_download_system(self.h, C.byref(n), C.byref(nnz), None, None, None, None,
+                                    None), self.h)
+        rs = np.zeros(n.value + 1, dtype=np.uint32)
+        col = np.zeros(nnz.value, dtype=np.uint32)
+        val = np.zeros(nnz.value)
+        b = np.zeros(n.value)
+        p = np.zeros(n.value)
+        _check(L.wc_download_system(self.h, C.byref(n), C.byref(nnz), _ptr(rs, _u32p),
+                                    _ptr(col, _u32p), _ptr(val), _ptr(b), _ptr(p)), self.h)
+        return rs, col, val, b, p
+
+    def phase_times(self) -> dict:
+        ms = (C.c_double * 7)()
+        _check(lib().wc_phase_times(self.h, ms), self.h)
+        return dict(zip(PHASES, list(ms)))
+
+
+# ---- robertbridson::PCGSolver<double> drop-in ----
+
+def _csr(rs, col):
+    return (np.ascontiguousarray(rs, dtype=np.uint32), np.ascontiguousarray(col, dtype=np.uint32))
+
+
+def pcg_solve_csr(rs, col, val, rhs, tol=1e-8, maxit=500):
+    """PCGSolver<double>::solve: returns (converged, x, iterations, residual)."""
+    rs, col = _csr(rs, col)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    n = len(rs) - 1
+    x = np.zeros(n)
+    res = C.c_double()
+    it = C.c_int32()
+    st = lib().wc_pcg_solve_csr(n, _ptr(rs, _u32p), _ptr(col, _u32p), _ptr(val), _ptr(rhs),
+                                _ptr(x), tol, maxit, C.byref(res), C.byref(it))
+    _check(st, None, allow_nc=True)
+    return st == WC_OK, x, it.value, res.value
+
+
+def mic0_factor_csr(rs, col, val, tau=0.97, mdr=0.25):
+    rs, col = _csr(rs, col)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    n = len(rs) - 1
+    invdiag = np.zeros(n)
+    lv = np.zeros(len(val))
+    _check(lib().wc_mic0_factor_csr(n, _ptr(rs, _u32p), _ptr(col, _u32p), _ptr(val), tau, mdr,
+                                    _ptr(invdiag), _ptr(lv)))
+    return invdiag, lv
+
+
+def mic0_apply_csr(rs, col, invdiag, lv, r):
+    rs, col = _csr(rs, col)
+    n = len(rs) - 1
+    z = np.zeros(n)
+    _check(lib().wc_mic0_apply_csr(n, _ptr(rs, _u32p), _ptr(col, _u32p),
+                                   _ptr(np.ascontiguousarray(invdiag, dtype=np.float64)),
+                                   _ptr(np.ascontiguousarray(lv, dtype=np.float64)),
+                                   _ptr(np.ascontiguousarray(r, dtype=np.float64)), _ptr(z)))
+    return z
+
+
+def spmv_csr(rs, col, val, x):
+    rs, col = _csr(rs, col)
+    n = len(rs) - 1
+    y = np.zeros(n)
+    _check(lib().wc_spmv_csr(n, _ptr(rs, _u32p), _ptr(col, _u32p),
+                             _ptr(np.ascontiguousarray(val, dtype=np.float64)),
+                             _ptr(np.ascontiguousarray(x, dtype=np.float64)), _ptr(y)))
+    return y

@@ -1,0 +1,633 @@
+// extern "C" entry points of libwcgpu.so (declared in include/wetcloth_amd.h).
+// Each call validates its arguments, runs its phase on the context's stream
+// and converts HIP failures into a status code plus wc_last_error().
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "wc_ctx.h"
+
+using wc::HipError;
+
+namespace {
+
+thread_local std::string g_free_err;  // errors of context-free calls
+
+wc_status fail(wc_ctx* ctx, wc_status code, const std::string& msg) {
+  if (ctx)
+    ctx->err = msg;
+  else
+    g_free_err = msg;
+  return code;
+}
+
+wc_status hip_fail(wc_ctx* ctx, const HipError& e) {
+  std::string m = std::string("HIP error ") + hipGetErrorString(e.code) +
+                  " in " + e.what + " (" + e.file + ":" + std::to_string(e.line) + ")";
+  return fail(ctx, WC_ERR_HIP, m);
+}
+
+#define WC_TRY(ctx, ...)                               \
+  try {                                                \
+    __VA_ARGS__                                        \
+  } catch (const HipError& e) {                        \
+    return hip_fail(ctx, e);                           \
+  } catch (const std::bad_alloc&) {                    \
+    return fail(ctx, WC_ERR_NOMEM, "host allocation failed"); \
+  }
+
+void alloc_grid(wc_ctx* c) {
+  const int64_t nn = c->g.nnode;
+  for (int d = 0; d < 3; ++d) {
+    c->mass[d].ensure(nn);
+    c->mom[d].ensure(nn);
+    c->vol[d].ensure(nn);
+    c->w[d].ensure(nn);
+    c->rhs[d].ensure(nn);
+    c->invC[d].ensure(nn);
+    c->ustar[d].ensure(nn);
+    c->unew[d].ensure(nn);
+    c->utmp[d].ensure(nn);
+    c->valid[d].ensure(nn);
+    c->vtmp[d].ensure(nn);
+  }
+  c->phikey.ensure(nn);
+  c->lphi.ensure(nn);
+  c->pressure.ensure(nn);
+  c->isdof.ensure(nn + 1);
+  c->dofscan.ensure(nn + 1);
+  c->touched.ensure(c->g.nbucket);
+  c->act.ensure(c->g.nbucket);
+  c->bstart.ensure(c->g.nbucket + 1);
+  hipStream_t st = c->stream;
+  for (int d = 0; d < 3; ++d) {
+    WC_HIP(hipMemsetAsync(c->mass[d].p, 0, sizeof(double) * nn, st));
+    WC_HIP(hipMemsetAsync(c->mom[d].p, 0, sizeof(double) * nn, st));
+    WC_HIP(hipMemsetAsync(c->vol[d].p, 0, sizeof(double) * nn, st));
+    WC_HIP(hipMemsetAsync(c->w[d].p, 0, sizeof(double) * nn, st));
+    WC_HIP(hipMemsetAsync(c->unew[d].p, 0, sizeof(double) * nn, st));
+    WC_HIP(hipMemsetAsync(c->valid[d].p, 0, nn, st));
+  }
+  WC_HIP(hipMemsetAsync(c->lphi.p, 0, sizeof(double) * nn, st));
+  WC_HIP(hipMemsetAsync(c->pressure.p, 0, sizeof(double) * nn, st));
+  WC_HIP(hipMemsetAsync(c->isdof.p, 0, sizeof(uint32_t) * (nn + 1), st));
+  WC_HIP(hipMemsetAsync(c->act.p, 0, c->g.nbucket, st));
+  WC_HIP(hipMemsetAsync(c->bstart.p, 0, sizeof(uint32_t) * (c->g.nbucket + 1), st));
+  WC_HIP(hipStreamSynchronize(st));
+}
+
+void release_all(wc_ctx* c) {
+  c->x.release();
+  c->v.release();
+  c->C.release();
+  c->x2.release();
+  c->v2.release();
+  c->C2.release();
+  c->pid.release();
+  c->pid2.release();
+  c->pkey.release();
+  c->bcount.release();
+  c->bstart.release();
+  c->bcursor.release();
+  c->touched.release();
+  c->act.release();
+  for (int d = 0; d < 3; ++d) {
+    c->mass[d].release();
+    c->mom[d].release();
+    c->vol[d].release();
+    c->w[d].release();
+    c->rhs[d].release();
+    c->invC[d].release();
+    c->ustar[d].release();
+    c->unew[d].release();
+    c->utmp[d].release();
+    c->valid[d].release();
+    c->vtmp[d].release();
+  }
+  c->phikey.release();
+  c->lphi.release();
+  c->isdof.release();
+  c->dofscan.release();
+  c->pressure.release();
+  c->rowlen.release();
+  c->col.release();
+  c->val.release();
+  c->b.release();
+  c->p.release();
+  c->dof_level.release();
+  c->pcg.release();
+  c->scan_tmp.release();
+}
+
+wc::DevCsr system_csr(const wc_ctx* c) {
+  wc::DevCsr A;
+  A.n = c->ndof;
+  A.stride = 7;
+  A.len = c->rowlen.p;
+  A.col = c->col.p;
+  A.val = c->val.p;
+  A.nnz_slots = 7 * c->ndof;
+  return A;
+}
+
+void record(wc_ctx* c, int i) {
+  if (c->events.have) WC_HIP(hipEventRecord(c->events.ev[i], c->stream));
+}
+
+// solve the assembled system into c->p and scatter it onto the grid
+bool solve_system(wc_ctx* c, double tol, int maxit, double* res, int* it) {
+  const wc::DevCsr A = system_csr(c);
+  wc::pcg_prepare(c->stream, c->pcg, A, c->dof_level.p,
+                  c->level_max - c->level_min + 1);
+  const bool ok = wc::pcg_solve(c->stream, c->pcg, A, c->b.p, c->p.p, tol, maxit,
+                                0.97, 0.25, res, it);
+  wc::grid_scatter_pressure(c);
+  c->last_residual = *res;
+  c->last_iters = *it;
+  return ok;
+}
+
+bool ready(wc_ctx* c) { return c && c->stream; }
+
+}  // namespace
+
+extern "C" {
+
+wc_status wc_create(const wc_desc* desc, wc_ctx** out) {
+  if (!desc || !out) return fail(nullptr, WC_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (desc->ni <= 0 || desc->nj <= 0 || desc->nk <= 0 || !(desc->dx > 0))
+    return fail(nullptr, WC_ERR_ARG, "grid size and dx must be positive");
+  if (desc->precision != WC_FP64 && desc->precision != WC_MIXED)
+    return fail(nullptr, WC_ERR_ARG, "unknown precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, WC_ERR_HIP, "no HIP device");
+  wc_ctx* c = new wc_ctx();
+  c->desc = *desc;
+  c->g = wc::make_grid(desc->ni, desc->nj, desc->nk, desc->dx);
+  try {
+    WC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 8; ++i) WC_HIP(hipEventCreate(&c->events.ev[i]));
+    c->events.have = true;
+    alloc_grid(c);
+  } catch (const HipError& e) {
+    g_free_err = std::string("wc_create: ") + hipGetErrorString(e.code) + " in " + e.what;
+    release_all(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return WC_ERR_HIP;
+  }
+  *out = c;
+  return WC_OK;
+}
+
+void wc_destroy(wc_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release_all(c);
+  if (c->events.have)
+    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(c->events.ev[i]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* wc_last_error(const wc_ctx* c) {
+  return c ? c->err.c_str() : g_free_err.c_str();
+}
+
+wc_status wc_set_particles(wc_ctx* c, int64_t n, const double* x4,
+                           const double* v4, const double* fB) {
+  if (!ready(c) || n < 0) return fail(c, WC_ERR_ARG, "bad context or count");
+  if (n >= 0xFFFFFFFFll) return fail(c, WC_ERR_ARG, "too many particles");
+  WC_TRY(c, { wc::particles_set_soa(c, n, x4, v4, fB, true); })
+  return WC_OK;
+}
+
+wc_status wc_get_particles(wc_ctx* c, double* x4, double* v4, double* fB) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, { wc::particles_get_soa(c, x4, v4, fB, true); })
+  return WC_OK;
+}
+
+wc_status wc_set_particles_soa(wc_ctx* c, int64_t n, const double* x,
+                               const double* v, const double* C) {
+  if (!ready(c) || n < 0) return fail(c, WC_ERR_ARG, "bad context or count");
+  if (n >= 0xFFFFFFFFll) return fail(c, WC_ERR_ARG, "too many particles");
+  WC_TRY(c, { wc::particles_set_soa(c, n, x, v, C, false); })
+  return WC_OK;
+}
+
+wc_status wc_get_particles_soa(wc_ctx* c, double* x, double* v, double* C) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, { wc::particles_get_soa(c, x, v, C, false); })
+  return WC_OK;
+}
+
+wc_status wc_num_particles(const wc_ctx* c, int64_t* n) {
+  if (!c || !n) return WC_ERR_ARG;
+  *n = c->np;
+  return WC_OK;
+}
+
+wc_status wc_seed_block(wc_ctx* c, const double lo[3], const double hi[3],
+                        uint64_t seed, double sigma, int64_t* n_out) {
+  if (!ready(c) || !lo || !hi) return fail(c, WC_ERR_ARG, "bad argument");
+  WC_TRY(c, {
+    const int64_t n = wc::particles_seed_block(c, lo, hi, seed, sigma);
+    if (n_out) *n_out = n;
+  })
+  return WC_OK;
+}
+
+wc_status wc_rebucketize(wc_ctx* c) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::particles_sort(c);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_map_particle_nodes(wc_ctx* c) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::particles_p2g(c);
+    wc::grid_activate(c);
+    wc::grid_nodes(c);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_step_velocity(wc_ctx* c, double dt) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::grid_step_velocity(c, dt);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_project(wc_ctx* c, double dt, double tol, int32_t maxit,
+                     double* residual_out, int32_t* iters_out) {
+  if (!ready(c) || maxit < 0) return fail(c, WC_ERR_ARG, "bad argument");
+  bool ok = true;
+  double res = 0;
+  int it = 0;
+  WC_TRY(c, {
+    wc::grid_assemble(c, dt);
+    ok = solve_system(c, tol, maxit, &res, &it);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  if (residual_out) *residual_out = res;
+  if (iters_out) *iters_out = it;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_apply_pressure(wc_ctx* c, double dt) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::grid_apply_pressure(c, dt);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_map_node_particles(wc_ctx* c, double dt) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::particles_g2p_advect(c, dt);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_step(wc_ctx* c, double dt, double tol, int32_t maxit,
+                  double* residual_out, int32_t* iters_out) {
+  if (!ready(c) || maxit < 0) return fail(c, WC_ERR_ARG, "bad argument");
+  bool ok = true;
+  double res = 0;
+  int it = 0;
+  WC_TRY(c, {
+    record(c, 0);
+    wc::particles_sort(c);
+    record(c, 1);
+    wc::particles_p2g(c);
+    wc::grid_activate(c);
+    wc::grid_nodes(c);
+    record(c, 2);
+    wc::grid_step_velocity(c, dt);
+    record(c, 3);
+    wc::grid_assemble(c, dt);
+    record(c, 4);
+    ok = solve_system(c, tol, maxit, &res, &it);
+    record(c, 5);
+    wc::grid_apply_pressure(c, dt);
+    record(c, 6);
+    wc::particles_g2p_advect(c, dt);
+    record(c, 7);
+    WC_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 7; ++i) {
+      float ms = 0.f;
+      WC_HIP(hipEventElapsedTime(&ms, c->events.ev[i], c->events.ev[i + 1]));
+      c->phase_ms[i] = ms;
+    }
+  })
+  if (residual_out) *residual_out = res;
+  if (iters_out) *iters_out = it;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_grid_info(const wc_ctx* c, int32_t nx[3], int64_t* nnode,
+                       int64_t* nbucket) {
+  if (!c) return WC_ERR_ARG;
+  if (nx)
+    for (int d = 0; d < 3; ++d) nx[d] = c->g.nx[d];
+  if (nnode) *nnode = c->g.nnode;
+  if (nbucket) *nbucket = c->g.nbucket;
+  return WC_OK;
+}
+
+wc_status wc_download_field(wc_ctx* c, int32_t field, int32_t axis, void* host) {
+  if (!ready(c) || !host) return fail(c, WC_ERR_ARG, "bad argument");
+  if (axis < 0 || axis > 2) return fail(c, WC_ERR_ARG, "axis must be 0..2");
+  const int64_t nn = c->g.nnode;
+  hipStream_t st = c->stream;
+  WC_TRY(c, {
+    WC_HIP(hipStreamSynchronize(st));
+    switch (field) {
+      case WC_FIELD_ACT:
+        WC_HIP(hipMemcpy(host, c->act.p, c->g.nbucket, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_LPHI:
+        WC_HIP(hipMemcpy(host, c->lphi.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_W:
+        WC_HIP(hipMemcpy(host, c->w[axis].p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_MASS:
+        WC_HIP(hipMemcpy(host, c->mass[axis].p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_VEL:
+        WC_HIP(hipMemcpy(host, c->mom[axis].p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_VOL:
+        WC_HIP(hipMemcpy(host, c->vol[axis].p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_RHS:
+        WC_HIP(hipMemcpy(host, c->rhs[axis].p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_USTAR:
+        WC_HIP(hipMemcpy(host, c->ustar[axis].p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_UNEW:
+        WC_HIP(hipMemcpy(host, c->unew[axis].p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_PRESSURE:
+        WC_HIP(hipMemcpy(host, c->pressure.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_VALID:
+        WC_HIP(hipMemcpy(host, c->valid[axis].p, nn, hipMemcpyDeviceToHost));
+        break;
+      case WC_FIELD_DOF: {
+        std::vector<uint32_t> f(nn), s(nn);
+        WC_HIP(hipMemcpy(f.data(), c->isdof.p, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost));
+        WC_HIP(hipMemcpy(s.data(), c->dofscan.p, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost));
+        int32_t* out = (int32_t*)host;
+        for (int64_t i = 0; i < nn; ++i) out[i] = f[i] ? (int32_t)s[i] : -1;
+        break;
+      }
+      default:
+        return fail(c, WC_ERR_ARG, "unknown field");
+    }
+  })
+  return WC_OK;
+}
+
+wc_status wc_download_system(wc_ctx* c, int64_t* ndof, int64_t* nnz,
+                             uint32_t* rowstart, uint32_t* col, double* val,
+                             double* rhs, double* solution) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  const int64_t n = c->ndof;
+  WC_TRY(c, {
+    WC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> len(n), hc(7 * n);
+    std::vector<double> hv(7 * n);
+    if (n > 0) {
+      WC_HIP(hipMemcpy(len.data(), c->rowlen.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    }
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; ++i) total += len[i];
+    if (ndof) *ndof = n;
+    if (nnz) *nnz = total;
+    if (rowstart || col || val) {
+      if (n > 0) {
+        WC_HIP(hipMemcpy(hc.data(), c->col.p, sizeof(uint32_t) * 7 * n, hipMemcpyDeviceToHost));
+        WC_HIP(hipMemcpy(hv.data(), c->val.p, sizeof(double) * 7 * n, hipMemcpyDeviceToHost));
+      }
+      int64_t e = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        if (rowstart) rowstart[i] = (uint32_t)e;
+        for (uint32_t t = 0; t < len[i]; ++t, ++e) {
+          if (col) col[e] = hc[7 * i + t];
+          if (val) val[e] = hv[7 * i + t];
+        }
+      }
+      if (rowstart) rowstart[n] = (uint32_t)e;
+    }
+    if (rhs && n > 0)
+      WC_HIP(hipMemcpy(rhs, c->b.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (solution && n > 0)
+      WC_HIP(hipMemcpy(solution, c->p.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  })
+  return WC_OK;
+}
+
+wc_status wc_phase_times(const wc_ctx* c, double ms[7]) {
+  if (!c || !ms) return WC_ERR_ARG;
+  for (int i = 0; i < 7; ++i) ms[i] = c->phase_ms[i];
+  return WC_OK;
+}
+
+}  // extern "C"
+
+// ---- context-free PCGSolver drop-in ----
+
+namespace {
+
+struct CsrUpload {
+  wc::DevBuf<uint32_t> rowptr, col;
+  wc::DevBuf<double> val;
+  wc::DevCsr A;
+  void up(hipStream_t st, int64_t n, const uint32_t* rs, const uint32_t* c,
+          const double* v) {
+    const int64_t nnz = rs[n];
+    rowptr.ensure(n + 1);
+    col.ensure(nnz);
+    val.ensure(nnz);
+    WC_HIP(hipMemcpyAsync(rowptr.p, rs, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st));
+    if (nnz) {
+      WC_HIP(hipMemcpyAsync(col.p, c, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
+      if (v) WC_HIP(hipMemcpyAsync(val.p, v, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    }
+    A.n = n;
+    A.rowptr = rowptr.p;
+    A.col = col.p;
+    A.val = val.p;
+    A.nnz_slots = nnz;
+  }
+  void release() {
+    rowptr.release();
+    col.release();
+    val.release();
+  }
+};
+
+struct FreeSession {
+  hipStream_t st = nullptr;
+  CsrUpload csr;
+  wc::PcgWork work;
+  FreeSession() { WC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); }
+  ~FreeSession() {
+    if (st) (void)hipStreamSynchronize(st);
+    csr.release();
+    work.release();
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+wc_status check_csr(int64_t n, const uint32_t* rs, const uint32_t* col) {
+  if (n < 0 || (n > 0 && (!rs || !col))) return WC_ERR_ARG;
+  if (n >= 0xFFFFFFFFll) return WC_ERR_ARG;
+  if (rs && rs[0] != 0) return WC_ERR_ARG;
+  for (int64_t i = 0; i < n; ++i) {
+    if (rs[i + 1] < rs[i]) return WC_ERR_ARG;
+    for (uint32_t e = rs[i]; e < rs[i + 1]; ++e) {
+      if (col[e] >= (uint64_t)n) return WC_ERR_ARG;
+      if (e > rs[i] && col[e] <= col[e - 1]) return WC_ERR_ARG;
+    }
+  }
+  return WC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+wc_status wc_pcg_solve_csr(int64_t n, const uint32_t* rowstart,
+                           const uint32_t* col, const double* val,
+                           const double* rhs, double* result, double tol,
+                           int32_t maxit, double* residual_out,
+                           int32_t* iters_out) {
+  if (check_csr(n, rowstart, col) != WC_OK || (n > 0 && (!val || !rhs || !result)))
+    return fail(nullptr, WC_ERR_ARG, "invalid CSR matrix (rows must be sorted, columns < n)");
+  if (maxit < 0) return fail(nullptr, WC_ERR_ARG, "maxit < 0");
+  double res = 0;
+  int it = 0;
+  bool ok = true;
+  try {
+    FreeSession S;
+    if (n > 0) {
+      S.csr.up(S.st, n, rowstart, col, val);
+      wc::DevBuf<double> b, x;
+      b.ensure(n);
+      x.ensure(n);
+      WC_HIP(hipMemcpyAsync(b.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, S.st));
+      wc::pcg_prepare(S.st, S.work, S.csr.A, nullptr, 0);
+      ok = wc::pcg_solve(S.st, S.work, S.csr.A, b.p, x.p, tol, maxit, 0.97, 0.25,
+                         &res, &it);
+      WC_HIP(hipMemcpyAsync(result, x.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
+      WC_HIP(hipStreamSynchronize(S.st));
+      b.release();
+      x.release();
+    }
+  } catch (const HipError& e) {
+    return hip_fail(nullptr, e);
+  }
+  if (residual_out) *residual_out = res;
+  if (iters_out) *iters_out = it;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_mic0_factor_csr(int64_t n, const uint32_t* rowstart,
+                             const uint32_t* col, const double* val, double tau,
+                             double mdr, double* invdiag, double* lv) {
+  if (check_csr(n, rowstart, col) != WC_OK || (n > 0 && (!val || !invdiag || !lv)))
+    return fail(nullptr, WC_ERR_ARG, "invalid CSR matrix");
+  if (n == 0) return WC_OK;
+  try {
+    FreeSession S;
+    S.csr.up(S.st, n, rowstart, col, val);
+    wc::pcg_prepare(S.st, S.work, S.csr.A, nullptr, 0);
+    wc::pcg_factor(S.st, S.work, S.csr.A, tau, mdr);
+    const int64_t nnz = rowstart[n];
+    WC_HIP(hipMemcpyAsync(invdiag, S.work.invdiag.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
+    // lv is defined only above the diagonal; the rest is returned as zero
+    std::vector<double> h(nnz);
+    if (nnz) WC_HIP(hipMemcpyAsync(h.data(), S.work.lv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, S.st));
+    WC_HIP(hipStreamSynchronize(S.st));
+    for (int64_t i = 0; i < n; ++i)
+      for (uint32_t e = rowstart[i]; e < rowstart[i + 1]; ++e)
+        lv[e] = col[e] > (uint32_t)i ? h[e] : 0.0;
+  } catch (const HipError& e) {
+    return hip_fail(nullptr, e);
+  }
+  return WC_OK;
+}
+
+wc_status wc_mic0_apply_csr(int64_t n, const uint32_t* rowstart,
+                            const uint32_t* col, const double* invdiag,
+                            const double* lv, const double* r, double* z) {
+  if (check_csr(n, rowstart, col) != WC_OK || (n > 0 && (!invdiag || !lv || !r || !z)))
+    return fail(nullptr, WC_ERR_ARG, "invalid CSR matrix");
+  if (n == 0) return WC_OK;
+  try {
+    FreeSession S;
+    S.csr.up(S.st, n, rowstart, col, nullptr);
+    wc::pcg_prepare(S.st, S.work, S.csr.A, nullptr, 0);
+    const int64_t nnz = rowstart[n];
+    WC_HIP(hipMemcpyAsync(S.work.invdiag.p, invdiag, sizeof(double) * n, hipMemcpyHostToDevice, S.st));
+    if (nnz) WC_HIP(hipMemcpyAsync(S.work.lv.p, lv, sizeof(double) * nnz, hipMemcpyHostToDevice, S.st));
+    wc::DevBuf<double> dr, dz;
+    dr.ensure(n);
+    dz.ensure(n);
+    WC_HIP(hipMemcpyAsync(dr.p, r, sizeof(double) * n, hipMemcpyHostToDevice, S.st));
+    wc::pcg_apply(S.st, S.work, S.csr.A, dr.p, dz.p);
+    WC_HIP(hipMemcpyAsync(z, dz.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
+    WC_HIP(hipStreamSynchronize(S.st));
+    dr.release();
+    dz.release();
+  } catch (const HipError& e) {
+    return hip_fail(nullptr, e);
+  }
+  return WC_OK;
+}
+
+wc_status wc_spmv_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                      const double* val, const double* x, double* y) {
+  if (check_csr(n, rowstart, col) != WC_OK || (n > 0 && (!val || !x || !y)))
+    return fail(nullptr, WC_ERR_ARG, "invalid CSR matrix");
+  if (n == 0) return WC_OK;
+  try {
+    FreeSession S;
+    S.csr.up(S.st, n, rowstart, col, val);
+    wc::DevBuf<double> dx, dy;
+    dx.ensure(n);
+    dy.ensure(n);
+    WC_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, S.st));
+    wc::pcg_spmv(S.st, S.csr.A, dx.p, dy.p);
+    WC_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
+    WC_HIP(hipStreamSynchronize(S.st));
+    dx.release();
+    dy.release();
+  } catch (const HipError& e) {
+    return hip_fail(nullptr, e);
+  }
+  return WC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+// Host-side context of the liquid-substep library and the launchers each
+// kernel module exports to the C ABI layer (wc_capi.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/wetcloth_amd.h"
+#include "wc_device.h"
+
+namespace wc {
+
+// A HIP error inside a launcher is turned into this and caught at the ABI.
+struct HipError {
+  hipError_t code;
+  const char* what;
+  const char* file;
+  int line;
+};
+
+#define WC_HIP(expr)                                         \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) throw ::wc::HipError{_e, #expr, __FILE__, __LINE__}; \
+  } while (0)
+#define WC_LAUNCH_CHECK() WC_HIP(hipGetLastError())
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;  // elements
+  void ensure(size_t n) {
+    if (n <= cap && p) return;
+    if (p) WC_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    size_t c = n ? n : 1;
+    WC_HIP(hipMalloc((void**)&p, c * sizeof(T)));
+    cap = c;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Matrix in "padded CSR": row i holds len[i] sorted entries starting at
+// start[i]; with start == nullptr rows start at i*stride.
+struct DevCsr {
+  int64_t n = 0;
+  const uint32_t* start = nullptr;
+  const uint32_t* len = nullptr;
+  const uint32_t* rowptr = nullptr;  // compact CSR (start[i] .. rowptr[i+1])
+  uint32_t stride = 0;
+  const uint32_t* col = nullptr;
+  const double* val = nullptr;
+  int64_t nnz_slots = 0;  // size of col/val arrays
+};
+
+// Device work space of the MIC(0)-PCG solver (PCGSolver.h:310-314).
+struct PcgWork {
+  DevBuf<double> invdiag, adiag, lv, r, z, s, x, partial, scal;
+  DevBuf<uint32_t> tpos;
+  DevBuf<int> flags;  // [0] relaxation changed, [1] factor overflow, [2] max level
+  DevBuf<int32_t> level;
+  DevBuf<uint32_t> lvl_count, lvl_cursor, lvl_list;
+  std::vector<uint32_t> lvl_start;  // host copy: start of each level
+  std::vector<uint32_t> lvl_size;
+  int nlevels = 0;
+  int red_blocks = 0;
+  double* host_scal = nullptr;  // pinned
+  void release();
+};
+
+struct PhaseEvents {
+  hipEvent_t ev[8] = {};
+  bool have = false;
+};
+
+}  // namespace wc
+
+struct wc_ctx {
+  wc_desc desc;
+  wc::Grid g;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // particles, SoA: x[d*cap+p], v[d*cap+p], C[(r*3+c)*cap+p]; pid = the
+  // caller's particle index (the device keeps particles sorted by bucket)
+  int64_t np = 0, pcap = 0;
+  wc::DevBuf<double> x, v, C, x2, v2, C2;
+  wc::DevBuf<uint32_t> pid, pid2, pkey;
+  wc::DevBuf<uint32_t> bcount, bstart, bcursor;  // per bucket (+1)
+  bool sorted = false;  // bstart describes the current particle arrays
+
+  // grid (nnode entries unless noted)
+  wc::DevBuf<uint8_t> touched, act;  // per bucket
+  wc::DevBuf<double> mass[3], mom[3], vol[3];  // mom holds velocity after P2G
+  wc::DevBuf<unsigned long long> phikey;
+  wc::DevBuf<double> lphi, w[3], rhs[3], invC[3], ustar[3], unew[3], utmp[3];
+  wc::DevBuf<uint8_t> valid[3], vtmp[3];
+  wc::DevBuf<uint32_t> isdof, dofscan;  // nnode + 1
+  wc::DevBuf<double> pressure;
+
+  // assembled system: ELL-7 rows in DOF order
+  int64_t ndof = 0;
+  wc::DevBuf<uint32_t> rowlen, col;
+  wc::DevBuf<double> val, b, p;
+  wc::DevBuf<int32_t> dof_level;
+  int32_t level_min = 0, level_max = 0;
+  wc::PcgWork pcg;
+
+  // scan scratch
+  wc::DevBuf<uint32_t> scan_tmp;
+
+  double last_residual = 0.0;
+  int last_iters = 0;
+  double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  wc::PhaseEvents events;
+};
+
+namespace wc {
+
+// ---- wc_scan.hip ----
+// In-place exclusive scan of n uint32 values; returns the total.
+uint32_t exclusive_scan(wc_ctx* ctx, uint32_t* data, int64_t n);
+
+// ---- wc_particles.hip ----
+void particles_sort(wc_ctx* ctx);
+void particles_p2g(wc_ctx* ctx);
+void particles_g2p_advect(wc_ctx* ctx, double dt);
+int64_t particles_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3],
+                             uint64_t seed, double sigma);
+void particles_set_soa(wc_ctx* ctx, int64_t n, const double* x, const double* v,
+                       const double* C, bool aos_ref_layout);
+void particles_get_soa(wc_ctx* ctx, double* x, double* v, double* C,
+                       bool aos_ref_layout);
+
+// ---- wc_grid.hip ----
+void grid_activate(wc_ctx* ctx);
+void grid_nodes(wc_ctx* ctx);
+void grid_step_velocity(wc_ctx* ctx, double dt);
+int64_t grid_assemble(wc_ctx* ctx, double dt);
+void grid_scatter_pressure(wc_ctx* ctx);
+void grid_apply_pressure(wc_ctx* ctx, double dt);
+
+// ---- wc_pcg.hip ----
+// Levels of the lower-triangular dependency DAG: given per row (structured
+// grid: i+j+k) or computed by relaxation when level == nullptr.
+void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
+                 const int32_t* level, int32_t level_span);
+void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
+                double mdr);
+void pcg_apply(hipStream_t st, PcgWork& w, const DevCsr& A, const double* r,
+               double* z);
+void pcg_spmv(hipStream_t st, const DevCsr& A, const double* x, double* y);
+// PCGSolver::solve on device vectors b -> x; returns true when converged.
+bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
+               double* x, double tol, int maxit, double tau, double mdr,
+               double* residual_out, int* iters_out);
+
+}  // namespace wc

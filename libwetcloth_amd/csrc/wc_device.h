@@ -1,0 +1,225 @@
+// Device helpers shared by the gfx950 kernels: the dense MAC grid in 4x4x4
+// bucket-major order, the quadratic B-spline, fraction_inside, the box
+// container's distance field and the counter-based RNG.
+//
+// Every floating-point expression here is written operation for operation
+// like the reference function it restates, and the library is compiled with
+// -ffp-contract=off, so the results are bitwise those of the FP64 CPU form.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wc {
+
+constexpr int kGhost = 4;   // one bucket of ghost cells below the container
+constexpr int kBucket = 4;  // cells per bucket edge (every reference scene)
+
+struct Grid {
+  int n[3];     // container cells per axis
+  double dx;
+  int nx[3];    // dense nodes per axis
+  int nb[3];    // buckets per axis
+  int64_t nnode;
+  int64_t nbucket;
+};
+
+inline Grid make_grid(int ni, int nj, int nk, double dx) {
+  Grid g;
+  g.n[0] = ni;
+  g.n[1] = nj;
+  g.n[2] = nk;
+  g.dx = dx;
+  for (int d = 0; d < 3; ++d) {
+    g.nb[d] = (g.n[d] + kBucket - 1) / kBucket + 3;
+    g.nx[d] = g.nb[d] * kBucket;
+  }
+  g.nbucket = (int64_t)g.nb[0] * g.nb[1] * g.nb[2];
+  g.nnode = g.nbucket * 64;
+  return g;
+}
+
+__host__ __device__ inline int64_t bucket_index(const Grid& g, int bi, int bj,
+                                                int bk) {
+  return ((int64_t)bk * g.nb[1] + bj) * g.nb[0] + bi;
+}
+
+// TwoDScene.cpp:3754-3761 local numbering k*n^2 + j*n + i inside a bucket,
+// buckets in Sorter::bucket_index order (Sorter.h:66-68).
+__host__ __device__ inline int64_t node_index(const Grid& g, int i, int j,
+                                              int k) {
+  return (bucket_index(g, i >> 2, j >> 2, k >> 2) << 6) |
+         ((int64_t)(k & 3) << 4) | ((j & 3) << 2) | (i & 3);
+}
+
+__host__ __device__ inline void node_coords(const Grid& g, int64_t n, int& i,
+                                            int& j, int& k) {
+  const int64_t b = n >> 6;
+  const int l = (int)(n & 63);
+  const int bi = (int)(b % g.nb[0]);
+  const int bj = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bk = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  i = bi * 4 + (l & 3);
+  j = bj * 4 + ((l >> 2) & 3);
+  k = bk * 4 + (l >> 4);
+}
+
+__host__ __device__ inline bool in_range(const Grid& g, int i, int j, int k) {
+  return i >= 0 && j >= 0 && k >= 0 && i < g.nx[0] && j < g.nx[1] &&
+         k < g.nx[2];
+}
+
+__device__ inline bool node_active(const Grid& g, const uint8_t* act, int i,
+                                   int j, int k) {
+  if (!in_range(g, i, j, k)) return false;
+  return act[bucket_index(g, i >> 2, j >> 2, k >> 2)] != 0;
+}
+
+__host__ __device__ inline double node_pos(const Grid& g, int i, double off) {
+  return ((double)(i - kGhost) + off) * g.dx;
+}
+
+// MathUtilities.h:416-424
+__host__ __device__ inline double quad_kernel(double x) {
+  const double ax = fabs(x);
+  if (ax < 0.5) return 0.75 - ax * ax;
+  if (ax < 1.5) return 0.5 * (1.5 - ax) * (1.5 - ax);
+  return 0.0;
+}
+
+// MathUtilities.h:286-293
+__host__ __device__ inline double fraction_inside2(double l, double r) {
+  if (l < 0 && r < 0) return 1;
+  if (l < 0 && r >= 0) return l / (l - r);
+  if (l >= 0 && r < 0) return r / (r - l);
+  return 0;
+}
+
+__host__ __device__ inline void rotate4(double* a) {
+  const double t = a[0];
+  a[0] = a[1];
+  a[1] = a[2];
+  a[2] = a[3];
+  a[3] = t;
+}
+
+// MathUtilities.h:301-375 (corners walked bl, br, tr, tl)
+__host__ __device__ inline double fraction_inside4(double bl, double br,
+                                                   double tl, double tr) {
+  const int inside = (bl < 0) + (tl < 0) + (br < 0) + (tr < 0);
+  double c[4] = {bl, br, tr, tl};
+  if (inside == 4) return 1.0;
+  if (inside == 3) {
+    while (c[0] < 0) rotate4(c);
+    const double s0 = 1.0 - fraction_inside2(c[0], c[3]);
+    const double s1 = 1.0 - fraction_inside2(c[0], c[1]);
+    return 1.0 - 0.5 * s0 * s1;
+  }
+  if (inside == 2) {
+    while (c[0] >= 0 || !(c[1] < 0 || c[2] < 0)) rotate4(c);
+    if (c[1] < 0) {
+      const double sl = fraction_inside2(c[0], c[3]);
+      const double sr = fraction_inside2(c[1], c[2]);
+      return 0.5 * (sl + sr);
+    }
+    const double mid = 0.25 * (c[0] + c[1] + c[2] + c[3]);
+    if (mid < 0) {
+      double area = 0.0;
+      const double s1 = 1.0 - fraction_inside2(c[0], c[3]);
+      const double s3 = 1.0 - fraction_inside2(c[2], c[3]);
+      area += 0.5 * s1 * s3;
+      const double s2 = 1 - fraction_inside2(c[2], c[1]);
+      const double s0 = 1 - fraction_inside2(c[0], c[1]);
+      area += 0.5 * s0 * s2;
+      return 1.0 - area;
+    }
+    double area = 0.0;
+    const double s0 = fraction_inside2(c[0], c[1]);
+    const double s1 = fraction_inside2(c[0], c[3]);
+    area += 0.5 * s0 * s1;
+    const double s2 = fraction_inside2(c[2], c[1]);
+    const double s3 = fraction_inside2(c[2], c[3]);
+    area += 0.5 * s2 * s3;
+    return area;
+  }
+  if (inside == 1) {
+    while (c[0] >= 0) rotate4(c);
+    const double s0 = fraction_inside2(c[0], c[3]);
+    const double s1 = fraction_inside2(c[0], c[1]);
+    return 0.5 * s0 * s1;
+  }
+  return 0;
+}
+
+// DistanceFields.cpp:68-78 box_phi (zero corner radius) of the container
+// [0, n*dx], negated (inside=1, DistanceFields.cpp:486): positive inside.
+__host__ __device__ inline double container_phi(const Grid& g, double p0,
+                                                double p1, double p2) {
+  const double p[3] = {p0, p1, p2};
+  double dd[3], da[3];
+  for (int d = 0; d < 3; ++d) {
+    const double lo = 0.0, hi = (double)g.n[d] * g.dx;
+    const double c = 0.5 * (lo + hi);
+    const double e = 0.5 * (hi - lo);
+    dd[d] = fabs(p[d] - c) - e;
+    da[d] = dd[d] > 0.0 ? dd[d] : 0.0;
+  }
+  double m = dd[0] > dd[1] ? dd[0] : dd[1];
+  m = m > dd[2] ? m : dd[2];
+  if (m > 0.0) m = 0.0;
+  const double phi = m + sqrt(da[0] * da[0] + da[1] * da[1] + da[2] * da[2]);
+  return -phi;
+}
+
+// Three-node stencil base on one axis for a grid offset by off cells
+// (TwoDScene.cpp:3700-3713), clamped into the dense grid.
+__host__ __device__ inline int stencil_base(const Grid& g, int axis, double x,
+                                            double off) {
+  const double t = x / g.dx + ((double)kGhost - off);
+  const double fl = floor(t);
+  const double fr = t - fl;
+  int b = (int)fl + (fr > 0.5 ? 0 : -1);
+  if (b < 0) b = 0;
+  if (b > g.nx[axis] - 3) b = g.nx[axis] - 3;
+  return b;
+}
+
+__host__ __device__ inline void stencil_weights(const Grid& g, double x,
+                                                int base, double off,
+                                                double w[3]) {
+  for (int a = 0; a < 3; ++a)
+    w[a] = quad_kernel((x - node_pos(g, base + a, off)) / g.dx);
+}
+
+// grid offsets (in cells) of the u, v, w face grids (TwoDScene.cpp:5076-5081)
+__host__ __device__ inline double face_off(int d, int a) {
+  return (d == a) ? 0.0 : 0.5;
+}
+
+// Counter-based RNG (splitmix64 finaliser chained over the counter words).
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t hash3(uint64_t seed, uint64_t a,
+                                          uint64_t b) {
+  return mix64(mix64(mix64(seed) ^ a) ^ b);
+}
+__host__ __device__ inline double uniform01(uint64_t h) {
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+
+// Order-preserving map of a double onto uint64 (for atomic min of phi).
+__device__ inline unsigned long long phi_key(double v) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double phi_from_key(unsigned long long k) {
+  const unsigned long long b =
+      (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+}  // namespace wc

@@ -1,0 +1,559 @@
+// Grid sweeps for gfx950: bucket activation, solid weights and liquid phi,
+// the velocity prediction, pressure-DOF numbering and matrix assembly, and
+// the pressure application with extrapolation and the boundary constraint.
+// Every sweep is one thread per node over the bucket-major node array, so a
+// wavefront covers one 4x4x4 bucket and its loads are contiguous.
+#include <math.h>
+
+#include "wc_ctx.h"
+
+namespace wc {
+namespace {
+
+constexpr double kThetaCriterion = 0.01;  // Pressure.cpp:20
+
+inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// expandFluidNodesMarked(1) (TwoDScene.cpp:5021-5064): a bucket is active if
+// it or any of its 26 neighbours holds a stencil node of a particle.
+__global__ void k_activate(Grid g, const uint8_t* touched, uint8_t* act) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= g.nbucket) return;
+  const int bi = (int)(b % g.nb[0]);
+  const int bj = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bk = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  uint8_t a = 0;
+  for (int t = -1; t <= 1; ++t)
+    for (int s = -1; s <= 1; ++s)
+      for (int r = -1; r <= 1; ++r) {
+        const int x = bi + r, y = bj + s, z = bk + t;
+        if (x < 0 || y < 0 || z < 0 || x >= g.nb[0] || y >= g.nb[1] ||
+            z >= g.nb[2])
+          continue;
+        if (touched[bucket_index(g, x, y, z)]) a = 1;
+      }
+  act[b] = a;
+}
+
+// corner phi seen by updateSolidWeights (TwoDScene.cpp:1899-1911)
+__device__ inline double corner_phi_w(const Grid& g, const uint8_t* act, int i,
+                                      int j, int k) {
+  if (!node_active(g, act, i, j, k)) return 0.5 * g.dx;
+  return container_phi(g, node_pos(g, i, 0.0), node_pos(g, j, 0.0),
+                       node_pos(g, k, 0.0));
+}
+
+__device__ inline double clamp01(double f) {
+  return f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+}
+
+struct NodeArgs {
+  Grid g;
+  const uint8_t* act;
+  const unsigned long long* phikey;
+  double* lphi;
+  double* w[3];
+  const double* mass[3];
+  double* mom[3];  // momentum in, velocity out
+};
+
+// updateSolidWeights (TwoDScene.cpp:1871-1962), updateLiquidPhi's solid
+// override and default (:3297-3348), and u = P/M of P2G (:6700-6713).
+__global__ void k_grid_nodes(NodeArgs A) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const Grid& g = A.g;
+  if (n >= g.nnode) return;
+  const double bucket_size = kBucket * g.dx;
+  if (!A.act[n >> 6]) {
+    A.lphi[n] = 3.0 * bucket_size;
+    for (int d = 0; d < 3; ++d) {
+      A.w[d][n] = 0.0;
+      A.mom[d][n] = 0.0;
+    }
+    return;
+  }
+  int i, j, k;
+  node_coords(g, n, i, j, k);
+  const uint8_t* act = A.act;
+  A.w[0][n] = clamp01(1.0 - fraction_inside4(corner_phi_w(g, act, i, j, k),
+                                              corner_phi_w(g, act, i, j + 1, k),
+                                              corner_phi_w(g, act, i, j, k + 1),
+                                              corner_phi_w(g, act, i, j + 1, k + 1)));
+  A.w[1][n] = clamp01(1.0 - fraction_inside4(corner_phi_w(g, act, i, j, k),
+                                              corner_phi_w(g, act, i, j, k + 1),
+                                              corner_phi_w(g, act, i + 1, j, k),
+                                              corner_phi_w(g, act, i + 1, j, k + 1)));
+  A.w[2][n] = clamp01(1.0 - fraction_inside4(corner_phi_w(g, act, i, j, k),
+                                              corner_phi_w(g, act, i, j + 1, k),
+                                              corner_phi_w(g, act, i + 1, j, k),
+                                              corner_phi_w(g, act, i + 1, j + 1, k)));
+  const unsigned long long key = A.phikey[n];
+  double phi = key == 0xFFFFFFFFFFFFFFFFull ? 3.0 * bucket_size : phi_from_key(key);
+  // the stencil minimum never exceeds the default (phi < 3 buckets there)
+  if (phi > 3.0 * bucket_size) phi = 3.0 * bucket_size;
+  if (container_phi(g, node_pos(g, i, 0.5), node_pos(g, j, 0.5),
+                    node_pos(g, k, 0.5)) < 0.0)
+    phi = -0.5 * g.dx;
+  A.lphi[n] = phi;
+  for (int d = 0; d < 3; ++d) {
+    const double M = A.mass[d][n];
+    A.mom[d][n] = M > 1e-20 ? A.mom[d][n] / M : 0.0;
+  }
+}
+
+struct StepArgs {
+  Grid g;
+  const uint8_t* act;
+  const double* mass[3];
+  const double* vel[3];
+  double* rhs[3];
+  double* invC[3];
+  double* ustar[3];
+  double gravity[3];
+  double dt;
+};
+
+// stepVelocity for liquid without solids (LinearizedImplicitEuler.cpp:
+// 224-337): rhs = 0 - M g (-dt) + M u; C = M + Q*0 with Q = 1; invC = 1/C or
+// 1; u* = rhs/M or rhs.
+__global__ void k_step_velocity(StepArgs A) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= A.g.nnode) return;
+  const bool on = A.act[n >> 6] != 0;
+  for (int d = 0; d < 3; ++d) {
+    if (!on) {
+      A.rhs[d][n] = 0.0;
+      A.invC[d][n] = 0.0;
+      A.ustar[d][n] = 0.0;
+      continue;
+    }
+    const double M = A.mass[d][n];
+    double rhs = 0.0;
+    rhs -= M * A.gravity[d] * (-A.dt);
+    rhs += M * A.vel[d][n];
+    A.rhs[d][n] = rhs;
+    const double Q = 1.0, Ms = 0.0;
+    const double Cc = M + Q * Ms;
+    A.invC[d][n] = Cc > 1e-20 ? 1.0 / Cc : 1.0;
+    A.ustar[d][n] = M > 1e-20 ? rhs / M : rhs;
+  }
+}
+
+// psi_fs = ((1-psi) invC + psi invCs P) M with psi = 0 (LIE:1052-1107)
+__device__ inline double psi_fs(double invC, double M) {
+  const double psi = 0.0, invCs = 0.0;
+  const double P = M > 1e-20 ? 0.0 / M : 1.0;
+  return ((1.0 - psi) * invC + psi * invCs * P) * M;
+}
+
+struct DofArgs {
+  Grid g;
+  const uint8_t* act;
+  const double* lphi;
+  const double* w[3];
+  uint32_t* isdof;
+};
+
+// Pressure.cpp:348-392: a cell is a DOF if phi < 0 and one of its existing
+// faces has positive solid weight.
+__global__ void k_dof_flags(DofArgs A) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const Grid& g = A.g;
+  if (n >= g.nnode) return;
+  uint32_t f = 0;
+  if (A.act[n >> 6] && A.lphi[n] < 0) {
+    int i, j, k;
+    node_coords(g, n, i, j, k);
+    for (int face = 0; face < 6 && !f; ++face) {
+      const int axis = face >> 1;
+      int q[3] = {i, j, k};
+      q[axis] += face & 1;
+      if (!node_active(g, A.act, q[0], q[1], q[2])) continue;
+      if (A.w[axis][node_index(g, q[0], q[1], q[2])] > 0) f = 1;
+    }
+  }
+  A.isdof[n] = f;
+}
+
+struct AsmArgs {
+  Grid g;
+  const uint8_t* act;
+  const double* lphi;
+  const double* w[3];
+  const double* mass[3];
+  const double* vol[3];
+  const double* invC[3];
+  const double* ustar[3];
+  const uint32_t* isdof;
+  const uint32_t* dofscan;
+  uint32_t* rowlen;
+  uint32_t* col;
+  double* val;
+  double* b;
+  int32_t* level;
+  int32_t level_min;
+  double dt;
+};
+
+__device__ inline void insert_sorted(uint32_t* cols, double* vals, int& len,
+                                     uint32_t c, double v) {
+  // SparseMatrix::add_to_element (SparseMatrix.h:88-103)
+  int k = 0;
+  for (; k < len; ++k) {
+    if (cols[k] == c) {
+      vals[k] += v;
+      return;
+    }
+    if (cols[k] > c) break;
+  }
+  for (int t = len; t > k; --t) {
+    cols[t] = cols[t - 1];
+    vals[t] = vals[t - 1];
+  }
+  cols[k] = c;
+  vals[k] = v;
+  ++len;
+}
+
+// Pressure.cpp:444-791: one row per DOF, faces left, right, bottom, top,
+// near, far; rhs += or -= ((u* psi_fs + u_s psi_sf) w + u_solid (1-w)) / dx;
+// open faces add term = w dt/dx^2 [(1-psi) invC (Vf + Vs Q) + ...] to the
+// diagonal and -term to a liquid neighbour, or term/theta for an air one.
+__global__ void k_assemble(AsmArgs A) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const Grid& g = A.g;
+  if (n >= g.nnode || !A.isdof[n]) return;
+  const uint32_t me = A.dofscan[n];
+  int i, j, k;
+  node_coords(g, n, i, j, k);
+  const double dx = g.dx;
+  const double coeff = A.dt / (dx * dx);
+  const double center_phi = A.lphi[n];
+  uint32_t cols[7];
+  double vals[7];
+  int len = 0;
+  double rhs = 0.0;
+  for (int f = 0; f < 6; ++f) {
+    const int axis = f >> 1, hi = f & 1;
+    int q[3] = {i, j, k};
+    q[axis] += hi;
+    if (!node_active(g, A.act, q[0], q[1], q[2])) continue;
+    const int64_t fn = node_index(g, q[0], q[1], q[2]);
+    const double w = A.w[axis][fn];
+    const double M = A.mass[axis][fn];
+    const double invC = A.invC[axis][fn];
+    const double uf = A.ustar[axis][fn], pfs = psi_fs(invC, M);
+    const double us = 0.0, psf = 0.0, usolid = 0.0;
+    const double flux = ((uf * pfs + us * psf) * w + usolid * (1.0 - w)) / dx;
+    if (hi)
+      rhs -= flux;
+    else
+      rhs += flux;
+    if (w > 0.0) {
+      const double psi = 0.0, Vs = 0.0, Q = 1.0, invCs = 0.0;
+      const double P = M > 1e-20 ? 0.0 / M : 1.0;
+      const double Vf = A.vol[axis][fn];
+      double term = w * coeff;
+      term *= (1.0 - psi) * invC * (Vf + Vs * Q) + psi * invCs * (Vs + Vf * P);
+      int nb[3] = {i, j, k};
+      nb[axis] += hi ? 1 : -1;
+      if (!node_active(g, A.act, nb[0], nb[1], nb[2])) continue;
+      const int64_t nn = node_index(g, nb[0], nb[1], nb[2]);
+      const double nphi = A.lphi[nn];
+      if (nphi < 0.0) {
+        if (A.isdof[nn]) {
+          insert_sorted(cols, vals, len, me, term);
+          insert_sorted(cols, vals, len, A.dofscan[nn], -term);
+        }
+      } else {
+        double theta = fraction_inside2(center_phi, nphi);
+        if (theta < kThetaCriterion) theta = kThetaCriterion;
+        insert_sorted(cols, vals, len, me, term / theta);
+      }
+    }
+  }
+  A.b[me] = rhs;
+  A.rowlen[me] = (uint32_t)len;
+  for (int t = 0; t < len; ++t) {
+    A.col[7 * (int64_t)me + t] = cols[t];
+    A.val[7 * (int64_t)me + t] = vals[t];
+  }
+  A.level[me] = i + j + k - A.level_min;
+}
+
+__global__ void k_scatter_pressure(int64_t nnode, const uint32_t* isdof,
+                                   const uint32_t* dofscan, const double* p,
+                                   double* pressure) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  pressure[n] = isdof[n] ? p[dofscan[n]] : 0.0;
+}
+
+struct ApplyArgs {
+  Grid g;
+  const uint8_t* act;
+  const double* lphi;
+  const double* pressure;
+  const double* w[3];
+  const double* vol[3];
+  const double* rhs[3];
+  const double* invC[3];
+  double* u[3];
+  uint8_t* valid[3];
+  double dt;
+};
+
+// applyPressureGradsFluidRHS (Pressure.cpp:1137-1314) after addSolidDragRHS
+// (+Q Ms us = 0, LIE:689-717), then performInvLocalSolve u = rhs invC.
+__global__ void k_apply_pressure(ApplyArgs A) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const Grid& g = A.g;
+  if (n >= g.nnode) return;
+  if (!A.act[n >> 6]) {
+    for (int d = 0; d < 3; ++d) {
+      A.u[d][n] = 0.0;
+      A.valid[d][n] = 0;
+    }
+    return;
+  }
+  int i, j, k;
+  node_coords(g, n, i, j, k);
+  const double dx = g.dx;
+  for (int d = 0; d < 3; ++d) {
+    const double Q = 1.0, Ms = 0.0, us = 0.0, Vs = 0.0;
+    double rhs = A.rhs[d][n];
+    rhs += Q * Ms * us;
+    uint8_t valid = 0;
+    if (A.w[d][n] > 0.) {
+      int l[3] = {i, j, k};
+      l[d] -= 1;
+      const bool lex = node_active(g, A.act, l[0], l[1], l[2]);
+      const int64_t ln = lex ? node_index(g, l[0], l[1], l[2]) : -1;
+      const double left_phi = lex ? A.lphi[ln] : 0.0;
+      const double right_phi = A.lphi[n];
+      if (left_phi < 0.0 || right_phi < 0.0) {
+        double theta = 1.;
+        if (right_phi >= 0.0 || left_phi >= 0.0) {
+          theta = fraction_inside2(left_phi, right_phi);
+          if (theta < kThetaCriterion) theta = kThetaCriterion;
+        }
+        double pg = 0;
+        if (lex) pg -= A.pressure[ln];
+        pg += A.pressure[n];
+        pg /= (dx * theta);
+        const double term = pg * A.dt * (A.vol[d][n] + Vs * Q);
+        rhs -= term;
+        valid = 1;
+      }
+    }
+    A.u[d][n] = rhs * A.invC[d][n];
+    A.valid[d][n] = valid;
+  }
+}
+
+struct ExtrapArgs {
+  Grid g;
+  const uint8_t* act;
+  const double* uin[3];
+  const uint8_t* vin[3];
+  double* uout[3];
+  uint8_t* vout[3];
+  const double* w[3];
+  double friction;
+  int constrain;
+};
+
+// one extrapolation layer (Pressure.cpp:1826-1903) reading the previous
+// layer's copies; the last layer also applies constrainLiquidVelocity
+// (TwoDScene.cpp:1827-1866).
+__global__ void k_extrapolate(ExtrapArgs A) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const Grid& g = A.g;
+  if (n >= g.nnode) return;
+  if (!A.act[n >> 6]) {
+    for (int d = 0; d < 3; ++d) {
+      A.uout[d][n] = 0.0;
+      A.vout[d][n] = 0;
+    }
+    return;
+  }
+  int i, j, k;
+  node_coords(g, n, i, j, k);
+  for (int d = 0; d < 3; ++d) {
+    double u = A.uin[d][n];
+    uint8_t valid = A.vin[d][n];
+    if (!valid) {
+      double sum = 0;
+      int count = 0;
+      for (int r = 0; r < 6; ++r) {
+        int q[3] = {i, j, k};
+        q[r >> 1] += (r & 1) ? 1 : -1;
+        if (!node_active(g, A.act, q[0], q[1], q[2])) continue;
+        const int64_t m = node_index(g, q[0], q[1], q[2]);
+        if (!A.vin[d][m]) continue;
+        sum += A.uin[d][m];
+        ++count;
+      }
+      if (count > 0) {
+        u = sum / (double)count;
+        valid = 1;
+      }
+    }
+    if (A.constrain && A.w[d][n] == 0.0) {
+      const double fr = A.friction, usolid = 0.0;
+      u = (1 - fr) * u + fr * usolid;
+    }
+    A.uout[d][n] = u;
+    A.vout[d][n] = valid;
+  }
+}
+
+}  // namespace
+
+void grid_activate(wc_ctx* ctx) {
+  const Grid& g = ctx->g;
+  hipLaunchKernelGGL(k_activate, dim3(blocks(g.nbucket, 256)), dim3(256), 0,
+                     ctx->stream, g, ctx->touched.p, ctx->act.p);
+  WC_LAUNCH_CHECK();
+}
+
+void grid_nodes(wc_ctx* ctx) {
+  NodeArgs A;
+  A.g = ctx->g;
+  A.act = ctx->act.p;
+  A.phikey = ctx->phikey.p;
+  A.lphi = ctx->lphi.p;
+  for (int d = 0; d < 3; ++d) {
+    A.w[d] = ctx->w[d].p;
+    A.mass[d] = ctx->mass[d].p;
+    A.mom[d] = ctx->mom[d].p;
+  }
+  hipLaunchKernelGGL(k_grid_nodes, dim3(blocks(ctx->g.nnode, 256)), dim3(256),
+                     0, ctx->stream, A);
+  WC_LAUNCH_CHECK();
+}
+
+void grid_step_velocity(wc_ctx* ctx, double dt) {
+  StepArgs A;
+  A.g = ctx->g;
+  A.act = ctx->act.p;
+  for (int d = 0; d < 3; ++d) {
+    A.mass[d] = ctx->mass[d].p;
+    A.vel[d] = ctx->mom[d].p;
+    A.rhs[d] = ctx->rhs[d].p;
+    A.invC[d] = ctx->invC[d].p;
+    A.ustar[d] = ctx->ustar[d].p;
+    A.gravity[d] = ctx->desc.gravity[d];
+  }
+  A.dt = dt;
+  hipLaunchKernelGGL(k_step_velocity, dim3(blocks(ctx->g.nnode, 256)),
+                     dim3(256), 0, ctx->stream, A);
+  WC_LAUNCH_CHECK();
+}
+
+int64_t grid_assemble(wc_ctx* ctx, double dt) {
+  const Grid& g = ctx->g;
+  hipStream_t st = ctx->stream;
+  ctx->isdof.ensure(g.nnode + 1);
+  ctx->dofscan.ensure(g.nnode + 1);
+  DofArgs D;
+  D.g = g;
+  D.act = ctx->act.p;
+  D.lphi = ctx->lphi.p;
+  for (int d = 0; d < 3; ++d) D.w[d] = ctx->w[d].p;
+  D.isdof = ctx->isdof.p;
+  hipLaunchKernelGGL(k_dof_flags, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, D);
+  WC_LAUNCH_CHECK();
+  WC_HIP(hipMemsetAsync(ctx->isdof.p + g.nnode, 0, sizeof(uint32_t), st));
+  WC_HIP(hipMemcpyAsync(ctx->dofscan.p, ctx->isdof.p,
+                        sizeof(uint32_t) * (g.nnode + 1), hipMemcpyDeviceToDevice, st));
+  const int64_t ndof = exclusive_scan(ctx, ctx->dofscan.p, g.nnode + 1);
+  ctx->ndof = ndof;
+  ctx->rowlen.ensure(ndof);
+  ctx->col.ensure(7 * ndof);
+  ctx->val.ensure(7 * ndof);
+  ctx->b.ensure(ndof);
+  ctx->p.ensure(ndof);
+  ctx->dof_level.ensure(ndof);
+  ctx->level_min = 0;
+  ctx->level_max = g.nx[0] + g.nx[1] + g.nx[2];
+  if (ndof > 0) {
+    AsmArgs A;
+    A.g = g;
+    A.act = ctx->act.p;
+    A.lphi = ctx->lphi.p;
+    for (int d = 0; d < 3; ++d) {
+      A.w[d] = ctx->w[d].p;
+      A.mass[d] = ctx->mass[d].p;
+      A.vol[d] = ctx->vol[d].p;
+      A.invC[d] = ctx->invC[d].p;
+      A.ustar[d] = ctx->ustar[d].p;
+    }
+    A.isdof = ctx->isdof.p;
+    A.dofscan = ctx->dofscan.p;
+    A.rowlen = ctx->rowlen.p;
+    A.col = ctx->col.p;
+    A.val = ctx->val.p;
+    A.b = ctx->b.p;
+    A.level = ctx->dof_level.p;
+    A.level_min = ctx->level_min;
+    A.dt = dt;
+    hipLaunchKernelGGL(k_assemble, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, A);
+    WC_LAUNCH_CHECK();
+  }
+  return ndof;
+}
+
+void grid_scatter_pressure(wc_ctx* ctx) {
+  const Grid& g = ctx->g;
+  if (ctx->ndof == 0) {
+    WC_HIP(hipMemsetAsync(ctx->pressure.p, 0, sizeof(double) * g.nnode, ctx->stream));
+    return;
+  }
+  hipLaunchKernelGGL(k_scatter_pressure, dim3(blocks(g.nnode, 256)), dim3(256),
+                     0, ctx->stream, g.nnode, ctx->isdof.p, ctx->dofscan.p,
+                     ctx->p.p, ctx->pressure.p);
+  WC_LAUNCH_CHECK();
+}
+
+void grid_apply_pressure(wc_ctx* ctx, double dt) {
+  const Grid& g = ctx->g;
+  hipStream_t st = ctx->stream;
+  ApplyArgs A;
+  A.g = g;
+  A.act = ctx->act.p;
+  A.lphi = ctx->lphi.p;
+  A.pressure = ctx->pressure.p;
+  for (int d = 0; d < 3; ++d) {
+    A.w[d] = ctx->w[d].p;
+    A.vol[d] = ctx->vol[d].p;
+    A.rhs[d] = ctx->rhs[d].p;
+    A.invC[d] = ctx->invC[d].p;
+    A.u[d] = ctx->unew[d].p;
+    A.valid[d] = ctx->valid[d].p;
+  }
+  A.dt = dt;
+  hipLaunchKernelGGL(k_apply_pressure, dim3(blocks(g.nnode, 256)), dim3(256), 0,
+                     st, A);
+  WC_LAUNCH_CHECK();
+  // layer 1: unew -> utmp; layer 2 (+ constraint): utmp -> unew
+  for (int layer = 0; layer < 2; ++layer) {
+    ExtrapArgs E;
+    E.g = g;
+    E.act = ctx->act.p;
+    for (int d = 0; d < 3; ++d) {
+      E.uin[d] = layer == 0 ? ctx->unew[d].p : ctx->utmp[d].p;
+      E.vin[d] = layer == 0 ? ctx->valid[d].p : ctx->vtmp[d].p;
+      E.uout[d] = layer == 0 ? ctx->utmp[d].p : ctx->unew[d].p;
+      E.vout[d] = layer == 0 ? ctx->vtmp[d].p : ctx->valid[d].p;
+      E.w[d] = ctx->w[d].p;
+    }
+    E.friction = ctx->desc.liquid_boundary_friction;
+    E.constrain = layer == 1;
+    hipLaunchKernelGGL(k_extrapolate, dim3(blocks(g.nnode, 256)), dim3(256), 0,
+                       st, E);
+    WC_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace wc

@@ -1,0 +1,680 @@
+// Particle-side kernels for gfx950: seeding, the bucket sort, P2G and G2P.
+//
+// Particles live in HBM as structure-of-arrays planes and are kept sorted by
+// the 4x4x4 bucket of their cell (the reference re-buckets every substep,
+// TwoDScene.cpp:1657-1708).  P2G runs one workgroup per bucket: the bucket's
+// particles scatter their 3x3x3 quadratic B-spline stencils into an LDS tile
+// of 8x8x8 nodes (the bucket plus a two-node halo) for the u, v and w grids,
+// and the tile is then added to HBM with one atomic per touched node.  The
+// same pass takes the stencil-limited minimum of the liquid distance
+// (updateLiquidPhi) and marks the buckets the stencils touch (resampleNodes'
+// activation).
+#include <math.h>
+
+#include <vector>
+
+#include "wc_ctx.h"
+
+namespace wc {
+namespace {
+
+constexpr int kP2GThreads = 256;
+constexpr int kTile = 8;  // LDS nodes per axis: bucket (4) + halo 2 below, 2 above
+constexpr int kTileNodes = kTile * kTile * kTile;
+constexpr unsigned long long kPhiUntouched = 0xFFFFFFFFFFFFFFFFull;
+
+struct PView {
+  int64_t n, cap;
+  const double* x;
+  const double* v;
+  const double* C;
+};
+
+__device__ inline int clampi(int a, int lo, int hi) {
+  return a < lo ? lo : (a > hi ? hi : a);
+}
+
+__device__ inline int particle_bucket_axis(const Grid& g, int axis, double x) {
+  const int c = (int)floor(x / g.dx) + kGhost;
+  return clampi(c, 0, g.nx[axis] - 1) >> 2;
+}
+
+__global__ void k_sort_keys(Grid g, const double* x, int64_t n, int64_t cap,
+                            uint32_t* key, uint32_t* count) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int bi = particle_bucket_axis(g, 0, x[p]);
+  const int bj = particle_bucket_axis(g, 1, x[cap + p]);
+  const int bk = particle_bucket_axis(g, 2, x[2 * cap + p]);
+  const uint32_t b = (uint32_t)bucket_index(g, bi, bj, bk);
+  key[p] = b;
+  atomicAdd(&count[b], 1u);
+}
+
+__global__ void k_sort_scatter(int64_t n, int64_t cap, const uint32_t* key,
+                               const uint32_t* start, uint32_t* cursor,
+                               const double* x, const double* v,
+                               const double* C, const uint32_t* pid, double* x2,
+                               double* v2, double* C2, uint32_t* pid2) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t b = key[p];
+  const int64_t q = start[b] + atomicAdd(&cursor[b], 1u);
+  for (int d = 0; d < 3; ++d) {
+    x2[d * cap + q] = x[d * cap + p];
+    v2[d * cap + q] = v[d * cap + p];
+  }
+  for (int t = 0; t < 9; ++t) C2[t * cap + q] = C[t * cap + p];
+  pid2[q] = pid[p];
+}
+
+struct P2GArgs {
+  Grid g;
+  const double* x;
+  const double* v;
+  const double* C;
+  int64_t cap;
+  const uint32_t* bstart;
+  double* mass[3];
+  double* mom[3];
+  double* vol[3];
+  unsigned long long* phikey;
+  uint8_t* touched;
+  double fm, fvol, r0;
+};
+
+// grid offsets of the five stencils resampleNodes builds (TwoDScene.cpp:
+// 5076-5085): u, v, w faces, corners (solid phi) and cell centres (pressure)
+__device__ inline double grid5_off(int gi, int a) {
+  if (gi < 3) return face_off(gi, a);
+  return gi == 3 ? 0.0 : 0.5;
+}
+
+__global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs A) {
+  const int64_t b = blockIdx.x;
+  const uint32_t pbeg = A.bstart[b], pend = A.bstart[b + 1];
+  if (pbeg == pend) return;
+  const Grid& g = A.g;
+  const int bi = (int)(b % g.nb[0]);
+  const int bj = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bk = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  const int o0 = bi * 4 - 2, o1 = bj * 4 - 2, o2 = bk * 4 - 2;
+
+  __shared__ double sm[3][3][kTileNodes];  // [axis][mass, momentum, volume]
+  __shared__ unsigned long long sphi[kTileNodes];
+  __shared__ int stouch[27];
+  for (int i = threadIdx.x; i < kTileNodes; i += kP2GThreads) {
+    for (int d = 0; d < 3; ++d) {
+      sm[d][0][i] = 0.0;
+      sm[d][1][i] = 0.0;
+      sm[d][2][i] = 0.0;
+    }
+    sphi[i] = kPhiUntouched;
+  }
+  if (threadIdx.x < 27) stouch[threadIdx.x] = 0;
+  __syncthreads();
+
+  for (uint32_t p = pbeg + threadIdx.x; p < pend; p += kP2GThreads) {
+    const double px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
+    // u, v, w faces: TwoDScene.cpp:6700-6713
+    for (int d = 0; d < 3; ++d) {
+      int base[3];
+      double w[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], face_off(d, a));
+        stencil_weights(g, px[a], base[a], face_off(d, a), w[a]);
+      }
+      const double vd = A.v[d * A.cap + p];
+      const double b0 = A.C[(d * 3 + 0) * A.cap + p];
+      const double b1 = A.C[(d * 3 + 1) * A.cap + p];
+      const double b2 = A.C[(d * 3 + 2) * A.cap + p];
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            const double e0 = node_pos(g, i, face_off(d, 0)) - px[0];
+            const double e1 = node_pos(g, j, face_off(d, 1)) - px[1];
+            const double e2 = node_pos(g, k, face_off(d, 2)) - px[2];
+            const double vel = vd + (b0 * e0 + b1 * e1 + b2 * e2);
+            const double dm = A.fm * ww;
+            const double dp = vel * A.fm * ww;
+            const double dv = A.fvol * ww;
+            const int li = i - o0, lj = j - o1, lk = k - o2;
+            if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
+                lk < kTile) {
+              const int l = (lk * kTile + lj) * kTile + li;
+              atomicAdd(&sm[d][0][l], dm);
+              atomicAdd(&sm[d][1][l], dp);
+              atomicAdd(&sm[d][2][l], dv);
+            } else {
+              const int64_t n = node_index(g, i, j, k);
+              unsafeAtomicAdd(&A.mass[d][n], dm);
+              unsafeAtomicAdd(&A.mom[d][n], dp);
+              unsafeAtomicAdd(&A.vol[d][n], dv);
+            }
+          }
+    }
+    // liquid phi at cell centres: TwoDScene.cpp:3307-3332
+    {
+      int base[3];
+      for (int a = 0; a < 3; ++a) base[a] = stencil_base(g, a, px[a], 0.5);
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            const double d0 = px[0] - node_pos(g, i, 0.5);
+            const double d1 = px[1] - node_pos(g, j, 0.5);
+            const double d2 = px[2] - node_pos(g, k, 0.5);
+            const double phi = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - A.r0;
+            const unsigned long long key = phi_key(phi);
+            const int li = i - o0, lj = j - o1, lk = k - o2;
+            if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
+                lk < kTile)
+              atomicMin(&sphi[(lk * kTile + lj) * kTile + li], key);
+            else
+              atomicMin(&A.phikey[node_index(g, i, j, k)], key);
+          }
+    }
+    // buckets touched by the five stencils (findNodes, TwoDScene.cpp:3683)
+    for (int gi = 0; gi < 5; ++gi) {
+      int blo[3], bhi[3];
+      for (int a = 0; a < 3; ++a) {
+        const int base = stencil_base(g, a, px[a], grid5_off(gi, a));
+        blo[a] = base >> 2;
+        bhi[a] = (base + 2) >> 2;
+      }
+      for (int zk = blo[2]; zk <= bhi[2]; ++zk)
+        for (int zj = blo[1]; zj <= bhi[1]; ++zj)
+          for (int zi = blo[0]; zi <= bhi[0]; ++zi) {
+            const int di = zi - bi, dj = zj - bj, dk = zk - bk;
+            if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 &&
+                dk <= 1)
+              stouch[(dk + 1) * 9 + (dj + 1) * 3 + (di + 1)] = 1;
+            else
+              A.touched[bucket_index(g, zi, zj, zk)] = 1;
+          }
+    }
+  }
+  __syncthreads();
+
+  for (int l = threadIdx.x; l < kTileNodes; l += kP2GThreads) {
+    const int i = o0 + (l % kTile), j = o1 + ((l / kTile) % kTile),
+              k = o2 + l / (kTile * kTile);
+    if (!in_range(g, i, j, k)) continue;
+    const int64_t n = node_index(g, i, j, k);
+    for (int d = 0; d < 3; ++d) {
+      const double m = sm[d][0][l];
+      if (m != 0.0) {
+        unsafeAtomicAdd(&A.mass[d][n], m);
+        unsafeAtomicAdd(&A.mom[d][n], sm[d][1][l]);
+        unsafeAtomicAdd(&A.vol[d][n], sm[d][2][l]);
+      }
+    }
+    const unsigned long long key = sphi[l];
+    if (key != kPhiUntouched) atomicMin(&A.phikey[n], key);
+  }
+  if (threadIdx.x < 27 && stouch[threadIdx.x]) {
+    const int di = threadIdx.x % 3 - 1, dj = (threadIdx.x / 3) % 3 - 1,
+              dk = threadIdx.x / 9 - 1;
+    const int zi = bi + di, zj = bj + dj, zk = bk + dk;
+    if (zi >= 0 && zj >= 0 && zk >= 0 && zi < g.nb[0] && zj < g.nb[1] &&
+        zk < g.nb[2])
+      A.touched[bucket_index(g, zi, zj, zk)] = 1;
+  }
+}
+
+struct G2PArgs {
+  Grid g;
+  double* x;
+  double* v;
+  double* C;
+  int64_t n, cap;
+  const double* u[3];
+  const uint8_t* act;
+  double dt, flip, invD;
+  int solid_projection;
+};
+
+// mapNodeParticlesAPIC fluid branch (TwoDScene.cpp:6958-7019), advectScene
+// (SceneStepper.cpp:30-31) and solidProjection (TwoDScene.cpp:1782-1821).
+__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs A) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= A.n) return;
+  const Grid& g = A.g;
+  const double px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
+  double fv[3] = {0, 0, 0};
+  double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    int base[3];
+    double w[3][3];
+    for (int a = 0; a < 3; ++a) {
+      base[a] = stencil_base(g, a, px[a], face_off(d, a));
+      stencil_weights(g, px[a], base[a], face_off(d, a), w[a]);
+    }
+    const double* ud = A.u[d];
+    for (int c2 = 0; c2 < 3; ++c2)
+      for (int c1 = 0; c1 < 3; ++c1)
+        for (int c0 = 0; c0 < 3; ++c0) {
+          const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+          if (!node_active(g, A.act, i, j, k)) continue;
+          const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+          const double fnv = ud[node_index(g, i, j, k)];
+          fv[d] += fnv * ww;
+          const double e0 = node_pos(g, i, face_off(d, 0)) - px[0];
+          const double e1 = node_pos(g, j, face_off(d, 1)) - px[1];
+          const double e2 = node_pos(g, k, face_off(d, 2)) - px[2];
+          B[d * 3 + 0] += fnv * ww * e0 * A.invD;
+          B[d * 3 + 1] += fnv * ww * e1 * A.invD;
+          B[d * 3 + 2] += fnv * ww * e2 * A.invD;
+        }
+  }
+  for (int t = 0; t < 9; ++t) A.C[t * A.cap + p] = B[t] * A.flip;
+  double nx[3];
+  for (int d = 0; d < 3; ++d) {
+    A.v[d * A.cap + p] = fv[d];
+    nx[d] = px[d] + fv[d] * A.dt;
+  }
+  if (A.solid_projection) {
+    int base[3];
+    double w[3][3];
+    for (int a = 0; a < 3; ++a) {
+      base[a] = stencil_base(g, a, px[a], 0.0);
+      stencil_weights(g, px[a], base[a], 0.0, w[a]);
+    }
+    double phi_ori = 0.0, gr[3] = {0, 0, 0};
+    for (int c2 = 0; c2 < 3; ++c2)
+      for (int c1 = 0; c1 < 3; ++c1)
+        for (int c0 = 0; c0 < 3; ++c0) {
+          const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+          const double q0 = node_pos(g, i, 0.0), q1 = node_pos(g, j, 0.0),
+                       q2 = node_pos(g, k, 0.0);
+          const double phi = node_active(g, A.act, i, j, k)
+                                 ? container_phi(g, q0, q1, q2)
+                                 : 3.0 * g.dx;
+          const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+          phi_ori += phi * ww;
+          gr[0] += phi * A.invD * ww * (q0 - nx[0]);
+          gr[1] += phi * A.invD * ww * (q1 - nx[1]);
+          gr[2] += phi * A.invD * ww * (q2 - nx[2]);
+        }
+    const double gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+    if (gn > 1e-20) {
+      gr[0] /= gn;
+      gr[1] /= gn;
+      gr[2] /= gn;
+    }
+    const double dp0 = fv[0] * A.dt, dp1 = fv[1] * A.dt, dp2 = fv[2] * A.dt;
+    const double phi_now = phi_ori + (gr[0] * dp0 + gr[1] * dp1 + gr[2] * dp2);
+    if (phi_now < 0.0) {
+      nx[0] -= phi_now * gr[0];
+      nx[1] -= phi_now * gr[1];
+      nx[2] -= phi_now * gr[2];
+    }
+  }
+  for (int d = 0; d < 3; ++d) A.x[d * A.cap + p] = nx[d];
+}
+
+// ---- seeding ----
+
+__global__ void k_seed_flags(Grid g, int c00, int c01, int c02, int cn0,
+                             int cn1, int cn2, double lo0, double lo1,
+                             double lo2, double hi0, double hi1, double hi2,
+                             uint64_t seed, uint32_t* flag) {
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ncand = (int64_t)cn0 * cn1 * cn2 * 8;
+  if (id >= ncand) return;
+  const int sub = (int)(id & 7);
+  const int64_t cell = id >> 3;
+  const int ii = (int)(cell % cn0), jj = (int)((cell / cn0) % cn1),
+            kk = (int)(cell / ((int64_t)cn0 * cn1));
+  const int cc[3] = {c00 + ii, c01 + jj, c02 + kk};
+  const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+  uint32_t keep = 1;
+  for (int d = 0; d < 3; ++d) {
+    const int sc = (sub >> d) & 1;
+    const double u = uniform01(hash3(seed, (uint64_t)id, (uint64_t)d));
+    const double p = ((double)cc[d] + ((double)sc + u) * 0.5) * g.dx;
+    if (!(p >= lo[d] && p < hi[d])) keep = 0;
+  }
+  flag[id] = keep;
+}
+
+__global__ void k_seed_write(Grid g, int c00, int c01, int c02, int cn0,
+                             int cn1, int cn2, uint64_t seed,
+                             const uint32_t* flag_scan, const uint32_t* flag,
+                             double* x, double* v, double* C, uint32_t* pid,
+                             int64_t cap) {
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ncand = (int64_t)cn0 * cn1 * cn2 * 8;
+  if (id >= ncand || !flag[id]) return;
+  const int64_t q = flag_scan[id];
+  const int sub = (int)(id & 7);
+  const int64_t cell = id >> 3;
+  const int ii = (int)(cell % cn0), jj = (int)((cell / cn0) % cn1),
+            kk = (int)(cell / ((int64_t)cn0 * cn1));
+  const int cc[3] = {c00 + ii, c01 + jj, c02 + kk};
+  for (int d = 0; d < 3; ++d) {
+    const int sc = (sub >> d) & 1;
+    const double u = uniform01(hash3(seed, (uint64_t)id, (uint64_t)d));
+    x[d * cap + q] = ((double)cc[d] + ((double)sc + u) * 0.5) * g.dx;
+    v[d * cap + q] = 0.0;
+  }
+  for (int t = 0; t < 9; ++t) C[t * cap + q] = 0.0;
+  pid[q] = (uint32_t)q;
+}
+
+// Irwin-Hall N(0,1) per world cell and component, cells [-2, n+1]
+__global__ void k_noise(Grid g, uint64_t seed, double* raw) {
+  const int q0 = g.n[0] + 4, q1 = g.n[1] + 4, q2 = g.n[2] + 4;
+  const int64_t qq = (int64_t)q0 * q1 * q2;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3 * qq) return;
+  const int c = (int)(t / qq);
+  const int64_t r = t % qq;
+  const int i = (int)(r % q0) - 2, j = (int)((r / q0) % q1) - 2,
+            k = (int)(r / ((int64_t)q0 * q1)) - 2;
+  const uint64_t key = ((uint64_t)(k + (1 << 20)) << 42) |
+                       ((uint64_t)(j + (1 << 20)) << 21) |
+                       (uint64_t)(i + (1 << 20));
+  double s = 0.0;
+  for (int u = 0; u < 12; ++u)
+    s += uniform01(hash3(seed ^ 0xA076D1F0C3B5E927ull, key,
+                         (uint64_t)(c * 16 + u)));
+  raw[t] = s - 6.0;
+}
+
+// 3x3x3 box filter, world cells [-1, n], scaled to std sigma
+__global__ void k_noise_filter(Grid g, const double* raw, double scale,
+                               double* fil) {
+  const int q0 = g.n[0] + 4, q1 = g.n[1] + 4, q2 = g.n[2] + 4;
+  const int m0 = g.n[0] + 2, m1 = g.n[1] + 2, m2 = g.n[2] + 2;
+  const int64_t qq = (int64_t)q0 * q1 * q2, mm = (int64_t)m0 * m1 * m2;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3 * mm) return;
+  const int c = (int)(t / mm);
+  const int64_t r = t % mm;
+  const int i = (int)(r % m0), j = (int)((r / m0) % m1),
+            k = (int)(r / ((int64_t)m0 * m1));
+  double s = 0.0;
+  for (int dk = 0; dk < 3; ++dk)
+    for (int dj = 0; dj < 3; ++dj)
+      for (int di = 0; di < 3; ++di)
+        s += raw[c * qq + ((int64_t)(k + dk) * q1 + (j + dj)) * q0 + (i + di)];
+  fil[t] = s * scale;
+}
+
+__global__ void k_seed_velocity(Grid g, const double* fil, const double* x,
+                                double* v, int64_t n, int64_t cap) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int m0 = g.n[0] + 2, m1 = g.n[1] + 2, m2 = g.n[2] + 2;
+  const int64_t mm = (int64_t)m0 * m1 * m2;
+  const double px[3] = {x[p], x[cap + p], x[2 * cap + p]};
+  int base[3];
+  double w[3][3];
+  for (int d = 0; d < 3; ++d) {
+    base[d] = stencil_base(g, d, px[d], 0.5);
+    stencil_weights(g, px[d], base[d], 0.5, w[d]);
+  }
+  for (int c = 0; c < 3; ++c) {
+    double acc = 0.0;
+    for (int c2 = 0; c2 < 3; ++c2)
+      for (int c1 = 0; c1 < 3; ++c1)
+        for (int c0 = 0; c0 < 3; ++c0) {
+          const int wi = base[0] + c0 - kGhost + 1;
+          const int wj = base[1] + c1 - kGhost + 1;
+          const int wk = base[2] + c2 - kGhost + 1;
+          double f = 0.0;
+          if (wi >= 0 && wj >= 0 && wk >= 0 && wi < m0 && wj < m1 && wk < m2)
+            f = fil[c * mm + ((int64_t)wk * m1 + wj) * m0 + wi];
+          const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+          acc += ww * f;
+        }
+    v[c * cap + p] = acc;
+  }
+}
+
+inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+void ensure_particles(wc_ctx* ctx, int64_t n) {
+  if (n > ctx->pcap || !ctx->x.p) {
+    const int64_t cap = n > 0 ? n : 1;
+    ctx->x.release();
+    ctx->v.release();
+    ctx->C.release();
+    ctx->x2.release();
+    ctx->v2.release();
+    ctx->C2.release();
+    ctx->pid.release();
+    ctx->pid2.release();
+    ctx->pkey.release();
+    ctx->x.ensure(3 * cap);
+    ctx->v.ensure(3 * cap);
+    ctx->C.ensure(9 * cap);
+    ctx->x2.ensure(3 * cap);
+    ctx->v2.ensure(3 * cap);
+    ctx->C2.ensure(9 * cap);
+    ctx->pid.ensure(cap);
+    ctx->pid2.ensure(cap);
+    ctx->pkey.ensure(cap);
+    ctx->pcap = cap;
+  }
+  ctx->np = n;
+  ctx->sorted = false;
+}
+
+}  // namespace
+
+void particles_set_soa(wc_ctx* ctx, int64_t n, const double* x, const double* v,
+                       const double* C, bool ref_layout) {
+  ensure_particles(ctx, n);
+  const int64_t cap = ctx->pcap;
+  std::vector<double> hx(3 * (size_t)cap, 0.0), hv(3 * (size_t)cap, 0.0),
+      hC(9 * (size_t)cap, 0.0);
+  std::vector<uint32_t> hp((size_t)cap);
+  for (int64_t p = 0; p < n; ++p) {
+    hp[p] = (uint32_t)p;
+    for (int d = 0; d < 3; ++d) {
+      if (x) hx[d * cap + p] = ref_layout ? x[4 * p + d] : x[d * n + p];
+      if (v) hv[d * cap + p] = ref_layout ? v[4 * p + d] : v[d * n + p];
+    }
+    if (C)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          hC[(r * 3 + c) * cap + p] =
+              ref_layout ? C[(int64_t)c * 3 * n + 3 * p + r]  // Eigen 3N x 3
+                         : C[(r * 3 + c) * n + p];
+  }
+  hipStream_t st = ctx->stream;
+  WC_HIP(hipMemcpyAsync(ctx->x.p, hx.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice, st));
+  WC_HIP(hipMemcpyAsync(ctx->v.p, hv.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice, st));
+  WC_HIP(hipMemcpyAsync(ctx->C.p, hC.data(), sizeof(double) * 9 * cap, hipMemcpyHostToDevice, st));
+  WC_HIP(hipMemcpyAsync(ctx->pid.p, hp.data(), sizeof(uint32_t) * cap, hipMemcpyHostToDevice, st));
+  WC_HIP(hipStreamSynchronize(st));
+}
+
+void particles_get_soa(wc_ctx* ctx, double* x, double* v, double* C,
+                       bool ref_layout) {
+  const int64_t n = ctx->np, cap = ctx->pcap;
+  std::vector<double> hx(3 * (size_t)cap), hv(3 * (size_t)cap),
+      hC(9 * (size_t)cap);
+  std::vector<uint32_t> hp((size_t)cap);
+  hipStream_t st = ctx->stream;
+  WC_HIP(hipMemcpyAsync(hx.data(), ctx->x.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, st));
+  WC_HIP(hipMemcpyAsync(hv.data(), ctx->v.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, st));
+  WC_HIP(hipMemcpyAsync(hC.data(), ctx->C.p, sizeof(double) * 9 * cap, hipMemcpyDeviceToHost, st));
+  WC_HIP(hipMemcpyAsync(hp.data(), ctx->pid.p, sizeof(uint32_t) * cap, hipMemcpyDeviceToHost, st));
+  WC_HIP(hipStreamSynchronize(st));
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t p = hp[q];
+    for (int d = 0; d < 3; ++d) {
+      if (x) {
+        if (ref_layout) {
+          x[4 * p + d] = hx[d * cap + q];
+        } else {
+          x[d * n + p] = hx[d * cap + q];
+        }
+      }
+      if (v) {
+        if (ref_layout) {
+          v[4 * p + d] = hv[d * cap + q];
+        } else {
+          v[d * n + p] = hv[d * cap + q];
+        }
+      }
+    }
+    if (ref_layout) {
+      if (x) x[4 * p + 3] = 0.0;
+      if (v) v[4 * p + 3] = 0.0;
+    }
+    if (C)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          const double val = hC[(r * 3 + c) * cap + q];
+          if (ref_layout)
+            C[(int64_t)c * 3 * n + 3 * p + r] = val;
+          else
+            C[(r * 3 + c) * n + p] = val;
+        }
+  }
+}
+
+int64_t particles_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3],
+                             uint64_t seed, double sigma) {
+  const Grid& g = ctx->g;
+  hipStream_t st = ctx->stream;
+  int c0[3], cn[3];
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = (int)floor(lo[d] / g.dx);
+    const int c1 = (int)ceil(hi[d] / g.dx);
+    cn[d] = c1 > c0[d] ? c1 - c0[d] : 0;
+  }
+  const int64_t ncand = (int64_t)cn[0] * cn[1] * cn[2] * 8;
+  if (ncand >= 0xFFFFFFFFll) throw HipError{hipErrorInvalidValue, "seed block too large", __FILE__, __LINE__};
+  DevBuf<uint32_t> flag, scan;
+  flag.ensure(ncand + 1);
+  scan.ensure(ncand + 1);
+  hipLaunchKernelGGL(k_seed_flags, dim3(blocks(ncand, 256)), dim3(256), 0, st, g,
+                     c0[0], c0[1], c0[2], cn[0], cn[1], cn[2], lo[0], lo[1],
+                     lo[2], hi[0], hi[1], hi[2], seed, flag.p);
+  WC_LAUNCH_CHECK();
+  WC_HIP(hipMemcpyAsync(scan.p, flag.p, sizeof(uint32_t) * ncand,
+                        hipMemcpyDeviceToDevice, st));
+  const uint32_t n = exclusive_scan(ctx, scan.p, ncand);
+  ensure_particles(ctx, n);
+  hipLaunchKernelGGL(k_seed_write, dim3(blocks(ncand, 256)), dim3(256), 0, st, g,
+                     c0[0], c0[1], c0[2], cn[0], cn[1], cn[2], seed, scan.p,
+                     flag.p, ctx->x.p, ctx->v.p, ctx->C.p, ctx->pid.p, ctx->pcap);
+  WC_LAUNCH_CHECK();
+  if (sigma > 0.0) {
+    const int64_t qq = (int64_t)(g.n[0] + 4) * (g.n[1] + 4) * (g.n[2] + 4);
+    const int64_t mm = (int64_t)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
+    DevBuf<double> raw, fil;
+    raw.ensure(3 * qq);
+    fil.ensure(3 * mm);
+    hipLaunchKernelGGL(k_noise, dim3(blocks(3 * qq, 256)), dim3(256), 0, st, g,
+                       seed, raw.p);
+    const double scale = sigma / sqrt(27.0);
+    hipLaunchKernelGGL(k_noise_filter, dim3(blocks(3 * mm, 256)), dim3(256), 0,
+                       st, g, raw.p, scale, fil.p);
+    hipLaunchKernelGGL(k_seed_velocity, dim3(blocks(n, 256)), dim3(256), 0, st,
+                       g, fil.p, ctx->x.p, ctx->v.p, (int64_t)n, ctx->pcap);
+    WC_LAUNCH_CHECK();
+    WC_HIP(hipStreamSynchronize(st));
+    raw.release();
+    fil.release();
+  }
+  WC_HIP(hipStreamSynchronize(st));
+  flag.release();
+  scan.release();
+  return n;
+}
+
+void particles_sort(wc_ctx* ctx) {
+  const Grid& g = ctx->g;
+  hipStream_t st = ctx->stream;
+  const int64_t n = ctx->np, cap = ctx->pcap;
+  ctx->bcount.ensure(g.nbucket + 1);
+  ctx->bcursor.ensure(g.nbucket + 1);
+  WC_HIP(hipMemsetAsync(ctx->bcount.p, 0, sizeof(uint32_t) * (g.nbucket + 1), st));
+  WC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(uint32_t) * (g.nbucket + 1), st));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks(n, 256)), dim3(256), 0, st, g,
+                       ctx->x.p, n, cap, ctx->pkey.p, ctx->bcount.p);
+    WC_LAUNCH_CHECK();
+  }
+  exclusive_scan(ctx, ctx->bcount.p, g.nbucket + 1);  // bcount becomes bstart
+  std::swap(ctx->bcount, ctx->bstart);
+  ctx->bcount.ensure(g.nbucket + 1);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_sort_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, n,
+                       cap, ctx->pkey.p, ctx->bstart.p, ctx->bcursor.p, ctx->x.p,
+                       ctx->v.p, ctx->C.p, ctx->pid.p, ctx->x2.p, ctx->v2.p,
+                       ctx->C2.p, ctx->pid2.p);
+    WC_LAUNCH_CHECK();
+  }
+  std::swap(ctx->x, ctx->x2);
+  std::swap(ctx->v, ctx->v2);
+  std::swap(ctx->C, ctx->C2);
+  std::swap(ctx->pid, ctx->pid2);
+  ctx->sorted = true;
+}
+
+void particles_p2g(wc_ctx* ctx) {
+  // the workgroup-per-bucket P2G reads the bucket offsets of the last sort
+  if (!ctx->sorted) particles_sort(ctx);
+  const Grid& g = ctx->g;
+  hipStream_t st = ctx->stream;
+  const int64_t nn = g.nnode;
+  for (int d = 0; d < 3; ++d) {
+    WC_HIP(hipMemsetAsync(ctx->mass[d].p, 0, sizeof(double) * nn, st));
+    WC_HIP(hipMemsetAsync(ctx->mom[d].p, 0, sizeof(double) * nn, st));
+    WC_HIP(hipMemsetAsync(ctx->vol[d].p, 0, sizeof(double) * nn, st));
+  }
+  WC_HIP(hipMemsetAsync(ctx->phikey.p, 0xFF, sizeof(unsigned long long) * nn, st));
+  WC_HIP(hipMemsetAsync(ctx->touched.p, 0, g.nbucket, st));
+  if (ctx->np == 0) return;
+  P2GArgs A;
+  A.g = g;
+  A.x = ctx->x.p;
+  A.v = ctx->v.p;
+  A.C = ctx->C.p;
+  A.cap = ctx->pcap;
+  A.bstart = ctx->bstart.p;
+  for (int d = 0; d < 3; ++d) {
+    A.mass[d] = ctx->mass[d].p;
+    A.mom[d] = ctx->mom[d].p;
+    A.vol[d] = ctx->vol[d].p;
+  }
+  A.phikey = ctx->phikey.p;
+  A.touched = ctx->touched.p;
+  A.fm = ctx->desc.particle_mass;
+  A.fvol = ctx->desc.particle_volume;
+  A.r0 = g.dx * 0.883644 > ctx->desc.particle_radius ? g.dx * 0.883644
+                                                      : ctx->desc.particle_radius;
+  hipLaunchKernelGGL(k_p2g, dim3((unsigned)g.nbucket), dim3(kP2GThreads), 0, st, A);
+  WC_LAUNCH_CHECK();
+}
+
+void particles_g2p_advect(wc_ctx* ctx, double dt) {
+  if (ctx->np == 0) return;
+  G2PArgs A;
+  A.g = ctx->g;
+  A.x = ctx->x.p;
+  A.v = ctx->v.p;
+  A.C = ctx->C.p;
+  A.n = ctx->np;
+  A.cap = ctx->pcap;
+  for (int d = 0; d < 3; ++d) A.u[d] = ctx->unew[d].p;
+  A.act = ctx->act.p;
+  A.dt = dt;
+  A.flip = ctx->desc.flip_coeff;
+  A.invD = 4.0 / (ctx->g.dx * ctx->g.dx);
+  A.solid_projection = ctx->desc.solid_projection;
+  hipLaunchKernelGGL(k_g2p_advect, dim3(blocks(ctx->np, 256)), dim3(256), 0,
+                     ctx->stream, A);
+  WC_LAUNCH_CHECK();
+}
+
+}  // namespace wc

@@ -1,0 +1,582 @@
+// MIC(0)-preconditioned conjugate gradients on gfx950
+// (robertbridson::PCGSolver, PCGSolver/PCGSolver.h:97-330).
+//
+// The factorization and both triangular solves are level-scheduled: rows are
+// grouped by their depth in the lower-triangular dependency graph (for the
+// pressure grid that is the i+j+k hyperplane) and each level is one launch in
+// which every row only reads rows of earlier levels.  Each row is computed in
+// gather ("left-looking") form, applying exactly the updates the reference's
+// column-sweeping loops apply to it and in the same order (ascending column),
+// so the factor and the preconditioned vector are bitwise the sequential
+// ones.  Inner products are two-stage block reductions in FP64 with a fixed
+// order (deterministic, but not the reference's SIMD order).
+#include <math.h>
+
+#include <algorithm>
+
+#include "wc_ctx.h"
+
+namespace wc {
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kRedThreads = 256;
+constexpr int kMaxUpper = 16;  // upper entries per row the factor supports
+
+// scalar slots in PcgWork::scal
+enum { S_RHO = 0, S_ALPHA = 1, S_BETA = 2, S_RES = 3, S_DOT = 4, S_NSCAL = 8 };
+
+__device__ inline void row_range(const DevCsr& A, int64_t i, uint32_t& b,
+                                 uint32_t& e) {
+  if (A.rowptr) {
+    b = A.rowptr[i];
+    e = A.rowptr[i + 1];
+  } else {
+    b = A.start ? A.start[i] : (uint32_t)(i * (int64_t)A.stride);
+    e = b + A.len[i];
+  }
+}
+
+// first entry of row i with column > i
+__device__ inline uint32_t upper_begin(const DevCsr& A, int64_t i, uint32_t b,
+                                       uint32_t e) {
+  while (b < e && A.col[b] <= (uint32_t)i) ++b;
+  return b;
+}
+
+__global__ void k_tpos_adiag(DevCsr A, uint32_t* tpos, double* adiag) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  uint32_t b, e;
+  row_range(A, i, b, e);
+  double d = 0.0;
+  for (uint32_t q = b; q < e; ++q) {
+    const uint32_t c = A.col[q];
+    uint32_t t = kNone;
+    if (c == (uint32_t)i) d = A.val[q];
+    if (c < (uint32_t)i) {
+      uint32_t kb, ke;
+      row_range(A, c, kb, ke);
+      for (uint32_t r = kb; r < ke; ++r)
+        if (A.col[r] == (uint32_t)i) {
+          t = r;
+          break;
+        }
+    }
+    tpos[q] = t;
+  }
+  adiag[i] = d;
+}
+
+__global__ void k_level_relax(DevCsr A, int32_t* level, int* changed) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  uint32_t b, e;
+  row_range(A, i, b, e);
+  int32_t lv = 0;
+  for (uint32_t q = b; q < e; ++q) {
+    const uint32_t c = A.col[q];
+    if (c < (uint32_t)i) lv = max(lv, level[c] + 1);
+  }
+  if (lv != level[i]) {
+    level[i] = lv;
+    *changed = 1;
+  }
+}
+
+__global__ void k_level_max(const int32_t* level, int64_t n, int* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicMax(out, level[i]);
+}
+
+__global__ void k_level_count(const int32_t* level, int64_t n, int32_t base,
+                              uint32_t* count) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(&count[level[i] - base], 1u);
+}
+
+__global__ void k_level_scatter(const int32_t* level, int64_t n, int32_t base,
+                                uint32_t* cursor, uint32_t* list) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t pos = atomicAdd(&cursor[level[i] - base], 1u);
+  list[pos] = (uint32_t)i;
+}
+
+// factor_modified_incomplete_cholesky0 (PCGSolver.h:97-205), one level.
+// Column j collects, in ascending k, every update the right-looking loop of
+// column k applies to it: the diagonal -= L_jk*L_jk, the fill-in matches
+// L_rj -= L_jk*L_rk, and the diagonal -= tau*L_jk*missing.
+__global__ void k_mic0_level(DevCsr A, const uint32_t* tpos,
+                             const double* adiag, const uint32_t* list,
+                             uint32_t count, double tau, double mdr,
+                             double* invdiag, double* lv, int* overflow) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t j = list[t];
+  uint32_t jb, je;
+  row_range(A, j, jb, je);
+  const uint32_t ju = upper_begin(A, j, jb, je);
+  const int nu = (int)(je - ju);
+  if (nu > kMaxUpper) {
+    *overflow = 1;
+    return;
+  }
+  double lvj[kMaxUpper];
+  for (int u = 0; u < nu; ++u) lvj[u] = A.val[ju + u];
+  const double aj = adiag[j];
+  double d = aj;
+  for (uint32_t q = jb; q < ju; ++q) {
+    const uint32_t k = A.col[q];
+    if (k >= (uint32_t)j) break;  // the diagonal
+    const uint32_t pk = tpos[q];
+    if (pk == kNone) continue;
+    if (adiag[k] == 0) continue;  // null column k is skipped (PCGSolver.h:141)
+    uint32_t kb, ke;
+    row_range(A, k, kb, ke);
+    const uint32_t c0 = upper_begin(A, k, kb, ke);
+    const double multiplier = lv[pk];
+    double missing = 0;
+    uint32_t a = c0;
+    uint32_t bb = jb;
+    while (a < ke && A.col[a] < (uint32_t)j) {
+      while (bb < je) {
+        if (A.col[bb] < A.col[a])
+          ++bb;
+        else if (A.col[bb] == A.col[a])
+          break;
+        else {
+          missing += lv[a];
+          break;
+        }
+      }
+      ++a;
+    }
+    if (a < ke && A.col[a] == (uint32_t)j) d -= multiplier * lv[a];
+    ++a;
+    int ub = 0;
+    while (a < ke && ub < nu) {
+      if (A.col[ju + ub] < A.col[a])
+        ++ub;
+      else if (A.col[ju + ub] == A.col[a]) {
+        lvj[ub] -= multiplier * lv[a];
+        ++a;
+        ++ub;
+      } else {
+        missing += lv[a];
+        ++a;
+      }
+    }
+    while (a < ke) {
+      missing += lv[a];
+      ++a;
+    }
+    d -= tau * multiplier * missing;
+  }
+  if (aj == 0) {
+    invdiag[j] = d;
+    for (int u = 0; u < nu; ++u) lv[ju + u] = lvj[u];
+    return;
+  }
+  double inv;
+  if (d < mdr * aj)
+    inv = 1 / sqrt(aj);
+  else
+    inv = 1 / sqrt(d);
+  invdiag[j] = inv;
+  for (int u = 0; u < nu; ++u) lv[ju + u] = lvj[u] * inv;
+}
+
+// solve_lower (PCGSolver.h:211-223) for one level: z_j = (r_j - sum over
+// ascending k of L_jk z_k) * invdiag_j.
+__global__ void k_lower_level(DevCsr A, const uint32_t* tpos,
+                              const double* invdiag, const double* lv,
+                              const uint32_t* list, uint32_t count,
+                              const double* r, double* z) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t j = list[t];
+  uint32_t jb, je;
+  row_range(A, j, jb, je);
+  double acc = r[j];
+  for (uint32_t q = jb; q < je; ++q) {
+    const uint32_t k = A.col[q];
+    if (k >= (uint32_t)j) break;
+    const uint32_t pk = tpos[q];
+    if (pk == kNone) continue;
+    acc -= lv[pk] * z[k];
+  }
+  z[j] = acc * invdiag[j];
+}
+
+// solve_lower_transpose_in_place (PCGSolver.h:226-239) for one level.
+__global__ void k_upper_level(DevCsr A, const double* invdiag,
+                              const double* lv, const uint32_t* list,
+                              uint32_t count, double* z) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t j = list[t];
+  uint32_t jb, je;
+  row_range(A, j, jb, je);
+  double acc = z[j];
+  for (uint32_t q = upper_begin(A, j, jb, je); q < je; ++q)
+    acc -= lv[q] * z[A.col[q]];
+  z[j] = acc * invdiag[j];
+}
+
+// multiply(FixedSparseMatrix) (SparseMatrix.h:298-322)
+__global__ void k_spmv(DevCsr A, const double* x, double* y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  uint32_t b, e;
+  row_range(A, i, b, e);
+  double acc = 0;
+  for (uint32_t q = b; q < e; ++q) acc += A.val[q] * x[A.col[q]];
+  y[i] = acc;
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__device__ inline double block_max(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial(
+    const double* a, const double* b, int64_t n, double* partial) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads)
+    acc += a[i] * b[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// modes of the final reduction
+enum { R_SET_RHO = 0, R_ALPHA = 1, R_BETA = 2, R_DOT = 3 };
+
+__global__ __launch_bounds__(kRedThreads) void k_dot_final(
+    const double* partial, int nb, double* scal, int mode) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) acc += partial[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    if (mode == R_SET_RHO) {
+      scal[S_RHO] = s;
+    } else if (mode == R_ALPHA) {
+      scal[S_ALPHA] = scal[S_RHO] / s;  // PCGSolver.h:291
+    } else if (mode == R_BETA) {
+      scal[S_BETA] = s / scal[S_RHO];  // PCGSolver.h:301
+      scal[S_RHO] = s;
+    } else {
+      scal[S_DOT] = s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_absmax_partial(
+    const double* a, int64_t n, double* partial) {
+  __shared__ double sh[kRedThreads];
+  double m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads)
+    m = fmax(m, fabs(a[i]));
+  const double s = block_max(m, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_absmax_final(
+    const double* partial, int nb, double* scal) {
+  __shared__ double sh[kRedThreads];
+  double m = 0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) m = fmax(m, partial[i]);
+  const double s = block_max(m, sh);
+  if (threadIdx.x == 0) scal[S_RES] = s;
+}
+
+// result += s*alpha; r += z*(-alpha) (PCGSolver.h:292-293), plus |r|_inf
+// block partials (:294).
+__global__ __launch_bounds__(kRedThreads) void k_update_xr(
+    double* x, double* r, const double* s, const double* z, int64_t n,
+    const double* scal, double* partial) {
+  __shared__ double sh[kRedThreads];
+  const double alpha = scal[S_ALPHA];
+  const double nalpha = -alpha;
+  double m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads) {
+    x[i] += s[i] * alpha;
+    const double ri = r[i] + z[i] * nalpha;
+    r[i] = ri;
+    m = fmax(m, fabs(ri));
+  }
+  const double mm = block_max(m, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = mm;
+}
+
+// s = z + s*beta (add_scaled then swap, PCGSolver.h:302-303)
+__global__ void k_update_s(double* s, const double* z, int64_t n,
+                           const double* scal) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  s[i] = z[i] + s[i] * scal[S_BETA];
+}
+
+inline unsigned blocks_for(int64_t n, int t) {
+  return (unsigned)((n + t - 1) / t);
+}
+
+}  // namespace
+
+void PcgWork::release() {
+  invdiag.release();
+  adiag.release();
+  flags.release();
+  lv.release();
+  r.release();
+  z.release();
+  s.release();
+  x.release();
+  partial.release();
+  scal.release();
+  tpos.release();
+  level.release();
+  lvl_count.release();
+  lvl_cursor.release();
+  lvl_list.release();
+  if (host_scal) (void)hipHostFree(host_scal);
+  host_scal = nullptr;
+}
+
+static size_t slot_count(const DevCsr& A) { return (size_t)A.nnz_slots; }
+
+void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
+                 const int32_t* level, int32_t level_span) {
+  const int64_t n = A.n;
+  w.tpos.ensure(slot_count(A));
+  w.invdiag.ensure(n);
+  w.adiag.ensure(n);
+  w.flags.ensure(4);
+  w.z.ensure(n);
+  w.lv.ensure(slot_count(A));
+  w.r.ensure(n);
+  w.s.ensure(n);
+  w.x.ensure(n);
+  w.red_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
+  w.partial.ensure(w.red_blocks);
+  w.scal.ensure(S_NSCAL);
+  if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
+  if (n == 0) {
+    w.nlevels = 0;
+    return;
+  }
+  hipLaunchKernelGGL(k_tpos_adiag, dim3(blocks_for(n, 256)), dim3(256), 0, st, A,
+                     w.tpos.p, w.adiag.p);
+  WC_LAUNCH_CHECK();
+
+  const int32_t* lev = level;
+  int32_t span = level_span;
+  if (!lev) {
+    w.level.ensure(n);
+    WC_HIP(hipMemsetAsync(w.level.p, 0, sizeof(int32_t) * n, st));
+    int* changed = w.flags.p;
+    int hchanged = 1;
+    int64_t iters = 0;
+    while (hchanged) {
+      WC_HIP(hipMemsetAsync(changed, 0, sizeof(int), st));
+      hipLaunchKernelGGL(k_level_relax, dim3(blocks_for(n, 256)), dim3(256), 0,
+                         st, A, w.level.p, changed);
+      WC_LAUNCH_CHECK();
+      WC_HIP(hipMemcpyAsync(&hchanged, changed, sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+      WC_HIP(hipStreamSynchronize(st));
+      if (++iters > n + 1) break;
+    }
+    // relaxation in place can converge in fewer passes than the depth
+    WC_HIP(hipMemsetAsync(w.flags.p + 2, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_level_max, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                       w.level.p, n, w.flags.p + 2);
+    WC_LAUNCH_CHECK();
+    int hmax = 0;
+    WC_HIP(hipMemcpyAsync(&hmax, w.flags.p + 2, sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+    WC_HIP(hipStreamSynchronize(st));
+    span = hmax + 1;
+    lev = w.level.p;
+  }
+  w.lvl_count.ensure(span);
+  w.lvl_cursor.ensure(span);
+  w.lvl_list.ensure(n);
+  WC_HIP(hipMemsetAsync(w.lvl_count.p, 0, sizeof(uint32_t) * span, st));
+  hipLaunchKernelGGL(k_level_count, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                     lev, n, 0, w.lvl_count.p);
+  WC_LAUNCH_CHECK();
+  std::vector<uint32_t> cnt(span);
+  WC_HIP(hipMemcpyAsync(cnt.data(), w.lvl_count.p, sizeof(uint32_t) * span,
+                        hipMemcpyDeviceToHost, st));
+  WC_HIP(hipStreamSynchronize(st));
+  w.lvl_start.clear();
+  w.lvl_size.clear();
+  std::vector<uint32_t> start(span);
+  uint32_t acc = 0;
+  for (int32_t l = 0; l < span; ++l) {
+    start[l] = acc;
+    if (cnt[l]) {
+      w.lvl_start.push_back(acc);
+      w.lvl_size.push_back(cnt[l]);
+    }
+    acc += cnt[l];
+  }
+  w.nlevels = (int)w.lvl_size.size();
+  WC_HIP(hipMemcpyAsync(w.lvl_cursor.p, start.data(), sizeof(uint32_t) * span,
+                        hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_level_scatter, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     st, lev, n, 0, w.lvl_cursor.p, w.lvl_list.p);
+  WC_LAUNCH_CHECK();
+  // the host vector 'start' must outlive the async copy
+  WC_HIP(hipStreamSynchronize(st));
+}
+
+void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
+                double mdr) {
+  if (A.n == 0) return;
+  int* overflow = w.flags.p + 1;
+  WC_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st));
+  for (int l = 0; l < w.nlevels; ++l) {
+    const uint32_t cnt = w.lvl_size[l];
+    hipLaunchKernelGGL(k_mic0_level, dim3(blocks_for(cnt, 64)), dim3(64), 0, st,
+                       A, w.tpos.p, w.adiag.p, w.lvl_list.p + w.lvl_start[l], cnt,
+                       tau, mdr, w.invdiag.p, w.lv.p, overflow);
+    WC_LAUNCH_CHECK();
+  }
+  int hov = 0;
+  WC_HIP(hipMemcpyAsync(&hov, overflow, sizeof(int), hipMemcpyDeviceToHost, st));
+  WC_HIP(hipStreamSynchronize(st));
+  if (hov) throw HipError{hipErrorInvalidValue, "MIC(0): a row has more than 16 entries above the diagonal", __FILE__, __LINE__};
+}
+
+void pcg_apply(hipStream_t st, PcgWork& w, const DevCsr& A, const double* r,
+               double* z) {
+  for (int l = 0; l < w.nlevels; ++l) {
+    const uint32_t cnt = w.lvl_size[l];
+    hipLaunchKernelGGL(k_lower_level, dim3(blocks_for(cnt, 64)), dim3(64), 0, st,
+                       A, w.tpos.p, w.invdiag.p, w.lv.p,
+                       w.lvl_list.p + w.lvl_start[l], cnt, r, z);
+    WC_LAUNCH_CHECK();
+  }
+  for (int l = w.nlevels - 1; l >= 0; --l) {
+    const uint32_t cnt = w.lvl_size[l];
+    hipLaunchKernelGGL(k_upper_level, dim3(blocks_for(cnt, 64)), dim3(64), 0, st,
+                       A, w.invdiag.p, w.lv.p, w.lvl_list.p + w.lvl_start[l],
+                       cnt, z);
+    WC_LAUNCH_CHECK();
+  }
+}
+
+void pcg_spmv(hipStream_t st, const DevCsr& A, const double* x, double* y) {
+  if (A.n == 0) return;
+  hipLaunchKernelGGL(k_spmv, dim3(blocks_for(A.n, 256)), dim3(256), 0, st, A,
+                     x, y);
+  WC_LAUNCH_CHECK();
+}
+
+static double read_scal(hipStream_t st, PcgWork& w, int slot) {
+  WC_HIP(hipMemcpyAsync(w.host_scal + slot, w.scal.p + slot, sizeof(double),
+                        hipMemcpyDeviceToHost, st));
+  WC_HIP(hipStreamSynchronize(st));
+  return w.host_scal[slot];
+}
+
+bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
+               double* x, double tol, int maxit, double tau, double mdr,
+               double* residual_out, int* iters_out) {
+  const int64_t n = A.n;
+  if (tol < 1e-30) tol = 1e-30;  // PCGSolver.h:253
+  if (n == 0) {
+    *residual_out = 0;
+    *iters_out = 0;
+    return true;
+  }
+  const int nb = w.red_blocks;
+  double* r = w.r.p;
+  double* z = w.z.p;
+  double* s = w.s.p;
+  WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+  WC_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_absmax_partial, dim3(nb), dim3(kRedThreads), 0, st, r, n,
+                     w.partial.p);
+  hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st,
+                     w.partial.p, nb, w.scal.p);
+  WC_LAUNCH_CHECK();
+  double res = read_scal(st, w, S_RES);
+  *residual_out = res;
+  if (res < 1e-30) {
+    *iters_out = 0;
+    return true;
+  }
+  const double tolr = tol * res;
+  pcg_factor(st, w, A, tau, mdr);
+  pcg_apply(st, w, A, r, z);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
+                     w.partial.p);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p,
+                     nb, w.scal.p, (int)R_SET_RHO);
+  WC_LAUNCH_CHECK();
+  const double rho = read_scal(st, w, S_RHO);
+  if (rho == 0 || rho != rho) {
+    *iters_out = 0;
+    return false;
+  }
+  WC_HIP(hipMemcpyAsync(s, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  const unsigned gb = blocks_for(n, 256);
+  int it;
+  for (it = 0; it < maxit; ++it) {
+    hipLaunchKernelGGL(k_spmv, dim3(gb), dim3(256), 0, st, A, s, z);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, s, z,
+                       n, w.partial.p);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st,
+                       w.partial.p, nb, w.scal.p, (int)R_ALPHA);
+    hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
+                       z, n, w.scal.p, w.partial.p);
+    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st,
+                       w.partial.p, nb, w.scal.p);
+    WC_LAUNCH_CHECK();
+    res = read_scal(st, w, S_RES);
+    *residual_out = res;
+    if (res <= tolr) {
+      *iters_out = it + 1;
+      return true;
+    }
+    pcg_apply(st, w, A, r, z);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r,
+                       n, w.partial.p);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st,
+                       w.partial.p, nb, w.scal.p, (int)R_BETA);
+    hipLaunchKernelGGL(k_update_s, dim3(gb), dim3(256), 0, st, s, z, n,
+                       w.scal.p);
+    WC_LAUNCH_CHECK();
+  }
+  *iters_out = it;
+  return false;
+}
+
+}  // namespace wc
