@@ -92,7 +92,7 @@ def lib():
                                        C.c_double]
         L.wco_scene_destroy.argtypes = [C.POINTER(SceneStruct)]
         for nm in ("wco_activate", "wco_update_liquid_phi", "wco_update_solid_phi",
-                   "wco_update_solid_weights", "wco_p2g"):
+                   "wco_update_solid_weights", "wco_p2g", "wco_copy_vel_to_unew"):
             getattr(L, nm).argtypes = [C.POINTER(SceneStruct)]
         L.wco_step_velocity.argtypes = [C.POINTER(SceneStruct), C.c_double]
         L.wco_assemble.restype = C.c_int64
@@ -260,6 +260,7 @@ class OracleScene:
     def update_solid_phi(self): lib().wco_update_solid_phi(self.s)
     def update_solid_weights(self): lib().wco_update_solid_weights(self.s)
     def p2g(self): lib().wco_p2g(self.s)
+    def copy_vel_to_unew(self): lib().wco_copy_vel_to_unew(self.s)
     def step_velocity(self, dt): lib().wco_step_velocity(self.s, dt)
     def assemble(self, dt): return lib().wco_assemble(self.s, dt)
     def solve_pressure(self, tol, maxit): return lib().wco_solve_pressure(self.s, tol, maxit)

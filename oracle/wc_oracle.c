@@ -1145,6 +1145,11 @@ void wco_g2p_advect(wco_scene* s, double dt) {
   }
 }
 
+void wco_copy_vel_to_unew(wco_scene* s) {
+  for (int d = 0; d < 3; ++d)
+    memcpy(s->unew[d], s->vel[d], sizeof(double) * s->g.nnode);
+}
+
 /* Core/WetClothCore.cpp:77-300 substep order for liquid-only scenes with
  * correctionStrength = 0 and no splitting/merging (parity settings). */
 int wco_substep(wco_scene* s, double dt, double tol, int maxit) {

@@ -139,6 +139,8 @@ int64_t wco_assemble(wco_scene* s, double dt);
 int wco_solve_pressure(wco_scene* s, double tol, int maxit);
 void wco_apply_pressure(wco_scene* s, double dt);
 void wco_g2p_advect(wco_scene* s, double dt);
+/* test hook: G2P reads the P2G velocity instead of the projected one */
+void wco_copy_vel_to_unew(wco_scene* s);
 /* one substep in the reference order (Core/WetClothCore.cpp:77-300) */
 int wco_substep(wco_scene* s, double dt, double tol, int maxit);
 
