@@ -135,6 +135,12 @@ wc_status wc_download_system(wc_ctx* ctx, int64_t* ndof, int64_t* nnz,
  * G2P -- the reference's timing_buffer phases (App/ParticleSimulation.cpp:19). */
 wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
 
+/* Runtime options of a context (no reference counterpart):
+ *   "pipelined_sweeps" (default 1): apply MIC(0) with the persistent tiled
+ *   wavefront kernels; 0 selects one launch per i+j+k level.  Both give
+ *   bitwise the same result. */
+wc_status wc_set_option(wc_ctx* ctx, const char* name, int64_t value);
+
 /* ---- robertbridson::PCGSolver<double> on a host CSR matrix ---- */
 
 /* Literal drop-in for PCGSolver<double>::solve(matrix, rhs, result,

@@ -94,6 +94,7 @@ def lib():
         L.wc_download_field.argtypes = [vp, C.c_int32, C.c_int32, vp]
         L.wc_download_system.argtypes = [vp, _i64p, _i64p, _u32p, _u32p, _dp, _dp, _dp]
         L.wc_phase_times.argtypes = [vp, _dp]
+        L.wc_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
         L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
                                        C.c_int32, _dp, _i32p]
         L.wc_mic0_factor_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double, C.c_double,
@@ -269,6 +270,9 @@ class Scene:
         _check(L.wc_download_system(self.h, C.byref(n), C.byref(nnz), _ptr(rs, _u32p),
                                     _ptr(col, _u32p), _ptr(val), _ptr(b), _ptr(p)), self.h)
         return rs, col, val, b, p
+
+    def set_option(self, name: str, value: int):
+        _check(lib().wc_set_option(self.h, name.encode(), int(value)), self.h)
 
     def phase_times(self) -> dict:
         ms = (C.c_double * 7)()

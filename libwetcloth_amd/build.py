@@ -17,7 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwcgpu.so")
 ARCH = os.environ.get("WC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["wc_scan.hip", "wc_pcg.hip", "wc_particles.hip", "wc_grid.hip", "wc_capi.cpp"]
+SOURCES = ["wc_scan.hip", "wc_pcg.hip", "wc_sweep.hip", "wc_particles.hip", "wc_grid.hip",
+           "wc_capi.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
@@ -51,7 +52,22 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_host_example(verbose)
     return LIB
+
+
+HOST_EXE = os.path.join(HERE, "host", "dam_break")
+
+
+def build_host_example(verbose: bool = False) -> str:
+    """The C++ host mirror's example program, linked against libwcgpu.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "host"),
+           os.path.join(HERE, "host", "dam_break.cpp"), "-o", HOST_EXE,
+           "-L", HERE, "-lwcgpu", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    return HOST_EXE
 
 
 def asm(src: str) -> str:

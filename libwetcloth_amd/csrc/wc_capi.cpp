@@ -140,6 +140,13 @@ void record(wc_ctx* c, int i) {
 // solve the assembled system into c->p and scatter it onto the grid
 bool solve_system(wc_ctx* c, double tol, int maxit, double* res, int* it) {
   const wc::DevCsr A = system_csr(c);
+  wc::GridSystem& gs = c->pcg.grid;
+  gs.g = &c->g;
+  gs.act = c->act.p;
+  gs.isdof = c->isdof.p;
+  gs.dofscan = c->dofscan.p;
+  gs.rowlen = c->rowlen.p;
+  gs.col = c->col.p;
   wc::pcg_prepare(c->stream, c->pcg, A, c->dof_level.p,
                   c->level_max - c->level_min + 1);
   const bool ok = wc::pcg_solve(c->stream, c->pcg, A, c->b.p, c->p.p, tol, maxit,
@@ -445,6 +452,16 @@ wc_status wc_download_system(wc_ctx* c, int64_t* ndof, int64_t* nnz,
       WC_HIP(hipMemcpy(solution, c->p.p, sizeof(double) * n, hipMemcpyDeviceToHost));
   })
   return WC_OK;
+}
+
+wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
+  if (!c || !name) return fail(c, WC_ERR_ARG, "bad argument");
+  const std::string n(name);
+  if (n == "pipelined_sweeps") {
+    c->pcg.use_sweeps = value != 0;
+    return WC_OK;
+  }
+  return fail(c, WC_ERR_ARG, "unknown option " + n);
 }
 
 wc_status wc_phase_times(const wc_ctx* c, double ms[7]) {

@@ -61,8 +61,37 @@ struct DevCsr {
   int64_t nnz_slots = 0;  // size of col/val arrays
 };
 
+// Pencil-layout copy of the 7-point factor for the pipelined sweeps
+// (wc_sweep.hip): the DOF bounding box, cut into 16x16 tiles of k-pencils.
+struct SweepPlan {
+  bool valid = false;
+  int o[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+  int ntile_a = 0, ntile_b = 0, ntiles = 0;
+  int64_t ncell = 0;
+  DevBuf<int32_t> pdof;
+  DevBuf<uint32_t> order;
+  DevBuf<double> cf, cb;  // packed 48-byte cells (forward / backward)
+  DevBuf<double> zf, zb;
+  DevBuf<int> progress, flags;
+  void release();
+};
+
+// Where the rows of a structured (grid) system live, so the solver can build
+// the pipelined sweeps; all null for a general CSR matrix.
+struct GridSystem {
+  const Grid* g = nullptr;
+  const uint8_t* act = nullptr;
+  const uint32_t* isdof = nullptr;
+  const uint32_t* dofscan = nullptr;
+  const uint32_t* rowlen = nullptr;
+  const uint32_t* col = nullptr;
+};
+
 // Device work space of the MIC(0)-PCG solver (PCGSolver.h:310-314).
 struct PcgWork {
+  GridSystem grid;   // set by the caller for grid systems
+  bool use_sweeps = true;
+  SweepPlan sweep;
   DevBuf<double> invdiag, adiag, lv, r, z, s, x, partial, scal;
   DevBuf<uint32_t> tpos;
   DevBuf<int> flags;  // [0] relaxation changed, [1] factor overflow, [2] max level
@@ -147,6 +176,14 @@ void grid_step_velocity(wc_ctx* ctx, double dt);
 int64_t grid_assemble(wc_ctx* ctx, double dt);
 void grid_scatter_pressure(wc_ctx* ctx);
 void grid_apply_pressure(wc_ctx* ctx, double dt);
+
+// ---- wc_sweep.hip ----
+void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* act,
+                   const uint32_t* isdof, const uint32_t* dofscan,
+                   const uint32_t* rowlen, const uint32_t* col,
+                   const double* invdiag, const double* lv, int64_t ndof);
+void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z);
+bool sweep_aborted(hipStream_t st, SweepPlan& P);
 
 // ---- wc_pcg.hip ----
 // Levels of the lower-triangular dependency DAG: given per row (structured

@@ -348,6 +348,7 @@ inline unsigned blocks_for(int64_t n, int t) {
 }  // namespace
 
 void PcgWork::release() {
+  sweep.release();
   invdiag.release();
   adiag.release();
   flags.release();
@@ -476,6 +477,10 @@ void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
 
 void pcg_apply(hipStream_t st, PcgWork& w, const DevCsr& A, const double* r,
                double* z) {
+  if (w.sweep.valid) {
+    sweep_apply(st, w.sweep, r, z);
+    return;
+  }
   for (int l = 0; l < w.nlevels; ++l) {
     const uint32_t cnt = w.lvl_size[l];
     hipLaunchKernelGGL(k_lower_level, dim3(blocks_for(cnt, 64)), dim3(64), 0, st,
@@ -535,6 +540,10 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
   }
   const double tolr = tol * res;
   pcg_factor(st, w, A, tau, mdr);
+  w.sweep.valid = false;
+  if (w.use_sweeps && w.grid.g)
+    sweep_prepare(st, w.sweep, *w.grid.g, w.grid.act, w.grid.isdof, w.grid.dofscan,
+                  w.grid.rowlen, w.grid.col, w.invdiag.p, w.lv.p, n);
   pcg_apply(st, w, A, r, z);
   hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
                      w.partial.p);
@@ -562,6 +571,8 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
     WC_LAUNCH_CHECK();
     res = read_scal(st, w, S_RES);
     *residual_out = res;
+    if (w.sweep.valid && sweep_aborted(st, w.sweep))
+      throw HipError{hipErrorLaunchTimeOut, "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__, __LINE__};
     if (res <= tolr) {
       *iters_out = it + 1;
       return true;

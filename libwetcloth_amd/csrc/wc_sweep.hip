@@ -1,0 +1,483 @@
+// Pipelined MIC(0) triangular solves on the structured pressure grid.
+//
+// solve_lower / solve_lower_transpose_in_place (PCGSolver.h:211-239) of the
+// 7-point factor, as one persistent kernel per sweep instead of one launch
+// per i+j+k hyperplane.  The DOF bounding box is cut into 16x16 tiles of
+// k-pencils; a workgroup walks its tile's local hyperplanes i'+j'+k = w, one
+// step per w, passing the newest value of each pencil to its +x/+y
+// neighbours through LDS.  Values on a tile's low faces come from the
+// upstream tiles (a-1, b) and (a, b-1), which run ahead.  The output array is
+// filled with a NaN sentinel before the sweep and every cell is written
+// exactly once with a single 64-bit device-coherent (agent-scope atomic)
+// store, so a consumer knows a value is final when it no longer reads the
+// sentinel: boundary values are prefetched kChunk steps ahead and re-read
+// only if they were not ready yet.  Tiles are claimed with a ticket counter
+// in (a+b, a) order, so a tile only waits for tiles already claimed by
+// running workgroups (no deadlock), and every wait is bounded.
+//
+// Each cell applies its <= 3 neighbour terms in ascending DOF order (the
+// order the column-oriented reference loop applies them), so the result is
+// bitwise that of the sequential solve.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "wc_ctx.h"
+
+namespace wc {
+namespace {
+
+constexpr int kT = 16;           // tile edge (pencils)
+constexpr int kThreads = kT * kT;
+constexpr int kChunk = 8;        // cells prefetched ahead per pencil
+constexpr long long kSpinLimit = 1ll << 22;  // ~seconds; a correct run waits us
+constexpr unsigned long long kSentinel = 0xFFFFFFFFFFFFFFFFull;
+
+// code bits per cell
+constexpr uint32_t C_DOF = 1u;
+constexpr int C_LOWER = 1;   // bits 1..3: coupling to -x, -y, -z
+constexpr int C_UPPER = 4;   // bits 4..6: coupling to +x, +y, +z
+constexpr int C_LORD = 7;    // bits 7..9: order of the lower terms (0..5)
+constexpr int C_UORD = 10;   // bits 10..12: order of the upper terms
+
+// the six orders of three axes, 2 bits per axis, 6 bits per order:
+// 0 xyz, 1 xzy, 2 yxz, 3 yzx, 4 zxy, 5 zyx
+constexpr uint64_t kPerms =
+    (uint64_t)(0 | 1 << 2 | 2 << 4) | (uint64_t)(0 | 2 << 2 | 1 << 4) << 6 |
+    (uint64_t)(1 | 0 << 2 | 2 << 4) << 12 | (uint64_t)(1 | 2 << 2 | 0 << 4) << 18 |
+    (uint64_t)(2 | 0 << 2 | 1 << 4) << 24 | (uint64_t)(2 | 1 << 2 | 0 << 4) << 30;
+__host__ __device__ inline int perm_axis(int p, int q) {
+  return (int)((kPerms >> (6 * p + 2 * q)) & 3);
+}
+
+__device__ inline int perm_of(int64_t k0, int64_t k1, int64_t k2) {
+  // index of the permutation that sorts (k0, k1, k2) ascending
+  if (k0 <= k1 && k1 <= k2) return 0;
+  if (k0 <= k2 && k2 <= k1) return 1;
+  if (k1 <= k0 && k0 <= k2) return 2;
+  if (k1 <= k2 && k2 <= k0) return 3;
+  if (k2 <= k0 && k0 <= k1) return 4;
+  return 5;
+}
+
+__device__ inline double load_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int load_flag(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_flag(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// one box cell's sweep inputs, read with three 16-byte loads
+struct __attribute__((aligned(16))) Cell {
+  double l0, l1, l2, inv, rhs;
+  uint32_t code, pad;
+};
+static_assert(sizeof(Cell) == 48, "Cell is three dwordx4");
+
+struct BoxArgs {
+  Grid g;
+  int o[3];        // box origin (dense node coordinates)
+  int B[3];        // box dims; B[0], B[1] multiples of kT
+  const uint8_t* act;
+  const uint32_t* isdof;
+  const uint32_t* dofscan;
+  const uint32_t* rowlen;  // ELL-7 rows of the assembled matrix
+  const uint32_t* col;
+  const double* invdiag;
+  const double* lv;
+  int32_t* pdof;
+  Cell* cf;  // forward inputs (lower terms)
+  Cell* cb;  // backward inputs (upper terms)
+};
+
+__device__ inline int64_t box_index(const int B[3], int x, int y, int z) {
+  return ((int64_t)y * B[0] + x) * B[2] + z;
+}
+
+__global__ void k_dof_box(Grid g, const uint32_t* isdof, int* bb) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= g.nnode || !isdof[n]) return;
+  int i, j, k;
+  node_coords(g, n, i, j, k);
+  atomicMin(&bb[0], i);
+  atomicMin(&bb[1], j);
+  atomicMin(&bb[2], k);
+  atomicMax(&bb[3], i);
+  atomicMax(&bb[4], j);
+  atomicMax(&bb[5], k);
+}
+
+// The factor's entries around each box cell, in pencil layout.
+__global__ void k_fill_box(BoxArgs A) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ncell = (int64_t)A.B[0] * A.B[1] * A.B[2];
+  if (t >= ncell) return;
+  const int z = (int)(t % A.B[2]);
+  const int64_t xy = t / A.B[2];
+  const int x = (int)(xy % A.B[0]);
+  const int y = (int)(xy / A.B[0]);
+  const int i = A.o[0] + x, j = A.o[1] + y, k = A.o[2] + z;
+  const Grid& g = A.g;
+  uint32_t code = 0;
+  int32_t me = -1;
+  double lm[3] = {0, 0, 0}, lp[3] = {0, 0, 0}, inv = 0;
+  if (in_range(g, i, j, k)) {
+    const int64_t n = node_index(g, i, j, k);
+    if (A.isdof[n]) {
+      me = (int32_t)A.dofscan[n];
+      code |= C_DOF;
+      inv = A.invdiag[me];
+      int64_t lkey[3], ukey[3];
+      for (int d = 0; d < 3; ++d) {
+        lkey[d] = ukey[d] = 0x7FFFFFFFFFFFll;
+        // lower neighbour: its row holds (k, me) -> L(me, k)
+        int q[3] = {i, j, k};
+        q[d] -= 1;
+        if (in_range(g, q[0], q[1], q[2])) {
+          const int64_t nn = node_index(g, q[0], q[1], q[2]);
+          if (A.isdof[nn]) {
+            const uint32_t other = A.dofscan[nn];
+            const uint32_t len = A.rowlen[other];
+            for (uint32_t s = 0; s < len; ++s)
+              if (A.col[7 * (int64_t)other + s] == (uint32_t)me) {
+                lm[d] = A.lv[7 * (int64_t)other + s];
+                code |= 1u << (C_LOWER + d);
+                lkey[d] = other;
+              }
+          }
+        }
+        // upper neighbour: my row holds (me, other) -> L(other, me)
+        int u[3] = {i, j, k};
+        u[d] += 1;
+        if (in_range(g, u[0], u[1], u[2])) {
+          const int64_t nn = node_index(g, u[0], u[1], u[2]);
+          if (A.isdof[nn]) {
+            const uint32_t other = A.dofscan[nn];
+            const uint32_t len = A.rowlen[me];
+            for (uint32_t s = 0; s < len; ++s)
+              if (A.col[7 * (int64_t)me + s] == other) {
+                lp[d] = A.lv[7 * (int64_t)me + s];
+                code |= 1u << (C_UPPER + d);
+                ukey[d] = other;
+              }
+          }
+        }
+      }
+      code |= (uint32_t)perm_of(lkey[0], lkey[1], lkey[2]) << C_LORD;
+      code |= (uint32_t)perm_of(ukey[0], ukey[1], ukey[2]) << C_UORD;
+    }
+  }
+  A.pdof[t] = me;
+  Cell f, bw;
+  f.l0 = lm[0];
+  f.l1 = lm[1];
+  f.l2 = lm[2];
+  bw.l0 = lp[0];
+  bw.l1 = lp[1];
+  bw.l2 = lp[2];
+  f.inv = bw.inv = inv;
+  f.rhs = bw.rhs = 0.0;
+  f.code = bw.code = code;
+  f.pad = bw.pad = 0;
+  A.cf[t] = f;
+  A.cb[t] = bw;
+}
+
+__global__ void k_gather(int64_t ncell, const int32_t* pdof, const double* src,
+                         Cell* dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  const int32_t d = pdof[t];
+  dst[t].rhs = d >= 0 ? src[d] : 0.0;
+}
+
+__global__ void k_scatter(int64_t ncell, const int32_t* pdof, const double* src,
+                          double* dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  const int32_t d = pdof[t];
+  if (d >= 0) dst[d] = src[t];
+}
+
+struct SweepArgs {
+  int B[3];
+  int ntile_a, ntile_b, ntiles;
+  const uint32_t* order;   // ticket -> a | b << 16
+  const Cell* cells;       // forward: lower terms and r; backward: upper terms
+  const double* in;        // backward only: the forward result
+  double* out;
+  int* ticket;
+  int* abort_flag;
+};
+
+
+// inputs of the kk-th cell of a pencil; kk is clamped into the pencil so the
+// load is always issued to a valid address (steps outside it discard it)
+template <bool BWD>
+__device__ inline Cell load_cell_dir(const SweepArgs& S, int64_t col, int Bz, int kk) {
+  const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
+  const int64_t idx = col + (BWD ? (Bz - 1 - kc) : kc);
+  Cell c = S.cells[idx];
+  if (BWD) c.rhs = S.in[idx];
+  return c;
+}
+#define load_cell(S, col, Bz, kk) load_cell_dir<BWD>(S, col, Bz, kk)
+
+// the value at p once it is no longer the sentinel (rare path: the
+// prefetched copy v was read before its tile had written it)
+__device__ __noinline__ double wait_value(const double* p, int* abort_flag) {
+  long long spins = 0;
+  double v = load_coherent(p);
+  while (__double_as_longlong(v) == (long long)kSentinel) {
+    if (++spins > kSpinLimit || load_flag(abort_flag)) {
+      store_flag(abort_flag, 1);
+      return 0.0;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = load_coherent(p);
+  }
+  return v;
+}
+
+__device__ inline double ready_value(const double* p, double v, int* abort_flag) {
+  if (__builtin_expect(__double_as_longlong(v) == (long long)kSentinel, 0))
+    return wait_value(p, abort_flag);
+  return v;
+}
+
+// One pipelined sweep.  BWD mirrors every axis: local coordinates run from
+// the high corner of the box, upstream tiles are (a+1, b) and (a, b+1).
+template <bool BWD>
+__global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
+  __shared__ double lz[2][kT][kT + 1];
+  __shared__ int s_tile;
+  const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
+  const int Bz = S.B[2];
+  const int nsteps = 2 * (kT - 1) + Bz;
+  const int ubits = BWD ? C_UPPER : C_LOWER;
+  const int obits = BWD ? C_UORD : C_LORD;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int t = atomicAdd(S.ticket, 1);
+      s_tile = t < S.ntiles ? (int)S.order[t] : -1;
+    }
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile < 0) return;
+    // logical tile (a, b) counted from the sweep's start corner
+    const int ta = tile & 0xFFFF, tb = tile >> 16;
+    const int a = BWD ? S.ntile_a - 1 - ta : ta;
+    const int b = BWD ? S.ntile_b - 1 - tb : tb;
+    const bool has_a = BWD ? a + 1 < S.ntile_a : a > 0;
+    const bool has_b = BWD ? b + 1 < S.ntile_b : b > 0;
+    // physical pencil of this thread
+    const int px = a * kT + (BWD ? kT - 1 - li : li);
+    const int py = b * kT + (BWD ? kT - 1 - lj : lj);
+    const int64_t col = ((int64_t)py * S.B[0] + px) * Bz;
+    const int64_t dstep_x = BWD ? Bz : -Bz;  // toward the upstream x pencil
+    const int64_t dstep_y = BWD ? (int64_t)S.B[0] * Bz : -(int64_t)S.B[0] * Bz;
+    const bool bnd_x = li == 0 && has_a;     // reads the upstream x face
+    const bool bnd_y = lj == 0 && has_b;
+    const int off = li + lj;  // this pencil starts at step off
+    auto out_index = [&](int kk) { return col + (BWD ? (Bz - 1 - kk) : kk); };
+    // the upstream tile's output beside this pencil's kk-th cell (threads on
+    // the tile's upstream faces only; others leave the register unwritten)
+    auto load_bnd = [&](bool on, int64_t dstep, int kk) -> double {
+      double v;
+      if (on) {
+        const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
+        v = load_coherent(&S.out[out_index(kc) + dstep]);
+      }
+      return v;
+    };
+    double zprev = 0.0;
+    // prefetch rings: slot s holds the inputs of step w0 + s.  The step loop
+    // is unrolled by kChunk and a slot is refilled only after its step has
+    // used it, so every load writes the registers it is consumed from and
+    // no wait on a load in flight is needed before its own step.
+    Cell win[kChunk];
+    double wx[kChunk], wy[kChunk];
+#pragma unroll
+    for (int s = 0; s < kChunk; ++s) {
+      win[s] = load_cell(S, col, Bz, s - off);
+      wx[s] = load_bnd(bnd_x, dstep_x, s - off);
+      wy[s] = load_bnd(bnd_y, dstep_y, s - off);
+    }
+    const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
+    const int lyj = lj > 0 ? lj - 1 : 0;
+    for (int w0 = 0; w0 < nsteps; w0 += kChunk) {
+#pragma clang loop unroll(full)
+      for (int s = 0; s < kChunk; ++s) {
+        const int w = w0 + s;
+        if (w < nsteps) {
+          __syncthreads();
+          const int kk = w - off;
+          const bool active = kk >= 0 && kk < Bz;
+          const Cell c = win[s];
+          const uint32_t code = active ? c.code : 0u;
+          const int64_t idx = out_index(kk);
+          const double lx = lz[(w - 1) & 1][lj][lxi];
+          const double ly = lz[(w - 1) & 1][lyj][li];
+          double zx = li > 0 ? lx : (bnd_x ? wx[s] : 0.0);
+          double zy = lj > 0 ? ly : (bnd_y ? wy[s] : 0.0);
+          // a face value prefetched before its tile wrote it is re-read
+          // (only where the term is present)
+          if (bnd_x && (code & (1u << ubits)))
+            zx = ready_value(&S.out[idx + dstep_x], zx, S.abort_flag);
+          if (bnd_y && (code & (2u << ubits)))
+            zy = ready_value(&S.out[idx + dstep_y], zy, S.abort_flag);
+          const int perm = (code >> obits) & 7;
+          double acc = c.rhs;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const int ax = perm_axis(perm, q);
+            const double L = ax == 0 ? c.l0 : (ax == 1 ? c.l1 : c.l2);
+            const double zn = ax == 0 ? zx : (ax == 1 ? zy : zprev);
+            // terms absent from the matrix subtract +0, an exact identity
+            const double term = (code & (1u << (ubits + ax))) ? L * zn : 0.0;
+            acc -= term;
+          }
+          // non-DOF cells carry rhs = inv = 0 and no terms, so z = 0
+          const double z = active ? acc * c.inv : 0.0;
+          if (active) store_coherent(&S.out[idx], z);
+          zprev = z;
+          lz[w & 1][lj][li] = z;
+        }
+        win[s] = load_cell(S, col, Bz, w + kChunk - off);
+        wx[s] = load_bnd(bnd_x, dstep_x, w + kChunk - off);
+        wy[s] = load_bnd(bnd_y, dstep_y, w + kChunk - off);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* act,
+                   const uint32_t* isdof, const uint32_t* dofscan,
+                   const uint32_t* rowlen, const uint32_t* col,
+                   const double* invdiag, const double* lv, int64_t ndof) {
+  P.valid = false;
+  if (ndof == 0) return;
+  P.flags.ensure(8);
+  int init[6] = {1 << 30, 1 << 30, 1 << 30, -1, -1, -1};
+  WC_HIP(hipMemcpyAsync(P.flags.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_dof_box, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, g,
+                     isdof, P.flags.p);
+  WC_LAUNCH_CHECK();
+  int bb[6];
+  WC_HIP(hipMemcpyAsync(bb, P.flags.p, sizeof(bb), hipMemcpyDeviceToHost, st));
+  WC_HIP(hipStreamSynchronize(st));
+  for (int d = 0; d < 3; ++d) {
+    P.o[d] = bb[d];
+    int ext = bb[3 + d] - bb[d] + 1;
+    if (d < 2) ext = (ext + kT - 1) / kT * kT;
+    P.B[d] = ext;
+  }
+  P.ntile_a = P.B[0] / kT;
+  P.ntile_b = P.B[1] / kT;
+  P.ntiles = P.ntile_a * P.ntile_b;
+  P.ncell = (int64_t)P.B[0] * P.B[1] * P.B[2];
+  P.pdof.ensure(P.ncell);
+  P.cf.ensure(P.ncell * sizeof(Cell) / sizeof(double));
+  P.cb.ensure(P.ncell * sizeof(Cell) / sizeof(double));
+  P.zf.ensure(P.ncell);
+  P.zb.ensure(P.ncell);
+  // tickets in (a+b, a) order
+  std::vector<uint32_t> order;
+  order.reserve(P.ntiles);
+  for (int s = 0; s <= P.ntile_a + P.ntile_b - 2; ++s)
+    for (int a = 0; a < P.ntile_a; ++a) {
+      const int b = s - a;
+      if (b >= 0 && b < P.ntile_b) order.push_back((uint32_t)a | ((uint32_t)b << 16));
+    }
+  P.order.ensure(P.ntiles);
+  WC_HIP(hipMemcpyAsync(P.order.p, order.data(), sizeof(uint32_t) * P.ntiles,
+                        hipMemcpyHostToDevice, st));
+  P.progress.ensure(8);
+  BoxArgs A;
+  A.g = g;
+  for (int d = 0; d < 3; ++d) {
+    A.o[d] = P.o[d];
+    A.B[d] = P.B[d];
+  }
+  A.act = act;
+  A.isdof = isdof;
+  A.dofscan = dofscan;
+  A.rowlen = rowlen;
+  A.col = col;
+  A.invdiag = invdiag;
+  A.lv = lv;
+  A.pdof = P.pdof.p;
+  A.cf = reinterpret_cast<Cell*>(P.cf.p);
+  A.cb = reinterpret_cast<Cell*>(P.cb.p);
+  hipLaunchKernelGGL(k_fill_box, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, A);
+  WC_LAUNCH_CHECK();
+  WC_HIP(hipStreamSynchronize(st));  // 'order' must outlive its copy
+  P.valid = true;
+}
+
+void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z) {
+  const int64_t nc = P.ncell;
+  // two ticket counters and the abort flag; outputs start as the sentinel
+  int* tick = P.progress.p;
+  int* abort_flag = tick + 2;
+  WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 8, st));
+  WC_HIP(hipMemsetAsync(P.zf.p, 0xFF, sizeof(double) * nc, st));
+  WC_HIP(hipMemsetAsync(P.zb.p, 0xFF, sizeof(double) * nc, st));
+  Cell* cf = reinterpret_cast<Cell*>(P.cf.p);
+  hipLaunchKernelGGL(k_gather, dim3(blocks(nc, 256)), dim3(256), 0, st, nc,
+                     P.pdof.p, r, cf);
+  SweepArgs S;
+  for (int d = 0; d < 3; ++d) S.B[d] = P.B[d];
+  S.ntile_a = P.ntile_a;
+  S.ntile_b = P.ntile_b;
+  S.ntiles = P.ntiles;
+  S.order = P.order.p;
+  S.abort_flag = abort_flag;
+  S.cells = cf;
+  S.in = nullptr;
+  S.out = P.zf.p;
+  S.ticket = tick;
+  hipLaunchKernelGGL(k_sweep<false>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  WC_LAUNCH_CHECK();
+  S.cells = reinterpret_cast<const Cell*>(P.cb.p);
+  S.in = P.zf.p;
+  S.out = P.zb.p;
+  S.ticket = tick + 1;
+  hipLaunchKernelGGL(k_sweep<true>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  WC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scatter, dim3(blocks(nc, 256)), dim3(256), 0, st, nc,
+                     P.pdof.p, P.zb.p, z);
+  WC_LAUNCH_CHECK();
+}
+
+bool sweep_aborted(hipStream_t st, SweepPlan& P) {
+  int v = 0;
+  WC_HIP(hipMemcpyAsync(&v, P.progress.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+  WC_HIP(hipStreamSynchronize(st));
+  return v != 0;
+}
+void SweepPlan::release() {
+  pdof.release();
+  cf.release();
+  cb.release();
+  zf.release();
+  zb.release();
+  order.release();
+  progress.release();
+  flags.release();
+  valid = false;
+}
+
+}  // namespace wc

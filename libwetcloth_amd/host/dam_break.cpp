@@ -1,0 +1,85 @@
+// Drives the C++ host mirror the way a reference user would: a
+// robertbridson::PCGSolver<double> solve on a matrix assembled with
+// SparseMatrix::add_to_element, then liquid substeps through WetClothCore.
+// Prints one JSON line.  Usage: dam_break <n> <steps>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "wetcloth_amd.hpp"
+
+int main(int argc, char** argv) {
+  const int n = argc > 1 ? std::atoi(argv[1]) : 32;
+  const int steps = argc > 2 ? std::atoi(argv[2]) : 5;
+  try {
+    // 1. PCGSolver<double> on a 2-D Poisson matrix with a known solution
+    const int m = 40, N = m * m;
+    robertbridson::SparseMatrix<double> A(N);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) {
+        const int r = j * m + i;
+        A.add_to_element(r, r, 4.0 + 0.01);
+        if (i > 0) A.add_to_element(r, r - 1, -1.0);
+        if (i + 1 < m) A.add_to_element(r, r + 1, -1.0);
+        if (j > 0) A.add_to_element(r, r - m, -1.0);
+        if (j + 1 < m) A.add_to_element(r, r + m, -1.0);
+      }
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<double> xt(N), b(N, 0.0), x;
+    for (auto& v : xt) v = nd(rng);
+    for (int r = 0; r < N; ++r)
+      for (size_t k = 0; k < A.index[r].size(); ++k) b[r] += A.value[r][k] * xt[A.index[r][k]];
+    robertbridson::PCGSolver<double> solver;
+    solver.set_solver_parameters(1e-12, 500);
+    double residual = 0.0;
+    int iterations = 0;
+    const bool ok = solver.solve(A, b, x, residual, iterations);
+    double err = 0.0, nrm = 0.0;
+    for (int r = 0; r < N; ++r) {
+      err = std::fmax(err, std::fabs(x[r] - xt[r]));
+      nrm = std::fmax(nrm, std::fabs(xt[r]));
+    }
+
+    // 2. liquid substeps
+    wc_desc d{};
+    d.ni = d.nj = d.nk = n;
+    d.dx = 1.0 / n;
+    d.liquid_density = 1000.0;
+    d.flip_coeff = 0.99;
+    d.liquid_boundary_friction = 1.0;
+    d.gravity[1] = -9.81;
+    d.particle_volume = d.dx * d.dx * d.dx / 8.0;
+    d.particle_mass = d.liquid_density * d.particle_volume;
+    d.particle_radius = std::cbrt(3.0 * d.particle_volume / (4.0 * M_PI));
+    d.solid_projection = 1;
+    d.precision = WC_FP64;
+    wcamd::TwoDScene scene(d);
+    const double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.4, 0.6, 1.0};
+    const int64_t np = scene.seedBlock(lo, hi, 0, 0.0);
+    wcamd::LinearizedImplicitEuler stepper(1e-8, 1000);
+    wcamd::WetClothCore core(scene, stepper);
+    const auto t0 = std::chrono::steady_clock::now();
+    bool all_ok = true;
+    for (int s = 0; s < steps; ++s) all_ok &= core.stepSystem(1.0 / 480);
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<double> x4, v4, fB;
+    scene.getParticles(x4, v4, fB);
+    double vy = 0.0;
+    for (int64_t p = 0; p < np; ++p) vy += v4[4 * p + 1];
+    std::printf(
+        "{\"pcg_ok\": %s, \"pcg_iterations\": %d, \"pcg_residual\": %.17g, "
+        "\"pcg_max_error\": %.17g, \"particles\": %lld, \"steps_ok\": %s, "
+        "\"last_iterations\": %d, \"mean_vy\": %.17g, \"seconds\": %.6f}\n",
+        ok ? "true" : "false", iterations, residual, err / nrm, (long long)np,
+        all_ok ? "true" : "false", stepper.lastIterations(), vy / (double)np, secs);
+  } catch (const wcamd::Error& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,258 @@
+// C++ host mirror of the reference interfaces on the liquid substep path,
+// implemented on the C ABI of libwcgpu.so (include/wetcloth_amd.h).  Header
+// only; link with -lwcgpu.
+//
+//  * robertbridson::SparseMatrix<double> / PCGSolver<double>
+//    (Core/PCGSolver/SparseMatrix.h:24-167, PCGSolver.h:245-330): same names,
+//    same argument meaning, same return convention (solve() returns false on
+//    non-convergence, on rho == 0 or NaN, and the residual/iteration outputs
+//    are filled as the reference fills them).
+//  * wcamd::TwoDScene / LinearizedImplicitEuler / WetClothCore: the phase calls
+//    of one liquid substep (Core/WetClothCore.cpp:77-300) on a scene whose
+//    particles and grids stay resident on the GPU.
+//
+// Errors of the library (bad arguments, HIP failures) throw wcamd::Error; the
+// reference has no exceptions because it has no failure modes of this kind.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/wetcloth_amd.h"
+
+namespace wcamd {
+
+struct Error : std::runtime_error {
+  wc_status status;
+  Error(wc_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+inline void check(wc_status st, const wc_ctx* ctx) {
+  if (st == WC_OK || st == WC_NOT_CONVERGED) return;
+  throw Error(st, std::string("wetcloth_amd: ") + wc_last_error(ctx));
+}
+
+// ---------------------------------------------------------------- scene
+
+class TwoDScene {
+ public:
+  explicit TwoDScene(const wc_desc& d) : desc_(d) {
+    wc_ctx* c = nullptr;
+    check(wc_create(&d, &c), nullptr);
+    ctx_ = c;
+  }
+  ~TwoDScene() { wc_destroy(ctx_); }
+  TwoDScene(const TwoDScene&) = delete;
+  TwoDScene& operator=(const TwoDScene&) = delete;
+
+  // m_x, m_fluid_v (4 doubles per particle) and m_fB (3N x 3 column-major)
+  void setParticles(const std::vector<double>& x4, const std::vector<double>& fluid_v4,
+                    const std::vector<double>& fB) {
+    const int64_t n = (int64_t)x4.size() / 4;
+    check(wc_set_particles(ctx_, n, x4.data(), fluid_v4.empty() ? nullptr : fluid_v4.data(),
+                           fB.empty() ? nullptr : fB.data()),
+          ctx_);
+  }
+  void getParticles(std::vector<double>& x4, std::vector<double>& fluid_v4,
+                    std::vector<double>& fB) const {
+    int64_t n = 0;
+    check(wc_num_particles(ctx_, &n), ctx_);
+    x4.assign(4 * n, 0.0);
+    fluid_v4.assign(4 * n, 0.0);
+    fB.assign(9 * n, 0.0);
+    check(wc_get_particles(ctx_, x4.data(), fluid_v4.data(), fB.data()), ctx_);
+  }
+  int64_t getNumParticles() const {
+    int64_t n = 0;
+    check(wc_num_particles(ctx_, &n), ctx_);
+    return n;
+  }
+  int64_t seedBlock(const double lo[3], const double hi[3], uint64_t seed, double sigma) {
+    int64_t n = 0;
+    check(wc_seed_block(ctx_, lo, hi, seed, sigma, &n), ctx_);
+    return n;
+  }
+
+  // TwoDScene.cpp:1588-1708
+  void rebucketizeParticles() { check(wc_rebucketize(ctx_), ctx_); }
+  // resampleNodes activation, updateLiquidPhi, updateSolidPhi, updateSolidWeights
+  // and TwoDScene::mapParticleNodesAPIC (TwoDScene.h:487)
+  void mapParticleNodesAPIC() { check(wc_map_particle_nodes(ctx_), ctx_); }
+  // TwoDScene::mapNodeParticlesAPIC (TwoDScene.h:493) fused with advectScene
+  // and solidProjection, which take the substep size
+  void mapNodeParticlesAPIC(double dt) { check(wc_map_node_particles(ctx_, dt), ctx_); }
+
+  wc_ctx* handle() const { return ctx_; }
+  const wc_desc& desc() const { return desc_; }
+
+ private:
+  wc_desc desc_;
+  wc_ctx* ctx_ = nullptr;
+};
+
+// LinearizedImplicitEuler's liquid phases (LinearizedImplicitEuler.cpp)
+class LinearizedImplicitEuler {
+ public:
+  LinearizedImplicitEuler(double pressure_criterion = 1e-6, int maxiters = 1000)
+      : criterion_(pressure_criterion), maxiters_(maxiters) {}
+  bool stepVelocity(TwoDScene& s, double dt) {
+    check(wc_step_velocity(s.handle(), dt), s.handle());
+    return true;
+  }
+  // projectFine -> pressure::solveNodePressure with the MIC(0) PCG solve;
+  // like the reference, a failed solve is reported and the substep goes on
+  bool projectFine(TwoDScene& s, double dt) {
+    const wc_status st = wc_project(s.handle(), dt, criterion_, maxiters_, &residual_, &iterations_);
+    check(st, s.handle());
+    return st == WC_OK;
+  }
+  bool applyPressureDragFluid(TwoDScene& s, double dt) {
+    check(wc_apply_pressure(s.handle(), dt), s.handle());
+    return true;
+  }
+  double lastResidual() const { return residual_; }
+  int lastIterations() const { return iterations_; }
+
+ private:
+  double criterion_;
+  int maxiters_;
+  double residual_ = 0.0;
+  int32_t iterations_ = 0;
+};
+
+// WetClothCore::stepSystem for liquid scenes: one substep per call
+class WetClothCore {
+ public:
+  WetClothCore(TwoDScene& scene, LinearizedImplicitEuler& stepper)
+      : scene_(scene), stepper_(stepper) {}
+  bool stepSystem(double dt) {
+    scene_.rebucketizeParticles();
+    scene_.mapParticleNodesAPIC();
+    stepper_.stepVelocity(scene_, dt);
+    const bool ok = stepper_.projectFine(scene_, dt);
+    stepper_.applyPressureDragFluid(scene_, dt);
+    scene_.mapNodeParticlesAPIC(dt);
+    return ok;
+  }
+
+ private:
+  TwoDScene& scene_;
+  LinearizedImplicitEuler& stepper_;
+};
+
+}  // namespace wcamd
+
+// ------------------------------------------------- PCGSolver<double> drop-in
+
+namespace robertbridson {
+
+// Dynamic compressed sparse row matrix with sorted rows
+// (Core/PCGSolver/SparseMatrix.h:24-167).
+template <class T>
+struct SparseMatrix {
+  unsigned int n;
+  std::vector<std::vector<unsigned int> > index;
+  std::vector<std::vector<T> > value;
+
+  explicit SparseMatrix(unsigned int n_ = 0, unsigned int expected_nonzeros_per_row = 7)
+      : n(n_), index(n_), value(n_) {
+    for (unsigned int i = 0; i < n; ++i) {
+      index[i].reserve(expected_nonzeros_per_row);
+      value[i].reserve(expected_nonzeros_per_row);
+    }
+  }
+  void clear() {
+    n = 0;
+    index.clear();
+    value.clear();
+  }
+  void zero() {
+    for (unsigned int i = 0; i < n; ++i) {
+      index[i].resize(0);
+      value[i].resize(0);
+    }
+  }
+  void resize(int n_) {
+    n = n_;
+    index.resize(n);
+    value.resize(n);
+  }
+  T operator()(unsigned int i, unsigned int j) const {
+    for (unsigned int k = 0; k < index[i].size(); ++k) {
+      if (index[i][k] == j) return value[i][k];
+      if (index[i][k] > j) return 0;
+    }
+    return 0;
+  }
+  void set_element(unsigned int i, unsigned int j, T v) { place(i, j, v, false); }
+  void add_to_element(unsigned int i, unsigned int j, T v) { place(i, j, v, true); }
+
+ private:
+  void place(unsigned int i, unsigned int j, T v, bool add) {
+    auto& ix = index[i];
+    auto& vx = value[i];
+    unsigned int k = 0;
+    for (; k < ix.size(); ++k) {
+      if (ix[k] == j) {
+        vx[k] = add ? vx[k] + v : v;
+        return;
+      }
+      if (ix[k] > j) break;
+    }
+    ix.insert(ix.begin() + k, j);
+    vx.insert(vx.begin() + k, v);
+  }
+};
+
+// PCGSolver<double> whose solve runs on the GPU (PCGSolver.h:245-330).
+template <class T>
+struct PCGSolver {
+  static_assert(sizeof(T) == 8, "the reference's BLAS wrappers exist only for double");
+  PCGSolver() { set_solver_parameters(1e-8, 500, 0.97, 0.25); }
+  void set_solver_parameters(T tolerance_factor_, int max_iterations_,
+                             T modified_incomplete_cholesky_parameter_ = 0.97,
+                             T min_diagonal_ratio_ = 0.25) {
+    tolerance_factor = tolerance_factor_;
+    if (tolerance_factor < 1e-30) tolerance_factor = 1e-30;
+    max_iterations = max_iterations_;
+    modified_incomplete_cholesky_parameter = modified_incomplete_cholesky_parameter_;
+    min_diagonal_ratio = min_diagonal_ratio_;
+  }
+  bool solve(const SparseMatrix<T>& matrix, const std::vector<T>& rhs, std::vector<T>& result,
+             T& residual_out, int& iterations_out) {
+    const unsigned int n = matrix.n;
+    rowstart.assign(n + 1, 0);
+    colindex.clear();
+    values.clear();
+    for (unsigned int i = 0; i < n; ++i) {
+      rowstart[i + 1] = rowstart[i] + (uint32_t)matrix.index[i].size();
+      colindex.insert(colindex.end(), matrix.index[i].begin(), matrix.index[i].end());
+      values.insert(values.end(), matrix.value[i].begin(), matrix.value[i].end());
+    }
+    result.assign(n, 0);
+    double res = 0.0;
+    int32_t it = 0;
+    // like form_preconditioner (PCGSolver.h:322-324) the factor always uses
+    // tau = 0.97 and min-diagonal ratio 0.25
+    const wc_status st = wc_pcg_solve_csr(n, rowstart.data(), colindex.data(), values.data(),
+                                          rhs.data(), result.data(), tolerance_factor,
+                                          max_iterations, &res, &it);
+    wcamd::check(st, nullptr);
+    residual_out = res;
+    iterations_out = it;
+    return st == WC_OK;
+  }
+
+  T tolerance_factor;
+  int max_iterations;
+  T modified_incomplete_cholesky_parameter;
+  T min_diagonal_ratio;
+
+ private:
+  std::vector<uint32_t> rowstart, colindex;
+  std::vector<double> values;
+};
+
+}  // namespace robertbridson
