@@ -287,15 +287,14 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     const bool bnd_y = lj == 0 && has_b;
     const int off = li + lj;  // this pencil starts at step off
     auto out_index = [&](int kk) { return col + (BWD ? (Bz - 1 - kk) : kk); };
-    // the upstream tile's output beside this pencil's kk-th cell (threads on
-    // the tile's upstream faces only; others leave the register unwritten)
+    // the upstream tile's output beside this pencil's kk-th cell.  Every lane
+    // loads (no branch, so no register of a load in flight is rewritten on
+    // another path); lanes off the upstream face, whose neighbour pencil may
+    // lie outside the box, read element 0 instead and discard it.
     auto load_bnd = [&](bool on, int64_t dstep, int kk) -> double {
-      double v;
-      if (on) {
-        const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
-        v = load_coherent(&S.out[out_index(kc) + dstep]);
-      }
-      return v;
+      const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
+      const int64_t at = on ? out_index(kc) + dstep : 0;
+      return load_coherent(&S.out[at]);
     };
     double zprev = 0.0;
     // prefetch rings: slot s holds the inputs of step w0 + s.  The step loop
