@@ -34,7 +34,73 @@ def parse():
     ap.add_argument("--maxit", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB five times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     return ap.parse_args()
+
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# algorithmic bytes per DOF of one launch (FP64 words): a sweep reads
+# invdiag, three factor entries and its input and writes its output; the
+# SpMV reads four coefficients and x and writes y
+SWEEP_WORDS = 6
+SPMV_WORDS = 6
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27  # doubles: 1 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}))
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of a kernel measured with rocprofv3 counters and
+    committed as data next to the summaries (None if absent or measured on
+    another workload)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(kernel)
+    if not e or e.get("workload") != workload:
+        return None
+    return e.get("bytes_per_launch")
+
+
+def roofline(stats: dict, word: int, workload: str) -> dict:
+    """HBM roofline of the dominant kernel (the slower MIC(0) sweep) from the
+    HIP-event times the library recorded on its own stream."""
+    per = {}
+    for name, words in (("sweep_fwd", SWEEP_WORDS), ("sweep_bwd", SWEEP_WORDS),
+                        ("spmv", SPMV_WORDS)):
+        ms, nl, units = stats[name]
+        if nl == 0:
+            continue
+        avg_s = ms / nl * 1e-3
+        bytes_per_launch = units / nl * words * word
+        per[name] = {"avg_ms": ms / nl, "launches": nl, "dofs": units // nl,
+                     "bytes_per_launch": bytes_per_launch,
+                     "GBs": bytes_per_launch / avg_s / 1e9}
+    if not per:
+        return None
+    dom = max(("sweep_fwd", "sweep_bwd"), key=lambda k: per.get(k, {}).get("avg_ms", -1))
+    d = per[dom]
+    kname = {"sweep_fwd": "k_sweep<false>", "sweep_bwd": "k_sweep<true>"}[dom]
+    return {
+        "bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(kname, workload),
+        "kernel": kname, "avg_ms": d["avg_ms"], "algorithmic_bytes": d["bytes_per_launch"],
+        "kernels": per,
+    }
 
 
 def cpu_baseline(n_particles_gpu: int, steps: int) -> dict:
@@ -67,6 +133,9 @@ def cpu_baseline(n_particles_gpu: int, steps: int) -> dict:
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
@@ -84,6 +153,7 @@ def main():
         ok, it, res = scene.step(dt, a.tol, a.maxit)
         iters.append(it)
     phase_tot = dict.fromkeys(W.PHASES, 0.0)
+    scene.set_option("kernel_timing", 1)  # also clears the counters
     t0 = time.perf_counter()
     for _ in range(a.steps):
         ok, it, res = scene.step(dt, a.tol, a.maxit)
@@ -92,6 +162,7 @@ def main():
             phase_tot[k] += v
     t1 = time.perf_counter()
     wall = t1 - t0
+    kstats = scene.kernel_stats()
     sps = a.steps / wall
     out = {
         "metric": METRIC,
@@ -115,6 +186,7 @@ def main():
         "pcg_iterations": iters[-a.steps:],
         "phase_ms_per_step": {k: v / a.steps for k, v in phase_tot.items()},
     }
+    out["roofline"] = roofline(kstats, 8, out["config"]["workload"])
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(npart, a.cpu_steps)
     scene.close()

@@ -33,7 +33,8 @@ typedef struct wc_ctx wc_ctx;
 
 /* arithmetic of the substep */
 #define WC_FP64 0  /* the reference's precision (scalar = double, MathDefs.h:24) */
-#define WC_MIXED 1 /* FP32 state/transfers/PCG vectors, FP64 reductions + refinement */
+#define WC_MIXED 1 /* FP32 state/transfers/PCG vectors, FP64 reductions +
+                      * refinement (not implemented yet: wc_create rejects it) */
 
 /* Scene description: the LiquidInfo / bucket fields the liquid path reads
  * (App/TwoDSceneXMLParser.cpp:1849-1904, TwoDScene::setBucketInfo
@@ -138,8 +139,20 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
 /* Runtime options of a context (no reference counterpart):
  *   "pipelined_sweeps" (default 1): apply MIC(0) with the persistent tiled
  *   wavefront kernels; 0 selects one launch per i+j+k level.  Both give
- *   bitwise the same result. */
+ *   bitwise the same result.
+ *   "kernel_timing" (default 0): bracket the solver's sweep and SpMV launches
+ *   with HIP events on the context's stream; setting it (to 0 or 1) clears
+ *   the counters read by wc_kernel_stats. */
 wc_status wc_set_option(wc_ctx* ctx, const char* name, int64_t value);
+
+/* Accumulated device time of one solver kernel class since "kernel_timing"
+ * was last set: total ms, number of launches, and DOFs processed (the sum
+ * over launches of the system size).  No reference counterpart. */
+#define WC_KERNEL_SWEEP_FWD 0 /* MIC(0) forward sweep (solve_lower) */
+#define WC_KERNEL_SWEEP_BWD 1 /* MIC(0) backward sweep (solve_lower_transpose) */
+#define WC_KERNEL_SPMV 2      /* multiply(FixedSparseMatrix, s, z) */
+wc_status wc_kernel_stats(const wc_ctx* ctx, int32_t kernel, double* ms,
+                          int64_t* launches, int64_t* units);
 
 /* ---- robertbridson::PCGSolver<double> on a host CSR matrix ---- */
 

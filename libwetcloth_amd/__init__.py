@@ -27,6 +27,7 @@ FIELD = {
 }
 _FIELD_DTYPE = {"act": np.uint8, "dof": np.int32, "valid": np.uint8}
 
+KERNELS = ("sweep_fwd", "sweep_bwd", "spmv")  # WC_KERNEL_* order
 PHASES = ("rebucketize", "p2g", "step_velocity", "assemble", "solve", "apply_pressure", "g2p")
 
 _dp = C.POINTER(C.c_double)
@@ -95,6 +96,7 @@ def lib():
         L.wc_download_system.argtypes = [vp, _i64p, _i64p, _u32p, _u32p, _dp, _dp, _dp]
         L.wc_phase_times.argtypes = [vp, _dp]
         L.wc_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
+        L.wc_kernel_stats.argtypes = [vp, C.c_int32, _dp, _i64p, _i64p]
         L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
                                        C.c_int32, _dp, _i32p]
         L.wc_mic0_factor_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double, C.c_double,
@@ -273,6 +275,19 @@ class Scene:
 
     def set_option(self, name: str, value: int):
         _check(lib().wc_set_option(self.h, name.encode(), int(value)), self.h)
+
+    def kernel_stats(self) -> dict:
+        """Per solver kernel class since set_option("kernel_timing", ...):
+        {name: (total ms, launches, DOFs processed)}."""
+        out = {}
+        for k, name in enumerate(KERNELS):
+            ms = C.c_double(0)
+            nl = C.c_int64(0)
+            nu = C.c_int64(0)
+            _check(lib().wc_kernel_stats(self.h, k, C.byref(ms), C.byref(nl), C.byref(nu)),
+                   self.h)
+            out[name] = (ms.value, nl.value, nu.value)
+        return out
 
     def phase_times(self) -> dict:
         ms = (C.c_double * 7)()

@@ -168,8 +168,9 @@ wc_status wc_create(const wc_desc* desc, wc_ctx** out) {
   *out = nullptr;
   if (desc->ni <= 0 || desc->nj <= 0 || desc->nk <= 0 || !(desc->dx > 0))
     return fail(nullptr, WC_ERR_ARG, "grid size and dx must be positive");
-  if (desc->precision != WC_FP64 && desc->precision != WC_MIXED)
-    return fail(nullptr, WC_ERR_ARG, "unknown precision");
+  if (desc->precision == WC_MIXED)
+    return fail(nullptr, WC_ERR_ARG, "precision WC_MIXED is not implemented yet");
+  if (desc->precision != WC_FP64) return fail(nullptr, WC_ERR_ARG, "unknown precision");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(nullptr, WC_ERR_HIP, "no HIP device");
@@ -461,7 +462,25 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
     c->pcg.use_sweeps = value != 0;
     return WC_OK;
   }
+  if (n == "kernel_timing") {
+    WC_TRY(c, {
+      WC_HIP(hipStreamSynchronize(c->stream));
+      c->pcg.timer.reset();
+      c->pcg.timer.on = value != 0;
+    })
+    return WC_OK;
+  }
   return fail(c, WC_ERR_ARG, "unknown option " + n);
+}
+
+wc_status wc_kernel_stats(const wc_ctx* c, int32_t kernel, double* ms, int64_t* launches,
+                          int64_t* units) {
+  if (!c || kernel < 0 || kernel >= wc::KTimer::COUNT) return WC_ERR_ARG;
+  const wc::KTimer& t = c->pcg.timer;
+  if (ms) *ms = t.ms[kernel];
+  if (launches) *launches = t.launches[kernel];
+  if (units) *units = t.units[kernel];
+  return WC_OK;
 }
 
 wc_status wc_phase_times(const wc_ctx* c, double ms[7]) {

@@ -61,6 +61,31 @@ struct DevCsr {
   int64_t nnz_slots = 0;  // size of col/val arrays
 };
 
+// Optional timing of the solver's kernels with HIP events recorded on the
+// launching stream around each launch; pending pairs are read back only at
+// host synchronization points the solver already has.
+struct KTimer {
+  enum Kind { SWEEP_FWD = 0, SWEEP_BWD = 1, SPMV = 2, COUNT = 3 };
+  bool on = false;
+  double ms[COUNT] = {};
+  int64_t launches[COUNT] = {};
+  int64_t units[COUNT] = {};
+  struct Rec {
+    int kind;
+    int64_t units;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> spare;
+  hipEvent_t mark(hipStream_t st);  // record a fresh event on st
+  void add(int kind, int64_t units, hipEvent_t a, hipEvent_t b) {
+    pending.push_back(Rec{kind, units, a, b});
+  }
+  void harvest();  // the stream has been synchronized: accumulate pending
+  void reset();
+  void release();
+};
+
 // Pencil-layout copy of the 7-point factor for the pipelined sweeps
 // (wc_sweep.hip): the DOF bounding box, cut into 16x16 tiles of k-pencils.
 struct SweepPlan {
@@ -90,6 +115,7 @@ struct GridSystem {
 // Device work space of the MIC(0)-PCG solver (PCGSolver.h:310-314).
 struct PcgWork {
   GridSystem grid;   // set by the caller for grid systems
+  KTimer timer;
   bool use_sweeps = true;
   SweepPlan sweep;
   DevBuf<double> invdiag, adiag, lv, r, z, s, x, partial, scal;
@@ -182,7 +208,8 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
                    const uint32_t* isdof, const uint32_t* dofscan,
                    const uint32_t* rowlen, const uint32_t* col,
                    const double* invdiag, const double* lv, int64_t ndof);
-void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z);
+void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z,
+                 KTimer* timer, int64_t ndof);
 bool sweep_aborted(hipStream_t st, SweepPlan& P);
 
 // ---- wc_pcg.hip ----

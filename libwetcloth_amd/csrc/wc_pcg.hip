@@ -347,7 +347,52 @@ inline unsigned blocks_for(int64_t n, int t) {
 
 }  // namespace
 
+hipEvent_t KTimer::mark(hipStream_t st) {
+  hipEvent_t e;
+  if (!spare.empty()) {
+    e = spare.back();
+    spare.pop_back();
+  } else {
+    WC_HIP(hipEventCreate(&e));
+  }
+  WC_HIP(hipEventRecord(e, st));
+  return e;
+}
+
+void KTimer::harvest() {
+  for (const Rec& r : pending) {
+    float t = 0.f;
+    WC_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    ms[r.kind] += t;
+    launches[r.kind] += 1;
+    units[r.kind] += r.units;
+    spare.push_back(r.a);
+    spare.push_back(r.b);
+  }
+  pending.clear();
+}
+
+void KTimer::reset() {
+  for (const Rec& r : pending) {
+    spare.push_back(r.a);
+    spare.push_back(r.b);
+  }
+  pending.clear();
+  for (int k = 0; k < COUNT; ++k) {
+    ms[k] = 0;
+    launches[k] = 0;
+    units[k] = 0;
+  }
+}
+
+void KTimer::release() {
+  reset();
+  for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+  spare.clear();
+}
+
 void PcgWork::release() {
+  timer.release();
   sweep.release();
   invdiag.release();
   adiag.release();
@@ -478,7 +523,7 @@ void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
 void pcg_apply(hipStream_t st, PcgWork& w, const DevCsr& A, const double* r,
                double* z) {
   if (w.sweep.valid) {
-    sweep_apply(st, w.sweep, r, z);
+    sweep_apply(st, w.sweep, r, z, w.timer.on ? &w.timer : nullptr, A.n);
     return;
   }
   for (int l = 0; l < w.nlevels; ++l) {
@@ -508,6 +553,7 @@ static double read_scal(hipStream_t st, PcgWork& w, int slot) {
   WC_HIP(hipMemcpyAsync(w.host_scal + slot, w.scal.p + slot, sizeof(double),
                         hipMemcpyDeviceToHost, st));
   WC_HIP(hipStreamSynchronize(st));
+  if (!w.timer.pending.empty()) w.timer.harvest();
   return w.host_scal[slot];
 }
 
@@ -559,7 +605,9 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
   const unsigned gb = blocks_for(n, 256);
   int it;
   for (it = 0; it < maxit; ++it) {
+    hipEvent_t e0 = w.timer.on ? w.timer.mark(st) : nullptr;
     hipLaunchKernelGGL(k_spmv, dim3(gb), dim3(256), 0, st, A, s, z);
+    if (e0) w.timer.add(KTimer::SPMV, n, e0, w.timer.mark(st));
     hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, s, z,
                        n, w.partial.p);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st,

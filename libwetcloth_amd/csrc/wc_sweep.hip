@@ -232,7 +232,7 @@ __device__ inline Cell load_cell_dir(const SweepArgs& S, int64_t col, int Bz, in
 
 // the value at p once it is no longer the sentinel (rare path: the
 // prefetched copy v was read before its tile had written it)
-__device__ __noinline__ double wait_value(const double* p, int* abort_flag) {
+__device__ __attribute__((always_inline)) inline double wait_value(const double* p, int* abort_flag) {
   long long spins = 0;
   double v = load_coherent(p);
   while (__double_as_longlong(v) == (long long)kSentinel) {
@@ -291,16 +291,22 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // loads (no branch, so no register of a load in flight is rewritten on
     // another path); lanes off the upstream face, whose neighbour pencil may
     // lie outside the box, read element 0 instead and discard it.
-    auto load_bnd = [&](bool on, int64_t dstep, int kk) -> double {
+    auto face_at = [&](bool on, int64_t dstep, int kk) -> int64_t {
       const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
-      const int64_t at = on ? out_index(kc) + dstep : 0;
-      return load_coherent(&S.out[at]);
+      return on ? out_index(kc) + dstep : 0;
+    };
+    auto load_bnd = [&](bool on, int64_t dstep, int kk) -> double {
+      return load_coherent(&S.out[face_at(on, dstep, kk)]);
     };
     double zprev = 0.0;
     // prefetch rings: slot s holds the inputs of step w0 + s.  The step loop
     // is unrolled by kChunk and a slot is refilled only after its step has
     // used it, so every load writes the registers it is consumed from and
     // no wait on a load in flight is needed before its own step.
+    // (the gate values are issued first: each is checked after the loads
+    // issued behind it, never before them)
+    double gx = load_bnd(bnd_x, dstep_x, 2 * kChunk - 1 - off);
+    double gy = load_bnd(bnd_y, dstep_y, 2 * kChunk - 1 - off);
     Cell win[kChunk];
     double wx[kChunk], wy[kChunk];
 #pragma unroll
@@ -311,7 +317,20 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     }
     const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
     const int lyj = lj > 0 ? lj - 1 : 0;
+    // Chunk gate: before a chunk issues its refills (the inputs of the next
+    // chunk), the face values they read must be final, else every refill
+    // reads the sentinel and each step pays a round trip in ready_value.  The
+    // gate value of a chunk (the face value of the next chunk's last step) is
+    // loaded one chunk earlier and only re-polled if it was not final then,
+    // so a tile settles at a lag behind its upstream tiles that needs no
+    // waits at all.
     for (int w0 = 0; w0 < nsteps; w0 += kChunk) {
+      if (bnd_x && __double_as_longlong(gx) == (long long)kSentinel)
+        wait_value(&S.out[face_at(true, dstep_x, w0 + 2 * kChunk - 1 - off)], S.abort_flag);
+      if (bnd_y && __double_as_longlong(gy) == (long long)kSentinel)
+        wait_value(&S.out[face_at(true, dstep_y, w0 + 2 * kChunk - 1 - off)], S.abort_flag);
+      gx = load_bnd(bnd_x, dstep_x, w0 + 3 * kChunk - 1 - off);
+      gy = load_bnd(bnd_y, dstep_y, w0 + 3 * kChunk - 1 - off);
 #pragma clang loop unroll(full)
       for (int s = 0; s < kChunk; ++s) {
         const int w = w0 + s;
@@ -426,7 +445,8 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.valid = true;
 }
 
-void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z) {
+void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z,
+                 KTimer* timer, int64_t ndof) {
   const int64_t nc = P.ncell;
   // two ticket counters and the abort flag; outputs start as the sentinel
   int* tick = P.progress.p;
@@ -448,14 +468,21 @@ void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z) {
   S.in = nullptr;
   S.out = P.zf.p;
   S.ticket = tick;
+  hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
   hipLaunchKernelGGL(k_sweep<false>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
+  hipEvent_t e1 = timer ? timer->mark(st) : nullptr;
   S.cells = reinterpret_cast<const Cell*>(P.cb.p);
   S.in = P.zf.p;
   S.out = P.zb.p;
   S.ticket = tick + 1;
+  hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
   hipLaunchKernelGGL(k_sweep<true>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
+  if (timer) {
+    timer->add(KTimer::SWEEP_FWD, ndof, e0, e1);
+    timer->add(KTimer::SWEEP_BWD, ndof, e2, timer->mark(st));
+  }
   hipLaunchKernelGGL(k_scatter, dim3(blocks(nc, 256)), dim3(256), 0, st, nc,
                      P.pdof.p, P.zb.p, z);
   WC_LAUNCH_CHECK();

@@ -26,7 +26,7 @@
  *   wco_apply_pressure  pinned by the reference's own divergence diagnostic
  *       (discrete divergence <= tol after projection).
  *   wco_update_liquid_phi / activation / extrapolation: pinned by hand cases
- *       on tiny grids (tests/test_oracle_grid.py).
+ *       on tiny grids (tests/test_oracle_scene.py).
  *   BLAS dot: a sequential sum; the reference uses Eigen's SIMD-blocked dot,
  *       so dot products match to rounding only (parity by tolerance).
  */
