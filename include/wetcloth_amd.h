@@ -131,6 +131,13 @@ wc_status wc_download_field(wc_ctx* ctx, int32_t field, int32_t axis,
 wc_status wc_download_system(wc_ctx* ctx, int64_t* ndof, int64_t* nnz,
                              uint32_t* rowstart, uint32_t* col, double* val,
                              double* rhs, double* solution);
+/* The context's last pressure system (after wc_project), with DOF-ordered
+ * host vectors of length ndof, on the kernels wc_project used (pipelined or
+ * level-scheduled): apply_preconditioner (PCGSolver.h:326-329) with the
+ * MIC(0) factor wc_project computed, and multiply(FixedSparseMatrix, x, y)
+ * (PCGSolver/SparseMatrix.h:298-322).  For parity checks. */
+wc_status wc_apply_preconditioner(wc_ctx* ctx, const double* r, double* z);
+wc_status wc_system_multiply(wc_ctx* ctx, const double* x, double* y);
 /* Wall time of the last wc_step's phases on the context's stream (HIP
  * events), in ms: rebucketize, P2G, grid update, assemble, solve, apply,
  * G2P -- the reference's timing_buffer phases (App/ParticleSimulation.cpp:19). */

@@ -97,6 +97,8 @@ def lib():
         L.wc_phase_times.argtypes = [vp, _dp]
         L.wc_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
         L.wc_kernel_stats.argtypes = [vp, C.c_int32, _dp, _i64p, _i64p]
+        L.wc_apply_preconditioner.argtypes = [vp, _dp, _dp]
+        L.wc_system_multiply.argtypes = [vp, _dp, _dp]
         L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
                                        C.c_int32, _dp, _i32p]
         L.wc_mic0_factor_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double, C.c_double,
@@ -275,6 +277,21 @@ class Scene:
 
     def set_option(self, name: str, value: int):
         _check(lib().wc_set_option(self.h, name.encode(), int(value)), self.h)
+
+    def apply_preconditioner(self, r):
+        """MIC(0) apply (PCGSolver.h:326-329) with the factor of the last
+        projectFine, on the kernels it used; DOF-ordered vectors."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        _check(lib().wc_apply_preconditioner(self.h, _ptr(r), _ptr(z)), self.h)
+        return z
+
+    def multiply(self, x):
+        """y = A x for the last assembled pressure system."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().wc_system_multiply(self.h, _ptr(x), _ptr(y)), self.h)
+        return y
 
     def kernel_stats(self) -> dict:
         """Per solver kernel class since set_option("kernel_timing", ...):

@@ -455,6 +455,71 @@ wc_status wc_download_system(wc_ctx* c, int64_t* ndof, int64_t* nnz,
   return WC_OK;
 }
 
+namespace {
+// the context's last factored system, or an error message
+const char* no_system(const wc_ctx* c) {
+  if (c->ndof == 0) return "no pressure system: call wc_project first";
+  if (!c->pcg.invdiag.p || c->pcg.invdiag.cap < (size_t)c->ndof)
+    return "the pressure system has not been factored: call wc_project first";
+  return nullptr;
+}
+}  // namespace
+
+wc_status wc_apply_preconditioner(wc_ctx* c, const double* r, double* z) {
+  if (!ready(c) || !r || !z) return fail(c, WC_ERR_ARG, "bad argument");
+  if (const char* m = no_system(c)) return fail(c, WC_ERR_STATE, m);
+  WC_TRY(c, {
+    const int64_t n = c->ndof;
+    wc::DevBuf<double> dr, dz;
+    dr.ensure(n);
+    dz.ensure(n);
+    WC_HIP(hipMemcpyAsync(dr.p, r, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    wc::SweepPlan& P = c->pcg.sweep;
+    if (P.valid) {
+      wc::box_gather(c->stream, P, dr.p, P.rb.p);
+      wc::sweep_apply(c->stream, P, P.rb.p, nullptr, n);
+      wc::box_scatter(c->stream, P, P.zb.p, dz.p);
+    } else {
+      wc::pcg_apply(c->stream, c->pcg, system_csr(c), dr.p, dz.p);
+    }
+    WC_HIP(hipMemcpyAsync(z, dz.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    const bool aborted = P.valid && wc::sweep_aborted(c->stream, P);
+    dr.release();
+    dz.release();
+    if (aborted)
+      throw wc::HipError{hipErrorLaunchTimeOut,
+                         "pipelined MIC(0) sweep: a tile wait exceeded its bound",
+                         __FILE__, __LINE__};
+  })
+  return WC_OK;
+}
+
+wc_status wc_system_multiply(wc_ctx* c, const double* x, double* y) {
+  if (!ready(c) || !x || !y) return fail(c, WC_ERR_ARG, "bad argument");
+  if (const char* m = no_system(c)) return fail(c, WC_ERR_STATE, m);
+  WC_TRY(c, {
+    const int64_t n = c->ndof;
+    wc::DevBuf<double> dx, dy;
+    dx.ensure(n);
+    dy.ensure(n);
+    WC_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    wc::SweepPlan& P = c->pcg.sweep;
+    if (P.valid) {
+      wc::box_gather(c->stream, P, dx.p, P.sb.p);
+      wc::box_spmv(c->stream, P, P.sb.p, P.zb.p);
+      wc::box_scatter(c->stream, P, P.zb.p, dy.p);
+    } else {
+      wc::pcg_spmv(c->stream, system_csr(c), dx.p, dy.p);
+    }
+    WC_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+    dx.release();
+    dy.release();
+  })
+  return WC_OK;
+}
+
 wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
   if (!c || !name) return fail(c, WC_ERR_ARG, "bad argument");
   const std::string n(name);

@@ -93,10 +93,12 @@ struct SweepPlan {
   int o[3] = {0, 0, 0}, B[3] = {0, 0, 0};
   int ntile_a = 0, ntile_b = 0, ntiles = 0;
   int64_t ncell = 0;
-  DevBuf<int32_t> pdof;
-  DevBuf<uint32_t> order;
-  DevBuf<double> cf, cb;  // packed 48-byte cells (forward / backward)
-  DevBuf<double> zf, zb;
+  DevBuf<int32_t> pdof;             // box cell -> DOF or -1
+  DevBuf<uint32_t> order, code;     // ticket order; per-cell coupling bits
+  DevBuf<double> cf, cb;            // 32-byte factor records (forward / backward)
+  DevBuf<double> coef;              // 7 matrix entries per cell
+  DevBuf<double> zf, zb;            // sweep outputs (zb = preconditioned vector)
+  DevBuf<double> rb, sb, xb;        // PCG vectors r, s, x in box layout
   DevBuf<int> progress, flags;
   void release();
 };
@@ -206,10 +208,14 @@ void grid_apply_pressure(wc_ctx* ctx, double dt);
 // ---- wc_sweep.hip ----
 void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* act,
                    const uint32_t* isdof, const uint32_t* dofscan,
-                   const uint32_t* rowlen, const uint32_t* col,
+                   const uint32_t* rowlen, const uint32_t* col, const double* val,
                    const double* invdiag, const double* lv, int64_t ndof);
-void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z,
-                 KTimer* timer, int64_t ndof);
+// MIC(0) apply on box vectors: r (box) -> P.zb
+void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
+                 int64_t ndof);
+void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y);
+void box_gather(hipStream_t st, const SweepPlan& P, const double* dof, double* box);
+void box_scatter(hipStream_t st, const SweepPlan& P, const double* box, double* dof);
 bool sweep_aborted(hipStream_t st, SweepPlan& P);
 
 // ---- wc_pcg.hip ----

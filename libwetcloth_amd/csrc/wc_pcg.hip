@@ -522,10 +522,6 @@ void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
 
 void pcg_apply(hipStream_t st, PcgWork& w, const DevCsr& A, const double* r,
                double* z) {
-  if (w.sweep.valid) {
-    sweep_apply(st, w.sweep, r, z, w.timer.on ? &w.timer : nullptr, A.n);
-    return;
-  }
   for (int l = 0; l < w.nlevels; ++l) {
     const uint32_t cnt = w.lvl_size[l];
     hipLaunchKernelGGL(k_lower_level, dim3(blocks_for(cnt, 64)), dim3(64), 0, st,
@@ -557,10 +553,78 @@ static double read_scal(hipStream_t st, PcgWork& w, int slot) {
   return w.host_scal[slot];
 }
 
+// The PCG loop of PCGSolver::solve (PCGSolver.h:280-304) on box vectors: the
+// pipelined sweeps apply MIC(0) in place and the SpMV is the box stencil.
+// b is gathered into the box once and x scattered back once.
+static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double* b,
+                         double* x_dof, double tolr, int maxit, double* residual_out,
+                         int* iters_out) {
+  SweepPlan& P = w.sweep;
+  const int64_t nc = P.ncell;
+  const int nb = w.red_blocks;
+  double* r = P.rb.p;
+  double* s = P.sb.p;
+  double* x = P.xb.p;
+  double* z = P.zb.p;  // the preconditioned vector, then A s
+  KTimer* tm = w.timer.on ? &w.timer : nullptr;
+  box_gather(st, P, b, r);
+  WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
+  sweep_apply(st, P, r, tm, ndof);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+                     w.partial.p);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                     w.scal.p, (int)R_SET_RHO);
+  WC_LAUNCH_CHECK();
+  const double rho = read_scal(st, w, S_RHO);
+  if (rho == 0 || rho != rho) {
+    *iters_out = 0;
+    return false;
+  }
+  WC_HIP(hipMemcpyAsync(s, z, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
+  const unsigned gb = blocks_for(nc, 256);
+  int it;
+  for (it = 0; it < maxit; ++it) {
+    hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
+    box_spmv(st, P, s, z);
+    if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, s, z, nc,
+                       w.partial.p);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                       w.scal.p, (int)R_ALPHA);
+    hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(kRedThreads), 0, st, x, r, s, z, nc,
+                       w.scal.p, w.partial.p);
+    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p,
+                       nb, w.scal.p);
+    WC_LAUNCH_CHECK();
+    const double res = read_scal(st, w, S_RES);
+    *residual_out = res;
+    if (sweep_aborted(st, P))
+      throw HipError{hipErrorLaunchTimeOut,
+                     "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__,
+                     __LINE__};
+    if (res <= tolr) {
+      *iters_out = it + 1;
+      box_scatter(st, P, x, x_dof);
+      return true;
+    }
+    sweep_apply(st, P, r, tm, ndof);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+                       w.partial.p);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                       w.scal.p, (int)R_BETA);
+    hipLaunchKernelGGL(k_update_s, dim3(gb), dim3(256), 0, st, s, z, nc, w.scal.p);
+    WC_LAUNCH_CHECK();
+  }
+  *iters_out = it;
+  box_scatter(st, P, x, x_dof);
+  return false;
+}
+
 bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
                double* x, double tol, int maxit, double tau, double mdr,
                double* residual_out, int* iters_out) {
   const int64_t n = A.n;
+  w.sweep.valid = false;  // the plan belongs to the system factored below
   if (tol < 1e-30) tol = 1e-30;  // PCGSolver.h:253
   if (n == 0) {
     *residual_out = 0;
@@ -589,7 +653,8 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
   w.sweep.valid = false;
   if (w.use_sweeps && w.grid.g)
     sweep_prepare(st, w.sweep, *w.grid.g, w.grid.act, w.grid.isdof, w.grid.dofscan,
-                  w.grid.rowlen, w.grid.col, w.invdiag.p, w.lv.p, n);
+                  w.grid.rowlen, w.grid.col, A.val, w.invdiag.p, w.lv.p, n);
+  if (w.sweep.valid) return pcg_loop_box(st, w, n, b, x, tolr, maxit, residual_out, iters_out);
   pcg_apply(st, w, A, r, z);
   hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
                      w.partial.p);
@@ -619,8 +684,6 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
     WC_LAUNCH_CHECK();
     res = read_scal(st, w, S_RES);
     *residual_out = res;
-    if (w.sweep.valid && sweep_aborted(st, w.sweep))
-      throw HipError{hipErrorLaunchTimeOut, "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__, __LINE__};
     if (res <= tolr) {
       *iters_out = it + 1;
       return true;

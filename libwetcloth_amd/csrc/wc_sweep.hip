@@ -1,7 +1,8 @@
-// Pipelined MIC(0) triangular solves on the structured pressure grid.
+// Pipelined MIC(0) triangular solves and the box-resident PCG vectors of a
+// structured pressure system.
 //
 // solve_lower / solve_lower_transpose_in_place (PCGSolver.h:211-239) of the
-// 7-point factor, as one persistent kernel per sweep instead of one launch
+// 7-point factor run as one persistent kernel per sweep instead of one launch
 // per i+j+k hyperplane.  The DOF bounding box is cut into 16x16 tiles of
 // k-pencils; a workgroup walks its tile's local hyperplanes i'+j'+k = w, one
 // step per w, passing the newest value of each pencil to its +x/+y
@@ -15,9 +16,15 @@
 // in (a+b, a) order, so a tile only waits for tiles already claimed by
 // running workgroups (no deadlock), and every wait is bounded.
 //
-// Each cell applies its <= 3 neighbour terms in ascending DOF order (the
-// order the column-oriented reference loop applies them), so the result is
-// bitwise that of the sequential solve.
+// While these sweeps are in use the PCG vectors live in the same box layout
+// (cells that are not DOFs hold zero), so the sweeps read and write them in
+// place and the SpMV is a 7-point stencil without column indices.
+//
+// Every sum keeps the reference's order: a sweep cell applies its <= 3
+// neighbour terms in ascending DOF order (the column loop's order) and a
+// SpMV row adds its entries in ascending column order (lower neighbours,
+// diagonal, upper neighbours), so results are bitwise those of the
+// sequential solve and of multiply(FixedSparseMatrix, ...).
 #include <math.h>
 
 #include <algorithm>
@@ -40,6 +47,9 @@ constexpr int C_LOWER = 1;   // bits 1..3: coupling to -x, -y, -z
 constexpr int C_UPPER = 4;   // bits 4..6: coupling to +x, +y, +z
 constexpr int C_LORD = 7;    // bits 7..9: order of the lower terms (0..5)
 constexpr int C_UORD = 10;   // bits 10..12: order of the upper terms
+constexpr uint32_t C_DIAG = 1u << 13;  // the row has a diagonal entry
+constexpr int C_ALO = 14;    // bits 14..16: A(me, -x/-y/-z) present
+constexpr int C_AUP = 17;    // bits 17..19: A(me, +x/+y/+z) present
 
 // the six orders of three axes, 2 bits per axis, 6 bits per order:
 // 0 xyz, 1 xzy, 2 yxz, 3 yzx, 4 zxy, 5 zyx
@@ -74,12 +84,12 @@ __device__ inline void store_flag(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// one box cell's sweep inputs, read with three 16-byte loads
+// one box cell's factor entries for one sweep direction (two 16-byte loads)
 struct __attribute__((aligned(16))) Cell {
-  double l0, l1, l2, inv, rhs;
-  uint32_t code, pad;
+  double l0, l1, l2, inv;
 };
-static_assert(sizeof(Cell) == 48, "Cell is three dwordx4");
+static_assert(sizeof(Cell) == 32, "Cell is two dwordx4");
+constexpr int kCoef = 7;  // per box cell: A diagonal, A(-x,-y,-z), A(+x,+y,+z)
 
 struct BoxArgs {
   Grid g;
@@ -92,29 +102,45 @@ struct BoxArgs {
   const uint32_t* col;
   const double* invdiag;
   const double* lv;
+  const double* val;
   int32_t* pdof;
-  Cell* cf;  // forward inputs (lower terms)
-  Cell* cb;  // backward inputs (upper terms)
+  uint32_t* code;
+  Cell* cf;      // forward sweep (lower terms)
+  Cell* cb;      // backward sweep (upper terms)
+  double* coef;  // kCoef entries of A per cell
 };
 
 __device__ inline int64_t box_index(const int B[3], int x, int y, int z) {
   return ((int64_t)y * B[0] + x) * B[2] + z;
 }
 
+// Bounding box of the DOF nodes: a wave reduces its lanes' coordinates with
+// shuffles and one lane issues the six atomics for the wave.
 __global__ void k_dof_box(Grid g, const uint32_t* isdof, int* bb) {
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (n >= g.nnode || !isdof[n]) return;
-  int i, j, k;
-  node_coords(g, n, i, j, k);
-  atomicMin(&bb[0], i);
-  atomicMin(&bb[1], j);
-  atomicMin(&bb[2], k);
-  atomicMax(&bb[3], i);
-  atomicMax(&bb[4], j);
-  atomicMax(&bb[5], k);
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+  if (n < g.nnode && isdof[n]) {
+    int c[3];
+    node_coords(g, n, c[0], c[1], c[2]);
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = c[d];
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  if ((threadIdx.x & 63) == 0 && hi[0] >= 0) {
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&bb[d], lo[d]);
+      atomicMax(&bb[3 + d], hi[d]);
+    }
+  }
 }
 
-// The factor's entries around each box cell, in pencil layout.
+// The factor's and the matrix's entries around each box cell, in pencil
+// layout.  The row of a DOF holds its lower neighbours, its diagonal and its
+// upper neighbours; L(me, lower) sits in the lower neighbour's row and
+// L(upper, me) in this row (lv is indexed like val).
 __global__ void k_fill_box(BoxArgs A) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t ncell = (int64_t)A.B[0] * A.B[1] * A.B[2];
@@ -128,44 +154,57 @@ __global__ void k_fill_box(BoxArgs A) {
   uint32_t code = 0;
   int32_t me = -1;
   double lm[3] = {0, 0, 0}, lp[3] = {0, 0, 0}, inv = 0;
+  double cf[kCoef] = {0, 0, 0, 0, 0, 0, 0};
   if (in_range(g, i, j, k)) {
     const int64_t n = node_index(g, i, j, k);
     if (A.isdof[n]) {
       me = (int32_t)A.dofscan[n];
       code |= C_DOF;
       inv = A.invdiag[me];
+      const int64_t row = 7 * (int64_t)me;
+      const uint32_t len = A.rowlen[me];
+      for (uint32_t s = 0; s < len; ++s)
+        if (A.col[row + s] == (uint32_t)me) {
+          code |= C_DIAG;
+          cf[0] = A.val[row + s];
+        }
       int64_t lkey[3], ukey[3];
       for (int d = 0; d < 3; ++d) {
         lkey[d] = ukey[d] = 0x7FFFFFFFFFFFll;
-        // lower neighbour: its row holds (k, me) -> L(me, k)
+        // lower neighbour: its row holds (k, me) -> L(me, k); this row holds A(me, k)
         int q[3] = {i, j, k};
         q[d] -= 1;
         if (in_range(g, q[0], q[1], q[2])) {
           const int64_t nn = node_index(g, q[0], q[1], q[2]);
           if (A.isdof[nn]) {
             const uint32_t other = A.dofscan[nn];
-            const uint32_t len = A.rowlen[other];
-            for (uint32_t s = 0; s < len; ++s)
+            lkey[d] = other;
+            const uint32_t olen = A.rowlen[other];
+            for (uint32_t s = 0; s < olen; ++s)
               if (A.col[7 * (int64_t)other + s] == (uint32_t)me) {
                 lm[d] = A.lv[7 * (int64_t)other + s];
                 code |= 1u << (C_LOWER + d);
-                lkey[d] = other;
+              }
+            for (uint32_t s = 0; s < len; ++s)
+              if (A.col[row + s] == other) {
+                cf[1 + d] = A.val[row + s];
+                code |= 1u << (C_ALO + d);
               }
           }
         }
-        // upper neighbour: my row holds (me, other) -> L(other, me)
+        // upper neighbour: this row holds (me, other) -> L(other, me) and A(me, other)
         int u[3] = {i, j, k};
         u[d] += 1;
         if (in_range(g, u[0], u[1], u[2])) {
           const int64_t nn = node_index(g, u[0], u[1], u[2]);
           if (A.isdof[nn]) {
             const uint32_t other = A.dofscan[nn];
-            const uint32_t len = A.rowlen[me];
+            ukey[d] = other;
             for (uint32_t s = 0; s < len; ++s)
-              if (A.col[7 * (int64_t)me + s] == other) {
-                lp[d] = A.lv[7 * (int64_t)me + s];
-                code |= 1u << (C_UPPER + d);
-                ukey[d] = other;
+              if (A.col[row + s] == other) {
+                lp[d] = A.lv[row + s];
+                cf[4 + d] = A.val[row + s];
+                code |= (1u << (C_UPPER + d)) | (1u << (C_AUP + d));
               }
           }
         }
@@ -175,6 +214,7 @@ __global__ void k_fill_box(BoxArgs A) {
     }
   }
   A.pdof[t] = me;
+  A.code[t] = code;
   Cell f, bw;
   f.l0 = lm[0];
   f.l1 = lm[1];
@@ -183,35 +223,62 @@ __global__ void k_fill_box(BoxArgs A) {
   bw.l1 = lp[1];
   bw.l2 = lp[2];
   f.inv = bw.inv = inv;
-  f.rhs = bw.rhs = 0.0;
-  f.code = bw.code = code;
-  f.pad = bw.pad = 0;
   A.cf[t] = f;
   A.cb[t] = bw;
+  for (int e = 0; e < kCoef; ++e) A.coef[kCoef * t + e] = cf[e];
 }
 
-__global__ void k_gather(int64_t ncell, const int32_t* pdof, const double* src,
-                         Cell* dst) {
+// DOF-ordered vector -> box (zero where the cell is not a DOF)
+__global__ void k_box_gather(int64_t ncell, const int32_t* pdof, const double* src,
+                             double* dst) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ncell) return;
   const int32_t d = pdof[t];
-  dst[t].rhs = d >= 0 ? src[d] : 0.0;
+  dst[t] = d >= 0 ? src[d] : 0.0;
 }
 
-__global__ void k_scatter(int64_t ncell, const int32_t* pdof, const double* src,
-                          double* dst) {
+// box -> DOF-ordered vector
+__global__ void k_box_scatter(int64_t ncell, const int32_t* pdof, const double* src,
+                              double* dst) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ncell) return;
   const int32_t d = pdof[t];
   if (d >= 0) dst[d] = src[t];
 }
 
+// y = A x on box vectors: multiply(FixedSparseMatrix, x, y) (SparseMatrix.h:298-322)
+// with each row's entries added in ascending column order.  A present entry
+// always couples two DOFs, so every x read stays inside the box.
+__global__ void k_box_spmv(int64_t ncell, int64_t sx, int64_t sy, const uint32_t* code,
+                           const double* coef, const double* x, double* y) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  const uint32_t c = code[t];
+  double acc = 0.0;
+  if (c & C_DOF) {
+    const double* a = coef + kCoef * t;
+    const int64_t stride[3] = {sx, sy, 1};
+    const int lo = (c >> C_LORD) & 7, up = (c >> C_UORD) & 7;
+    for (int q = 0; q < 3; ++q) {
+      const int ax = perm_axis(lo, q);
+      if (c & (1u << (C_ALO + ax))) acc += a[1 + ax] * x[t - stride[ax]];
+    }
+    if (c & C_DIAG) acc += a[0] * x[t];
+    for (int q = 0; q < 3; ++q) {
+      const int ax = perm_axis(up, q);
+      if (c & (1u << (C_AUP + ax))) acc += a[4 + ax] * x[t + stride[ax]];
+    }
+  }
+  y[t] = acc;
+}
+
 struct SweepArgs {
   int B[3];
   int ntile_a, ntile_b, ntiles;
   const uint32_t* order;   // ticket -> a | b << 16
-  const Cell* cells;       // forward: lower terms and r; backward: upper terms
-  const double* in;        // backward only: the forward result
+  const Cell* cells;       // forward: lower terms; backward: upper terms
+  const uint32_t* code;
+  const double* in;        // forward: r; backward: the forward result
   double* out;
   int* ticket;
   int* abort_flag;
@@ -219,14 +286,21 @@ struct SweepArgs {
 
 
 // inputs of the kk-th cell of a pencil; kk is clamped into the pencil so the
-// load is always issued to a valid address (steps outside it discard it)
+// loads are always issued to a valid address (steps outside it discard them)
+struct CellIn {
+  Cell c;
+  double rhs;
+  uint32_t code;
+};
 template <bool BWD>
-__device__ inline Cell load_cell_dir(const SweepArgs& S, int64_t col, int Bz, int kk) {
+__device__ inline CellIn load_cell_dir(const SweepArgs& S, int64_t col, int Bz, int kk) {
   const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
   const int64_t idx = col + (BWD ? (Bz - 1 - kc) : kc);
-  Cell c = S.cells[idx];
-  if (BWD) c.rhs = S.in[idx];
-  return c;
+  CellIn v;
+  v.c = S.cells[idx];
+  v.rhs = S.in[idx];
+  v.code = S.code[idx];
+  return v;
 }
 #define load_cell(S, col, Bz, kk) load_cell_dir<BWD>(S, col, Bz, kk)
 
@@ -307,7 +381,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // issued behind it, never before them)
     double gx = load_bnd(bnd_x, dstep_x, 2 * kChunk - 1 - off);
     double gy = load_bnd(bnd_y, dstep_y, 2 * kChunk - 1 - off);
-    Cell win[kChunk];
+    CellIn win[kChunk];
     double wx[kChunk], wy[kChunk];
 #pragma unroll
     for (int s = 0; s < kChunk; ++s) {
@@ -338,8 +412,9 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           __syncthreads();
           const int kk = w - off;
           const bool active = kk >= 0 && kk < Bz;
-          const Cell c = win[s];
-          const uint32_t code = active ? c.code : 0u;
+          const CellIn ci = win[s];
+          const Cell c = ci.c;
+          const uint32_t code = active ? ci.code : 0u;
           const int64_t idx = out_index(kk);
           const double lx = lz[(w - 1) & 1][lj][lxi];
           const double ly = lz[(w - 1) & 1][lyj][li];
@@ -352,7 +427,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           if (bnd_y && (code & (2u << ubits)))
             zy = ready_value(&S.out[idx + dstep_y], zy, S.abort_flag);
           const int perm = (code >> obits) & 7;
-          double acc = c.rhs;
+          double acc = ci.rhs;
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int ax = perm_axis(perm, q);
@@ -383,7 +458,7 @@ inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* act,
                    const uint32_t* isdof, const uint32_t* dofscan,
-                   const uint32_t* rowlen, const uint32_t* col,
+                   const uint32_t* rowlen, const uint32_t* col, const double* val,
                    const double* invdiag, const double* lv, int64_t ndof) {
   P.valid = false;
   if (ndof == 0) return;
@@ -407,10 +482,15 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.ntiles = P.ntile_a * P.ntile_b;
   P.ncell = (int64_t)P.B[0] * P.B[1] * P.B[2];
   P.pdof.ensure(P.ncell);
+  P.code.ensure(P.ncell);
   P.cf.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.cb.ensure(P.ncell * sizeof(Cell) / sizeof(double));
+  P.coef.ensure(P.ncell * kCoef);
   P.zf.ensure(P.ncell);
   P.zb.ensure(P.ncell);
+  P.rb.ensure(P.ncell);
+  P.sb.ensure(P.ncell);
+  P.xb.ensure(P.ncell);
   // tickets in (a+b, a) order
   std::vector<uint32_t> order;
   order.reserve(P.ntiles);
@@ -436,27 +516,26 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   A.col = col;
   A.invdiag = invdiag;
   A.lv = lv;
+  A.val = val;
   A.pdof = P.pdof.p;
+  A.code = P.code.p;
   A.cf = reinterpret_cast<Cell*>(P.cf.p);
   A.cb = reinterpret_cast<Cell*>(P.cb.p);
+  A.coef = P.coef.p;
   hipLaunchKernelGGL(k_fill_box, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, A);
   WC_LAUNCH_CHECK();
   WC_HIP(hipStreamSynchronize(st));  // 'order' must outlive its copy
   P.valid = true;
 }
 
-void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z,
-                 KTimer* timer, int64_t ndof) {
-  const int64_t nc = P.ncell;
+void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
+                 int64_t ndof) {
   // two ticket counters and the abort flag; outputs start as the sentinel
   int* tick = P.progress.p;
   int* abort_flag = tick + 2;
   WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 8, st));
-  WC_HIP(hipMemsetAsync(P.zf.p, 0xFF, sizeof(double) * nc, st));
-  WC_HIP(hipMemsetAsync(P.zb.p, 0xFF, sizeof(double) * nc, st));
-  Cell* cf = reinterpret_cast<Cell*>(P.cf.p);
-  hipLaunchKernelGGL(k_gather, dim3(blocks(nc, 256)), dim3(256), 0, st, nc,
-                     P.pdof.p, r, cf);
+  WC_HIP(hipMemsetAsync(P.zf.p, 0xFF, sizeof(double) * P.ncell, st));
+  WC_HIP(hipMemsetAsync(P.zb.p, 0xFF, sizeof(double) * P.ncell, st));
   SweepArgs S;
   for (int d = 0; d < 3; ++d) S.B[d] = P.B[d];
   S.ntile_a = P.ntile_a;
@@ -464,8 +543,9 @@ void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z,
   S.ntiles = P.ntiles;
   S.order = P.order.p;
   S.abort_flag = abort_flag;
-  S.cells = cf;
-  S.in = nullptr;
+  S.code = P.code.p;
+  S.cells = reinterpret_cast<const Cell*>(P.cf.p);
+  S.in = r;
   S.out = P.zf.p;
   S.ticket = tick;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
@@ -483,8 +563,24 @@ void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, double* z,
     timer->add(KTimer::SWEEP_FWD, ndof, e0, e1);
     timer->add(KTimer::SWEEP_BWD, ndof, e2, timer->mark(st));
   }
-  hipLaunchKernelGGL(k_scatter, dim3(blocks(nc, 256)), dim3(256), 0, st, nc,
-                     P.pdof.p, P.zb.p, z);
+}
+
+void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y) {
+  const int64_t sx = P.B[2], sy = (int64_t)P.B[0] * P.B[2];
+  hipLaunchKernelGGL(k_box_spmv, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, P.ncell,
+                     sx, sy, P.code.p, P.coef.p, x, y);
+  WC_LAUNCH_CHECK();
+}
+
+void box_gather(hipStream_t st, const SweepPlan& P, const double* dof, double* box) {
+  hipLaunchKernelGGL(k_box_gather, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
+                     P.ncell, P.pdof.p, dof, box);
+  WC_LAUNCH_CHECK();
+}
+
+void box_scatter(hipStream_t st, const SweepPlan& P, const double* box, double* dof) {
+  hipLaunchKernelGGL(k_box_scatter, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
+                     P.ncell, P.pdof.p, box, dof);
   WC_LAUNCH_CHECK();
 }
 
@@ -496,8 +592,13 @@ bool sweep_aborted(hipStream_t st, SweepPlan& P) {
 }
 void SweepPlan::release() {
   pdof.release();
+  code.release();
   cf.release();
   cb.release();
+  coef.release();
+  rb.release();
+  sb.release();
+  xb.release();
   zf.release();
   zb.release();
   order.release();
