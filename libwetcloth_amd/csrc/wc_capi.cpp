@@ -459,7 +459,7 @@ namespace {
 // the context's last factored system, or an error message
 const char* no_system(const wc_ctx* c) {
   if (c->ndof == 0) return "no pressure system: call wc_project first";
-  if (!c->pcg.invdiag.p || c->pcg.invdiag.cap < (size_t)c->ndof)
+  if (!c->pcg.factored)
     return "the pressure system has not been factored: call wc_project first";
   return nullptr;
 }

@@ -97,6 +97,7 @@ struct SweepPlan {
   DevBuf<uint32_t> order, code;     // ticket order; per-cell coupling bits
   DevBuf<double> cf, cb;            // 32-byte factor records (forward / backward)
   DevBuf<double> coef;              // 7 matrix entries per cell
+  DevBuf<double> fin;               // 96-byte factor-pass inputs per cell
   DevBuf<double> zf, zb;            // sweep outputs (zb = preconditioned vector)
   DevBuf<double> rb, sb, xb;        // PCG vectors r, s, x in box layout
   DevBuf<int> progress, flags;
@@ -118,6 +119,7 @@ struct GridSystem {
 struct PcgWork {
   GridSystem grid;   // set by the caller for grid systems
   KTimer timer;
+  bool factored = false;  // a factor of the last system exists (either path)
   bool use_sweeps = true;
   SweepPlan sweep;
   DevBuf<double> invdiag, adiag, lv, r, z, s, x, partial, scal;
@@ -128,6 +130,9 @@ struct PcgWork {
   std::vector<uint32_t> lvl_start;  // host copy: start of each level
   std::vector<uint32_t> lvl_size;
   int nlevels = 0;
+  const int32_t* level_in = nullptr;  // caller's per-row level, or null
+  int32_t level_span = 0;
+  bool levels_ready = false;
   int red_blocks = 0;
   double* host_scal = nullptr;  // pinned
   void release();
@@ -209,7 +214,9 @@ void grid_apply_pressure(wc_ctx* ctx, double dt);
 void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* act,
                    const uint32_t* isdof, const uint32_t* dofscan,
                    const uint32_t* rowlen, const uint32_t* col, const double* val,
-                   const double* invdiag, const double* lv, int64_t ndof);
+                   int64_t ndof);
+// MIC(0) factor of the prepared box (pipelined), into the sweep records
+void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr);
 // MIC(0) apply on box vectors: r (box) -> P.zb
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
                  int64_t ndof);

@@ -431,6 +431,18 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
   w.partial.ensure(w.red_blocks);
   w.scal.ensure(S_NSCAL);
   if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
+  w.level_in = level;
+  w.level_span = level_span;
+  w.levels_ready = false;
+  w.nlevels = 0;
+}
+
+// The level schedule of the level-scheduled kernels (built on first use, so
+// systems solved with the pipelined sweeps never pay for it).
+static void pcg_levels(hipStream_t st, PcgWork& w, const DevCsr& A) {
+  if (w.levels_ready) return;
+  w.levels_ready = true;
+  const int64_t n = A.n;
   if (n == 0) {
     w.nlevels = 0;
     return;
@@ -439,8 +451,8 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
                      w.tpos.p, w.adiag.p);
   WC_LAUNCH_CHECK();
 
-  const int32_t* lev = level;
-  int32_t span = level_span;
+  const int32_t* lev = w.level_in;
+  int32_t span = w.level_span;
   if (!lev) {
     w.level.ensure(n);
     WC_HIP(hipMemsetAsync(w.level.p, 0, sizeof(int32_t) * n, st));
@@ -505,6 +517,7 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
 void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
                 double mdr) {
   if (A.n == 0) return;
+  pcg_levels(st, w, A);
   int* overflow = w.flags.p + 1;
   WC_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st));
   for (int l = 0; l < w.nlevels; ++l) {
@@ -522,6 +535,7 @@ void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
 
 void pcg_apply(hipStream_t st, PcgWork& w, const DevCsr& A, const double* r,
                double* z) {
+  pcg_levels(st, w, A);
   for (int l = 0; l < w.nlevels; ++l) {
     const uint32_t cnt = w.lvl_size[l];
     hipLaunchKernelGGL(k_lower_level, dim3(blocks_for(cnt, 64)), dim3(64), 0, st,
@@ -625,6 +639,7 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
                double* residual_out, int* iters_out) {
   const int64_t n = A.n;
   w.sweep.valid = false;  // the plan belongs to the system factored below
+  w.factored = false;
   if (tol < 1e-30) tol = 1e-30;  // PCGSolver.h:253
   if (n == 0) {
     *residual_out = 0;
@@ -649,12 +664,16 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
     return true;
   }
   const double tolr = tol * res;
-  pcg_factor(st, w, A, tau, mdr);
-  w.sweep.valid = false;
   if (w.use_sweeps && w.grid.g)
     sweep_prepare(st, w.sweep, *w.grid.g, w.grid.act, w.grid.isdof, w.grid.dofscan,
-                  w.grid.rowlen, w.grid.col, A.val, w.invdiag.p, w.lv.p, n);
-  if (w.sweep.valid) return pcg_loop_box(st, w, n, b, x, tolr, maxit, residual_out, iters_out);
+                  w.grid.rowlen, w.grid.col, A.val, n);
+  if (w.sweep.valid) {
+    sweep_factor(st, w.sweep, tau, mdr);
+    w.factored = true;
+    return pcg_loop_box(st, w, n, b, x, tolr, maxit, residual_out, iters_out);
+  }
+  pcg_factor(st, w, A, tau, mdr);
+  w.factored = true;
   pcg_apply(st, w, A, r, z);
   hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
                      w.partial.p);

@@ -91,6 +91,20 @@ struct __attribute__((aligned(16))) Cell {
 static_assert(sizeof(Cell) == 32, "Cell is two dwordx4");
 constexpr int kCoef = 7;  // per box cell: A diagonal, A(-x,-y,-z), A(+x,+y,+z)
 
+// one box cell's inputs of the factor pass (six 16-byte loads): the lower
+// triangle of A as factor_modified_incomplete_cholesky0 copies it
+// (PCGSolver.h:110-121), seen from this cell's row of L
+struct __attribute__((aligned(16))) FacIn {
+  double adiag;      // A(me, me), 0 if the row has no diagonal entry
+  double lo[3];      // A(k_d, me) from the row of the lower neighbour k_d = me - e_d
+  double oth[3][2];  // row k_d's other entries above its diagonal, ascending column
+  uint32_t fcode;    // F_* bits below
+  uint32_t code;     // the cell's C_* bits
+};
+static_assert(sizeof(FacIn) == 96, "FacIn is six dwordx4");
+constexpr int F_KNULL = 0;  // bits 0..2: adiag(k_d) == 0 (column k_d is skipped)
+constexpr int F_NOTH = 3;   // bits 3..8: number of oth entries per d (2 bits each)
+
 struct BoxArgs {
   Grid g;
   int o[3];        // box origin (dense node coordinates)
@@ -105,9 +119,10 @@ struct BoxArgs {
   const double* val;
   int32_t* pdof;
   uint32_t* code;
-  Cell* cf;      // forward sweep (lower terms)
-  Cell* cb;      // backward sweep (upper terms)
+  Cell* cf;      // forward sweep (lower terms), zeroed here, filled by the factor pass
+  Cell* cb;      // backward sweep (upper terms), likewise
   double* coef;  // kCoef entries of A per cell
+  FacIn* fin;    // factor pass inputs
 };
 
 __device__ inline int64_t box_index(const int B[3], int x, int y, int z) {
@@ -151,27 +166,30 @@ __global__ void k_fill_box(BoxArgs A) {
   const int y = (int)(xy / A.B[0]);
   const int i = A.o[0] + x, j = A.o[1] + y, k = A.o[2] + z;
   const Grid& g = A.g;
-  uint32_t code = 0;
+  uint32_t code = 0, fcode = 0;
   int32_t me = -1;
-  double lm[3] = {0, 0, 0}, lp[3] = {0, 0, 0}, inv = 0;
   double cf[kCoef] = {0, 0, 0, 0, 0, 0, 0};
+  FacIn f;
+  f.adiag = 0;
+  for (int d = 0; d < 3; ++d) f.lo[d] = f.oth[d][0] = f.oth[d][1] = 0;
   if (in_range(g, i, j, k)) {
     const int64_t n = node_index(g, i, j, k);
     if (A.isdof[n]) {
       me = (int32_t)A.dofscan[n];
       code |= C_DOF;
-      inv = A.invdiag[me];
       const int64_t row = 7 * (int64_t)me;
       const uint32_t len = A.rowlen[me];
       for (uint32_t s = 0; s < len; ++s)
         if (A.col[row + s] == (uint32_t)me) {
           code |= C_DIAG;
           cf[0] = A.val[row + s];
+          f.adiag = A.val[row + s];
         }
       int64_t lkey[3], ukey[3];
       for (int d = 0; d < 3; ++d) {
         lkey[d] = ukey[d] = 0x7FFFFFFFFFFFll;
-        // lower neighbour: its row holds (k, me) -> L(me, k); this row holds A(me, k)
+        // lower neighbour k_d: its row holds (k_d, me) -> column k_d of L;
+        // this row holds A(me, k_d) for the SpMV
         int q[3] = {i, j, k};
         q[d] -= 1;
         if (in_range(g, q[0], q[1], q[2])) {
@@ -179,12 +197,23 @@ __global__ void k_fill_box(BoxArgs A) {
           if (A.isdof[nn]) {
             const uint32_t other = A.dofscan[nn];
             lkey[d] = other;
+            const int64_t orow = 7 * (int64_t)other;
             const uint32_t olen = A.rowlen[other];
-            for (uint32_t s = 0; s < olen; ++s)
-              if (A.col[7 * (int64_t)other + s] == (uint32_t)me) {
-                lm[d] = A.lv[7 * (int64_t)other + s];
+            double kdiag = 0;
+            int noth = 0;
+            for (uint32_t s = 0; s < olen; ++s) {
+              const uint32_t c = A.col[orow + s];
+              if (c == (uint32_t)me) {
+                f.lo[d] = A.val[orow + s];
                 code |= 1u << (C_LOWER + d);
+              } else if (c == other) {
+                kdiag = A.val[orow + s];
+              } else if (c > other && noth < 2) {
+                f.oth[d][noth++] = A.val[orow + s];  // ascending: rows are sorted
               }
+            }
+            if (kdiag == 0) fcode |= 1u << (F_KNULL + d);
+            fcode |= (uint32_t)noth << (F_NOTH + 2 * d);
             for (uint32_t s = 0; s < len; ++s)
               if (A.col[row + s] == other) {
                 cf[1 + d] = A.val[row + s];
@@ -192,7 +221,8 @@ __global__ void k_fill_box(BoxArgs A) {
               }
           }
         }
-        // upper neighbour: this row holds (me, other) -> L(other, me) and A(me, other)
+        // upper neighbour: this row holds A(me, other); L(other, me) is
+        // written here by the factor pass of the neighbour
         int u[3] = {i, j, k};
         u[d] += 1;
         if (in_range(g, u[0], u[1], u[2])) {
@@ -202,7 +232,6 @@ __global__ void k_fill_box(BoxArgs A) {
             ukey[d] = other;
             for (uint32_t s = 0; s < len; ++s)
               if (A.col[row + s] == other) {
-                lp[d] = A.lv[row + s];
                 cf[4 + d] = A.val[row + s];
                 code |= (1u << (C_UPPER + d)) | (1u << (C_AUP + d));
               }
@@ -215,16 +244,12 @@ __global__ void k_fill_box(BoxArgs A) {
   }
   A.pdof[t] = me;
   A.code[t] = code;
-  Cell f, bw;
-  f.l0 = lm[0];
-  f.l1 = lm[1];
-  f.l2 = lm[2];
-  bw.l0 = lp[0];
-  bw.l1 = lp[1];
-  bw.l2 = lp[2];
-  f.inv = bw.inv = inv;
-  A.cf[t] = f;
-  A.cb[t] = bw;
+  f.fcode = fcode;
+  f.code = code;
+  A.fin[t] = f;
+  const Cell zero = {0, 0, 0, 0};
+  A.cf[t] = zero;
+  A.cb[t] = zero;
   for (int e = 0; e < kCoef; ++e) A.coef[kCoef * t + e] = cf[e];
 }
 
@@ -272,6 +297,8 @@ __global__ void k_box_spmv(int64_t ncell, int64_t sx, int64_t sy, const uint32_t
   y[t] = acc;
 }
 
+enum { K_FWD = 0, K_BWD = 1, K_FAC = 2 };
+
 struct SweepArgs {
   int B[3];
   int ntile_a, ntile_b, ntiles;
@@ -279,33 +306,53 @@ struct SweepArgs {
   const Cell* cells;       // forward: lower terms; backward: upper terms
   const uint32_t* code;
   const double* in;        // forward: r; backward: the forward result
-  double* out;
+  double* out;             // the sweep result, or the factor's invdiag
   int* ticket;
   int* abort_flag;
+  // factor pass
+  const FacIn* fin;
+  Cell* cf;
+  Cell* cb;
+  double tau, mdr;
 };
 
-
-// inputs of the kk-th cell of a pencil; kk is clamped into the pencil so the
-// loads are always issued to a valid address (steps outside it discard them)
+// inputs of one cell of a sweep
 struct CellIn {
   Cell c;
   double rhs;
   uint32_t code;
 };
-template <bool BWD>
-__device__ inline CellIn load_cell_dir(const SweepArgs& S, int64_t col, int Bz, int kk) {
-  const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
-  const int64_t idx = col + (BWD ? (Bz - 1 - kc) : kc);
-  CellIn v;
-  v.c = S.cells[idx];
-  v.rhs = S.in[idx];
-  v.code = S.code[idx];
-  return v;
+
+template <int KIND>
+struct PassIn {
+  using T = CellIn;
+  static constexpr int chunk = 8;  // steps prefetched ahead
+};
+template <>
+struct PassIn<K_FAC> {
+  using T = FacIn;
+  static constexpr int chunk = 4;
+};
+
+// inputs of the kk-th cell of a pencil; kk is clamped into the pencil so the
+// loads are always issued to a valid address (steps outside it discard them)
+template <int KIND>
+__device__ inline typename PassIn<KIND>::T load_in(const SweepArgs& S, int64_t idx) {
+  if constexpr (KIND == K_FAC) {
+    return S.fin[idx];
+  } else {
+    CellIn v;
+    v.c = S.cells[idx];
+    v.rhs = S.in[idx];
+    v.code = S.code[idx];
+    return v;
+  }
 }
-#define load_cell(S, col, Bz, kk) load_cell_dir<BWD>(S, col, Bz, kk)
+__device__ inline uint32_t code_of(const CellIn& v) { return v.code; }
+__device__ inline uint32_t code_of(const FacIn& v) { return v.code; }
 
 // the value at p once it is no longer the sentinel (rare path: the
-// prefetched copy v was read before its tile had written it)
+// prefetched copy was read before its tile had written it)
 __device__ __attribute__((always_inline)) inline double wait_value(const double* p, int* abort_flag) {
   long long spins = 0;
   double v = load_coherent(p);
@@ -326,17 +373,101 @@ __device__ inline double ready_value(const double* p, double v, int* abort_flag)
   return v;
 }
 
-// One pipelined sweep.  BWD mirrors every axis: local coordinates run from
+// solve_lower / solve_lower_transpose_in_place for one cell: the rhs minus
+// the present neighbour terms in ascending DOF order, times invdiag.
+// Absent terms subtract +0, an exact identity; cells that are not DOFs
+// carry rhs = inv = 0 and no terms, so their value is 0.
+template <int KIND>
+__device__ inline double sweep_cell(const CellIn& ci, uint32_t code, double zx, double zy,
+                                    double zprev) {
+  const int ubits = KIND == K_BWD ? C_UPPER : C_LOWER;
+  const int perm = (code >> (KIND == K_BWD ? C_UORD : C_LORD)) & 7;
+  double acc = ci.rhs;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(perm, q);
+    const double L = ax == 0 ? ci.c.l0 : (ax == 1 ? ci.c.l1 : ci.c.l2);
+    const double zn = ax == 0 ? zx : (ax == 1 ? zy : zprev);
+    const double term = (code & (1u << (ubits + ax))) ? L * zn : 0.0;
+    acc -= term;
+  }
+  return acc * ci.c.inv;
+}
+
+// factor_modified_incomplete_cholesky0 (PCGSolver.h:136-205) for one cell of
+// a 7-point system, where no update ever fills in: each lower neighbour
+// column k (ascending DOF order, skipped when adiag(k) == 0) scales
+// L(me, k) = A(k, me) invdiag(k) and modifies this diagonal by -L^2 and by
+// -(tau L) missing, missing being column k's other scaled entries summed in
+// ascending row order.  Returns invdiag(me) and writes the scaled entries
+// into the sweep records: L(me, k_d) into this cell's forward record and
+// into the backward record of k_d.
+__device__ inline double sel3(int ax, double v0, double v1, double v2) {
+  return ax == 0 ? v0 : (ax == 1 ? v1 : v2);
+}
+
+__device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_t code,
+                                     double ix, double iy, double iprev, int64_t idx,
+                                     const int64_t stride[3]) {
+  const int perm = (code >> C_LORD) & 7;
+  double L0 = 0, L1 = 0, L2 = 0;
+  double d = f.adiag;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(perm, q);
+    if (!(code & (1u << (C_LOWER + ax)))) continue;
+    const double ik = sel3(ax, ix, iy, iprev);
+    const double a = sel3(ax, f.lo[0], f.lo[1], f.lo[2]);
+    double m = a;  // column k skipped (adiag(k) == 0): its entries stay unscaled
+    if (!(f.fcode & (1u << (F_KNULL + ax)))) {
+      m = a * ik;
+      const int noth = (f.fcode >> (F_NOTH + 2 * ax)) & 3;
+      const double o0 = sel3(ax, f.oth[0][0], f.oth[1][0], f.oth[2][0]);
+      const double o1 = sel3(ax, f.oth[0][1], f.oth[1][1], f.oth[2][1]);
+      double missing = 0;
+      if (noth > 0) missing += o0 * ik;
+      if (noth > 1) missing += o1 * ik;
+      d -= m * m;
+      d -= S.tau * m * missing;
+    }
+    if (ax == 0) L0 = m;
+    else if (ax == 1) L1 = m;
+    else L2 = m;
+  }
+  double inv;
+  if (f.adiag == 0)
+    inv = d;  // null row: left as accumulated (PCGSolver.h:141)
+  else if (d < S.mdr * f.adiag)
+    inv = 1 / sqrt(f.adiag);
+  else
+    inv = 1 / sqrt(d);
+  Cell c;
+  c.l0 = L0;
+  c.l1 = L1;
+  c.l2 = L2;
+  c.inv = inv;
+  S.cf[idx] = c;
+  S.cb[idx].inv = inv;
+  if (code & (1u << (C_LOWER + 0))) S.cb[idx - stride[0]].l0 = L0;
+  if (code & (1u << (C_LOWER + 1))) S.cb[idx - stride[1]].l1 = L1;
+  if (code & (1u << (C_LOWER + 2))) S.cb[idx - stride[2]].l2 = L2;
+  return inv;
+}
+
+// One pipelined pass.  K_BWD mirrors every axis: local coordinates run from
 // the high corner of the box, upstream tiles are (a+1, b) and (a, b+1).
-template <bool BWD>
+template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
+  constexpr bool BWD = KIND == K_BWD;
+  constexpr int kCh = PassIn<KIND>::chunk;
+  using In = typename PassIn<KIND>::T;
   __shared__ double lz[2][kT][kT + 1];
   __shared__ int s_tile;
   const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
   const int Bz = S.B[2];
   const int nsteps = 2 * (kT - 1) + Bz;
   const int ubits = BWD ? C_UPPER : C_LOWER;
-  const int obits = BWD ? C_UORD : C_LORD;
+  const int64_t stride[3] = {Bz, (int64_t)S.B[0] * Bz, 1};
   for (;;) {
     if (threadIdx.x == 0) {
       const int t = atomicAdd(S.ticket, 1);
@@ -361,31 +492,33 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     const bool bnd_y = lj == 0 && has_b;
     const int off = li + lj;  // this pencil starts at step off
     auto out_index = [&](int kk) { return col + (BWD ? (Bz - 1 - kk) : kk); };
+    auto in_index = [&](int kk) {
+      const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
+      return out_index(kc);
+    };
     // the upstream tile's output beside this pencil's kk-th cell.  Every lane
     // loads (no branch, so no register of a load in flight is rewritten on
     // another path); lanes off the upstream face, whose neighbour pencil may
     // lie outside the box, read element 0 instead and discard it.
     auto face_at = [&](bool on, int64_t dstep, int kk) -> int64_t {
-      const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
-      return on ? out_index(kc) + dstep : 0;
+      return on ? in_index(kk) + dstep : 0;
     };
     auto load_bnd = [&](bool on, int64_t dstep, int kk) -> double {
       return load_coherent(&S.out[face_at(on, dstep, kk)]);
     };
     double zprev = 0.0;
-    // prefetch rings: slot s holds the inputs of step w0 + s.  The step loop
-    // is unrolled by kChunk and a slot is refilled only after its step has
-    // used it, so every load writes the registers it is consumed from and
-    // no wait on a load in flight is needed before its own step.
-    // (the gate values are issued first: each is checked after the loads
-    // issued behind it, never before them)
-    double gx = load_bnd(bnd_x, dstep_x, 2 * kChunk - 1 - off);
-    double gy = load_bnd(bnd_y, dstep_y, 2 * kChunk - 1 - off);
-    CellIn win[kChunk];
-    double wx[kChunk], wy[kChunk];
+    // Prefetch rings: slot s holds the inputs of step w0 + s.  The step loop
+    // is unrolled by kCh and a slot is refilled only after its step has used
+    // it, so every load writes the registers it is consumed from and no wait
+    // on a load in flight is needed before its own step.  The gate values
+    // are issued first: each is checked after the loads issued behind it.
+    double gx = load_bnd(bnd_x, dstep_x, 2 * kCh - 1 - off);
+    double gy = load_bnd(bnd_y, dstep_y, 2 * kCh - 1 - off);
+    In win[kCh];
+    double wx[kCh], wy[kCh];
 #pragma unroll
-    for (int s = 0; s < kChunk; ++s) {
-      win[s] = load_cell(S, col, Bz, s - off);
+    for (int s = 0; s < kCh; ++s) {
+      win[s] = load_in<KIND>(S, in_index(s - off));
       wx[s] = load_bnd(bnd_x, dstep_x, s - off);
       wy[s] = load_bnd(bnd_y, dstep_y, s - off);
     }
@@ -395,26 +528,23 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // chunk), the face values they read must be final, else every refill
     // reads the sentinel and each step pays a round trip in ready_value.  The
     // gate value of a chunk (the face value of the next chunk's last step) is
-    // loaded one chunk earlier and only re-polled if it was not final then,
-    // so a tile settles at a lag behind its upstream tiles that needs no
-    // waits at all.
-    for (int w0 = 0; w0 < nsteps; w0 += kChunk) {
+    // loaded one chunk earlier and only re-polled if it was not final then.
+    for (int w0 = 0; w0 < nsteps; w0 += kCh) {
       if (bnd_x && __double_as_longlong(gx) == (long long)kSentinel)
-        wait_value(&S.out[face_at(true, dstep_x, w0 + 2 * kChunk - 1 - off)], S.abort_flag);
+        wait_value(&S.out[face_at(true, dstep_x, w0 + 2 * kCh - 1 - off)], S.abort_flag);
       if (bnd_y && __double_as_longlong(gy) == (long long)kSentinel)
-        wait_value(&S.out[face_at(true, dstep_y, w0 + 2 * kChunk - 1 - off)], S.abort_flag);
-      gx = load_bnd(bnd_x, dstep_x, w0 + 3 * kChunk - 1 - off);
-      gy = load_bnd(bnd_y, dstep_y, w0 + 3 * kChunk - 1 - off);
+        wait_value(&S.out[face_at(true, dstep_y, w0 + 2 * kCh - 1 - off)], S.abort_flag);
+      gx = load_bnd(bnd_x, dstep_x, w0 + 3 * kCh - 1 - off);
+      gy = load_bnd(bnd_y, dstep_y, w0 + 3 * kCh - 1 - off);
 #pragma clang loop unroll(full)
-      for (int s = 0; s < kChunk; ++s) {
+      for (int s = 0; s < kCh; ++s) {
         const int w = w0 + s;
         if (w < nsteps) {
           __syncthreads();
           const int kk = w - off;
           const bool active = kk >= 0 && kk < Bz;
-          const CellIn ci = win[s];
-          const Cell c = ci.c;
-          const uint32_t code = active ? ci.code : 0u;
+          const In ci = win[s];
+          const uint32_t code = active ? code_of(ci) : 0u;
           const int64_t idx = out_index(kk);
           const double lx = lz[(w - 1) & 1][lj][lxi];
           const double ly = lz[(w - 1) & 1][lyj][li];
@@ -426,26 +556,20 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
             zx = ready_value(&S.out[idx + dstep_x], zx, S.abort_flag);
           if (bnd_y && (code & (2u << ubits)))
             zy = ready_value(&S.out[idx + dstep_y], zy, S.abort_flag);
-          const int perm = (code >> obits) & 7;
-          double acc = ci.rhs;
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            const int ax = perm_axis(perm, q);
-            const double L = ax == 0 ? c.l0 : (ax == 1 ? c.l1 : c.l2);
-            const double zn = ax == 0 ? zx : (ax == 1 ? zy : zprev);
-            // terms absent from the matrix subtract +0, an exact identity
-            const double term = (code & (1u << (ubits + ax))) ? L * zn : 0.0;
-            acc -= term;
+          double z;
+          if constexpr (KIND == K_FAC) {
+            z = 0.0;
+            if (code & C_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx, stride);
+          } else {
+            z = active ? sweep_cell<KIND>(ci, code, zx, zy, zprev) : 0.0;
           }
-          // non-DOF cells carry rhs = inv = 0 and no terms, so z = 0
-          const double z = active ? acc * c.inv : 0.0;
           if (active) store_coherent(&S.out[idx], z);
           zprev = z;
           lz[w & 1][lj][li] = z;
         }
-        win[s] = load_cell(S, col, Bz, w + kChunk - off);
-        wx[s] = load_bnd(bnd_x, dstep_x, w + kChunk - off);
-        wy[s] = load_bnd(bnd_y, dstep_y, w + kChunk - off);
+        win[s] = load_in<KIND>(S, in_index(w + kCh - off));
+        wx[s] = load_bnd(bnd_x, dstep_x, w + kCh - off);
+        wy[s] = load_bnd(bnd_y, dstep_y, w + kCh - off);
       }
     }
     __syncthreads();
@@ -459,7 +583,7 @@ inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* act,
                    const uint32_t* isdof, const uint32_t* dofscan,
                    const uint32_t* rowlen, const uint32_t* col, const double* val,
-                   const double* invdiag, const double* lv, int64_t ndof) {
+                   int64_t ndof) {
   P.valid = false;
   if (ndof == 0) return;
   P.flags.ensure(8);
@@ -486,6 +610,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.cf.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.cb.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.coef.ensure(P.ncell * kCoef);
+  P.fin.ensure(P.ncell * sizeof(FacIn) / sizeof(double));
   P.zf.ensure(P.ncell);
   P.zb.ensure(P.ncell);
   P.rb.ensure(P.ncell);
@@ -514,18 +639,45 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   A.dofscan = dofscan;
   A.rowlen = rowlen;
   A.col = col;
-  A.invdiag = invdiag;
-  A.lv = lv;
   A.val = val;
   A.pdof = P.pdof.p;
   A.code = P.code.p;
   A.cf = reinterpret_cast<Cell*>(P.cf.p);
   A.cb = reinterpret_cast<Cell*>(P.cb.p);
   A.coef = P.coef.p;
+  A.fin = reinterpret_cast<FacIn*>(P.fin.p);
   hipLaunchKernelGGL(k_fill_box, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, A);
   WC_LAUNCH_CHECK();
   WC_HIP(hipStreamSynchronize(st));  // 'order' must outlive its copy
   P.valid = true;
+}
+
+static SweepArgs sweep_args(SweepPlan& P) {
+  SweepArgs S = {};
+  for (int d = 0; d < 3; ++d) S.B[d] = P.B[d];
+  S.ntile_a = P.ntile_a;
+  S.ntile_b = P.ntile_b;
+  S.ntiles = P.ntiles;
+  S.order = P.order.p;
+  S.code = P.code.p;
+  S.abort_flag = P.progress.p + 2;
+  return S;
+}
+
+void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
+  int* tick = P.progress.p;
+  WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 8, st));
+  WC_HIP(hipMemsetAsync(P.zf.p, 0xFF, sizeof(double) * P.ncell, st));
+  SweepArgs S = sweep_args(P);
+  S.fin = reinterpret_cast<const FacIn*>(P.fin.p);
+  S.cf = reinterpret_cast<Cell*>(P.cf.p);
+  S.cb = reinterpret_cast<Cell*>(P.cb.p);
+  S.tau = tau;
+  S.mdr = mdr;
+  S.out = P.zf.p;
+  S.ticket = tick + 3;
+  hipLaunchKernelGGL(k_sweep<K_FAC>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  WC_LAUNCH_CHECK();
 }
 
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
@@ -536,20 +688,14 @@ void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
   WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 8, st));
   WC_HIP(hipMemsetAsync(P.zf.p, 0xFF, sizeof(double) * P.ncell, st));
   WC_HIP(hipMemsetAsync(P.zb.p, 0xFF, sizeof(double) * P.ncell, st));
-  SweepArgs S;
-  for (int d = 0; d < 3; ++d) S.B[d] = P.B[d];
-  S.ntile_a = P.ntile_a;
-  S.ntile_b = P.ntile_b;
-  S.ntiles = P.ntiles;
-  S.order = P.order.p;
-  S.abort_flag = abort_flag;
-  S.code = P.code.p;
+  SweepArgs S = sweep_args(P);
+  (void)abort_flag;
   S.cells = reinterpret_cast<const Cell*>(P.cf.p);
   S.in = r;
   S.out = P.zf.p;
   S.ticket = tick;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL(k_sweep<false>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL(k_sweep<K_FWD>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
   hipEvent_t e1 = timer ? timer->mark(st) : nullptr;
   S.cells = reinterpret_cast<const Cell*>(P.cb.p);
@@ -557,7 +703,7 @@ void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
   S.out = P.zb.p;
   S.ticket = tick + 1;
   hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL(k_sweep<true>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL(k_sweep<K_BWD>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
   if (timer) {
     timer->add(KTimer::SWEEP_FWD, ndof, e0, e1);
@@ -596,6 +742,7 @@ void SweepPlan::release() {
   cf.release();
   cb.release();
   coef.release();
+  fin.release();
   rb.release();
   sb.release();
   xb.release();
