@@ -86,13 +86,16 @@ struct KTimer {
   void release();
 };
 
-// Pencil-layout copy of the 7-point factor for the pipelined sweeps
-// (wc_sweep.hip): the DOF bounding box, cut into 16x16 tiles of k-pencils.
+// The pipelined sweeps' copy of a 7-point system (wc_sweep.hip): the DOF
+// bounding box, cut into 16x16 tiles of k-pencils and stored in the skewed
+// tile layout, where the cells a sweep step touches are consecutive.
 struct SweepPlan {
   bool valid = false;
   int o[3] = {0, 0, 0}, B[3] = {0, 0, 0};
   int ntile_a = 0, ntile_b = 0, ntiles = 0;
-  int64_t ncell = 0;
+  int nsteps = 0;       // skewed steps per tile (B[2] + 30)
+  int64_t ncell = 0;    // slots of the skewed tile layout
+  int64_t kdx = 0, kdy = 0;  // slot offsets to the upstream x / y face cell
   DevBuf<int32_t> pdof;             // box cell -> DOF or -1
   DevBuf<uint32_t> order, code;     // ticket order; per-cell coupling bits
   DevBuf<double> cf, cb;            // 32-byte factor records (forward / backward)

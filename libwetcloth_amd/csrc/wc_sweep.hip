@@ -105,10 +105,31 @@ static_assert(sizeof(FacIn) == 96, "FacIn is six dwordx4");
 constexpr int F_KNULL = 0;  // bits 0..2: adiag(k_d) == 0 (column k_d is skipped)
 constexpr int F_NOTH = 3;   // bits 3..8: number of oth entries per d (2 bits each)
 
+// Skewed tile layout of the box.  Cell (px, py, kk) belongs to tile
+// (a, b) = (px / 16, py / 16) and lane l = (py % 16) * 16 + px % 16, and is
+// stored at the tile's step w = kk + px % 16 + py % 16, the step at which the
+// forward sweep computes it:
+//     slot = ((b * ntile_a + a) * nsteps + w) * 256 + l,  nsteps = Bz + 30.
+// At every step of a sweep (forward, or backward with w reversed) the 256
+// lanes of a tile therefore touch 256 consecutive slots.  Slots whose kk lies
+// outside [0, Bz) are padding and hold no DOF.
+struct Skew {
+  int ntile_a, ntile_b, nsteps, Bz;
+  // slot offset from a lane on a tile's low x (y) face to the cell before it
+  // in the tile below: (15, lj, kk) of tile a-1 (resp. (li, 15, kk) of b-1)
+  int64_t dx, dy;
+};
+
+__host__ __device__ inline int64_t skew_slot(const Skew& K, int a, int b, int li, int lj,
+                                             int w) {
+  return (((int64_t)b * K.ntile_a + a) * K.nsteps + w) * kThreads + lj * kT + li;
+}
+
 struct BoxArgs {
   Grid g;
   int o[3];        // box origin (dense node coordinates)
-  int B[3];        // box dims; B[0], B[1] multiples of kT
+  Skew K;
+  int64_t nslot;
   const uint8_t* act;
   const uint32_t* isdof;
   const uint32_t* dofscan;
@@ -124,10 +145,6 @@ struct BoxArgs {
   double* coef;  // kCoef entries of A per cell
   FacIn* fin;    // factor pass inputs
 };
-
-__device__ inline int64_t box_index(const int B[3], int x, int y, int z) {
-  return ((int64_t)y * B[0] + x) * B[2] + z;
-}
 
 // Bounding box of the DOF nodes: a wave reduces its lanes' coordinates with
 // shuffles and one lane issues the six atomics for the wave.
@@ -158,13 +175,17 @@ __global__ void k_dof_box(Grid g, const uint32_t* isdof, int* bb) {
 // L(upper, me) in this row (lv is indexed like val).
 __global__ void k_fill_box(BoxArgs A) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t ncell = (int64_t)A.B[0] * A.B[1] * A.B[2];
-  if (t >= ncell) return;
-  const int z = (int)(t % A.B[2]);
-  const int64_t xy = t / A.B[2];
-  const int x = (int)(xy % A.B[0]);
-  const int y = (int)(xy / A.B[0]);
+  if (t >= A.nslot) return;
+  const int lane = (int)(t % kThreads);
+  const int64_t q = t / kThreads;
+  const int w = (int)(q % A.K.nsteps);
+  const int tile = (int)(q / A.K.nsteps);
+  const int li = lane % kT, lj = lane / kT;
+  const int z = w - li - lj;
+  const int x = (tile % A.K.ntile_a) * kT + li;
+  const int y = (tile / A.K.ntile_a) * kT + lj;
   const int i = A.o[0] + x, j = A.o[1] + y, k = A.o[2] + z;
+  const bool inbox = z >= 0 && z < A.K.Bz;
   const Grid& g = A.g;
   uint32_t code = 0, fcode = 0;
   int32_t me = -1;
@@ -172,7 +193,7 @@ __global__ void k_fill_box(BoxArgs A) {
   FacIn f;
   f.adiag = 0;
   for (int d = 0; d < 3; ++d) f.lo[d] = f.oth[d][0] = f.oth[d][1] = 0;
-  if (in_range(g, i, j, k)) {
+  if (inbox && in_range(g, i, j, k)) {
     const int64_t n = node_index(g, i, j, k);
     if (A.isdof[n]) {
       me = (int32_t)A.dofscan[n];
@@ -271,27 +292,45 @@ __global__ void k_box_scatter(int64_t ncell, const int32_t* pdof, const double* 
   if (d >= 0) dst[d] = src[t];
 }
 
+// The slots of the six neighbours of slot t (valid where the coupling to that
+// neighbour is present, which implies the neighbour is a DOF in the box).
+struct Nbr {
+  int64_t lo[3], up[3];
+};
+__device__ inline Nbr neighbours(const Skew& K, int64_t t) {
+  const int lane = (int)(t % kThreads);
+  const int li = lane % kT, lj = lane / kT;
+  Nbr n;
+  n.lo[0] = li > 0 ? t - kThreads - 1 : t + K.dx;
+  n.up[0] = li < kT - 1 ? t + kThreads + 1 : t - K.dx;
+  n.lo[1] = lj > 0 ? t - kThreads - kT : t + K.dy;
+  n.up[1] = lj < kT - 1 ? t + kThreads + kT : t - K.dy;
+  n.lo[2] = t - kThreads;
+  n.up[2] = t + kThreads;
+  return n;
+}
+
 // y = A x on box vectors: multiply(FixedSparseMatrix, x, y) (SparseMatrix.h:298-322)
 // with each row's entries added in ascending column order.  A present entry
 // always couples two DOFs, so every x read stays inside the box.
-__global__ void k_box_spmv(int64_t ncell, int64_t sx, int64_t sy, const uint32_t* code,
-                           const double* coef, const double* x, double* y) {
+__global__ void k_box_spmv(int64_t nslot, Skew K, const uint32_t* code, const double* coef,
+                           const double* x, double* y) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
+  if (t >= nslot) return;
   const uint32_t c = code[t];
   double acc = 0.0;
   if (c & C_DOF) {
     const double* a = coef + kCoef * t;
-    const int64_t stride[3] = {sx, sy, 1};
+    const Nbr n = neighbours(K, t);
     const int lo = (c >> C_LORD) & 7, up = (c >> C_UORD) & 7;
     for (int q = 0; q < 3; ++q) {
       const int ax = perm_axis(lo, q);
-      if (c & (1u << (C_ALO + ax))) acc += a[1 + ax] * x[t - stride[ax]];
+      if (c & (1u << (C_ALO + ax))) acc += a[1 + ax] * x[n.lo[ax]];
     }
     if (c & C_DIAG) acc += a[0] * x[t];
     for (int q = 0; q < 3; ++q) {
       const int ax = perm_axis(up, q);
-      if (c & (1u << (C_AUP + ax))) acc += a[4 + ax] * x[t + stride[ax]];
+      if (c & (1u << (C_AUP + ax))) acc += a[4 + ax] * x[n.up[ax]];
     }
   }
   y[t] = acc;
@@ -300,8 +339,8 @@ __global__ void k_box_spmv(int64_t ncell, int64_t sx, int64_t sy, const uint32_t
 enum { K_FWD = 0, K_BWD = 1, K_FAC = 2 };
 
 struct SweepArgs {
-  int B[3];
-  int ntile_a, ntile_b, ntiles;
+  Skew K;
+  int ntiles;
   const uint32_t* order;   // ticket -> a | b << 16
   const Cell* cells;       // forward: lower terms; backward: upper terms
   const uint32_t* code;
@@ -408,7 +447,7 @@ __device__ inline double sel3(int ax, double v0, double v1, double v2) {
 
 __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_t code,
                                      double ix, double iy, double iprev, int64_t idx,
-                                     const int64_t stride[3]) {
+                                     int li, int lj) {
   const int perm = (code >> C_LORD) & 7;
   double L0 = 0, L1 = 0, L2 = 0;
   double d = f.adiag;
@@ -448,9 +487,10 @@ __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_
   c.inv = inv;
   S.cf[idx] = c;
   S.cb[idx].inv = inv;
-  if (code & (1u << (C_LOWER + 0))) S.cb[idx - stride[0]].l0 = L0;
-  if (code & (1u << (C_LOWER + 1))) S.cb[idx - stride[1]].l1 = L1;
-  if (code & (1u << (C_LOWER + 2))) S.cb[idx - stride[2]].l2 = L2;
+  // the lower neighbours' slots (skewed layout, forward lanes)
+  if (code & (1u << (C_LOWER + 0))) S.cb[li > 0 ? idx - kThreads - 1 : idx + S.K.dx].l0 = L0;
+  if (code & (1u << (C_LOWER + 1))) S.cb[lj > 0 ? idx - kThreads - kT : idx + S.K.dy].l1 = L1;
+  if (code & (1u << (C_LOWER + 2))) S.cb[idx - kThreads].l2 = L2;
   return inv;
 }
 
@@ -464,10 +504,9 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
   __shared__ double lz[2][kT][kT + 1];
   __shared__ int s_tile;
   const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
-  const int Bz = S.B[2];
-  const int nsteps = 2 * (kT - 1) + Bz;
+  const int Bz = S.K.Bz;
+  const int nsteps = S.K.nsteps;
   const int ubits = BWD ? C_UPPER : C_LOWER;
-  const int64_t stride[3] = {Bz, (int64_t)S.B[0] * Bz, 1};
   for (;;) {
     if (threadIdx.x == 0) {
       const int t = atomicAdd(S.ticket, 1);
@@ -478,30 +517,33 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     if (tile < 0) return;
     // logical tile (a, b) counted from the sweep's start corner
     const int ta = tile & 0xFFFF, tb = tile >> 16;
-    const int a = BWD ? S.ntile_a - 1 - ta : ta;
-    const int b = BWD ? S.ntile_b - 1 - tb : tb;
-    const bool has_a = BWD ? a + 1 < S.ntile_a : a > 0;
-    const bool has_b = BWD ? b + 1 < S.ntile_b : b > 0;
-    // physical pencil of this thread
-    const int px = a * kT + (BWD ? kT - 1 - li : li);
-    const int py = b * kT + (BWD ? kT - 1 - lj : lj);
-    const int64_t col = ((int64_t)py * S.B[0] + px) * Bz;
-    const int64_t dstep_x = BWD ? Bz : -Bz;  // toward the upstream x pencil
-    const int64_t dstep_y = BWD ? (int64_t)S.B[0] * Bz : -(int64_t)S.B[0] * Bz;
+    const int a = BWD ? S.K.ntile_a - 1 - ta : ta;
+    const int b = BWD ? S.K.ntile_b - 1 - tb : tb;
+    const bool has_a = BWD ? a + 1 < S.K.ntile_a : a > 0;
+    const bool has_b = BWD ? b + 1 < S.K.ntile_b : b > 0;
+    // physical lane of this thread, and the slot of its cell computed at
+    // step w (= kk + off): base + sgn * w (the backward sweep runs the
+    // skewed steps in reverse)
+    const int lip = BWD ? kT - 1 - li : li, ljp = BWD ? kT - 1 - lj : lj;
+    const int64_t base = skew_slot(S.K, a, b, lip, ljp, BWD ? nsteps - 1 : 0);
+    const int64_t sgn = BWD ? -kThreads : kThreads;
+    const int64_t dstep_x = BWD ? -S.K.dx : S.K.dx;  // to the upstream tile's face cell
+    const int64_t dstep_y = BWD ? -S.K.dy : S.K.dy;
     const bool bnd_x = li == 0 && has_a;     // reads the upstream x face
     const bool bnd_y = lj == 0 && has_b;
     const int off = li + lj;  // this pencil starts at step off
-    auto out_index = [&](int kk) { return col + (BWD ? (Bz - 1 - kk) : kk); };
+    auto out_index = [&](int kk) { return base + sgn * (kk + off); };
     auto in_index = [&](int kk) {
       const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
       return out_index(kc);
     };
     // the upstream tile's output beside this pencil's kk-th cell.  Every lane
     // loads (no branch, so no register of a load in flight is rewritten on
-    // another path); lanes off the upstream face, whose neighbour pencil may
-    // lie outside the box, read element 0 instead and discard it.
+    // another path); lanes off the upstream face, whose neighbour may lie
+    // outside the box, read their own slot instead (consecutive across the
+    // wave) and discard it.
     auto face_at = [&](bool on, int64_t dstep, int kk) -> int64_t {
-      return on ? in_index(kk) + dstep : 0;
+      return in_index(kk) + (on ? dstep : 0);
     };
     auto load_bnd = [&](bool on, int64_t dstep, int kk) -> double {
       return load_coherent(&S.out[face_at(on, dstep, kk)]);
@@ -559,7 +601,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           double z;
           if constexpr (KIND == K_FAC) {
             z = 0.0;
-            if (code & C_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx, stride);
+            if (code & C_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx, li, lj);
           } else {
             z = active ? sweep_cell<KIND>(ci, code, zx, zy, zprev) : 0.0;
           }
@@ -577,6 +619,17 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
 }
 
 inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+Skew skew_of(const SweepPlan& P) {
+  Skew K;
+  K.ntile_a = P.ntile_a;
+  K.ntile_b = P.ntile_b;
+  K.nsteps = P.nsteps;
+  K.Bz = P.B[2];
+  K.dx = P.kdx;
+  K.dy = P.kdy;
+  return K;
+}
 
 }  // namespace
 
@@ -604,7 +657,11 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.ntile_a = P.B[0] / kT;
   P.ntile_b = P.B[1] / kT;
   P.ntiles = P.ntile_a * P.ntile_b;
-  P.ncell = (int64_t)P.B[0] * P.B[1] * P.B[2];
+  P.nsteps = P.B[2] + 2 * (kT - 1);
+  // slots of the skewed tile layout (every vector and record of the plan)
+  P.ncell = (int64_t)P.ntiles * P.nsteps * kThreads;
+  P.kdx = -(int64_t)P.nsteps * kThreads + (kT - 1) * kThreads + (kT - 1);
+  P.kdy = -(int64_t)P.ntile_a * P.nsteps * kThreads + (kT - 1) * kThreads + (kT - 1) * kT;
   P.pdof.ensure(P.ncell);
   P.code.ensure(P.ncell);
   P.cf.ensure(P.ncell * sizeof(Cell) / sizeof(double));
@@ -630,10 +687,9 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.progress.ensure(8);
   BoxArgs A;
   A.g = g;
-  for (int d = 0; d < 3; ++d) {
-    A.o[d] = P.o[d];
-    A.B[d] = P.B[d];
-  }
+  for (int d = 0; d < 3; ++d) A.o[d] = P.o[d];
+  A.K = skew_of(P);
+  A.nslot = P.ncell;
   A.act = act;
   A.isdof = isdof;
   A.dofscan = dofscan;
@@ -654,9 +710,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
 
 static SweepArgs sweep_args(SweepPlan& P) {
   SweepArgs S = {};
-  for (int d = 0; d < 3; ++d) S.B[d] = P.B[d];
-  S.ntile_a = P.ntile_a;
-  S.ntile_b = P.ntile_b;
+  S.K = skew_of(P);
   S.ntiles = P.ntiles;
   S.order = P.order.p;
   S.code = P.code.p;
@@ -712,9 +766,8 @@ void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
 }
 
 void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y) {
-  const int64_t sx = P.B[2], sy = (int64_t)P.B[0] * P.B[2];
   hipLaunchKernelGGL(k_box_spmv, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, P.ncell,
-                     sx, sy, P.code.p, P.coef.p, x, y);
+                     skew_of(P), P.code.p, P.coef.p, x, y);
   WC_LAUNCH_CHECK();
 }
 
