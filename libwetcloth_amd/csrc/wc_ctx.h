@@ -138,6 +138,7 @@ struct PcgWork {
   bool levels_ready = false;
   int red_blocks = 0;
   double* host_scal = nullptr;  // pinned
+  int* host_flag = nullptr;     // pinned
   void release();
 };
 

@@ -411,6 +411,8 @@ void PcgWork::release() {
   lvl_list.release();
   if (host_scal) (void)hipHostFree(host_scal);
   host_scal = nullptr;
+  if (host_flag) (void)hipHostFree(host_flag);
+  host_flag = nullptr;
 }
 
 static size_t slot_count(const DevCsr& A) { return (size_t)A.nnz_slots; }
@@ -431,6 +433,7 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
   w.partial.ensure(w.red_blocks);
   w.scal.ensure(S_NSCAL);
   if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
+  if (!w.host_flag) WC_HIP(hipHostMalloc((void**)&w.host_flag, sizeof(int)));
   w.level_in = level;
   w.level_span = level_span;
   w.levels_ready = false;
@@ -610,9 +613,16 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p,
                        nb, w.scal.p);
     WC_LAUNCH_CHECK();
-    const double res = read_scal(st, w, S_RES);
+    // the residual and the sweeps' abort flag in one round trip
+    WC_HIP(hipMemcpyAsync(w.host_scal + S_RES, w.scal.p + S_RES, sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    WC_HIP(hipMemcpyAsync(w.host_flag, P.progress.p + 2, sizeof(int), hipMemcpyDeviceToHost,
+                          st));
+    WC_HIP(hipStreamSynchronize(st));
+    if (!w.timer.pending.empty()) w.timer.harvest();
+    const double res = w.host_scal[S_RES];
     *residual_out = res;
-    if (sweep_aborted(st, P))
+    if (*w.host_flag)
       throw HipError{hipErrorLaunchTimeOut,
                      "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__,
                      __LINE__};
@@ -669,6 +679,10 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
                   w.grid.rowlen, w.grid.col, A.val, n);
   if (w.sweep.valid) {
     sweep_factor(st, w.sweep, tau, mdr);
+    if (sweep_aborted(st, w.sweep))
+      throw HipError{hipErrorLaunchTimeOut,
+                     "pipelined MIC(0) factor: a tile wait exceeded its bound", __FILE__,
+                     __LINE__};
     w.factored = true;
     return pcg_loop_box(st, w, n, b, x, tolr, maxit, residual_out, iters_out);
   }

@@ -336,6 +336,24 @@ __global__ void k_box_spmv(int64_t nslot, Skew K, const uint32_t* code, const do
   y[t] = acc;
 }
 
+// The backward sweep's record of cell k: L(k + e_d, k) from the forward
+// record of each upper neighbour (present couplings only) and invdiag(k).
+__global__ void k_backward_records(int64_t nslot, Skew K, const uint32_t* code,
+                                   const Cell* cf, Cell* cb) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nslot) return;
+  const uint32_t c = code[t];
+  Cell r = {0, 0, 0, 0};
+  if (c & C_DOF) {
+    const Nbr n = neighbours(K, t);
+    if (c & (1u << (C_UPPER + 0))) r.l0 = cf[n.up[0]].l0;
+    if (c & (1u << (C_UPPER + 1))) r.l1 = cf[n.up[1]].l1;
+    if (c & (1u << (C_UPPER + 2))) r.l2 = cf[n.up[2]].l2;
+    r.inv = cf[t].inv;
+  }
+  cb[t] = r;
+}
+
 enum { K_FWD = 0, K_BWD = 1, K_FAC = 2 };
 
 struct SweepArgs {
@@ -351,7 +369,6 @@ struct SweepArgs {
   // factor pass
   const FacIn* fin;
   Cell* cf;
-  Cell* cb;
   double tau, mdr;
 };
 
@@ -439,15 +456,14 @@ __device__ inline double sweep_cell(const CellIn& ci, uint32_t code, double zx, 
 // L(me, k) = A(k, me) invdiag(k) and modifies this diagonal by -L^2 and by
 // -(tau L) missing, missing being column k's other scaled entries summed in
 // ascending row order.  Returns invdiag(me) and writes the scaled entries
-// into the sweep records: L(me, k_d) into this cell's forward record and
-// into the backward record of k_d.
+// into this cell's forward record (the backward records are gathered from
+// the forward ones after the pass, so every record has a single writer).
 __device__ inline double sel3(int ax, double v0, double v1, double v2) {
   return ax == 0 ? v0 : (ax == 1 ? v1 : v2);
 }
 
 __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_t code,
-                                     double ix, double iy, double iprev, int64_t idx,
-                                     int li, int lj) {
+                                     double ix, double iy, double iprev, int64_t idx) {
   const int perm = (code >> C_LORD) & 7;
   double L0 = 0, L1 = 0, L2 = 0;
   double d = f.adiag;
@@ -485,12 +501,7 @@ __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_
   c.l1 = L1;
   c.l2 = L2;
   c.inv = inv;
-  S.cf[idx] = c;
-  S.cb[idx].inv = inv;
-  // the lower neighbours' slots (skewed layout, forward lanes)
-  if (code & (1u << (C_LOWER + 0))) S.cb[li > 0 ? idx - kThreads - 1 : idx + S.K.dx].l0 = L0;
-  if (code & (1u << (C_LOWER + 1))) S.cb[lj > 0 ? idx - kThreads - kT : idx + S.K.dy].l1 = L1;
-  if (code & (1u << (C_LOWER + 2))) S.cb[idx - kThreads].l2 = L2;
+  S.cf[idx] = c;  // the backward records are gathered from these afterwards
   return inv;
 }
 
@@ -601,11 +612,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           double z;
           if constexpr (KIND == K_FAC) {
             z = 0.0;
-            if (code & C_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx, li, lj);
+            if (code & C_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx);
           } else {
             z = active ? sweep_cell<KIND>(ci, code, zx, zy, zprev) : 0.0;
           }
-          if (active) store_coherent(&S.out[idx], z);
+          // every step writes the lane's slot of that step (0 on the padding
+          // slots), so no slot keeps the sentinel after the pass
+          store_coherent(&S.out[idx], z);
           zprev = z;
           lz[w & 1][lj][li] = z;
         }
@@ -725,12 +738,15 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
   SweepArgs S = sweep_args(P);
   S.fin = reinterpret_cast<const FacIn*>(P.fin.p);
   S.cf = reinterpret_cast<Cell*>(P.cf.p);
-  S.cb = reinterpret_cast<Cell*>(P.cb.p);
   S.tau = tau;
   S.mdr = mdr;
   S.out = P.zf.p;
   S.ticket = tick + 3;
   hipLaunchKernelGGL(k_sweep<K_FAC>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  WC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_backward_records, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
+                     P.ncell, skew_of(P), P.code.p, reinterpret_cast<const Cell*>(P.cf.p),
+                     reinterpret_cast<Cell*>(P.cb.p));
   WC_LAUNCH_CHECK();
 }
 
