@@ -98,6 +98,7 @@ struct SweepPlan {
   int64_t kdx = 0, kdy = 0;  // slot offsets to the upstream x / y face cell
   DevBuf<int32_t> pdof;             // box cell -> DOF or -1
   DevBuf<uint32_t> order, code;     // ticket order; per-cell coupling bits
+  DevBuf<uint32_t> scode;           // per-cell sweep code (orders applied)
   DevBuf<double> cf, cb;            // 32-byte factor records (forward / backward)
   DevBuf<double> coef;              // 7 matrix entries per cell
   DevBuf<double> fin;               // 96-byte factor-pass inputs per cell

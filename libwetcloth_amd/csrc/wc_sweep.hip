@@ -51,6 +51,17 @@ constexpr uint32_t C_DIAG = 1u << 13;  // the row has a diagonal entry
 constexpr int C_ALO = 14;    // bits 14..16: A(me, -x/-y/-z) present
 constexpr int C_AUP = 17;    // bits 17..19: A(me, +x/+y/+z) present
 
+// Sweep code word per slot, with each sweep's neighbour order applied: term
+// q of a sweep is the q-th neighbour in ascending DOF order (the order the
+// column loops of the reference apply them), present or not.
+constexpr uint32_t S_DOF = 1u;
+constexpr int S_FPRES = 1;   // bits 1..3: forward term q present
+constexpr int S_FSRC = 4;    // bits 4..9: forward term q axis (0 x, 1 y, 2 z)
+constexpr int S_BPRES = 10;  // bits 10..12: backward term q present
+constexpr int S_BSRC = 13;   // bits 13..18: backward term q axis
+constexpr int S_FX = 19, S_FY = 20;  // a forward term comes from -x / -y
+constexpr int S_BX = 21, S_BY = 22;  // a backward term comes from +x / +y
+
 // the six orders of three axes, 2 bits per axis, 6 bits per order:
 // 0 xyz, 1 xzy, 2 yxz, 3 yzx, 4 zxy, 5 zyx
 constexpr uint64_t kPerms =
@@ -84,7 +95,8 @@ __device__ inline void store_flag(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// one box cell's factor entries for one sweep direction (two 16-byte loads)
+// one box cell's factor entries for one sweep direction, in that sweep's
+// term order (two 16-byte loads)
 struct __attribute__((aligned(16))) Cell {
   double l0, l1, l2, inv;
 };
@@ -96,14 +108,14 @@ constexpr int kCoef = 7;  // per box cell: A diagonal, A(-x,-y,-z), A(+x,+y,+z)
 // (PCGSolver.h:110-121), seen from this cell's row of L
 struct __attribute__((aligned(16))) FacIn {
   double adiag;      // A(me, me), 0 if the row has no diagonal entry
-  double lo[3];      // A(k_d, me) from the row of the lower neighbour k_d = me - e_d
-  double oth[3][2];  // row k_d's other entries above its diagonal, ascending column
-  uint32_t fcode;    // F_* bits below
-  uint32_t code;     // the cell's C_* bits
+  double lo[3];      // per forward term q: A(k, me) from the row of that lower neighbour k
+  double oth[3][2];  // per term: row k's other entries above its diagonal, ascending column
+  uint32_t fcode;    // F_* bits below (per term q)
+  uint32_t code;     // the cell's S_* bits
 };
 static_assert(sizeof(FacIn) == 96, "FacIn is six dwordx4");
-constexpr int F_KNULL = 0;  // bits 0..2: adiag(k_d) == 0 (column k_d is skipped)
-constexpr int F_NOTH = 3;   // bits 3..8: number of oth entries per d (2 bits each)
+constexpr int F_KNULL = 0;  // bits 0..2: adiag(k) == 0 for term q (column k is skipped)
+constexpr int F_NOTH = 3;   // bits 3..8: number of oth entries per term (2 bits each)
 
 // Skewed tile layout of the box.  Cell (px, py, kk) belongs to tile
 // (a, b) = (px / 16, py / 16) and lane l = (py % 16) * 16 + px % 16, and is
@@ -139,7 +151,8 @@ struct BoxArgs {
   const double* lv;
   const double* val;
   int32_t* pdof;
-  uint32_t* code;
+  uint32_t* code;   // per-axis coupling bits and orders (C_*)
+  uint32_t* scode;  // sweep code (S_*)
   Cell* cf;      // forward sweep (lower terms), zeroed here, filled by the factor pass
   Cell* cb;      // backward sweep (upper terms), likewise
   double* coef;  // kCoef entries of A per cell
@@ -263,11 +276,39 @@ __global__ void k_fill_box(BoxArgs A) {
       code |= (uint32_t)perm_of(ukey[0], ukey[1], ukey[2]) << C_UORD;
     }
   }
+  // the sweeps' and the factor's term order
+  uint32_t sc = 0;
+  FacIn fo;
+  fo.adiag = f.adiag;
+  uint32_t fco = 0;
+  for (int q = 0; q < 3; ++q) fo.lo[q] = fo.oth[q][0] = fo.oth[q][1] = 0;
+  if (code & C_DOF) {
+    sc |= S_DOF;
+    const int lord = (code >> C_LORD) & 7, uord = (code >> C_UORD) & 7;
+    for (int q = 0; q < 3; ++q) {
+      const int ax = perm_axis(lord, q);
+      sc |= (uint32_t)ax << (S_FSRC + 2 * q);
+      if (code & (1u << (C_LOWER + ax))) sc |= 1u << (S_FPRES + q);
+      fo.lo[q] = f.lo[ax];
+      fo.oth[q][0] = f.oth[ax][0];
+      fo.oth[q][1] = f.oth[ax][1];
+      if (fcode & (1u << (F_KNULL + ax))) fco |= 1u << (F_KNULL + q);
+      fco |= ((fcode >> (F_NOTH + 2 * ax)) & 3u) << (F_NOTH + 2 * q);
+      const int bx = perm_axis(uord, q);
+      sc |= (uint32_t)bx << (S_BSRC + 2 * q);
+      if (code & (1u << (C_UPPER + bx))) sc |= 1u << (S_BPRES + q);
+    }
+    if (code & (1u << (C_LOWER + 0))) sc |= 1u << S_FX;
+    if (code & (1u << (C_LOWER + 1))) sc |= 1u << S_FY;
+    if (code & (1u << (C_UPPER + 0))) sc |= 1u << S_BX;
+    if (code & (1u << (C_UPPER + 1))) sc |= 1u << S_BY;
+  }
   A.pdof[t] = me;
   A.code[t] = code;
-  f.fcode = fcode;
-  f.code = code;
-  A.fin[t] = f;
+  A.scode[t] = sc;
+  fo.fcode = fco;
+  fo.code = sc;
+  A.fin[t] = fo;
   const Cell zero = {0, 0, 0, 0};
   A.cf[t] = zero;
   A.cb[t] = zero;
@@ -336,19 +377,37 @@ __global__ void k_box_spmv(int64_t nslot, Skew K, const uint32_t* code, const do
   y[t] = acc;
 }
 
-// The backward sweep's record of cell k: L(k + e_d, k) from the forward
-// record of each upper neighbour (present couplings only) and invdiag(k).
-__global__ void k_backward_records(int64_t nslot, Skew K, const uint32_t* code,
+// The slot of the upper neighbour of slot t along axis d.
+__device__ inline int64_t upper_slot(const Skew& K, int64_t t, int d) {
+  const int lane = (int)(t % kThreads);
+  const int li = lane % kT, lj = lane / kT;
+  if (d == 0) return li < kT - 1 ? t + kThreads + 1 : t - K.dx;
+  if (d == 1) return lj < kT - 1 ? t + kThreads + kT : t - K.dy;
+  return t + kThreads;
+}
+
+// The backward sweep's record of cell k, in its term order: L(u, k) for the
+// upper neighbour u of each present term, read from u's forward record at
+// the position of u's term that points back to k, and invdiag(k).
+__global__ void k_backward_records(int64_t nslot, Skew K, const uint32_t* scode,
                                    const Cell* cf, Cell* cb) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nslot) return;
-  const uint32_t c = code[t];
+  const uint32_t c = scode[t];
+  double v[3] = {0, 0, 0};
   Cell r = {0, 0, 0, 0};
-  if (c & C_DOF) {
-    const Nbr n = neighbours(K, t);
-    if (c & (1u << (C_UPPER + 0))) r.l0 = cf[n.up[0]].l0;
-    if (c & (1u << (C_UPPER + 1))) r.l1 = cf[n.up[1]].l1;
-    if (c & (1u << (C_UPPER + 2))) r.l2 = cf[n.up[2]].l2;
+  if (c & S_DOF) {
+    for (int q = 0; q < 3; ++q) {
+      if (!(c & (1u << (S_BPRES + q)))) continue;
+      const int d = (c >> (S_BSRC + 2 * q)) & 3;
+      const int64_t u = upper_slot(K, t, d);
+      const uint32_t uc = scode[u];
+      const int qq = (int)((uc >> S_FSRC) & 3u) == d ? 0 : ((int)((uc >> (S_FSRC + 2)) & 3u) == d ? 1 : 2);
+      v[q] = reinterpret_cast<const double*>(&cf[u])[qq];
+    }
+    r.l0 = v[0];
+    r.l1 = v[1];
+    r.l2 = v[2];
     r.inv = cf[t].inv;
   }
   cb[t] = r;
@@ -436,15 +495,15 @@ __device__ inline double ready_value(const double* p, double v, int* abort_flag)
 template <int KIND>
 __device__ inline double sweep_cell(const CellIn& ci, uint32_t code, double zx, double zy,
                                     double zprev) {
-  const int ubits = KIND == K_BWD ? C_UPPER : C_LOWER;
-  const int perm = (code >> (KIND == K_BWD ? C_UORD : C_LORD)) & 7;
+  const int pres = KIND == K_BWD ? S_BPRES : S_FPRES;
+  const int src = KIND == K_BWD ? S_BSRC : S_FSRC;
   double acc = ci.rhs;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const int ax = perm_axis(perm, q);
-    const double L = ax == 0 ? ci.c.l0 : (ax == 1 ? ci.c.l1 : ci.c.l2);
-    const double zn = ax == 0 ? zx : (ax == 1 ? zy : zprev);
-    const double term = (code & (1u << (ubits + ax))) ? L * zn : 0.0;
+    const uint32_t a = (code >> (src + 2 * q)) & 3u;
+    const double zn = a == 0 ? zx : (a == 1 ? zy : zprev);
+    const double L = q == 0 ? ci.c.l0 : (q == 1 ? ci.c.l1 : ci.c.l2);
+    const double term = (code & (1u << (pres + q))) ? L * zn : 0.0;
     acc -= term;
   }
   return acc * ci.c.inv;
@@ -458,35 +517,28 @@ __device__ inline double sweep_cell(const CellIn& ci, uint32_t code, double zx, 
 // ascending row order.  Returns invdiag(me) and writes the scaled entries
 // into this cell's forward record (the backward records are gathered from
 // the forward ones after the pass, so every record has a single writer).
-__device__ inline double sel3(int ax, double v0, double v1, double v2) {
-  return ax == 0 ? v0 : (ax == 1 ? v1 : v2);
-}
-
 __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_t code,
                                      double ix, double iy, double iprev, int64_t idx) {
-  const int perm = (code >> C_LORD) & 7;
   double L0 = 0, L1 = 0, L2 = 0;
   double d = f.adiag;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const int ax = perm_axis(perm, q);
-    if (!(code & (1u << (C_LOWER + ax)))) continue;
-    const double ik = sel3(ax, ix, iy, iprev);
-    const double a = sel3(ax, f.lo[0], f.lo[1], f.lo[2]);
+    if (!(code & (1u << (S_FPRES + q)))) continue;
+    const uint32_t ax = (code >> (S_FSRC + 2 * q)) & 3u;
+    const double ik = ax == 0 ? ix : (ax == 1 ? iy : iprev);
+    const double a = f.lo[q];
     double m = a;  // column k skipped (adiag(k) == 0): its entries stay unscaled
-    if (!(f.fcode & (1u << (F_KNULL + ax)))) {
+    if (!(f.fcode & (1u << (F_KNULL + q)))) {
       m = a * ik;
-      const int noth = (f.fcode >> (F_NOTH + 2 * ax)) & 3;
-      const double o0 = sel3(ax, f.oth[0][0], f.oth[1][0], f.oth[2][0]);
-      const double o1 = sel3(ax, f.oth[0][1], f.oth[1][1], f.oth[2][1]);
+      const int noth = (f.fcode >> (F_NOTH + 2 * q)) & 3;
       double missing = 0;
-      if (noth > 0) missing += o0 * ik;
-      if (noth > 1) missing += o1 * ik;
+      if (noth > 0) missing += f.oth[q][0] * ik;
+      if (noth > 1) missing += f.oth[q][1] * ik;
       d -= m * m;
       d -= S.tau * m * missing;
     }
-    if (ax == 0) L0 = m;
-    else if (ax == 1) L1 = m;
+    if (q == 0) L0 = m;
+    else if (q == 1) L1 = m;
     else L2 = m;
   }
   double inv;
@@ -517,7 +569,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
   const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
   const int Bz = S.K.Bz;
   const int nsteps = S.K.nsteps;
-  const int ubits = BWD ? C_UPPER : C_LOWER;
+  const uint32_t xbit = 1u << (BWD ? S_BX : S_FX), ybit = 1u << (BWD ? S_BY : S_FY);
   for (;;) {
     if (threadIdx.x == 0) {
       const int t = atomicAdd(S.ticket, 1);
@@ -605,14 +657,14 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           double zy = lj > 0 ? ly : (bnd_y ? wy[s] : 0.0);
           // a face value prefetched before its tile wrote it is re-read
           // (only where the term is present)
-          if (bnd_x && (code & (1u << ubits)))
+          if (bnd_x && (code & xbit))
             zx = ready_value(&S.out[idx + dstep_x], zx, S.abort_flag);
-          if (bnd_y && (code & (2u << ubits)))
+          if (bnd_y && (code & ybit))
             zy = ready_value(&S.out[idx + dstep_y], zy, S.abort_flag);
           double z;
           if constexpr (KIND == K_FAC) {
             z = 0.0;
-            if (code & C_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx);
+            if (code & S_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx);
           } else {
             z = active ? sweep_cell<KIND>(ci, code, zx, zy, zprev) : 0.0;
           }
@@ -677,6 +729,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.kdy = -(int64_t)P.ntile_a * P.nsteps * kThreads + (kT - 1) * kThreads + (kT - 1) * kT;
   P.pdof.ensure(P.ncell);
   P.code.ensure(P.ncell);
+  P.scode.ensure(P.ncell);
   P.cf.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.cb.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.coef.ensure(P.ncell * kCoef);
@@ -711,6 +764,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   A.val = val;
   A.pdof = P.pdof.p;
   A.code = P.code.p;
+  A.scode = P.scode.p;
   A.cf = reinterpret_cast<Cell*>(P.cf.p);
   A.cb = reinterpret_cast<Cell*>(P.cb.p);
   A.coef = P.coef.p;
@@ -726,7 +780,7 @@ static SweepArgs sweep_args(SweepPlan& P) {
   S.K = skew_of(P);
   S.ntiles = P.ntiles;
   S.order = P.order.p;
-  S.code = P.code.p;
+  S.code = P.scode.p;
   S.abort_flag = P.progress.p + 2;
   return S;
 }
@@ -745,7 +799,7 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
   hipLaunchKernelGGL(k_sweep<K_FAC>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_backward_records, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
-                     P.ncell, skew_of(P), P.code.p, reinterpret_cast<const Cell*>(P.cf.p),
+                     P.ncell, skew_of(P), P.scode.p, reinterpret_cast<const Cell*>(P.cf.p),
                      reinterpret_cast<Cell*>(P.cb.p));
   WC_LAUNCH_CHECK();
 }
@@ -808,6 +862,7 @@ bool sweep_aborted(hipStream_t st, SweepPlan& P) {
 void SweepPlan::release() {
   pdof.release();
   code.release();
+  scode.release();
   cf.release();
   cb.release();
   coef.release();
