@@ -94,7 +94,8 @@ def roofline(stats: dict, word: int, workload: str) -> dict:
         return None
     dom = max(("sweep_fwd", "sweep_bwd"), key=lambda k: per.get(k, {}).get("avg_ms", -1))
     d = per[dom]
-    kname = {"sweep_fwd": "k_sweep<false>", "sweep_bwd": "k_sweep<true>"}[dom]
+    # the kernels' names as rocprofv3 reports them
+    kname = {"sweep_fwd": "k_sweep<0>", "sweep_bwd": "k_sweep<1>"}[dom]
     return {
         "bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(kname, workload),

@@ -159,25 +159,45 @@ struct BoxArgs {
   FacIn* fin;    // factor pass inputs
 };
 
-// Bounding box of the DOF nodes: a wave reduces its lanes' coordinates with
-// shuffles and one lane issues the six atomics for the wave.
-__global__ void k_dof_box(Grid g, const uint32_t* isdof, int* bb) {
-  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// Bounding box of the DOF nodes: each block strides over the nodes, reduces
+// its threads' bounds through shuffles and LDS, and issues one set of six
+// atomics (the grid is bounded, so a few thousand atomics in all).
+constexpr int kBoxBlocks = 2048;
+__global__ __launch_bounds__(256) void k_dof_box(Grid g, const uint32_t* isdof, int* bb) {
+  __shared__ int sh[6][4];
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
-  if (n < g.nnode && isdof[n]) {
+  for (int64_t n = blockIdx.x * 256ll + threadIdx.x; n < g.nnode; n += (int64_t)gridDim.x * 256) {
+    if (!isdof[n]) continue;
     int c[3];
     node_coords(g, n, c[0], c[1], c[2]);
-    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = c[d];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = min(lo[d], c[d]);
+      hi[d] = max(hi[d], c[d]);
+    }
   }
   for (int o = 32; o > 0; o >>= 1)
     for (int d = 0; d < 3; ++d) {
       lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
       hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
     }
-  if ((threadIdx.x & 63) == 0 && hi[0] >= 0) {
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
     for (int d = 0; d < 3; ++d) {
-      atomicMin(&bb[d], lo[d]);
-      atomicMax(&bb[3 + d], hi[d]);
+      sh[d][wv] = lo[d];
+      sh[3 + d][wv] = hi[d];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) {
+      int l = sh[d][0], h = sh[3 + d][0];
+      for (int w = 1; w < 4; ++w) {
+        l = min(l, sh[d][w]);
+        h = max(h, sh[3 + d][w]);
+      }
+      if (h >= 0) {
+        atomicMin(&bb[d], l);
+        atomicMax(&bb[3 + d], h);
+      }
     }
   }
 }
@@ -707,8 +727,8 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.flags.ensure(8);
   int init[6] = {1 << 30, 1 << 30, 1 << 30, -1, -1, -1};
   WC_HIP(hipMemcpyAsync(P.flags.p, init, sizeof(init), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_dof_box, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, g,
-                     isdof, P.flags.p);
+  hipLaunchKernelGGL(k_dof_box, dim3((unsigned)std::min<int64_t>(kBoxBlocks, blocks(g.nnode, 256))),
+                     dim3(256), 0, st, g, isdof, P.flags.p);
   WC_LAUNCH_CHECK();
   int bb[6];
   WC_HIP(hipMemcpyAsync(bb, P.flags.p, sizeof(bb), hipMemcpyDeviceToHost, st));
