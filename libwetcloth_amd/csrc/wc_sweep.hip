@@ -37,7 +37,7 @@ namespace {
 
 constexpr int kT = 16;           // tile edge (pencils)
 constexpr int kThreads = kT * kT;
-constexpr int kChunk = 8;        // cells prefetched ahead per pencil
+constexpr int kChunk = 4;        // cells prefetched ahead per pencil
 constexpr long long kSpinLimit = 1ll << 22;  // ~seconds; a correct run waits us
 constexpr unsigned long long kSentinel = 0xFFFFFFFFFFFFFFFFull;
 
@@ -461,25 +461,37 @@ struct CellIn {
 template <int KIND>
 struct PassIn {
   using T = CellIn;
-  static constexpr int chunk = 8;  // steps prefetched ahead
+  static constexpr int chunk = kChunk;  // steps prefetched ahead
 };
 template <>
 struct PassIn<K_FAC> {
   using T = FacIn;
-  static constexpr int chunk = 4;
+  static constexpr int chunk = 2;
 };
 
 // inputs of the kk-th cell of a pencil; kk is clamped into the pencil so the
 // loads are always issued to a valid address (steps outside it discard them)
+// base[i] addressed as a 32-bit byte offset from a uniform base (the plan
+// keeps every array below 4 GiB), so no 64-bit address arithmetic is needed
+template <class T>
+__device__ inline const T* at(const T* base, uint32_t i) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                    i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ inline T* at(T* base, uint32_t i) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T));
+}
+
 template <int KIND>
-__device__ inline typename PassIn<KIND>::T load_in(const SweepArgs& S, int64_t idx) {
+__device__ inline typename PassIn<KIND>::T load_in(const SweepArgs& S, uint32_t idx) {
   if constexpr (KIND == K_FAC) {
-    return S.fin[idx];
+    return *at(S.fin, idx);
   } else {
     CellIn v;
-    v.c = S.cells[idx];
-    v.rhs = S.in[idx];
-    v.code = S.code[idx];
+    v.c = *at(S.cells, idx);
+    v.rhs = *at(S.in, idx);
+    v.code = *at(S.code, idx);
     return v;
   }
 }
@@ -538,7 +550,7 @@ __device__ inline double sweep_cell(const CellIn& ci, uint32_t code, double zx, 
 // into this cell's forward record (the backward records are gathered from
 // the forward ones after the pass, so every record has a single writer).
 __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_t code,
-                                     double ix, double iy, double iprev, int64_t idx) {
+                                     double ix, double iy, double iprev, uint32_t idx) {
   double L0 = 0, L1 = 0, L2 = 0;
   double d = f.adiag;
 #pragma unroll
@@ -573,7 +585,7 @@ __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_
   c.l1 = L1;
   c.l2 = L2;
   c.inv = inv;
-  S.cf[idx] = c;  // the backward records are gathered from these afterwards
+  *at(S.cf, idx) = c;  // the backward records are gathered from these afterwards
   return inv;
 }
 
@@ -608,14 +620,14 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // step w (= kk + off): base + sgn * w (the backward sweep runs the
     // skewed steps in reverse)
     const int lip = BWD ? kT - 1 - li : li, ljp = BWD ? kT - 1 - lj : lj;
-    const int64_t base = skew_slot(S.K, a, b, lip, ljp, BWD ? nsteps - 1 : 0);
-    const int64_t sgn = BWD ? -kThreads : kThreads;
-    const int64_t dstep_x = BWD ? -S.K.dx : S.K.dx;  // to the upstream tile's face cell
-    const int64_t dstep_y = BWD ? -S.K.dy : S.K.dy;
+    const uint32_t base = (uint32_t)skew_slot(S.K, a, b, lip, ljp, BWD ? nsteps - 1 : 0);
+    const int sgn = BWD ? -kThreads : kThreads;
+    const int dstep_x = (int)(BWD ? -S.K.dx : S.K.dx);  // to the upstream tile's face cell
+    const int dstep_y = (int)(BWD ? -S.K.dy : S.K.dy);
     const bool bnd_x = li == 0 && has_a;     // reads the upstream x face
     const bool bnd_y = lj == 0 && has_b;
     const int off = li + lj;  // this pencil starts at step off
-    auto out_index = [&](int kk) { return base + sgn * (kk + off); };
+    auto out_index = [&](int kk) -> uint32_t { return base + (uint32_t)(sgn * (kk + off)); };
     auto in_index = [&](int kk) {
       const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
       return out_index(kc);
@@ -625,12 +637,14 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // another path); lanes off the upstream face, whose neighbour may lie
     // outside the box, read their own slot instead (consecutive across the
     // wave) and discard it.
-    auto face_at = [&](bool on, int64_t dstep, int kk) -> int64_t {
-      return in_index(kk) + (on ? dstep : 0);
+    auto face_at = [&](bool on, int dstep, int kk) -> uint32_t {
+      return in_index(kk) + (uint32_t)(on ? dstep : 0);
     };
-    auto load_bnd = [&](bool on, int64_t dstep, int kk) -> double {
-      return load_coherent(&S.out[face_at(on, dstep, kk)]);
+    auto load_bnd = [&](bool on, int dstep, int kk) -> double {
+      return load_coherent(at(S.out, face_at(on, dstep, kk)));
     };
+    // only the first wave holds lanes on the upstream y face (lj == 0)
+    const bool ywave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == 0;
     double zprev = 0.0;
     // Prefetch rings: slot s holds the inputs of step w0 + s.  The step loop
     // is unrolled by kCh and a slot is refilled only after its step has used
@@ -638,14 +652,16 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // on a load in flight is needed before its own step.  The gate values
     // are issued first: each is checked after the loads issued behind it.
     double gx = load_bnd(bnd_x, dstep_x, 2 * kCh - 1 - off);
-    double gy = load_bnd(bnd_y, dstep_y, 2 * kCh - 1 - off);
+    double gy = 0.0;
+    if (ywave) gy = load_bnd(bnd_y, dstep_y, 2 * kCh - 1 - off);
     In win[kCh];
     double wx[kCh], wy[kCh];
 #pragma unroll
     for (int s = 0; s < kCh; ++s) {
       win[s] = load_in<KIND>(S, in_index(s - off));
       wx[s] = load_bnd(bnd_x, dstep_x, s - off);
-      wy[s] = load_bnd(bnd_y, dstep_y, s - off);
+      wy[s] = 0.0;
+      if (ywave) wy[s] = load_bnd(bnd_y, dstep_y, s - off);
     }
     const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
     const int lyj = lj > 0 ? lj - 1 : 0;
@@ -656,11 +672,11 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // loaded one chunk earlier and only re-polled if it was not final then.
     for (int w0 = 0; w0 < nsteps; w0 += kCh) {
       if (bnd_x && __double_as_longlong(gx) == (long long)kSentinel)
-        wait_value(&S.out[face_at(true, dstep_x, w0 + 2 * kCh - 1 - off)], S.abort_flag);
+        wait_value(at(S.out, face_at(true, dstep_x, w0 + 2 * kCh - 1 - off)), S.abort_flag);
       if (bnd_y && __double_as_longlong(gy) == (long long)kSentinel)
-        wait_value(&S.out[face_at(true, dstep_y, w0 + 2 * kCh - 1 - off)], S.abort_flag);
+        wait_value(at(S.out, face_at(true, dstep_y, w0 + 2 * kCh - 1 - off)), S.abort_flag);
       gx = load_bnd(bnd_x, dstep_x, w0 + 3 * kCh - 1 - off);
-      gy = load_bnd(bnd_y, dstep_y, w0 + 3 * kCh - 1 - off);
+      if (ywave) gy = load_bnd(bnd_y, dstep_y, w0 + 3 * kCh - 1 - off);
 #pragma clang loop unroll(full)
       for (int s = 0; s < kCh; ++s) {
         const int w = w0 + s;
@@ -670,17 +686,21 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           const bool active = kk >= 0 && kk < Bz;
           const In ci = win[s];
           const uint32_t code = active ? code_of(ci) : 0u;
-          const int64_t idx = out_index(kk);
+          const uint32_t idx = out_index(kk);
           const double lx = lz[(w - 1) & 1][lj][lxi];
           const double ly = lz[(w - 1) & 1][lyj][li];
           double zx = li > 0 ? lx : (bnd_x ? wx[s] : 0.0);
           double zy = lj > 0 ? ly : (bnd_y ? wy[s] : 0.0);
           // a face value prefetched before its tile wrote it is re-read
           // (only where the term is present)
-          if (bnd_x && (code & xbit))
-            zx = ready_value(&S.out[idx + dstep_x], zx, S.abort_flag);
-          if (bnd_y && (code & ybit))
-            zy = ready_value(&S.out[idx + dstep_y], zy, S.abort_flag);
+          const bool sx = bnd_x && (code & xbit) &&
+                          __double_as_longlong(zx) == (long long)kSentinel;
+          const bool sy = bnd_y && (code & ybit) &&
+                          __double_as_longlong(zy) == (long long)kSentinel;
+          if (__builtin_expect(sx || sy, 0)) {
+            if (sx) zx = wait_value(at(S.out, idx + (uint32_t)dstep_x), S.abort_flag);
+            if (sy) zy = wait_value(at(S.out, idx + (uint32_t)dstep_y), S.abort_flag);
+          }
           double z;
           if constexpr (KIND == K_FAC) {
             z = 0.0;
@@ -690,13 +710,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           }
           // every step writes the lane's slot of that step (0 on the padding
           // slots), so no slot keeps the sentinel after the pass
-          store_coherent(&S.out[idx], z);
+          store_coherent(at(S.out, idx), z);
           zprev = z;
           lz[w & 1][lj][li] = z;
         }
         win[s] = load_in<KIND>(S, in_index(w + kCh - off));
         wx[s] = load_bnd(bnd_x, dstep_x, w + kCh - off);
-        wy[s] = load_bnd(bnd_y, dstep_y, w + kCh - off);
+        if (ywave) wy[s] = load_bnd(bnd_y, dstep_y, w + kCh - off);
       }
     }
     __syncthreads();
@@ -747,6 +767,9 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.ncell = (int64_t)P.ntiles * P.nsteps * kThreads;
   P.kdx = -(int64_t)P.nsteps * kThreads + (kT - 1) * kThreads + (kT - 1);
   P.kdy = -(int64_t)P.ntile_a * P.nsteps * kThreads + (kT - 1) * kThreads + (kT - 1) * kT;
+  // the sweep kernels address every array with 32-bit byte offsets; larger
+  // systems are solved with the level-scheduled kernels instead
+  if (P.ncell * (int64_t)sizeof(FacIn) >= (int64_t(1) << 32)) return;
   P.pdof.ensure(P.ncell);
   P.code.ensure(P.ncell);
   P.scode.ensure(P.ncell);
