@@ -37,7 +37,8 @@ namespace {
 
 constexpr int kT = 16;           // tile edge (pencils)
 constexpr int kThreads = kT * kT;
-constexpr int kChunk = 4;        // cells prefetched ahead per pencil
+constexpr int kChunk = 8;        // cell records prefetched ahead per pencil
+constexpr int kFace = 4;         // face values prefetched ahead (divides every chunk)
 constexpr long long kSpinLimit = 1ll << 22;  // ~seconds; a correct run waits us
 constexpr unsigned long long kSentinel = 0xFFFFFFFFFFFFFFFFull;
 
@@ -466,7 +467,7 @@ struct PassIn {
 template <>
 struct PassIn<K_FAC> {
   using T = FacIn;
-  static constexpr int chunk = 2;
+  static constexpr int chunk = 4;
 };
 
 // inputs of the kk-th cell of a pencil; kk is clamped into the pencil so the
@@ -646,40 +647,50 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // only the first wave holds lanes on the upstream y face (lj == 0)
     const bool ywave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == 0;
     double zprev = 0.0;
-    // Prefetch rings: slot s holds the inputs of step w0 + s.  The step loop
-    // is unrolled by kCh and a slot is refilled only after its step has used
-    // it, so every load writes the registers it is consumed from and no wait
-    // on a load in flight is needed before its own step.  The gate values
-    // are issued first: each is checked after the loads issued behind it.
-    double gx = load_bnd(bnd_x, dstep_x, 2 * kCh - 1 - off);
+    // Prefetch rings.  The static inputs of a cell (its records, rhs and
+    // code) are read kCh steps ahead: slot s holds those of step w0 + s.  The
+    // face values, which the upstream tiles are still producing, are read
+    // only kF steps ahead (ring slot s % kF), so a tile needs to trail its
+    // upstream tiles by fewer steps.  The step loop is unrolled by kCh and a
+    // slot is refilled only after its step has used it, so every load writes
+    // the registers it is consumed from and no wait on a load in flight is
+    // needed before its own step.  The gate values are issued first: each is
+    // checked after the loads issued behind it.
+    constexpr int kF = kFace < kCh ? kFace : kCh;
+    static_assert(kCh % kF == 0, "the face ring must divide the chunk");
+    double gx = load_bnd(bnd_x, dstep_x, 2 * kF - 1 - off);
     double gy = 0.0;
-    if (ywave) gy = load_bnd(bnd_y, dstep_y, 2 * kCh - 1 - off);
+    if (ywave) gy = load_bnd(bnd_y, dstep_y, 2 * kF - 1 - off);
     In win[kCh];
-    double wx[kCh], wy[kCh];
+    double wx[kF], wy[kF];
 #pragma unroll
-    for (int s = 0; s < kCh; ++s) {
-      win[s] = load_in<KIND>(S, in_index(s - off));
+    for (int s = 0; s < kF; ++s) {
       wx[s] = load_bnd(bnd_x, dstep_x, s - off);
       wy[s] = 0.0;
       if (ywave) wy[s] = load_bnd(bnd_y, dstep_y, s - off);
     }
+#pragma unroll
+    for (int s = 0; s < kCh; ++s) win[s] = load_in<KIND>(S, in_index(s - off));
     const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
     const int lyj = lj > 0 ? lj - 1 : 0;
-    // Chunk gate: before a chunk issues its refills (the inputs of the next
-    // chunk), the face values they read must be final, else every refill
-    // reads the sentinel and each step pays a round trip in ready_value.  The
-    // gate value of a chunk (the face value of the next chunk's last step) is
-    // loaded one chunk earlier and only re-polled if it was not final then.
     for (int w0 = 0; w0 < nsteps; w0 += kCh) {
-      if (bnd_x && __double_as_longlong(gx) == (long long)kSentinel)
-        wait_value(at(S.out, face_at(true, dstep_x, w0 + 2 * kCh - 1 - off)), S.abort_flag);
-      if (bnd_y && __double_as_longlong(gy) == (long long)kSentinel)
-        wait_value(at(S.out, face_at(true, dstep_y, w0 + 2 * kCh - 1 - off)), S.abort_flag);
-      gx = load_bnd(bnd_x, dstep_x, w0 + 3 * kCh - 1 - off);
-      if (ywave) gy = load_bnd(bnd_y, dstep_y, w0 + 3 * kCh - 1 - off);
 #pragma clang loop unroll(full)
       for (int s = 0; s < kCh; ++s) {
         const int w = w0 + s;
+        if (s % kF == 0) {
+          // Face gate: the face refills of the next kF steps (for steps
+          // w + kF .. w + 2kF - 1) must read final values, else each of them
+          // reads the sentinel and every step pays a round trip in
+          // ready_value.  The gate value (the face value of step
+          // w + 2kF - 1) was loaded kF steps ago and is only re-polled if it
+          // was not final then.
+          if (bnd_x && __double_as_longlong(gx) == (long long)kSentinel)
+            wait_value(at(S.out, face_at(true, dstep_x, w + 2 * kF - 1 - off)), S.abort_flag);
+          if (bnd_y && __double_as_longlong(gy) == (long long)kSentinel)
+            wait_value(at(S.out, face_at(true, dstep_y, w + 2 * kF - 1 - off)), S.abort_flag);
+          gx = load_bnd(bnd_x, dstep_x, w + 3 * kF - 1 - off);
+          if (ywave) gy = load_bnd(bnd_y, dstep_y, w + 3 * kF - 1 - off);
+        }
         if (w < nsteps) {
           __syncthreads();
           const int kk = w - off;
@@ -689,8 +700,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           const uint32_t idx = out_index(kk);
           const double lx = lz[(w - 1) & 1][lj][lxi];
           const double ly = lz[(w - 1) & 1][lyj][li];
-          double zx = li > 0 ? lx : (bnd_x ? wx[s] : 0.0);
-          double zy = lj > 0 ? ly : (bnd_y ? wy[s] : 0.0);
+          double zx = li > 0 ? lx : (bnd_x ? wx[s % kF] : 0.0);
+          double zy = lj > 0 ? ly : (bnd_y ? wy[s % kF] : 0.0);
           // a face value prefetched before its tile wrote it is re-read
           // (only where the term is present)
           const bool sx = bnd_x && (code & xbit) &&
@@ -715,8 +726,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           lz[w & 1][lj][li] = z;
         }
         win[s] = load_in<KIND>(S, in_index(w + kCh - off));
-        wx[s] = load_bnd(bnd_x, dstep_x, w + kCh - off);
-        if (ywave) wy[s] = load_bnd(bnd_y, dstep_y, w + kCh - off);
+        wx[s % kF] = load_bnd(bnd_x, dstep_x, w + kF - off);
+        if (ywave) wy[s % kF] = load_bnd(bnd_y, dstep_y, w + kF - off);
       }
     }
     __syncthreads();
