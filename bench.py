@@ -7,6 +7,10 @@ relative residual.  One step = one full substep (sort, P2G, grid update,
 assembly, pressure solve, pressure application, G2P + advection) with every
 input resident in HBM.
 
+The same workload runs twice from the same seeds: in the mixed-precision
+form (the reported value) and in the plain FP64 form (reported beside it,
+with the ratio and the positions' difference after the same steps).
+
 Prints ONE JSON line (see the keys below).  --gpus N > 1 exits with a message:
 this build runs on a single GPU.
 """
@@ -41,9 +45,10 @@ def parse():
 
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# algorithmic bytes per DOF of one launch (FP64 words): a sweep reads
-# invdiag, three factor entries and its input and writes its output; the
-# SpMV reads four coefficients and x and writes y
+# algorithmic words per DOF of one launch (8-byte words in FP64, 4-byte in
+# the mixed form's FP32 solve): a sweep reads invdiag, three factor entries
+# and its input and writes its output; the SpMV reads four coefficients and
+# x and writes y
 SWEEP_WORDS = 6
 SPMV_WORDS = 6
 
@@ -78,7 +83,8 @@ def measured_traffic(kernel: str, workload: str):
 
 def roofline(stats: dict, word: int, workload: str) -> dict:
     """HBM roofline of the dominant kernel (the slower MIC(0) sweep) from the
-    HIP-event times the library recorded on its own stream."""
+    HIP-event times the library recorded on its own stream; word is the
+    solve's storage word (8 FP64, 4 FP32)."""
     per = {}
     for name, words in (("sweep_fwd", SWEEP_WORDS), ("sweep_bwd", SWEEP_WORDS),
                         ("spmv", SPMV_WORDS)):
@@ -95,7 +101,8 @@ def roofline(stats: dict, word: int, workload: str) -> dict:
     dom = max(("sweep_fwd", "sweep_bwd"), key=lambda k: per.get(k, {}).get("avg_ms", -1))
     d = per[dom]
     # the kernels' names as rocprofv3 reports them
-    kname = {"sweep_fwd": "k_sweep<0>", "sweep_bwd": "k_sweep<1>"}[dom]
+    vt = "double" if word == 8 else "float"
+    kname = {"sweep_fwd": f"k_sweep<0, {vt}>", "sweep_bwd": f"k_sweep<1, {vt}>"}[dom]
     return {
         "bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(kname, workload),
@@ -132,6 +139,38 @@ def cpu_baseline(n_particles_gpu: int, steps: int) -> dict:
     }
 
 
+def run(W, precision, n, dt, a):
+    """Seed the workload in one precision, step it warmup + steps times and
+    time the last steps (the library's stream is synchronized by every step:
+    the solver reads its residual back each iteration)."""
+    scene = W.Scene((n, n, n), 1.0 / n, precision=precision)
+    npart = scene.seed_block((0.0, 0.0, 0.0), (0.5, 0.6, 1.0), seed=4, sigma=1.0)
+    iters, passes = [], []
+    for _ in range(a.warmup):
+        ok, it, res = scene.step(dt, a.tol, a.maxit)
+    phase_tot = dict.fromkeys(W.PHASES, 0.0)
+    scene.set_option("kernel_timing", 1)  # also clears the counters
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ok, it, res = scene.step(dt, a.tol, a.maxit)
+        if not ok:
+            raise RuntimeError(f"pressure solve did not converge ({it} iterations)")
+        iters.append(it)
+        passes.append(scene.refinement_passes())
+        for k, v in scene.phase_times().items():
+            phase_tot[k] += v
+    wall = time.perf_counter() - t0
+    return scene, {
+        "particles": npart,
+        "value": a.steps / wall,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "pcg_iterations": iters,
+        "refinement_passes": passes,
+        "phase_ms_per_step": {k: v / a.steps for k, v in phase_tot.items()},
+        "kernel_stats": scene.kernel_stats(),
+    }
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -141,30 +180,24 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    import numpy as np
     import libwetcloth_amd as W
 
     n = a.n
-    dx = 1.0 / n
     dt = 1.0 / 1920 if n >= 256 else (1.0 / 960 if n >= 128 else 1.0 / 480)
-    scene = W.Scene((n, n, n), dx, precision=W.WC_FP64)
-    npart = scene.seed_block((0.0, 0.0, 0.0), (0.5, 0.6, 1.0), seed=4, sigma=1.0)
-
-    iters = []
-    for _ in range(a.warmup):
-        ok, it, res = scene.step(dt, a.tol, a.maxit)
-        iters.append(it)
-    phase_tot = dict.fromkeys(W.PHASES, 0.0)
-    scene.set_option("kernel_timing", 1)  # also clears the counters
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        ok, it, res = scene.step(dt, a.tol, a.maxit)
-        iters.append(it)
-        for k, v in scene.phase_times().items():
-            phase_tot[k] += v
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    kstats = scene.kernel_stats()
-    sps = a.steps / wall
+    workload = (f"dam break {n}^3, block [0,0.5]x[0,0.6]x[0,1], 8 ppc, "
+                f"sigma 1 m/s, dt {dt:.6g}, MIC(0)-PCG tol {a.tol:g}")
+    mixed, m = run(W, W.WC_MIXED, n, dt, a)
+    fp64, f = run(W, W.WC_FP64, n, dt, a)
+    # both forms took warmup + steps substeps from the same seeds
+    xm, _, _ = mixed.particles()
+    xf, _, _ = fp64.particles()
+    pos_dx = float(np.abs(xm - xf).max() * n)
+    del xm, xf
+    mixed.close()
+    fp64.close()
+    npart = m["particles"]
+    sps = m["value"]
     out = {
         "metric": METRIC,
         "value": sps,
@@ -172,25 +205,28 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / a.steps,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {
-            "workload": (f"dam break {n}^3, block [0,0.5]x[0,0.6]x[0,1], 8 ppc, "
-                         f"sigma 1 m/s, dt {dt:.6g}, MIC(0)-PCG tol {a.tol:g}"),
-            "grid": n, "particles": npart, "precision": "fp64",
-        },
+        "config": {"workload": workload, "grid": n, "particles": npart,
+                   "precision": "mixed (FP32 particles/transfers/PCG vectors; FP64 grid, "
+                                "factor, inner products, refinement residual)"},
         "particle_updates_per_s": npart * sps,
-        "pcg_iterations": iters[-a.steps:],
-        "phase_ms_per_step": {k: v / a.steps for k, v in phase_tot.items()},
+        "pcg_iterations": m["pcg_iterations"],
+        "refinement_passes": m["refinement_passes"],
+        "phase_ms_per_step": m["phase_ms_per_step"],
+        "fp64": {k: f[k] for k in ("value", "ms_per_step", "pcg_iterations",
+                                   "phase_ms_per_step")},
+        "mixed_vs_fp64": sps / f["value"],
+        "mixed_vs_fp64_position_diff_dx": pos_dx,
     }
-    out["roofline"] = roofline(kstats, 8, out["config"]["workload"])
+    out["roofline"] = roofline(m["kernel_stats"], 4, workload)
+    out["fp64"]["roofline"] = roofline(f["kernel_stats"], 8, workload)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(npart, a.cpu_steps)
-    scene.close()
     print(json.dumps(out))
 
 

@@ -33,8 +33,14 @@ typedef struct wc_ctx wc_ctx;
 
 /* arithmetic of the substep */
 #define WC_FP64 0  /* the reference's precision (scalar = double, MathDefs.h:24) */
-#define WC_MIXED 1 /* FP32 state/transfers/PCG vectors, FP64 reductions +
-                      * refinement (not implemented yet: wc_create rejects it) */
+#define WC_MIXED 1 /* no reference counterpart: FP32 particle state and
+                      * transfers (P2G sums flushed into the FP64 grid), FP64
+                      * grid, assembly and MIC(0) factor; the pressure PCG runs
+                      * on FP32 vectors and rounded factor/matrix copies with
+                      * FP64 inner products and alpha/beta, inside FP64
+                      * iterative refinement on r = b - A p, and converges on
+                      * the FP64 residual.  Systems too large for the pipelined
+                      * sweeps are solved in FP64. */
 
 /* Scene description: the LiquidInfo / bucket fields the liquid path reads
  * (App/TwoDSceneXMLParser.cpp:1849-1904, TwoDScene::setBucketInfo
@@ -149,8 +155,18 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
  *   bitwise the same result.
  *   "kernel_timing" (default 0): bracket the solver's sweep and SpMV launches
  *   with HIP events on the context's stream; setting it (to 0 or 1) clears
- *   the counters read by wc_kernel_stats. */
+ *   the counters read by wc_kernel_stats.
+ *   "mixed_digits" (default 6, 1..15): WC_MIXED contexts; each refinement
+ *   pass iterates until its FP32 residual has dropped by 10^-digits (or
+ *   meets the solve's tolerance).
+ *   "mixed_refinements" (default 2): WC_MIXED contexts; refinement passes
+ *   allowed after the first (the solve reports not converged if the FP64
+ *   residual still misses the tolerance after them). */
 wc_status wc_set_option(wc_ctx* ctx, const char* name, int64_t value);
+
+/* Refinement passes of the context's last mixed-precision solve (0 for a
+ * WC_FP64 context).  No reference counterpart. */
+wc_status wc_refinement_passes(const wc_ctx* ctx, int32_t* passes);
 
 /* Accumulated device time of one solver kernel class since "kernel_timing"
  * was last set: total ms, number of launches, and DOFs processed (the sum

@@ -97,6 +97,7 @@ def lib():
         L.wc_phase_times.argtypes = [vp, _dp]
         L.wc_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
         L.wc_kernel_stats.argtypes = [vp, C.c_int32, _dp, _i64p, _i64p]
+        L.wc_refinement_passes.argtypes = [vp, _i32p]
         L.wc_apply_preconditioner.argtypes = [vp, _dp, _dp]
         L.wc_system_multiply.argtypes = [vp, _dp, _dp]
         L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
@@ -292,6 +293,12 @@ class Scene:
         y = np.zeros_like(x)
         _check(lib().wc_system_multiply(self.h, _ptr(x), _ptr(y)), self.h)
         return y
+
+    def refinement_passes(self) -> int:
+        """Refinement passes of the last mixed-precision solve (0 in FP64)."""
+        p = C.c_int32()
+        _check(lib().wc_refinement_passes(self.h, C.byref(p)), self.h)
+        return int(p.value)
 
     def kernel_stats(self) -> dict:
         """Per solver kernel class since set_option("kernel_timing", ...):

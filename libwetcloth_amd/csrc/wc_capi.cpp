@@ -86,6 +86,12 @@ void release_all(wc_ctx* c) {
   c->x2.release();
   c->v2.release();
   c->C2.release();
+  c->fx.release();
+  c->fv.release();
+  c->fC.release();
+  c->fx2.release();
+  c->fv2.release();
+  c->fC2.release();
   c->pid.release();
   c->pid2.release();
   c->pkey.release();
@@ -168,14 +174,15 @@ wc_status wc_create(const wc_desc* desc, wc_ctx** out) {
   *out = nullptr;
   if (desc->ni <= 0 || desc->nj <= 0 || desc->nk <= 0 || !(desc->dx > 0))
     return fail(nullptr, WC_ERR_ARG, "grid size and dx must be positive");
-  if (desc->precision == WC_MIXED)
-    return fail(nullptr, WC_ERR_ARG, "precision WC_MIXED is not implemented yet");
-  if (desc->precision != WC_FP64) return fail(nullptr, WC_ERR_ARG, "unknown precision");
+  if (desc->precision != WC_FP64 && desc->precision != WC_MIXED)
+    return fail(nullptr, WC_ERR_ARG, "unknown precision");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(nullptr, WC_ERR_HIP, "no HIP device");
   wc_ctx* c = new wc_ctx();
   c->desc = *desc;
+  c->mixed = desc->precision == WC_MIXED;
+  c->pcg.mixed = c->mixed;
   c->g = wc::make_grid(desc->ni, desc->nj, desc->nk, desc->dx);
   try {
     WC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -527,6 +534,17 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
     c->pcg.use_sweeps = value != 0;
     return WC_OK;
   }
+  if (n == "mixed_digits") {
+    if (value < 1 || value > 15) return fail(c, WC_ERR_ARG, "mixed_digits must be 1..15");
+    c->pcg.mixed_digits = (int)value;
+    return WC_OK;
+  }
+  if (n == "mixed_refinements") {
+    if (value < 0 || value > 100)
+      return fail(c, WC_ERR_ARG, "mixed_refinements must be 0..100");
+    c->pcg.mixed_refinements = (int)value;
+    return WC_OK;
+  }
   if (n == "kernel_timing") {
     WC_TRY(c, {
       WC_HIP(hipStreamSynchronize(c->stream));
@@ -536,6 +554,12 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
     return WC_OK;
   }
   return fail(c, WC_ERR_ARG, "unknown option " + n);
+}
+
+wc_status wc_refinement_passes(const wc_ctx* c, int32_t* passes) {
+  if (!c || !passes) return WC_ERR_ARG;
+  *passes = c->mixed ? c->pcg.passes : 0;
+  return WC_OK;
 }
 
 wc_status wc_kernel_stats(const wc_ctx* c, int32_t kernel, double* ms, int64_t* launches,

@@ -104,6 +104,10 @@ struct SweepPlan {
   DevBuf<double> fin;               // 96-byte factor-pass inputs per cell
   DevBuf<double> zf, zb;            // sweep outputs (zb = preconditioned vector)
   DevBuf<double> rb, sb, xb;        // PCG vectors r, s, x in box layout
+  // mixed-precision solve: FP32 copies of the records and entries, FP32
+  // sweep outputs and PCG vectors, and b in box layout (FP64)
+  DevBuf<float> cf32, cb32, coef32, zf32, zb32, rb32, sb32;
+  DevBuf<double> bb;
   DevBuf<int> progress, flags;
   void release();
 };
@@ -125,6 +129,11 @@ struct PcgWork {
   KTimer timer;
   bool factored = false;  // a factor of the last system exists (either path)
   bool use_sweeps = true;
+  // mixed-precision solve (grid systems with the pipelined sweeps)
+  bool mixed = false;
+  int mixed_digits = 6;       // residual reduction per refinement pass, 10^-digits
+  int mixed_refinements = 2;  // refinement passes after the first
+  int passes = 0;             // passes of the last mixed solve
   SweepPlan sweep;
   DevBuf<double> invdiag, adiag, lv, r, z, s, x, partial, scal;
   DevBuf<uint32_t> tpos;
@@ -157,9 +166,13 @@ struct wc_ctx {
   std::string err;
 
   // particles, SoA: x[d*cap+p], v[d*cap+p], C[(r*3+c)*cap+p]; pid = the
-  // caller's particle index (the device keeps particles sorted by bucket)
+  // caller's particle index (the device keeps particles sorted by bucket).
+  // FP64 planes for WC_FP64, FP32 planes (fx, ...) for WC_MIXED; the other
+  // set stays unallocated.
   int64_t np = 0, pcap = 0;
+  bool mixed = false;
   wc::DevBuf<double> x, v, C, x2, v2, C2;
+  wc::DevBuf<float> fx, fv, fC, fx2, fv2, fC2;
   wc::DevBuf<uint32_t> pid, pid2, pkey;
   wc::DevBuf<uint32_t> bcount, bstart, bcursor;  // per bucket (+1)
   bool sorted = false;  // bstart describes the current particle arrays
@@ -226,6 +239,15 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr);
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
                  int64_t ndof);
 void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y);
+// mixed-precision solve: FP32 records/entries from the FP64 factor and A,
+// FP32 sweeps (result in P.zb32), FP32 SpMV with FP64 row sums, and the FP64
+// refinement residual r = b - A x rounded to FP32 with |r|_inf partials
+void sweep_narrow(hipStream_t st, SweepPlan& P);
+void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
+                   int64_t ndof);
+void box_spmv32(hipStream_t st, const SweepPlan& P, const float* x, float* y);
+void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const double* x,
+                  float* r, double* partial, int nblocks);
 void box_gather(hipStream_t st, const SweepPlan& P, const double* dof, double* box);
 void box_scatter(hipStream_t st, const SweepPlan& P, const double* box, double* dof);
 bool sweep_aborted(hipStream_t st, SweepPlan& P);

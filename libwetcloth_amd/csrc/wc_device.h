@@ -196,6 +196,59 @@ __host__ __device__ inline double face_off(int d, int a) {
   return (d == a) ? 0.0 : 0.5;
 }
 
+// The transfer helpers above in a chosen arithmetic type R (float for the
+// mixed-precision form).  With R = double they perform exactly the
+// operations of the double versions.
+template <class R>
+__host__ __device__ inline R node_pos_r(const Grid& g, int i, R off) {
+  return ((R)(i - kGhost) + off) * (R)g.dx;
+}
+template <class R>
+__host__ __device__ inline R quad_kernel_r(R x) {
+  const R ax = x < (R)0 ? -x : x;
+  if (ax < (R)0.5) return (R)0.75 - ax * ax;
+  if (ax < (R)1.5) return (R)0.5 * ((R)1.5 - ax) * ((R)1.5 - ax);
+  return (R)0;
+}
+template <class R>
+__host__ __device__ inline int stencil_base_r(const Grid& g, int axis, R x, R off) {
+  const R t = x / (R)g.dx + ((R)kGhost - off);
+  const R fl = floor(t);
+  const R fr = t - fl;
+  int b = (int)fl + (fr > (R)0.5 ? 0 : -1);
+  if (b < 0) b = 0;
+  if (b > g.nx[axis] - 3) b = g.nx[axis] - 3;
+  return b;
+}
+template <class R>
+__host__ __device__ inline void stencil_weights_r(const Grid& g, R x, int base, R off,
+                                                  R w[3]) {
+  for (int a = 0; a < 3; ++a)
+    w[a] = quad_kernel_r<R>((x - node_pos_r<R>(g, base + a, off)) / (R)g.dx);
+}
+template <class R>
+__host__ __device__ inline R face_off_r(int d, int a) {
+  return (d == a) ? (R)0 : (R)0.5;
+}
+template <class R>
+__host__ __device__ inline R container_phi_r(const Grid& g, R p0, R p1, R p2) {
+  const R p[3] = {p0, p1, p2};
+  R dd[3], da[3];
+  for (int d = 0; d < 3; ++d) {
+    const R lo = (R)0, hi = (R)g.n[d] * (R)g.dx;
+    const R c = (R)0.5 * (lo + hi);
+    const R e = (R)0.5 * (hi - lo);
+    const R q = p[d] - c;
+    dd[d] = (q < (R)0 ? -q : q) - e;
+    da[d] = dd[d] > (R)0 ? dd[d] : (R)0;
+  }
+  R m = dd[0] > dd[1] ? dd[0] : dd[1];
+  m = m > dd[2] ? m : dd[2];
+  if (m > (R)0) m = (R)0;
+  const R phi = m + sqrt(da[0] * da[0] + da[1] * da[1] + da[2] * da[2]);
+  return -phi;
+}
+
 // Counter-based RNG (splitmix64 finaliser chained over the counter words).
 __host__ __device__ inline uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;

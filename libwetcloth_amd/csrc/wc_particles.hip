@@ -23,39 +23,34 @@ constexpr int kTile = 8;  // LDS nodes per axis: bucket (4) + halo 2 below, 2 ab
 constexpr int kTileNodes = kTile * kTile * kTile;
 constexpr unsigned long long kPhiUntouched = 0xFFFFFFFFFFFFFFFFull;
 
-struct PView {
-  int64_t n, cap;
-  const double* x;
-  const double* v;
-  const double* C;
-};
-
 __device__ inline int clampi(int a, int lo, int hi) {
   return a < lo ? lo : (a > hi ? hi : a);
 }
 
-__device__ inline int particle_bucket_axis(const Grid& g, int axis, double x) {
-  const int c = (int)floor(x / g.dx) + kGhost;
+template <class R>
+__device__ inline int particle_bucket_axis(const Grid& g, int axis, R x) {
+  const int c = (int)floor(x / (R)g.dx) + kGhost;
   return clampi(c, 0, g.nx[axis] - 1) >> 2;
 }
 
-__global__ void k_sort_keys(Grid g, const double* x, int64_t n, int64_t cap,
+template <class R>
+__global__ void k_sort_keys(Grid g, const R* x, int64_t n, int64_t cap,
                             uint32_t* key, uint32_t* count) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int bi = particle_bucket_axis(g, 0, x[p]);
-  const int bj = particle_bucket_axis(g, 1, x[cap + p]);
-  const int bk = particle_bucket_axis(g, 2, x[2 * cap + p]);
+  const int bi = particle_bucket_axis<R>(g, 0, x[p]);
+  const int bj = particle_bucket_axis<R>(g, 1, x[cap + p]);
+  const int bk = particle_bucket_axis<R>(g, 2, x[2 * cap + p]);
   const uint32_t b = (uint32_t)bucket_index(g, bi, bj, bk);
   key[p] = b;
   atomicAdd(&count[b], 1u);
 }
 
+template <class R>
 __global__ void k_sort_scatter(int64_t n, int64_t cap, const uint32_t* key,
                                const uint32_t* start, uint32_t* cursor,
-                               const double* x, const double* v,
-                               const double* C, const uint32_t* pid, double* x2,
-                               double* v2, double* C2, uint32_t* pid2) {
+                               const R* x, const R* v, const R* C, const uint32_t* pid,
+                               R* x2, R* v2, R* C2, uint32_t* pid2) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint32_t b = key[p];
@@ -68,11 +63,12 @@ __global__ void k_sort_scatter(int64_t n, int64_t cap, const uint32_t* key,
   pid2[q] = pid[p];
 }
 
+template <class R>
 struct P2GArgs {
   Grid g;
-  const double* x;
-  const double* v;
-  const double* C;
+  const R* x;
+  const R* v;
+  const R* C;
   int64_t cap;
   const uint32_t* bstart;
   double* mass[3];
@@ -80,7 +76,7 @@ struct P2GArgs {
   double* vol[3];
   unsigned long long* phikey;
   uint8_t* touched;
-  double fm, fvol, r0;
+  R fm, fvol, r0;
 };
 
 // grid offsets of the five stencils resampleNodes builds (TwoDScene.cpp:
@@ -90,7 +86,11 @@ __device__ inline double grid5_off(int gi, int a) {
   return gi == 3 ? 0.0 : 0.5;
 }
 
-__global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs A) {
+// R is the particle arithmetic (double, or float in the mixed-precision
+// form); each particle's contributions are summed in FP64 (LDS, then the
+// FP64 grid).
+template <class R>
+__global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   const int64_t b = blockIdx.x;
   const uint32_t pbeg = A.bstart[b], pend = A.bstart[b + 1];
   if (pbeg == pend) return;
@@ -100,7 +100,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs A) {
   const int bk = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
   const int o0 = bi * 4 - 2, o1 = bj * 4 - 2, o2 = bk * 4 - 2;
 
-  __shared__ double sm[3][3][kTileNodes];  // [axis][mass, momentum, volume]
+  // [axis][mass, momentum, volume], summed in FP64 for either R
+  __shared__ double sm[3][3][kTileNodes];
   __shared__ unsigned long long sphi[kTileNodes];
   __shared__ int stouch[27];
   for (int i = threadIdx.x; i < kTileNodes; i += kP2GThreads) {
@@ -115,59 +116,59 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs A) {
   __syncthreads();
 
   for (uint32_t p = pbeg + threadIdx.x; p < pend; p += kP2GThreads) {
-    const double px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
+    const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
     // u, v, w faces: TwoDScene.cpp:6700-6713
     for (int d = 0; d < 3; ++d) {
       int base[3];
-      double w[3][3];
+      R w[3][3];
       for (int a = 0; a < 3; ++a) {
-        base[a] = stencil_base(g, a, px[a], face_off(d, a));
-        stencil_weights(g, px[a], base[a], face_off(d, a), w[a]);
+        base[a] = stencil_base_r<R>(g, a, px[a], face_off_r<R>(d, a));
+        stencil_weights_r<R>(g, px[a], base[a], face_off_r<R>(d, a), w[a]);
       }
-      const double vd = A.v[d * A.cap + p];
-      const double b0 = A.C[(d * 3 + 0) * A.cap + p];
-      const double b1 = A.C[(d * 3 + 1) * A.cap + p];
-      const double b2 = A.C[(d * 3 + 2) * A.cap + p];
+      const R vd = A.v[d * A.cap + p];
+      const R b0 = A.C[(d * 3 + 0) * A.cap + p];
+      const R b1 = A.C[(d * 3 + 1) * A.cap + p];
+      const R b2 = A.C[(d * 3 + 2) * A.cap + p];
       for (int c2 = 0; c2 < 3; ++c2)
         for (int c1 = 0; c1 < 3; ++c1)
           for (int c0 = 0; c0 < 3; ++c0) {
             const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
-            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
-            const double e0 = node_pos(g, i, face_off(d, 0)) - px[0];
-            const double e1 = node_pos(g, j, face_off(d, 1)) - px[1];
-            const double e2 = node_pos(g, k, face_off(d, 2)) - px[2];
-            const double vel = vd + (b0 * e0 + b1 * e1 + b2 * e2);
-            const double dm = A.fm * ww;
-            const double dp = vel * A.fm * ww;
-            const double dv = A.fvol * ww;
+            const R ww = w[0][c0] * w[1][c1] * w[2][c2];
+            const R e0 = node_pos_r<R>(g, i, face_off_r<R>(d, 0)) - px[0];
+            const R e1 = node_pos_r<R>(g, j, face_off_r<R>(d, 1)) - px[1];
+            const R e2 = node_pos_r<R>(g, k, face_off_r<R>(d, 2)) - px[2];
+            const R vel = vd + (b0 * e0 + b1 * e1 + b2 * e2);
+            const R dm = A.fm * ww;
+            const R dp = vel * A.fm * ww;
+            const R dv = A.fvol * ww;
             const int li = i - o0, lj = j - o1, lk = k - o2;
             if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
                 lk < kTile) {
               const int l = (lk * kTile + lj) * kTile + li;
-              atomicAdd(&sm[d][0][l], dm);
-              atomicAdd(&sm[d][1][l], dp);
-              atomicAdd(&sm[d][2][l], dv);
+              atomicAdd(&sm[d][0][l], (double)dm);
+              atomicAdd(&sm[d][1][l], (double)dp);
+              atomicAdd(&sm[d][2][l], (double)dv);
             } else {
               const int64_t n = node_index(g, i, j, k);
-              unsafeAtomicAdd(&A.mass[d][n], dm);
-              unsafeAtomicAdd(&A.mom[d][n], dp);
-              unsafeAtomicAdd(&A.vol[d][n], dv);
+              unsafeAtomicAdd(&A.mass[d][n], (double)dm);
+              unsafeAtomicAdd(&A.mom[d][n], (double)dp);
+              unsafeAtomicAdd(&A.vol[d][n], (double)dv);
             }
           }
     }
     // liquid phi at cell centres: TwoDScene.cpp:3307-3332
     {
       int base[3];
-      for (int a = 0; a < 3; ++a) base[a] = stencil_base(g, a, px[a], 0.5);
+      for (int a = 0; a < 3; ++a) base[a] = stencil_base_r<R>(g, a, px[a], (R)0.5);
       for (int c2 = 0; c2 < 3; ++c2)
         for (int c1 = 0; c1 < 3; ++c1)
           for (int c0 = 0; c0 < 3; ++c0) {
             const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
-            const double d0 = px[0] - node_pos(g, i, 0.5);
-            const double d1 = px[1] - node_pos(g, j, 0.5);
-            const double d2 = px[2] - node_pos(g, k, 0.5);
-            const double phi = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - A.r0;
-            const unsigned long long key = phi_key(phi);
+            const R d0 = px[0] - node_pos_r<R>(g, i, (R)0.5);
+            const R d1 = px[1] - node_pos_r<R>(g, j, (R)0.5);
+            const R d2 = px[2] - node_pos_r<R>(g, k, (R)0.5);
+            const R phi = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - A.r0;
+            const unsigned long long key = phi_key((double)phi);
             const int li = i - o0, lj = j - o1, lk = k - o2;
             if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
                 lk < kTile)
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs A) {
     for (int gi = 0; gi < 5; ++gi) {
       int blo[3], bhi[3];
       for (int a = 0; a < 3; ++a) {
-        const int base = stencil_base(g, a, px[a], grid5_off(gi, a));
+        const int base = stencil_base_r<R>(g, a, px[a], (R)grid5_off(gi, a));
         blo[a] = base >> 2;
         bhi[a] = (base + 2) >> 2;
       }
@@ -224,33 +225,35 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs A) {
   }
 }
 
+template <class R>
 struct G2PArgs {
   Grid g;
-  double* x;
-  double* v;
-  double* C;
+  R* x;
+  R* v;
+  R* C;
   int64_t n, cap;
   const double* u[3];
   const uint8_t* act;
-  double dt, flip, invD;
+  R dt, flip, invD;
   int solid_projection;
 };
 
 // mapNodeParticlesAPIC fluid branch (TwoDScene.cpp:6958-7019), advectScene
 // (SceneStepper.cpp:30-31) and solidProjection (TwoDScene.cpp:1782-1821).
-__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs A) {
+template <class R>
+__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= A.n) return;
   const Grid& g = A.g;
-  const double px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
-  double fv[3] = {0, 0, 0};
-  double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
+  R fv[3] = {0, 0, 0};
+  R B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int d = 0; d < 3; ++d) {
     int base[3];
-    double w[3][3];
+    R w[3][3];
     for (int a = 0; a < 3; ++a) {
-      base[a] = stencil_base(g, a, px[a], face_off(d, a));
-      stencil_weights(g, px[a], base[a], face_off(d, a), w[a]);
+      base[a] = stencil_base_r<R>(g, a, px[a], face_off_r<R>(d, a));
+      stencil_weights_r<R>(g, px[a], base[a], face_off_r<R>(d, a), w[a]);
     }
     const double* ud = A.u[d];
     for (int c2 = 0; c2 < 3; ++c2)
@@ -258,55 +261,54 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs A) {
         for (int c0 = 0; c0 < 3; ++c0) {
           const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
           if (!node_active(g, A.act, i, j, k)) continue;
-          const double ww = w[0][c0] * w[1][c1] * w[2][c2];
-          const double fnv = ud[node_index(g, i, j, k)];
+          const R ww = w[0][c0] * w[1][c1] * w[2][c2];
+          const R fnv = (R)ud[node_index(g, i, j, k)];
           fv[d] += fnv * ww;
-          const double e0 = node_pos(g, i, face_off(d, 0)) - px[0];
-          const double e1 = node_pos(g, j, face_off(d, 1)) - px[1];
-          const double e2 = node_pos(g, k, face_off(d, 2)) - px[2];
+          const R e0 = node_pos_r<R>(g, i, face_off_r<R>(d, 0)) - px[0];
+          const R e1 = node_pos_r<R>(g, j, face_off_r<R>(d, 1)) - px[1];
+          const R e2 = node_pos_r<R>(g, k, face_off_r<R>(d, 2)) - px[2];
           B[d * 3 + 0] += fnv * ww * e0 * A.invD;
           B[d * 3 + 1] += fnv * ww * e1 * A.invD;
           B[d * 3 + 2] += fnv * ww * e2 * A.invD;
         }
   }
   for (int t = 0; t < 9; ++t) A.C[t * A.cap + p] = B[t] * A.flip;
-  double nx[3];
+  R nx[3];
   for (int d = 0; d < 3; ++d) {
     A.v[d * A.cap + p] = fv[d];
     nx[d] = px[d] + fv[d] * A.dt;
   }
   if (A.solid_projection) {
     int base[3];
-    double w[3][3];
+    R w[3][3];
     for (int a = 0; a < 3; ++a) {
-      base[a] = stencil_base(g, a, px[a], 0.0);
-      stencil_weights(g, px[a], base[a], 0.0, w[a]);
+      base[a] = stencil_base_r<R>(g, a, px[a], (R)0);
+      stencil_weights_r<R>(g, px[a], base[a], (R)0, w[a]);
     }
-    double phi_ori = 0.0, gr[3] = {0, 0, 0};
+    R phi_ori = 0, gr[3] = {0, 0, 0};
     for (int c2 = 0; c2 < 3; ++c2)
       for (int c1 = 0; c1 < 3; ++c1)
         for (int c0 = 0; c0 < 3; ++c0) {
           const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
-          const double q0 = node_pos(g, i, 0.0), q1 = node_pos(g, j, 0.0),
-                       q2 = node_pos(g, k, 0.0);
-          const double phi = node_active(g, A.act, i, j, k)
-                                 ? container_phi(g, q0, q1, q2)
-                                 : 3.0 * g.dx;
-          const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+          const R q0 = node_pos_r<R>(g, i, (R)0), q1 = node_pos_r<R>(g, j, (R)0),
+                  q2 = node_pos_r<R>(g, k, (R)0);
+          const R phi = node_active(g, A.act, i, j, k) ? container_phi_r<R>(g, q0, q1, q2)
+                                                         : (R)3 * (R)g.dx;
+          const R ww = w[0][c0] * w[1][c1] * w[2][c2];
           phi_ori += phi * ww;
           gr[0] += phi * A.invD * ww * (q0 - nx[0]);
           gr[1] += phi * A.invD * ww * (q1 - nx[1]);
           gr[2] += phi * A.invD * ww * (q2 - nx[2]);
         }
-    const double gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
-    if (gn > 1e-20) {
+    const R gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+    if (gn > (R)1e-20) {
       gr[0] /= gn;
       gr[1] /= gn;
       gr[2] /= gn;
     }
-    const double dp0 = fv[0] * A.dt, dp1 = fv[1] * A.dt, dp2 = fv[2] * A.dt;
-    const double phi_now = phi_ori + (gr[0] * dp0 + gr[1] * dp1 + gr[2] * dp2);
-    if (phi_now < 0.0) {
+    const R dp0 = fv[0] * A.dt, dp1 = fv[1] * A.dt, dp2 = fv[2] * A.dt;
+    const R phi_now = phi_ori + (gr[0] * dp0 + gr[1] * dp1 + gr[2] * dp2);
+    if (phi_now < (R)0) {
       nx[0] -= phi_now * gr[0];
       nx[1] -= phi_now * gr[1];
       nx[2] -= phi_now * gr[2];
@@ -437,24 +439,49 @@ __global__ void k_seed_velocity(Grid g, const double* fil, const double* x,
 
 inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// The particle planes of one precision: the current set and the sort target.
+template <class R>
+struct Planes {
+  DevBuf<R>*x, *v, *C, *x2, *v2, *C2;
+};
+inline Planes<double> planes(wc_ctx* c, double) {
+  return {&c->x, &c->v, &c->C, &c->x2, &c->v2, &c->C2};
+}
+inline Planes<float> planes(wc_ctx* c, float) {
+  return {&c->fx, &c->fv, &c->fC, &c->fx2, &c->fv2, &c->fC2};
+}
+
+template <class R>
+void release_planes(Planes<R> P) {
+  P.x->release();
+  P.v->release();
+  P.C->release();
+  P.x2->release();
+  P.v2->release();
+  P.C2->release();
+}
+
 void ensure_particles(wc_ctx* ctx, int64_t n) {
-  if (n > ctx->pcap || !ctx->x.p) {
+  const bool have = ctx->mixed ? ctx->fx.p != nullptr : ctx->x.p != nullptr;
+  if (n > ctx->pcap || !have) {
     const int64_t cap = n > 0 ? n : 1;
-    ctx->x.release();
-    ctx->v.release();
-    ctx->C.release();
-    ctx->x2.release();
-    ctx->v2.release();
-    ctx->C2.release();
+    release_planes(planes(ctx, 0.0));
+    release_planes(planes(ctx, 0.0f));
     ctx->pid.release();
     ctx->pid2.release();
     ctx->pkey.release();
-    ctx->x.ensure(3 * cap);
-    ctx->v.ensure(3 * cap);
-    ctx->C.ensure(9 * cap);
-    ctx->x2.ensure(3 * cap);
-    ctx->v2.ensure(3 * cap);
-    ctx->C2.ensure(9 * cap);
+    auto alloc = [&](auto P) {
+      P.x->ensure(3 * cap);
+      P.v->ensure(3 * cap);
+      P.C->ensure(9 * cap);
+      P.x2->ensure(3 * cap);
+      P.v2->ensure(3 * cap);
+      P.C2->ensure(9 * cap);
+    };
+    if (ctx->mixed)
+      alloc(planes(ctx, 0.0f));
+    else
+      alloc(planes(ctx, 0.0));
     ctx->pid.ensure(cap);
     ctx->pid2.ensure(cap);
     ctx->pkey.ensure(cap);
@@ -462,6 +489,114 @@ void ensure_particles(wc_ctx* ctx, int64_t n) {
   }
   ctx->np = n;
   ctx->sorted = false;
+}
+
+__global__ void k_narrow(const double* a, float* b, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// Upload host planes (FP64) into the current set, rounding to FP32 for the
+// mixed form.
+template <class R>
+void upload(hipStream_t st, DevBuf<R>& dst, const std::vector<double>& h) {
+  if constexpr (sizeof(R) == sizeof(double)) {
+    WC_HIP(hipMemcpyAsync(dst.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
+  } else {
+    std::vector<R> t(h.size());
+    for (size_t i = 0; i < h.size(); ++i) t[i] = (R)h[i];
+    WC_HIP(hipMemcpyAsync(dst.p, t.data(), sizeof(R) * t.size(), hipMemcpyHostToDevice, st));
+    WC_HIP(hipStreamSynchronize(st));
+  }
+}
+
+template <class R>
+void download(hipStream_t st, const DevBuf<R>& src, std::vector<double>& h) {
+  if constexpr (sizeof(R) == sizeof(double)) {
+    WC_HIP(hipMemcpyAsync(h.data(), src.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  } else {
+    std::vector<R> t(h.size());
+    WC_HIP(hipMemcpyAsync(t.data(), src.p, sizeof(R) * t.size(), hipMemcpyDeviceToHost, st));
+    WC_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < h.size(); ++i) h[i] = (double)t[i];
+  }
+}
+
+template <class R>
+void sort_impl(wc_ctx* ctx, Planes<R> P) {
+  const Grid& g = ctx->g;
+  hipStream_t st = ctx->stream;
+  const int64_t n = ctx->np, cap = ctx->pcap;
+  ctx->bcount.ensure(g.nbucket + 1);
+  ctx->bcursor.ensure(g.nbucket + 1);
+  WC_HIP(hipMemsetAsync(ctx->bcount.p, 0, sizeof(uint32_t) * (g.nbucket + 1), st));
+  WC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(uint32_t) * (g.nbucket + 1), st));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_sort_keys<R>, dim3(blocks(n, 256)), dim3(256), 0, st, g,
+                       P.x->p, n, cap, ctx->pkey.p, ctx->bcount.p);
+    WC_LAUNCH_CHECK();
+  }
+  exclusive_scan(ctx, ctx->bcount.p, g.nbucket + 1);  // bcount becomes bstart
+  std::swap(ctx->bcount, ctx->bstart);
+  ctx->bcount.ensure(g.nbucket + 1);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_sort_scatter<R>, dim3(blocks(n, 256)), dim3(256), 0, st, n,
+                       cap, ctx->pkey.p, ctx->bstart.p, ctx->bcursor.p, P.x->p,
+                       P.v->p, P.C->p, ctx->pid.p, P.x2->p, P.v2->p, P.C2->p,
+                       ctx->pid2.p);
+    WC_LAUNCH_CHECK();
+  }
+  std::swap(*P.x, *P.x2);
+  std::swap(*P.v, *P.v2);
+  std::swap(*P.C, *P.C2);
+  std::swap(ctx->pid, ctx->pid2);
+  ctx->sorted = true;
+}
+
+template <class R>
+void p2g_launch(wc_ctx* ctx, Planes<R> P) {
+  const Grid& g = ctx->g;
+  P2GArgs<R> A;
+  A.g = g;
+  A.x = P.x->p;
+  A.v = P.v->p;
+  A.C = P.C->p;
+  A.cap = ctx->pcap;
+  A.bstart = ctx->bstart.p;
+  for (int d = 0; d < 3; ++d) {
+    A.mass[d] = ctx->mass[d].p;
+    A.mom[d] = ctx->mom[d].p;
+    A.vol[d] = ctx->vol[d].p;
+  }
+  A.phikey = ctx->phikey.p;
+  A.touched = ctx->touched.p;
+  A.fm = (R)ctx->desc.particle_mass;
+  A.fvol = (R)ctx->desc.particle_volume;
+  A.r0 = (R)(g.dx * 0.883644 > ctx->desc.particle_radius ? g.dx * 0.883644
+                                                          : ctx->desc.particle_radius);
+  hipLaunchKernelGGL(k_p2g<R>, dim3((unsigned)g.nbucket), dim3(kP2GThreads), 0,
+                     ctx->stream, A);
+  WC_LAUNCH_CHECK();
+}
+
+template <class R>
+void g2p_launch(wc_ctx* ctx, Planes<R> P, double dt) {
+  G2PArgs<R> A;
+  A.g = ctx->g;
+  A.x = P.x->p;
+  A.v = P.v->p;
+  A.C = P.C->p;
+  A.n = ctx->np;
+  A.cap = ctx->pcap;
+  for (int d = 0; d < 3; ++d) A.u[d] = ctx->unew[d].p;
+  A.act = ctx->act.p;
+  A.dt = (R)dt;
+  A.flip = (R)ctx->desc.flip_coeff;
+  A.invD = (R)(4.0 / (ctx->g.dx * ctx->g.dx));
+  A.solid_projection = ctx->desc.solid_projection;
+  hipLaunchKernelGGL(k_g2p_advect<R>, dim3(blocks(ctx->np, 256)), dim3(256), 0,
+                     ctx->stream, A);
+  WC_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -487,9 +622,15 @@ void particles_set_soa(wc_ctx* ctx, int64_t n, const double* x, const double* v,
                          : C[(r * 3 + c) * n + p];
   }
   hipStream_t st = ctx->stream;
-  WC_HIP(hipMemcpyAsync(ctx->x.p, hx.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice, st));
-  WC_HIP(hipMemcpyAsync(ctx->v.p, hv.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice, st));
-  WC_HIP(hipMemcpyAsync(ctx->C.p, hC.data(), sizeof(double) * 9 * cap, hipMemcpyHostToDevice, st));
+  auto put = [&](auto P) {
+    upload(st, *P.x, hx);
+    upload(st, *P.v, hv);
+    upload(st, *P.C, hC);
+  };
+  if (ctx->mixed)
+    put(planes(ctx, 0.0f));
+  else
+    put(planes(ctx, 0.0));
   WC_HIP(hipMemcpyAsync(ctx->pid.p, hp.data(), sizeof(uint32_t) * cap, hipMemcpyHostToDevice, st));
   WC_HIP(hipStreamSynchronize(st));
 }
@@ -501,9 +642,15 @@ void particles_get_soa(wc_ctx* ctx, double* x, double* v, double* C,
       hC(9 * (size_t)cap);
   std::vector<uint32_t> hp((size_t)cap);
   hipStream_t st = ctx->stream;
-  WC_HIP(hipMemcpyAsync(hx.data(), ctx->x.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, st));
-  WC_HIP(hipMemcpyAsync(hv.data(), ctx->v.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, st));
-  WC_HIP(hipMemcpyAsync(hC.data(), ctx->C.p, sizeof(double) * 9 * cap, hipMemcpyDeviceToHost, st));
+  auto get = [&](auto P) {
+    download(st, *P.x, hx);
+    download(st, *P.v, hv);
+    download(st, *P.C, hC);
+  };
+  if (ctx->mixed)
+    get(planes(ctx, 0.0f));
+  else
+    get(planes(ctx, 0.0));
   WC_HIP(hipMemcpyAsync(hp.data(), ctx->pid.p, sizeof(uint32_t) * cap, hipMemcpyDeviceToHost, st));
   WC_HIP(hipStreamSynchronize(st));
   for (int64_t q = 0; q < n; ++q) {
@@ -563,9 +710,21 @@ int64_t particles_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3]
                         hipMemcpyDeviceToDevice, st));
   const uint32_t n = exclusive_scan(ctx, scan.p, ncand);
   ensure_particles(ctx, n);
+  // Seeds are generated in FP64; the mixed form rounds them once to FP32.
+  const int64_t cap = ctx->pcap;
+  DevBuf<double> tx, tv, tC;
+  double *sx = ctx->x.p, *sv = ctx->v.p, *sC = ctx->C.p;
+  if (ctx->mixed) {
+    tx.ensure(3 * cap);
+    tv.ensure(3 * cap);
+    tC.ensure(9 * cap);
+    sx = tx.p;
+    sv = tv.p;
+    sC = tC.p;
+  }
   hipLaunchKernelGGL(k_seed_write, dim3(blocks(ncand, 256)), dim3(256), 0, st, g,
                      c0[0], c0[1], c0[2], cn[0], cn[1], cn[2], seed, scan.p,
-                     flag.p, ctx->x.p, ctx->v.p, ctx->C.p, ctx->pid.p, ctx->pcap);
+                     flag.p, sx, sv, sC, ctx->pid.p, cap);
   WC_LAUNCH_CHECK();
   if (sigma > 0.0) {
     const int64_t qq = (int64_t)(g.n[0] + 4) * (g.n[1] + 4) * (g.n[2] + 4);
@@ -579,46 +738,35 @@ int64_t particles_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3]
     hipLaunchKernelGGL(k_noise_filter, dim3(blocks(3 * mm, 256)), dim3(256), 0,
                        st, g, raw.p, scale, fil.p);
     hipLaunchKernelGGL(k_seed_velocity, dim3(blocks(n, 256)), dim3(256), 0, st,
-                       g, fil.p, ctx->x.p, ctx->v.p, (int64_t)n, ctx->pcap);
+                       g, fil.p, sx, sv, (int64_t)n, cap);
     WC_LAUNCH_CHECK();
     WC_HIP(hipStreamSynchronize(st));
     raw.release();
     fil.release();
   }
+  if (ctx->mixed) {
+    hipLaunchKernelGGL(k_narrow, dim3(blocks(3 * cap, 256)), dim3(256), 0, st, sx,
+                       ctx->fx.p, 3 * cap);
+    hipLaunchKernelGGL(k_narrow, dim3(blocks(3 * cap, 256)), dim3(256), 0, st, sv,
+                       ctx->fv.p, 3 * cap);
+    hipLaunchKernelGGL(k_narrow, dim3(blocks(9 * cap, 256)), dim3(256), 0, st, sC,
+                       ctx->fC.p, 9 * cap);
+    WC_LAUNCH_CHECK();
+  }
   WC_HIP(hipStreamSynchronize(st));
+  tx.release();
+  tv.release();
+  tC.release();
   flag.release();
   scan.release();
   return n;
 }
 
 void particles_sort(wc_ctx* ctx) {
-  const Grid& g = ctx->g;
-  hipStream_t st = ctx->stream;
-  const int64_t n = ctx->np, cap = ctx->pcap;
-  ctx->bcount.ensure(g.nbucket + 1);
-  ctx->bcursor.ensure(g.nbucket + 1);
-  WC_HIP(hipMemsetAsync(ctx->bcount.p, 0, sizeof(uint32_t) * (g.nbucket + 1), st));
-  WC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(uint32_t) * (g.nbucket + 1), st));
-  if (n > 0) {
-    hipLaunchKernelGGL(k_sort_keys, dim3(blocks(n, 256)), dim3(256), 0, st, g,
-                       ctx->x.p, n, cap, ctx->pkey.p, ctx->bcount.p);
-    WC_LAUNCH_CHECK();
-  }
-  exclusive_scan(ctx, ctx->bcount.p, g.nbucket + 1);  // bcount becomes bstart
-  std::swap(ctx->bcount, ctx->bstart);
-  ctx->bcount.ensure(g.nbucket + 1);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_sort_scatter, dim3(blocks(n, 256)), dim3(256), 0, st, n,
-                       cap, ctx->pkey.p, ctx->bstart.p, ctx->bcursor.p, ctx->x.p,
-                       ctx->v.p, ctx->C.p, ctx->pid.p, ctx->x2.p, ctx->v2.p,
-                       ctx->C2.p, ctx->pid2.p);
-    WC_LAUNCH_CHECK();
-  }
-  std::swap(ctx->x, ctx->x2);
-  std::swap(ctx->v, ctx->v2);
-  std::swap(ctx->C, ctx->C2);
-  std::swap(ctx->pid, ctx->pid2);
-  ctx->sorted = true;
+  if (ctx->mixed)
+    sort_impl(ctx, planes(ctx, 0.0f));
+  else
+    sort_impl(ctx, planes(ctx, 0.0));
 }
 
 void particles_p2g(wc_ctx* ctx) {
@@ -635,46 +783,18 @@ void particles_p2g(wc_ctx* ctx) {
   WC_HIP(hipMemsetAsync(ctx->phikey.p, 0xFF, sizeof(unsigned long long) * nn, st));
   WC_HIP(hipMemsetAsync(ctx->touched.p, 0, g.nbucket, st));
   if (ctx->np == 0) return;
-  P2GArgs A;
-  A.g = g;
-  A.x = ctx->x.p;
-  A.v = ctx->v.p;
-  A.C = ctx->C.p;
-  A.cap = ctx->pcap;
-  A.bstart = ctx->bstart.p;
-  for (int d = 0; d < 3; ++d) {
-    A.mass[d] = ctx->mass[d].p;
-    A.mom[d] = ctx->mom[d].p;
-    A.vol[d] = ctx->vol[d].p;
-  }
-  A.phikey = ctx->phikey.p;
-  A.touched = ctx->touched.p;
-  A.fm = ctx->desc.particle_mass;
-  A.fvol = ctx->desc.particle_volume;
-  A.r0 = g.dx * 0.883644 > ctx->desc.particle_radius ? g.dx * 0.883644
-                                                      : ctx->desc.particle_radius;
-  hipLaunchKernelGGL(k_p2g, dim3((unsigned)g.nbucket), dim3(kP2GThreads), 0, st, A);
-  WC_LAUNCH_CHECK();
+  if (ctx->mixed)
+    p2g_launch(ctx, planes(ctx, 0.0f));
+  else
+    p2g_launch(ctx, planes(ctx, 0.0));
 }
 
 void particles_g2p_advect(wc_ctx* ctx, double dt) {
   if (ctx->np == 0) return;
-  G2PArgs A;
-  A.g = ctx->g;
-  A.x = ctx->x.p;
-  A.v = ctx->v.p;
-  A.C = ctx->C.p;
-  A.n = ctx->np;
-  A.cap = ctx->pcap;
-  for (int d = 0; d < 3; ++d) A.u[d] = ctx->unew[d].p;
-  A.act = ctx->act.p;
-  A.dt = dt;
-  A.flip = ctx->desc.flip_coeff;
-  A.invD = 4.0 / (ctx->g.dx * ctx->g.dx);
-  A.solid_projection = ctx->desc.solid_projection;
-  hipLaunchKernelGGL(k_g2p_advect, dim3(blocks(ctx->np, 256)), dim3(256), 0,
-                     ctx->stream, A);
-  WC_LAUNCH_CHECK();
+  if (ctx->mixed)
+    g2p_launch(ctx, planes(ctx, 0.0f), dt);
+  else
+    g2p_launch(ctx, planes(ctx, 0.0), dt);
 }
 
 }  // namespace wc

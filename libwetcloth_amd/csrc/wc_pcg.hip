@@ -259,13 +259,15 @@ __device__ inline double block_max(double v, double* sh) {
   return sh[0];
 }
 
+// FP64 block partials of a . b (FP64 products and sums for FP32 vectors too)
+template <class V>
 __global__ __launch_bounds__(kRedThreads) void k_dot_partial(
-    const double* a, const double* b, int64_t n, double* partial) {
+    const V* a, const V* b, int64_t n, double* partial) {
   __shared__ double sh[kRedThreads];
   double acc = 0;
   for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedThreads)
-    acc += a[i] * b[i];
+    acc += (double)a[i] * (double)b[i];
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -333,12 +335,33 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr(
   if (threadIdx.x == 0) partial[blockIdx.x] = mm;
 }
 
-// s = z + s*beta (add_scaled then swap, PCGSolver.h:302-303)
-__global__ void k_update_s(double* s, const double* z, int64_t n,
-                           const double* scal) {
+// The inner iteration of the mixed-precision solve: the FP64 solution takes
+// s*alpha directly and the FP32 residual r -= z*alpha is formed in FP64
+// before rounding; |r|_inf block partials.
+__global__ __launch_bounds__(kRedThreads) void k_update_xr_mixed(
+    double* x, float* r, const float* s, const float* z, int64_t n,
+    const double* scal, double* partial) {
+  __shared__ double sh[kRedThreads];
+  const double alpha = scal[S_ALPHA];
+  double m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads) {
+    x[i] += (double)s[i] * alpha;
+    const float ri = (float)((double)r[i] - (double)z[i] * alpha);
+    r[i] = ri;
+    m = fmax(m, (double)fabsf(ri));
+  }
+  const double mm = block_max(m, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = mm;
+}
+
+// s = z + s*beta (add_scaled then swap, PCGSolver.h:302-303); FP32 vectors
+// are updated in FP64 and rounded
+template <class V>
+__global__ void k_update_s(V* s, const V* z, int64_t n, const double* scal) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  s[i] = z[i] + s[i] * scal[S_BETA];
+  s[i] = (V)((double)z[i] + (double)s[i] * scal[S_BETA]);
 }
 
 inline unsigned blocks_for(int64_t n, int t) {
@@ -587,7 +610,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   box_gather(st, P, b, r);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
   sweep_apply(st, P, r, tm, ndof);
-  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
                      w.partial.p);
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
                      w.scal.p, (int)R_SET_RHO);
@@ -604,7 +627,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
     box_spmv(st, P, s, z);
     if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
-    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, s, z, nc,
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z, nc,
                        w.partial.p);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
                        w.scal.p, (int)R_ALPHA);
@@ -632,16 +655,113 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
       return true;
     }
     sweep_apply(st, P, r, tm, ndof);
-    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
                        w.partial.p);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
                        w.scal.p, (int)R_BETA);
-    hipLaunchKernelGGL(k_update_s, dim3(gb), dim3(256), 0, st, s, z, nc, w.scal.p);
+    hipLaunchKernelGGL(k_update_s<double>, dim3(gb), dim3(256), 0, st, s, z, nc, w.scal.p);
     WC_LAUNCH_CHECK();
   }
   *iters_out = it;
   box_scatter(st, P, x, x_dof);
   return false;
+}
+
+// The mixed-precision form of the solve (no reference counterpart): PCG
+// with FP32 vectors, FP32 factor records and matrix entries, FP64 inner
+// products and alpha/beta, inside FP64 iterative refinement.  The factor is
+// computed in FP64 (pivots and the min-diagonal guard included) and rounded.
+// Each pass solves A d = r from d = 0, adding d into the FP64 solution as it
+// goes, until its FP32 residual has dropped by 10^-digits (or to the target);
+// the FP64 residual r = b - A x is then recomputed, and the solve stops when
+// it meets |r|_inf <= tol |b|_inf, after 1 + refinements passes, or when the
+// iterations of all passes reach maxit.  iterations_out counts every inner
+// iteration.
+static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const double* b,
+                               double* x_dof, double tolr, int maxit,
+                               double* residual_out, int* iters_out) {
+  SweepPlan& P = w.sweep;
+  const int64_t nc = P.ncell;
+  const int nb = w.red_blocks;
+  sweep_narrow(st, P);
+  double* x = P.xb.p;
+  float* r = P.rb32.p;
+  float* s = P.sb32.p;
+  float* z = P.zb32.p;  // the preconditioned vector, then A s
+  KTimer* tm = w.timer.on ? &w.timer : nullptr;
+  box_gather(st, P, b, P.bb.p);
+  WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
+  const unsigned gb = blocks_for(nc, 256);
+  double eta = 1.0;
+  for (int d = 0; d < w.mixed_digits; ++d) eta *= 0.1;
+  int total = 0;
+  w.passes = 0;
+  for (;;) {
+    // FP64 residual of the current solution
+    box_residual(st, P, P.bb.p, x, r, w.partial.p, nb);
+    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                       w.scal.p);
+    WC_LAUNCH_CHECK();
+    const double res = read_scal(st, w, S_RES);
+    *residual_out = res;
+    *iters_out = total;
+    if (res <= tolr) {
+      box_scatter(st, P, x, x_dof);
+      return true;
+    }
+    if (total >= maxit || w.passes > w.mixed_refinements || res != res) {
+      box_scatter(st, P, x, x_dof);
+      return false;
+    }
+    ++w.passes;
+    const double inner_tol = std::max(tolr, eta * res);
+    sweep_apply32(st, P, r, tm, ndof);
+    hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+                       w.partial.p);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                       w.scal.p, (int)R_SET_RHO);
+    WC_LAUNCH_CHECK();
+    const double rho = read_scal(st, w, S_RHO);
+    if (rho == 0 || rho != rho) {
+      box_scatter(st, P, x, x_dof);
+      return false;
+    }
+    WC_HIP(hipMemcpyAsync(s, z, sizeof(float) * nc, hipMemcpyDeviceToDevice, st));
+    while (total < maxit) {
+      hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
+      box_spmv32(st, P, s, z);
+      if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
+      hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z, nc,
+                         w.partial.p);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                         w.scal.p, (int)R_ALPHA);
+      hipLaunchKernelGGL(k_update_xr_mixed, dim3(nb), dim3(kRedThreads), 0, st, x, r, s, z,
+                         nc, w.scal.p, w.partial.p);
+      hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p,
+                         nb, w.scal.p);
+      WC_LAUNCH_CHECK();
+      WC_HIP(hipMemcpyAsync(w.host_scal + S_RES, w.scal.p + S_RES, sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+      WC_HIP(hipMemcpyAsync(w.host_flag, P.progress.p + 2, sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+      WC_HIP(hipStreamSynchronize(st));
+      if (!w.timer.pending.empty()) w.timer.harvest();
+      ++total;
+      if (*w.host_flag)
+        throw HipError{hipErrorLaunchTimeOut,
+                       "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__,
+                       __LINE__};
+      const double r32 = w.host_scal[S_RES];
+      if (r32 <= inner_tol || r32 != r32) break;
+      sweep_apply32(st, P, r, tm, ndof);
+      hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+                         w.partial.p);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                         w.scal.p, (int)R_BETA);
+      hipLaunchKernelGGL(k_update_s<float>, dim3(gb), dim3(256), 0, st, s, z, nc, w.scal.p);
+      WC_LAUNCH_CHECK();
+    }
+  }
 }
 
 bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
@@ -684,12 +804,14 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
                      "pipelined MIC(0) factor: a tile wait exceeded its bound", __FILE__,
                      __LINE__};
     w.factored = true;
+    if (w.mixed)
+      return pcg_loop_box_mixed(st, w, n, b, x, tolr, maxit, residual_out, iters_out);
     return pcg_loop_box(st, w, n, b, x, tolr, maxit, residual_out, iters_out);
   }
   pcg_factor(st, w, A, tau, mdr);
   w.factored = true;
   pcg_apply(st, w, A, r, z);
-  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
                      w.partial.p);
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p,
                      nb, w.scal.p, (int)R_SET_RHO);
@@ -706,7 +828,7 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
     hipEvent_t e0 = w.timer.on ? w.timer.mark(st) : nullptr;
     hipLaunchKernelGGL(k_spmv, dim3(gb), dim3(256), 0, st, A, s, z);
     if (e0) w.timer.add(KTimer::SPMV, n, e0, w.timer.mark(st));
-    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, s, z,
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
                        n, w.partial.p);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st,
                        w.partial.p, nb, w.scal.p, (int)R_ALPHA);
@@ -722,11 +844,11 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
       return true;
     }
     pcg_apply(st, w, A, r, z);
-    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, st, z, r,
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r,
                        n, w.partial.p);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st,
                        w.partial.p, nb, w.scal.p, (int)R_BETA);
-    hipLaunchKernelGGL(k_update_s, dim3(gb), dim3(256), 0, st, s, z, n,
+    hipLaunchKernelGGL(k_update_s<double>, dim3(gb), dim3(256), 0, st, s, z, n,
                        w.scal.p);
     WC_LAUNCH_CHECK();
   }

@@ -89,6 +89,17 @@ __device__ inline double load_coherent(const double* p) {
 __device__ inline void store_coherent(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ inline float load_coherent(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_coherent(float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// the all-ones NaN that marks an output not yet written
+__device__ inline bool is_sentinel(double v) {
+  return __double_as_longlong(v) == (long long)kSentinel;
+}
+__device__ inline bool is_sentinel(float v) { return __float_as_uint(v) == 0xFFFFFFFFu; }
 __device__ inline int load_flag(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -97,11 +108,14 @@ __device__ inline void store_flag(int* p, int v) {
 }
 
 // one box cell's factor entries for one sweep direction, in that sweep's
-// term order (two 16-byte loads)
-struct __attribute__((aligned(16))) Cell {
-  double l0, l1, l2, inv;
+// term order (two 16-byte loads in FP64, one in FP32)
+template <class V>
+struct __attribute__((aligned(16))) CellT {
+  V l0, l1, l2, inv;
 };
+using Cell = CellT<double>;
 static_assert(sizeof(Cell) == 32, "Cell is two dwordx4");
+static_assert(sizeof(CellT<float>) == 16, "an FP32 cell is one dwordx4");
 constexpr int kCoef = 7;  // per box cell: A diagonal, A(-x,-y,-z), A(+x,+y,+z)
 
 // one box cell's inputs of the factor pass (six 16-byte loads): the lower
@@ -372,30 +386,68 @@ __device__ inline Nbr neighbours(const Skew& K, int64_t t) {
   return n;
 }
 
-// y = A x on box vectors: multiply(FixedSparseMatrix, x, y) (SparseMatrix.h:298-322)
-// with each row's entries added in ascending column order.  A present entry
-// always couples two DOFs, so every x read stays inside the box.
-__global__ void k_box_spmv(int64_t nslot, Skew K, const uint32_t* code, const double* coef,
-                           const double* x, double* y) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nslot) return;
-  const uint32_t c = code[t];
+// Row t of A x on box vectors: multiply(FixedSparseMatrix, x, y)
+// (SparseMatrix.h:298-322) with the row's entries added in ascending column
+// order, in FP64 whatever the storage type of A (CV) and x (V).  A present
+// entry always couples two DOFs, so every x read stays inside the box.
+template <class CV, class V>
+__device__ inline double box_row(const Skew& K, int64_t t, uint32_t c, const CV* coef,
+                                 const V* x) {
   double acc = 0.0;
   if (c & C_DOF) {
-    const double* a = coef + kCoef * t;
+    const CV* a = coef + kCoef * t;
     const Nbr n = neighbours(K, t);
     const int lo = (c >> C_LORD) & 7, up = (c >> C_UORD) & 7;
     for (int q = 0; q < 3; ++q) {
       const int ax = perm_axis(lo, q);
-      if (c & (1u << (C_ALO + ax))) acc += a[1 + ax] * x[n.lo[ax]];
+      if (c & (1u << (C_ALO + ax))) acc += (double)a[1 + ax] * (double)x[n.lo[ax]];
     }
-    if (c & C_DIAG) acc += a[0] * x[t];
+    if (c & C_DIAG) acc += (double)a[0] * (double)x[t];
     for (int q = 0; q < 3; ++q) {
       const int ax = perm_axis(up, q);
-      if (c & (1u << (C_AUP + ax))) acc += a[4 + ax] * x[n.up[ax]];
+      if (c & (1u << (C_AUP + ax))) acc += (double)a[4 + ax] * (double)x[n.up[ax]];
     }
   }
-  y[t] = acc;
+  return acc;
+}
+
+// y = A x (FP64, or FP32 storage with FP64 row sums)
+template <class V>
+__global__ void k_box_spmv(int64_t nslot, Skew K, const uint32_t* code, const V* coef,
+                           const V* x, V* y) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nslot) return;
+  y[t] = (V)box_row(K, t, code[t], coef, x);
+}
+
+// The refinement residual of the mixed-precision solve, r = b - A x in FP64
+// (FP64 A, b and x), stored rounded to FP32, with |r|_inf block partials.
+__global__ __launch_bounds__(256) void k_box_residual(int64_t nslot, Skew K,
+                                                      const uint32_t* code,
+                                                      const double* coef, const double* b,
+                                                      const double* x, float* r,
+                                                      double* partial) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x; t < nslot;
+       t += (int64_t)gridDim.x * 256) {
+    const double v = b[t] - box_row(K, t, code[t], coef, x);
+    r[t] = (float)v;
+    m = fmax(m, fabs(v));
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// FP64 -> FP32 copies of the factor records and matrix entries
+__global__ void k_narrow_box(const double* a, float* b, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 // The slot of the upper neighbour of slot t along axis d.
@@ -436,14 +488,17 @@ __global__ void k_backward_records(int64_t nslot, Skew K, const uint32_t* scode,
 
 enum { K_FWD = 0, K_BWD = 1, K_FAC = 2 };
 
+// V is the sweep's value type: double, or float for the FP32 sweeps of the
+// mixed-precision solve (the factor pass is always FP64).
+template <class V>
 struct SweepArgs {
   Skew K;
   int ntiles;
   const uint32_t* order;   // ticket -> a | b << 16
-  const Cell* cells;       // forward: lower terms; backward: upper terms
+  const CellT<V>* cells;   // forward: lower terms; backward: upper terms
   const uint32_t* code;
-  const double* in;        // forward: r; backward: the forward result
-  double* out;             // the sweep result, or the factor's invdiag
+  const V* in;             // forward: r; backward: the forward result
+  V* out;                  // the sweep result, or the factor's invdiag
   int* ticket;
   int* abort_flag;
   // factor pass
@@ -453,19 +508,20 @@ struct SweepArgs {
 };
 
 // inputs of one cell of a sweep
+template <class V>
 struct CellIn {
-  Cell c;
-  double rhs;
+  CellT<V> c;
+  V rhs;
   uint32_t code;
 };
 
-template <int KIND>
+template <int KIND, class V>
 struct PassIn {
-  using T = CellIn;
+  using T = CellIn<V>;
   static constexpr int chunk = kChunk;  // steps prefetched ahead
 };
-template <>
-struct PassIn<K_FAC> {
+template <class V>
+struct PassIn<K_FAC, V> {
   using T = FacIn;
   static constexpr int chunk = 4;
 };
@@ -484,30 +540,34 @@ __device__ inline T* at(T* base, uint32_t i) {
   return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T));
 }
 
-template <int KIND>
-__device__ inline typename PassIn<KIND>::T load_in(const SweepArgs& S, uint32_t idx) {
+template <int KIND, class V>
+__device__ inline typename PassIn<KIND, V>::T load_in(const SweepArgs<V>& S, uint32_t idx) {
   if constexpr (KIND == K_FAC) {
     return *at(S.fin, idx);
   } else {
-    CellIn v;
+    CellIn<V> v;
     v.c = *at(S.cells, idx);
     v.rhs = *at(S.in, idx);
     v.code = *at(S.code, idx);
     return v;
   }
 }
-__device__ inline uint32_t code_of(const CellIn& v) { return v.code; }
+template <class V>
+__device__ inline uint32_t code_of(const CellIn<V>& v) {
+  return v.code;
+}
 __device__ inline uint32_t code_of(const FacIn& v) { return v.code; }
 
 // the value at p once it is no longer the sentinel (rare path: the
 // prefetched copy was read before its tile had written it)
-__device__ __attribute__((always_inline)) inline double wait_value(const double* p, int* abort_flag) {
+template <class V>
+__device__ __attribute__((always_inline)) inline V wait_value(const V* p, int* abort_flag) {
   long long spins = 0;
-  double v = load_coherent(p);
-  while (__double_as_longlong(v) == (long long)kSentinel) {
+  V v = load_coherent(p);
+  while (is_sentinel(v)) {
     if (++spins > kSpinLimit || load_flag(abort_flag)) {
       store_flag(abort_flag, 1);
-      return 0.0;
+      return (V)0;
     }
     __builtin_amdgcn_s_sleep(1);
     v = load_coherent(p);
@@ -515,28 +575,21 @@ __device__ __attribute__((always_inline)) inline double wait_value(const double*
   return v;
 }
 
-__device__ inline double ready_value(const double* p, double v, int* abort_flag) {
-  if (__builtin_expect(__double_as_longlong(v) == (long long)kSentinel, 0))
-    return wait_value(p, abort_flag);
-  return v;
-}
-
 // solve_lower / solve_lower_transpose_in_place for one cell: the rhs minus
 // the present neighbour terms in ascending DOF order, times invdiag.
 // Absent terms subtract +0, an exact identity; cells that are not DOFs
 // carry rhs = inv = 0 and no terms, so their value is 0.
-template <int KIND>
-__device__ inline double sweep_cell(const CellIn& ci, uint32_t code, double zx, double zy,
-                                    double zprev) {
+template <int KIND, class V>
+__device__ inline V sweep_cell(const CellIn<V>& ci, uint32_t code, V zx, V zy, V zprev) {
   const int pres = KIND == K_BWD ? S_BPRES : S_FPRES;
   const int src = KIND == K_BWD ? S_BSRC : S_FSRC;
-  double acc = ci.rhs;
+  V acc = ci.rhs;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const uint32_t a = (code >> (src + 2 * q)) & 3u;
-    const double zn = a == 0 ? zx : (a == 1 ? zy : zprev);
-    const double L = q == 0 ? ci.c.l0 : (q == 1 ? ci.c.l1 : ci.c.l2);
-    const double term = (code & (1u << (pres + q))) ? L * zn : 0.0;
+    const V zn = a == 0 ? zx : (a == 1 ? zy : zprev);
+    const V L = q == 0 ? ci.c.l0 : (q == 1 ? ci.c.l1 : ci.c.l2);
+    const V term = (code & (1u << (pres + q))) ? L * zn : (V)0;
     acc -= term;
   }
   return acc * ci.c.inv;
@@ -550,7 +603,7 @@ __device__ inline double sweep_cell(const CellIn& ci, uint32_t code, double zx, 
 // ascending row order.  Returns invdiag(me) and writes the scaled entries
 // into this cell's forward record (the backward records are gathered from
 // the forward ones after the pass, so every record has a single writer).
-__device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_t code,
+__device__ inline double factor_cell(const SweepArgs<double>& S, const FacIn& f, uint32_t code,
                                      double ix, double iy, double iprev, uint32_t idx) {
   double L0 = 0, L1 = 0, L2 = 0;
   double d = f.adiag;
@@ -592,12 +645,12 @@ __device__ inline double factor_cell(const SweepArgs& S, const FacIn& f, uint32_
 
 // One pipelined pass.  K_BWD mirrors every axis: local coordinates run from
 // the high corner of the box, upstream tiles are (a+1, b) and (a, b+1).
-template <int KIND>
-__global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
+template <int KIND, class V>
+__global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<V> S) {
   constexpr bool BWD = KIND == K_BWD;
-  constexpr int kCh = PassIn<KIND>::chunk;
-  using In = typename PassIn<KIND>::T;
-  __shared__ double lz[2][kT][kT + 1];
+  constexpr int kCh = PassIn<KIND, V>::chunk;
+  using In = typename PassIn<KIND, V>::T;
+  __shared__ V lz[2][kT][kT + 1];
   __shared__ int s_tile;
   const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
   const int Bz = S.K.Bz;
@@ -641,12 +694,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     auto face_at = [&](bool on, int dstep, int kk) -> uint32_t {
       return in_index(kk) + (uint32_t)(on ? dstep : 0);
     };
-    auto load_bnd = [&](bool on, int dstep, int kk) -> double {
+    auto load_bnd = [&](bool on, int dstep, int kk) -> V {
       return load_coherent(at(S.out, face_at(on, dstep, kk)));
     };
     // only the first wave holds lanes on the upstream y face (lj == 0)
     const bool ywave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == 0;
-    double zprev = 0.0;
+    V zprev = 0;
     // Prefetch rings.  The static inputs of a cell (its records, rhs and
     // code) are read kCh steps ahead: slot s holds those of step w0 + s.  The
     // face values, which the upstream tiles are still producing, are read
@@ -658,19 +711,19 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
     // checked after the loads issued behind it.
     constexpr int kF = kFace < kCh ? kFace : kCh;
     static_assert(kCh % kF == 0, "the face ring must divide the chunk");
-    double gx = load_bnd(bnd_x, dstep_x, 2 * kF - 1 - off);
-    double gy = 0.0;
+    V gx = load_bnd(bnd_x, dstep_x, 2 * kF - 1 - off);
+    V gy = 0;
     if (ywave) gy = load_bnd(bnd_y, dstep_y, 2 * kF - 1 - off);
     In win[kCh];
-    double wx[kF], wy[kF];
+    V wx[kF], wy[kF];
 #pragma unroll
     for (int s = 0; s < kF; ++s) {
       wx[s] = load_bnd(bnd_x, dstep_x, s - off);
-      wy[s] = 0.0;
+      wy[s] = 0;
       if (ywave) wy[s] = load_bnd(bnd_y, dstep_y, s - off);
     }
 #pragma unroll
-    for (int s = 0; s < kCh; ++s) win[s] = load_in<KIND>(S, in_index(s - off));
+    for (int s = 0; s < kCh; ++s) win[s] = load_in<KIND, V>(S, in_index(s - off));
     const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
     const int lyj = lj > 0 ? lj - 1 : 0;
     for (int w0 = 0; w0 < nsteps; w0 += kCh) {
@@ -681,12 +734,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           // Face gate: the face refills of the next kF steps (for steps
           // w + kF .. w + 2kF - 1) must read final values, else each of them
           // reads the sentinel and every step pays a round trip in
-          // ready_value.  The gate value (the face value of step
+          // wait_value.  The gate value (the face value of step
           // w + 2kF - 1) was loaded kF steps ago and is only re-polled if it
           // was not final then.
-          if (bnd_x && __double_as_longlong(gx) == (long long)kSentinel)
+          if (bnd_x && is_sentinel(gx))
             wait_value(at(S.out, face_at(true, dstep_x, w + 2 * kF - 1 - off)), S.abort_flag);
-          if (bnd_y && __double_as_longlong(gy) == (long long)kSentinel)
+          if (bnd_y && is_sentinel(gy))
             wait_value(at(S.out, face_at(true, dstep_y, w + 2 * kF - 1 - off)), S.abort_flag);
           gx = load_bnd(bnd_x, dstep_x, w + 3 * kF - 1 - off);
           if (ywave) gy = load_bnd(bnd_y, dstep_y, w + 3 * kF - 1 - off);
@@ -698,26 +751,24 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           const In ci = win[s];
           const uint32_t code = active ? code_of(ci) : 0u;
           const uint32_t idx = out_index(kk);
-          const double lx = lz[(w - 1) & 1][lj][lxi];
-          const double ly = lz[(w - 1) & 1][lyj][li];
-          double zx = li > 0 ? lx : (bnd_x ? wx[s % kF] : 0.0);
-          double zy = lj > 0 ? ly : (bnd_y ? wy[s % kF] : 0.0);
+          const V lx = lz[(w - 1) & 1][lj][lxi];
+          const V ly = lz[(w - 1) & 1][lyj][li];
+          V zx = li > 0 ? lx : (bnd_x ? wx[s % kF] : (V)0);
+          V zy = lj > 0 ? ly : (bnd_y ? wy[s % kF] : (V)0);
           // a face value prefetched before its tile wrote it is re-read
           // (only where the term is present)
-          const bool sx = bnd_x && (code & xbit) &&
-                          __double_as_longlong(zx) == (long long)kSentinel;
-          const bool sy = bnd_y && (code & ybit) &&
-                          __double_as_longlong(zy) == (long long)kSentinel;
+          const bool sx = bnd_x && (code & xbit) && is_sentinel(zx);
+          const bool sy = bnd_y && (code & ybit) && is_sentinel(zy);
           if (__builtin_expect(sx || sy, 0)) {
             if (sx) zx = wait_value(at(S.out, idx + (uint32_t)dstep_x), S.abort_flag);
             if (sy) zy = wait_value(at(S.out, idx + (uint32_t)dstep_y), S.abort_flag);
           }
-          double z;
+          V z;
           if constexpr (KIND == K_FAC) {
             z = 0.0;
             if (code & S_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx);
           } else {
-            z = active ? sweep_cell<KIND>(ci, code, zx, zy, zprev) : 0.0;
+            z = active ? sweep_cell<KIND, V>(ci, code, zx, zy, zprev) : (V)0;
           }
           // every step writes the lane's slot of that step (0 on the padding
           // slots), so no slot keeps the sentinel after the pass
@@ -725,7 +776,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs S) {
           zprev = z;
           lz[w & 1][lj][li] = z;
         }
-        win[s] = load_in<KIND>(S, in_index(w + kCh - off));
+        win[s] = load_in<KIND, V>(S, in_index(w + kCh - off));
         wx[s % kF] = load_bnd(bnd_x, dstep_x, w + kF - off);
         if (ywave) wy[s % kF] = load_bnd(bnd_y, dstep_y, w + kF - off);
       }
@@ -829,8 +880,9 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.valid = true;
 }
 
-static SweepArgs sweep_args(SweepPlan& P) {
-  SweepArgs S = {};
+template <class V>
+static SweepArgs<V> sweep_args(SweepPlan& P) {
+  SweepArgs<V> S = {};
   S.K = skew_of(P);
   S.ntiles = P.ntiles;
   S.order = P.order.p;
@@ -843,14 +895,14 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
   int* tick = P.progress.p;
   WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 8, st));
   WC_HIP(hipMemsetAsync(P.zf.p, 0xFF, sizeof(double) * P.ncell, st));
-  SweepArgs S = sweep_args(P);
+  SweepArgs<double> S = sweep_args<double>(P);
   S.fin = reinterpret_cast<const FacIn*>(P.fin.p);
   S.cf = reinterpret_cast<Cell*>(P.cf.p);
   S.tau = tau;
   S.mdr = mdr;
   S.out = P.zf.p;
   S.ticket = tick + 3;
-  hipLaunchKernelGGL(k_sweep<K_FAC>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_FAC, double>), dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_backward_records, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
                      P.ncell, skew_of(P), P.scode.p, reinterpret_cast<const Cell*>(P.cf.p),
@@ -858,30 +910,48 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
   WC_LAUNCH_CHECK();
 }
 
-void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
-                 int64_t ndof) {
+void sweep_narrow(hipStream_t st, SweepPlan& P) {
+  const int64_t nrec = P.ncell * 4;
+  P.cf32.ensure(nrec);
+  P.cb32.ensure(nrec);
+  P.coef32.ensure(P.ncell * kCoef);
+  P.zf32.ensure(P.ncell);
+  P.zb32.ensure(P.ncell);
+  P.rb32.ensure(P.ncell);
+  P.sb32.ensure(P.ncell);
+  P.bb.ensure(P.ncell);
+  hipLaunchKernelGGL(k_narrow_box, dim3(blocks(nrec, 256)), dim3(256), 0, st, P.cf.p,
+                     P.cf32.p, nrec);
+  hipLaunchKernelGGL(k_narrow_box, dim3(blocks(nrec, 256)), dim3(256), 0, st, P.cb.p,
+                     P.cb32.p, nrec);
+  hipLaunchKernelGGL(k_narrow_box, dim3(blocks(P.ncell * kCoef, 256)), dim3(256), 0, st,
+                     P.coef.p, P.coef32.p, P.ncell * kCoef);
+  WC_LAUNCH_CHECK();
+}
+
+template <class V>
+static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, const V* cb,
+                       V* zf, V* zb, KTimer* timer, int64_t ndof) {
   // two ticket counters and the abort flag; outputs start as the sentinel
   int* tick = P.progress.p;
-  int* abort_flag = tick + 2;
   WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 8, st));
-  WC_HIP(hipMemsetAsync(P.zf.p, 0xFF, sizeof(double) * P.ncell, st));
-  WC_HIP(hipMemsetAsync(P.zb.p, 0xFF, sizeof(double) * P.ncell, st));
-  SweepArgs S = sweep_args(P);
-  (void)abort_flag;
-  S.cells = reinterpret_cast<const Cell*>(P.cf.p);
+  WC_HIP(hipMemsetAsync(zf, 0xFF, sizeof(V) * P.ncell, st));
+  WC_HIP(hipMemsetAsync(zb, 0xFF, sizeof(V) * P.ncell, st));
+  SweepArgs<V> S = sweep_args<V>(P);
+  S.cells = reinterpret_cast<const CellT<V>*>(cf);
   S.in = r;
-  S.out = P.zf.p;
+  S.out = zf;
   S.ticket = tick;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL(k_sweep<K_FWD>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
   hipEvent_t e1 = timer ? timer->mark(st) : nullptr;
-  S.cells = reinterpret_cast<const Cell*>(P.cb.p);
-  S.in = P.zf.p;
-  S.out = P.zb.p;
+  S.cells = reinterpret_cast<const CellT<V>*>(cb);
+  S.in = zf;
+  S.out = zb;
   S.ticket = tick + 1;
   hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL(k_sweep<K_BWD>, dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(kThreads), 0, st, S);
   WC_LAUNCH_CHECK();
   if (timer) {
     timer->add(KTimer::SWEEP_FWD, ndof, e0, e1);
@@ -889,9 +959,32 @@ void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
   }
 }
 
+void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
+                 int64_t ndof) {
+  apply_impl<double>(st, P, r, P.cf.p, P.cb.p, P.zf.p, P.zb.p, timer, ndof);
+}
+
+void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
+                   int64_t ndof) {
+  apply_impl<float>(st, P, r, P.cf32.p, P.cb32.p, P.zf32.p, P.zb32.p, timer, ndof);
+}
+
 void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y) {
-  hipLaunchKernelGGL(k_box_spmv, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, P.ncell,
-                     skew_of(P), P.code.p, P.coef.p, x, y);
+  hipLaunchKernelGGL(k_box_spmv<double>, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
+                     P.ncell, skew_of(P), P.code.p, P.coef.p, x, y);
+  WC_LAUNCH_CHECK();
+}
+
+void box_spmv32(hipStream_t st, const SweepPlan& P, const float* x, float* y) {
+  hipLaunchKernelGGL(k_box_spmv<float>, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
+                     P.ncell, skew_of(P), P.code.p, P.coef32.p, x, y);
+  WC_LAUNCH_CHECK();
+}
+
+void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const double* x,
+                  float* r, double* partial, int nblocks) {
+  hipLaunchKernelGGL(k_box_residual, dim3(nblocks), dim3(256), 0, st, P.ncell, skew_of(P),
+                     P.code.p, P.coef.p, b, x, r, partial);
   WC_LAUNCH_CHECK();
 }
 
@@ -926,6 +1019,14 @@ void SweepPlan::release() {
   xb.release();
   zf.release();
   zb.release();
+  cf32.release();
+  cb32.release();
+  coef32.release();
+  zf32.release();
+  zb32.release();
+  rb32.release();
+  sb32.release();
+  bb.release();
   order.release();
   progress.release();
   flags.release();
