@@ -38,7 +38,6 @@ namespace {
 constexpr int kT = 16;           // tile edge (pencils)
 constexpr int kThreads = kT * kT;
 constexpr int kChunk = 8;        // cell records prefetched ahead per pencil
-constexpr int kFace = 4;         // face values prefetched ahead (divides every chunk)
 constexpr long long kSpinLimit = 1ll << 22;  // ~seconds; a correct run waits us
 constexpr unsigned long long kSentinel = 0xFFFFFFFFFFFFFFFFull;
 
@@ -645,24 +644,39 @@ __device__ inline double factor_cell(const SweepArgs<double>& S, const FacIn& f,
 
 // One pipelined pass.  K_BWD mirrors every axis: local coordinates run from
 // the high corner of the box, upstream tiles are (a+1, b) and (a, b+1).
+//
+// A workgroup is four compute waves (one lane per pencil of the 16x16 tile)
+// and one face wave.  The compute lanes exchange each step's values through
+// the LDS array lz, whose row 0 and column 0 are a halo: the face wave
+// writes there the upstream tiles' values beside the tile's low x and y
+// faces, so a compute lane reads its two neighbours from lz unconditionally
+// and the compute waves never issue a device-coherent load (which would sit
+// in their in-order load counter ahead of the record prefetches).  The face
+// wave reads the face values kF steps ahead, and re-reads one that was not
+// final yet with a bounded wait before publishing it.
+constexpr int kFaceAhead = 8;  // face values read ahead by the face wave
+constexpr int kBlock = kThreads + 64;
+
 template <int KIND, class V>
-__global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<V> S) {
+__global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
   constexpr bool BWD = KIND == K_BWD;
   constexpr int kCh = PassIn<KIND, V>::chunk;
+  constexpr int kF = kFaceAhead;
   using In = typename PassIn<KIND, V>::T;
-  __shared__ V lz[2][kT][kT + 1];
+  // [step parity][1 + lj][1 + li]; row 0 / column 0 hold the upstream faces
+  __shared__ V lz[2][kT + 1][kT + 1];
   __shared__ int s_tile;
-  const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
+  // wave-uniform (scalar) branch: the face wave is wave 4
+  const bool face_wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == kThreads / 64;
   const int Bz = S.K.Bz;
   const int nsteps = S.K.nsteps;
-  const uint32_t xbit = 1u << (BWD ? S_BX : S_FX), ybit = 1u << (BWD ? S_BY : S_FY);
   for (;;) {
     if (threadIdx.x == 0) {
       const int t = atomicAdd(S.ticket, 1);
       s_tile = t < S.ntiles ? (int)S.order[t] : -1;
     }
     __syncthreads();
-    const int tile = s_tile;
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
     if (tile < 0) return;
     // logical tile (a, b) counted from the sweep's start corner
     const int ta = tile & 0xFFFF, tb = tile >> 16;
@@ -670,99 +684,95 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<V> S) {
     const int b = BWD ? S.K.ntile_b - 1 - tb : tb;
     const bool has_a = BWD ? a + 1 < S.K.ntile_a : a > 0;
     const bool has_b = BWD ? b + 1 < S.K.ntile_b : b > 0;
-    // physical lane of this thread, and the slot of its cell computed at
-    // step w (= kk + off): base + sgn * w (the backward sweep runs the
-    // skewed steps in reverse)
-    const int lip = BWD ? kT - 1 - li : li, ljp = BWD ? kT - 1 - lj : lj;
-    const uint32_t base = (uint32_t)skew_slot(S.K, a, b, lip, ljp, BWD ? nsteps - 1 : 0);
     const int sgn = BWD ? -kThreads : kThreads;
-    const int dstep_x = (int)(BWD ? -S.K.dx : S.K.dx);  // to the upstream tile's face cell
-    const int dstep_y = (int)(BWD ? -S.K.dy : S.K.dy);
-    const bool bnd_x = li == 0 && has_a;     // reads the upstream x face
-    const bool bnd_y = lj == 0 && has_b;
-    const int off = li + lj;  // this pencil starts at step off
-    auto out_index = [&](int kk) -> uint32_t { return base + (uint32_t)(sgn * (kk + off)); };
-    auto in_index = [&](int kk) {
+    // The pencil of local lane (li, lj) computes its cell kk at step
+    // w = kk + li + lj, stored at slot base + sgn * w (the backward sweep runs
+    // the skewed steps in reverse).  kk is clamped into the pencil for loads,
+    // so every load address is valid (steps outside it discard the value).
+    auto base_of = [&](int li, int lj) -> uint32_t {
+      const int lip = BWD ? kT - 1 - li : li, ljp = BWD ? kT - 1 - lj : lj;
+      return (uint32_t)skew_slot(S.K, a, b, lip, ljp, BWD ? nsteps - 1 : 0);
+    };
+    auto in_index = [&](uint32_t base, int off, int kk) -> uint32_t {
       const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
-      return out_index(kc);
+      return base + (uint32_t)(sgn * (kc + off));
     };
-    // the upstream tile's output beside this pencil's kk-th cell.  Every lane
-    // loads (no branch, so no register of a load in flight is rewritten on
-    // another path); lanes off the upstream face, whose neighbour may lie
-    // outside the box, read their own slot instead (consecutive across the
-    // wave) and discard it.
-    auto face_at = [&](bool on, int dstep, int kk) -> uint32_t {
-      return in_index(kk) + (uint32_t)(on ? dstep : 0);
-    };
-    auto load_bnd = [&](bool on, int dstep, int kk) -> V {
-      return load_coherent(at(S.out, face_at(on, dstep, kk)));
-    };
-    // only the first wave holds lanes on the upstream y face (lj == 0)
-    const bool ywave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == 0;
-    V zprev = 0;
-    // Prefetch rings.  The static inputs of a cell (its records, rhs and
-    // code) are read kCh steps ahead: slot s holds those of step w0 + s.  The
-    // face values, which the upstream tiles are still producing, are read
-    // only kF steps ahead (ring slot s % kF), so a tile needs to trail its
-    // upstream tiles by fewer steps.  The step loop is unrolled by kCh and a
-    // slot is refilled only after its step has used it, so every load writes
-    // the registers it is consumed from and no wait on a load in flight is
-    // needed before its own step.  The gate values are issued first: each is
-    // checked after the loads issued behind it.
-    constexpr int kF = kFace < kCh ? kFace : kCh;
-    static_assert(kCh % kF == 0, "the face ring must divide the chunk");
-    V gx = load_bnd(bnd_x, dstep_x, 2 * kF - 1 - off);
-    V gy = 0;
-    if (ywave) gy = load_bnd(bnd_y, dstep_y, 2 * kF - 1 - off);
-    In win[kCh];
-    V wx[kF], wy[kF];
+    if (face_wave) {
+      // lanes 0..15: x face of local row lj = lane (the pencil (0, lj));
+      // lanes 16..31: y face of column li = lane - 16 (the pencil (li, 0));
+      // lanes 32..63 repeat lanes 0..31 and publish nothing
+      const int fl = (threadIdx.x - kThreads) & 31;
+      const bool xf = fl < kT;
+      const int li = xf ? 0 : fl - kT, lj = xf ? fl : 0;
+      const bool pub = threadIdx.x - kThreads < 32;
+      const bool on = pub && (xf ? has_a : has_b);
+      const uint32_t base = base_of(li, lj);
+      const int off = li + lj;
+      const int dstep = on ? (int)(xf ? (BWD ? -S.K.dx : S.K.dx) : (BWD ? -S.K.dy : S.K.dy)) : 0;
+      // the upstream value beside this pencil's cell of step w (own slot if
+      // there is no upstream tile: read, never waited for, never used)
+      auto face_addr = [&](int w) -> uint32_t { return in_index(base, off, w - off) + (uint32_t)dstep; };
+      auto publish = [&](int w, V v) {
+        // value for step w, in the parity of step w - 1; it is waited for
+        // only where the pencil's cell of step w exists (elsewhere the
+        // compute lane discards it and 0 is published)
+        const int kk = w - off;
+        const bool need = on && kk >= 0 && kk < Bz;
+        if (need && is_sentinel(v)) v = wait_value(at(S.out, face_addr(w)), S.abort_flag);
+        if (!need) v = (V)0;
+        if (pub) {
+          if (xf)
+            lz[(w - 1) & 1][lj + 1][0] = v;
+          else
+            lz[(w - 1) & 1][0][li + 1] = v;
+        }
+      };
+      V fr[kF];
 #pragma unroll
-    for (int s = 0; s < kF; ++s) {
-      wx[s] = load_bnd(bnd_x, dstep_x, s - off);
-      wy[s] = 0;
-      if (ywave) wy[s] = load_bnd(bnd_y, dstep_y, s - off);
+      for (int q = 0; q < kF; ++q) fr[q] = load_coherent(at(S.out, face_addr(q)));
+      publish(0, fr[0]);
+      fr[0] = load_coherent(at(S.out, face_addr(kF)));
+      for (int w0 = 0; w0 < nsteps; w0 += kF) {
+#pragma clang loop unroll(full)
+        for (int q = 0; q < kF; ++q) {
+          const int w = w0 + q;
+          if (w < nsteps) {
+            __syncthreads();
+            if (w + 1 < nsteps) {
+              publish(w + 1, fr[(q + 1) % kF]);
+              fr[(q + 1) % kF] = load_coherent(at(S.out, face_addr(w + 1 + kF)));
+            }
+          }
+        }
+      }
+      __syncthreads();
+      continue;
     }
+    const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
+    const uint32_t base = base_of(li, lj);
+    const int off = li + lj;  // this pencil starts at step off
+    V zprev = 0;
+    // Record prefetch ring: slot s holds the inputs of step w0 + s, read kCh
+    // steps ahead.  The step loop is unrolled by kCh and a slot is refilled
+    // only after its step has used it, so every load writes the registers it
+    // is consumed from and no wait on a load in flight is needed before its
+    // own step.
+    In win[kCh];
 #pragma unroll
-    for (int s = 0; s < kCh; ++s) win[s] = load_in<KIND, V>(S, in_index(s - off));
-    const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
-    const int lyj = lj > 0 ? lj - 1 : 0;
+    for (int s = 0; s < kCh; ++s) win[s] = load_in<KIND, V>(S, in_index(base, off, s - off));
     for (int w0 = 0; w0 < nsteps; w0 += kCh) {
 #pragma clang loop unroll(full)
       for (int s = 0; s < kCh; ++s) {
         const int w = w0 + s;
-        if (s % kF == 0) {
-          // Face gate: the face refills of the next kF steps (for steps
-          // w + kF .. w + 2kF - 1) must read final values, else each of them
-          // reads the sentinel and every step pays a round trip in
-          // wait_value.  The gate value (the face value of step
-          // w + 2kF - 1) was loaded kF steps ago and is only re-polled if it
-          // was not final then.
-          if (bnd_x && is_sentinel(gx))
-            wait_value(at(S.out, face_at(true, dstep_x, w + 2 * kF - 1 - off)), S.abort_flag);
-          if (bnd_y && is_sentinel(gy))
-            wait_value(at(S.out, face_at(true, dstep_y, w + 2 * kF - 1 - off)), S.abort_flag);
-          gx = load_bnd(bnd_x, dstep_x, w + 3 * kF - 1 - off);
-          if (ywave) gy = load_bnd(bnd_y, dstep_y, w + 3 * kF - 1 - off);
-        }
         if (w < nsteps) {
           __syncthreads();
           const int kk = w - off;
           const bool active = kk >= 0 && kk < Bz;
           const In ci = win[s];
           const uint32_t code = active ? code_of(ci) : 0u;
-          const uint32_t idx = out_index(kk);
-          const V lx = lz[(w - 1) & 1][lj][lxi];
-          const V ly = lz[(w - 1) & 1][lyj][li];
-          V zx = li > 0 ? lx : (bnd_x ? wx[s % kF] : (V)0);
-          V zy = lj > 0 ? ly : (bnd_y ? wy[s % kF] : (V)0);
-          // a face value prefetched before its tile wrote it is re-read
-          // (only where the term is present)
-          const bool sx = bnd_x && (code & xbit) && is_sentinel(zx);
-          const bool sy = bnd_y && (code & ybit) && is_sentinel(zy);
-          if (__builtin_expect(sx || sy, 0)) {
-            if (sx) zx = wait_value(at(S.out, idx + (uint32_t)dstep_x), S.abort_flag);
-            if (sy) zy = wait_value(at(S.out, idx + (uint32_t)dstep_y), S.abort_flag);
-          }
+          const uint32_t idx = base + (uint32_t)(sgn * w);
+          const V zx = lz[(w - 1) & 1][lj + 1][li];
+          const V zy = lz[(w - 1) & 1][lj][li + 1];
           V z;
           if constexpr (KIND == K_FAC) {
             z = 0.0;
@@ -774,11 +784,9 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<V> S) {
           // slots), so no slot keeps the sentinel after the pass
           store_coherent(at(S.out, idx), z);
           zprev = z;
-          lz[w & 1][lj][li] = z;
+          lz[w & 1][lj + 1][li + 1] = z;
         }
-        win[s] = load_in<KIND, V>(S, in_index(w + kCh - off));
-        wx[s % kF] = load_bnd(bnd_x, dstep_x, w + kF - off);
-        if (ywave) wy[s % kF] = load_bnd(bnd_y, dstep_y, w + kF - off);
+        win[s] = load_in<KIND, V>(S, in_index(base, off, w + kCh - off));
       }
     }
     __syncthreads();
@@ -902,7 +910,7 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
   S.mdr = mdr;
   S.out = P.zf.p;
   S.ticket = tick + 3;
-  hipLaunchKernelGGL((k_sweep<K_FAC, double>), dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_FAC, double>), dim3(P.ntiles), dim3(kBlock), 0, st, S);
   WC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_backward_records, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
                      P.ncell, skew_of(P), P.scode.p, reinterpret_cast<const Cell*>(P.cf.p),
@@ -943,7 +951,7 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.out = zf;
   S.ticket = tick;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(kBlock), 0, st, S);
   WC_LAUNCH_CHECK();
   hipEvent_t e1 = timer ? timer->mark(st) : nullptr;
   S.cells = reinterpret_cast<const CellT<V>*>(cb);
@@ -951,7 +959,7 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.out = zb;
   S.ticket = tick + 1;
   hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(kThreads), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(kBlock), 0, st, S);
   WC_LAUNCH_CHECK();
   if (timer) {
     timer->add(KTimer::SWEEP_FWD, ndof, e0, e1);
