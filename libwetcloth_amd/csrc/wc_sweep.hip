@@ -9,12 +9,13 @@
 // neighbours through LDS.  Values on a tile's low faces come from the
 // upstream tiles (a-1, b) and (a, b-1), which run ahead.  The output array is
 // filled with a NaN sentinel before the sweep and every cell is written
-// exactly once with a single 64-bit device-coherent (agent-scope atomic)
-// store, so a consumer knows a value is final when it no longer reads the
-// sentinel: boundary values are prefetched kChunk steps ahead and re-read
-// only if they were not ready yet.  Tiles are claimed with a ticket counter
-// in (a+b, a) order, so a tile only waits for tiles already claimed by
-// running workgroups (no deadlock), and every wait is bounded.
+// exactly once with a single device-coherent (agent-scope atomic) store, so
+// a consumer knows a value is final when it no longer reads the sentinel.
+// A dedicated face wave per workgroup reads the upstream face values a batch
+// of steps ahead and re-reads only those that were not final yet.  Tiles are
+// claimed with a ticket counter in (a+b, a) order, so a tile only waits for
+// tiles already claimed by running workgroups (no deadlock), and every wait
+// is bounded.
 //
 // While these sweeps are in use the PCG vectors live in the same box layout
 // (cells that are not DOFs hold zero), so the sweeps read and write them in
@@ -647,24 +648,27 @@ __device__ inline double factor_cell(const SweepArgs<double>& S, const FacIn& f,
 //
 // A workgroup is four compute waves (one lane per pencil of the 16x16 tile)
 // and one face wave.  The compute lanes exchange each step's values through
-// the LDS array lz, whose row 0 and column 0 are a halo: the face wave
-// writes there the upstream tiles' values beside the tile's low x and y
-// faces, so a compute lane reads its two neighbours from lz unconditionally
-// and the compute waves never issue a device-coherent load (which would sit
-// in their in-order load counter ahead of the record prefetches).  The face
-// wave reads the face values kF steps ahead, and re-reads one that was not
-// final yet with a bounded wait before publishing it.
-constexpr int kFaceAhead = 8;  // face values read ahead by the face wave
+// the LDS array lz; the values beside the tile's low x and y faces, which
+// the upstream tiles produce, reach them through the LDS rings hx / hy,
+// filled by the face wave.  So the compute waves never issue a
+// device-coherent load (which would sit in their in-order load counter
+// ahead of the record prefetches, and takes a round trip to memory).  The
+// face wave works in batches of kFB steps: once per batch it takes the face
+// values of the next kFB steps, read one batch earlier (re-reading, with a
+// bounded wait, a value that was not final yet), publishes them into the
+// rings and reads the batch after.
+constexpr int kFB = 4;             // steps per face batch
+constexpr int kRing = 2 * kFB;     // ring slots (a power of two)
 constexpr int kBlock = kThreads + 64;
 
 template <int KIND, class V>
 __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
   constexpr bool BWD = KIND == K_BWD;
   constexpr int kCh = PassIn<KIND, V>::chunk;
-  constexpr int kF = kFaceAhead;
   using In = typename PassIn<KIND, V>::T;
-  // [step parity][1 + lj][1 + li]; row 0 / column 0 hold the upstream faces
-  __shared__ V lz[2][kT + 1][kT + 1];
+  __shared__ V lz[2][kT][kT + 1];
+  __shared__ V hx[kRing][kT];  // [step % kRing][lj]: upstream x face
+  __shared__ V hy[kRing][kT];  // [step % kRing][li]: upstream y face
   __shared__ int s_tile;
   // wave-uniform (scalar) branch: the face wave is wave 4
   const bool face_wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == kThreads / 64;
@@ -712,35 +716,45 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
       // the upstream value beside this pencil's cell of step w (own slot if
       // there is no upstream tile: read, never waited for, never used)
       auto face_addr = [&](int w) -> uint32_t { return in_index(base, off, w - off) + (uint32_t)dstep; };
-      auto publish = [&](int w, V v) {
-        // value for step w, in the parity of step w - 1; it is waited for
-        // only where the pencil's cell of step w exists (elsewhere the
-        // compute lane discards it and 0 is published)
-        const int kk = w - off;
-        const bool need = on && kk >= 0 && kk < Bz;
-        if (need && is_sentinel(v)) v = wait_value(at(S.out, face_addr(w)), S.abort_flag);
-        if (!need) v = (V)0;
-        if (pub) {
-          if (xf)
-            lz[(w - 1) & 1][lj + 1][0] = v;
-          else
-            lz[(w - 1) & 1][0][li + 1] = v;
+      V fr[kFB];
+      auto fetch = [&](int w0) {
+#pragma unroll
+        for (int q = 0; q < kFB; ++q) fr[q] = load_coherent(at(S.out, face_addr(w0 + q)));
+      };
+      // steps w0 .. w0 + kFB - 1; a value is waited for only where the
+      // pencil's cell of that step exists (elsewhere the compute lane
+      // discards it and 0 is published)
+      auto publish = [&](int w0) {
+#pragma unroll
+        for (int q = 0; q < kFB; ++q) {
+          const int w = w0 + q;
+          const int kk = w - off;
+          const bool need = on && kk >= 0 && kk < Bz;
+          V v = fr[q];
+          if (need && is_sentinel(v)) v = wait_value(at(S.out, face_addr(w)), S.abort_flag);
+          if (!need) v = (V)0;
+          if (pub) {
+            if (xf)
+              hx[w & (kRing - 1)][lj] = v;
+            else
+              hy[w & (kRing - 1)][li] = v;
+          }
         }
       };
-      V fr[kF];
-#pragma unroll
-      for (int q = 0; q < kF; ++q) fr[q] = load_coherent(at(S.out, face_addr(q)));
-      publish(0, fr[0]);
-      fr[0] = load_coherent(at(S.out, face_addr(kF)));
-      for (int w0 = 0; w0 < nsteps; w0 += kF) {
+      fetch(0);
+      publish(0);
+      fetch(kFB);
+      for (int w0 = 0; w0 < nsteps; w0 += kFB) {
 #pragma clang loop unroll(full)
-        for (int q = 0; q < kF; ++q) {
+        for (int q = 0; q < kFB; ++q) {
           const int w = w0 + q;
           if (w < nsteps) {
             __syncthreads();
-            if (w + 1 < nsteps) {
-              publish(w + 1, fr[(q + 1) % kF]);
-              fr[(q + 1) % kF] = load_coherent(at(S.out, face_addr(w + 1 + kF)));
+            // during the batch's last step: publish the next batch (read
+            // one batch ago) and read the one after
+            if (q == kFB - 1 && w + 1 < nsteps) {
+              publish(w + 1);
+              fetch(w + 1 + kFB);
             }
           }
         }
@@ -751,15 +765,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
     const int li = threadIdx.x % kT, lj = threadIdx.x / kT;
     const uint32_t base = base_of(li, lj);
     const int off = li + lj;  // this pencil starts at step off
+    const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
+    const int lyj = lj > 0 ? lj - 1 : 0;
     V zprev = 0;
     // Record prefetch ring: slot s holds the inputs of step w0 + s, read kCh
     // steps ahead.  The step loop is unrolled by kCh and a slot is refilled
     // only after its step has used it, so every load writes the registers it
     // is consumed from and no wait on a load in flight is needed before its
-    // own step.
+    // own step.  (The scheduling barriers keep the loads in consumption
+    // order: the counter wait before a slot's first use then leaves the
+    // loads of the later slots in flight.)
     In win[kCh];
 #pragma unroll
-    for (int s = 0; s < kCh; ++s) win[s] = load_in<KIND, V>(S, in_index(base, off, s - off));
+    for (int s = 0; s < kCh; ++s) {
+      win[s] = load_in<KIND, V>(S, in_index(base, off, s - off));
+      __builtin_amdgcn_sched_barrier(0);
+    }
     for (int w0 = 0; w0 < nsteps; w0 += kCh) {
 #pragma clang loop unroll(full)
       for (int s = 0; s < kCh; ++s) {
@@ -771,8 +792,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
           const In ci = win[s];
           const uint32_t code = active ? code_of(ci) : 0u;
           const uint32_t idx = base + (uint32_t)(sgn * w);
-          const V zx = lz[(w - 1) & 1][lj + 1][li];
-          const V zy = lz[(w - 1) & 1][lj][li + 1];
+          const V lx = lz[(w - 1) & 1][lj][lxi];
+          const V ly = lz[(w - 1) & 1][lyj][li];
+          const V fx = hx[w & (kRing - 1)][lj];
+          const V fy = hy[w & (kRing - 1)][li];
+          const V zx = li > 0 ? lx : fx;
+          const V zy = lj > 0 ? ly : fy;
           V z;
           if constexpr (KIND == K_FAC) {
             z = 0.0;
@@ -784,9 +809,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
           // slots), so no slot keeps the sentinel after the pass
           store_coherent(at(S.out, idx), z);
           zprev = z;
-          lz[w & 1][lj + 1][li + 1] = z;
+          lz[w & 1][lj][li] = z;
         }
         win[s] = load_in<KIND, V>(S, in_index(base, off, w + kCh - off));
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     __syncthreads();
