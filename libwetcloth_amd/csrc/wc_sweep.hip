@@ -576,22 +576,18 @@ __device__ __attribute__((always_inline)) inline V wait_value(const V* p, int* a
 }
 
 // solve_lower / solve_lower_transpose_in_place for one cell: the rhs minus
-// the present neighbour terms in ascending DOF order, times invdiag.
-// Absent terms subtract +0, an exact identity; cells that are not DOFs
-// carry rhs = inv = 0 and no terms, so their value is 0.
-template <int KIND, class V>
-__device__ inline V sweep_cell(const CellIn<V>& ci, uint32_t code, V zx, V zy, V zprev) {
-  const int pres = KIND == K_BWD ? S_BPRES : S_FPRES;
-  const int src = KIND == K_BWD ? S_BSRC : S_FSRC;
+// the present neighbour terms in ascending DOF order, times invdiag.  v_q is
+// the neighbour value of term q in that order, or +0 for an absent term,
+// whose factor entry is +0 as well (records are zeroed and only present
+// terms are written), so an absent term subtracts +0 * +0 = +0, an exact
+// identity.  Cells that are not DOFs carry rhs = inv = 0 and no terms, so
+// their value is 0.
+template <class V>
+__device__ inline V sweep_cell_ordered(const CellIn<V>& ci, V v0, V v1, V v2) {
   V acc = ci.rhs;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const uint32_t a = (code >> (src + 2 * q)) & 3u;
-    const V zn = a == 0 ? zx : (a == 1 ? zy : zprev);
-    const V L = q == 0 ? ci.c.l0 : (q == 1 ? ci.c.l1 : ci.c.l2);
-    const V term = (code & (1u << (pres + q))) ? L * zn : (V)0;
-    acc -= term;
-  }
+  acc -= ci.c.l0 * v0;
+  acc -= ci.c.l1 * v1;
+  acc -= ci.c.l2 * v2;
   return acc * ci.c.inv;
 }
 
@@ -666,10 +662,20 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
   constexpr bool BWD = KIND == K_BWD;
   constexpr int kCh = PassIn<KIND, V>::chunk;
   using In = typename PassIn<KIND, V>::T;
-  __shared__ V lz[2][kT][kT + 1];
-  __shared__ V hx[kRing][kT];  // [step % kRing][lj]: upstream x face
-  __shared__ V hy[kRing][kT];  // [step % kRing][li]: upstream y face
+  // One LDS array, indexed by element: the step-parity exchange planes
+  // [parity][lj][li] (row stride kT + 1), the face rings [step % kRing][16]
+  // for the upstream x face (indexed by lj) and y face (by li), and one +0,
+  // the operand of an absent term.
+  constexpr uint32_t kLz = 0, kHx = 2 * kT * (kT + 1), kHy = kHx + kRing * kT,
+                     kZero = kHy + kRing * kT;
+  __shared__ V lds[kZero + 1];
+  auto LZ = [](int par, int lj, int li) -> uint32_t {
+    return kLz + (uint32_t)((par * kT + lj) * (kT + 1) + li);
+  };
+  auto HX = [](int w, int lj) -> uint32_t { return kHx + (uint32_t)((w & (kRing - 1)) * kT + lj); };
+  auto HY = [](int w, int li) -> uint32_t { return kHy + (uint32_t)((w & (kRing - 1)) * kT + li); };
   __shared__ int s_tile;
+  if (threadIdx.x == 0) lds[kZero] = (V)0;
   // wave-uniform (scalar) branch: the face wave is wave 4
   const bool face_wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == kThreads / 64;
   const int Bz = S.K.Bz;
@@ -735,9 +741,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
           if (!need) v = (V)0;
           if (pub) {
             if (xf)
-              hx[w & (kRing - 1)][lj] = v;
+              lds[HX(w, lj)] = v;
             else
-              hy[w & (kRing - 1)][li] = v;
+              lds[HY(w, li)] = v;
           }
         }
       };
@@ -786,30 +792,53 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
       for (int s = 0; s < kCh; ++s) {
         const int w = w0 + s;
         if (w < nsteps) {
-          __syncthreads();
           const int kk = w - off;
           const bool active = kk >= 0 && kk < Bz;
           const In ci = win[s];
           const uint32_t code = active ? code_of(ci) : 0u;
           const uint32_t idx = base + (uint32_t)(sgn * w);
-          const V lx = lz[(w - 1) & 1][lj][lxi];
-          const V ly = lz[(w - 1) & 1][lyj][li];
-          const V fx = hx[w & (kRing - 1)][lj];
-          const V fy = hy[w & (kRing - 1)][li];
-          const V zx = li > 0 ? lx : fx;
-          const V zy = lj > 0 ? ly : fy;
+          // the LDS operand of each ordered term (before the barrier, off
+          // the step's critical path): the -x / -y neighbour's value of the
+          // previous step (an upstream face value at the tile's low faces),
+          // this pencil's previous value, or +0 for an absent term
+          uint32_t src[3] = {kZero, kZero, kZero};
+          if constexpr (KIND != K_FAC) {
+            const int pres = BWD ? S_BPRES : S_FPRES;
+            const int srcb = BWD ? S_BSRC : S_FSRC;
+            const uint32_t ax = li > 0 ? LZ((w - 1) & 1, lj, lxi) : HX(w, lj);
+            const uint32_t ay = lj > 0 ? LZ((w - 1) & 1, lyj, li) : HY(w, li);
+            const uint32_t az = LZ((w - 1) & 1, lj, li);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const uint32_t a = (code >> (srcb + 2 * q)) & 3u;
+              const uint32_t p = a == 0 ? ax : (a == 1 ? ay : az);
+              src[q] = (code & (1u << (pres + q))) ? p : kZero;
+              // materialized here, not sunk past the barrier
+              asm volatile("" : "+v"(src[q]));
+            }
+          }
+          __syncthreads();
           V z;
           if constexpr (KIND == K_FAC) {
+            const V lx = lds[LZ((w - 1) & 1, lj, lxi)];
+            const V ly = lds[LZ((w - 1) & 1, lyj, li)];
+            const V fx = lds[HX(w, lj)];
+            const V fy = lds[HY(w, li)];
+            const V zx = li > 0 ? lx : fx;
+            const V zy = lj > 0 ? ly : fy;
             z = 0.0;
             if (code & S_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx);
           } else {
-            z = active ? sweep_cell<KIND, V>(ci, code, zx, zy, zprev) : (V)0;
+            // each term reads its neighbour value straight from LDS at the
+            // address chosen before the barrier
+            const V v0 = lds[src[0]], v1 = lds[src[1]], v2 = lds[src[2]];
+            z = active ? sweep_cell_ordered<V>(ci, v0, v1, v2) : (V)0;
           }
           // every step writes the lane's slot of that step (0 on the padding
           // slots), so no slot keeps the sentinel after the pass
           store_coherent(at(S.out, idx), z);
           zprev = z;
-          lz[w & 1][lj][li] = z;
+          lds[LZ(w & 1, lj, li)] = z;
         }
         win[s] = load_in<KIND, V>(S, in_index(base, off, w + kCh - off));
         __builtin_amdgcn_sched_barrier(0);
