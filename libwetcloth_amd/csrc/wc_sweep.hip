@@ -655,13 +655,45 @@ __device__ inline double factor_cell(const SweepArgs<double>& S, const FacIn& f,
 // rings and reads the batch after.
 constexpr int kFB = 4;             // steps per face batch
 constexpr int kRing = 2 * kFB;     // ring slots (a power of two)
-constexpr int kBlock = kThreads + 64;
+
+// The forward and backward sweeps stage each step's records (factor
+// entries, rhs, code) through an LDS ring filled by two loader waves with
+// direct global-to-LDS loads (one wave-wide 16-byte load copies 1 KB), kAhead
+// steps ahead.  The compute waves read a step's records from LDS one step
+// early, so their only global-memory operations are the output stores, which
+// nothing waits for; the loaders' in-order load counters hold nothing but
+// their own ring loads, and each waits for exactly the step it publishes.
+template <class V>
+struct ApplyRing {
+  static constexpr int kAhead = sizeof(V) == 8 ? 7 : 14;   // steps loaded ahead
+  static constexpr int kSlots = sizeof(V) == 8 ? 8 : 16;   // > kAhead, a power of two
+  static constexpr int kCellOps = kThreads * (int)sizeof(CellT<V>) / 1024;
+  static constexpr int kRhsOps = kThreads * (int)sizeof(V) / 1024;
+  static constexpr int kCodeOps = kThreads * 4 / 1024;
+  static_assert(kCellOps * kAhead <= 63 && (kRhsOps + kCodeOps) * kAhead <= 63,
+                "a loader's loads in flight fit its 6-bit load counter");
+};
+template <int KIND>
+struct SweepBlock {
+  static constexpr int value = KIND == K_FAC ? kThreads + 64 : kThreads + 192;
+};
+
+// s_waitcnt with only the vector-memory count set (expcnt and lgkmcnt at
+// their maxima, so they are not waited for); gfx9 encoding
+template <int N>
+__device__ inline void wait_vmcnt() {
+  static_assert(N >= 0 && N <= 63, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
 
 template <int KIND, class V>
-__global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
+__global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> S) {
   constexpr bool BWD = KIND == K_BWD;
+  constexpr bool FAC = KIND == K_FAC;
   constexpr int kCh = PassIn<KIND, V>::chunk;
   using In = typename PassIn<KIND, V>::T;
+  using Ring = ApplyRing<V>;
+  constexpr int kR = FAC ? 1 : Ring::kSlots;
   // One LDS array, indexed by element: the step-parity exchange planes
   // [parity][lj][li] (row stride kT + 1), the face rings [step % kRing][16]
   // for the upstream x face (indexed by lj) and y face (by li), and one +0,
@@ -669,6 +701,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
   constexpr uint32_t kLz = 0, kHx = 2 * kT * (kT + 1), kHy = kHx + kRing * kT,
                      kZero = kHy + kRing * kT;
   __shared__ V lds[kZero + 1];
+  // record rings of the sweeps: [step % kR][position of the slot in the step]
+  __shared__ CellT<V> rc[kR][kThreads];
+  __shared__ V rr[kR][kThreads];
+  __shared__ uint32_t rk[kR][kThreads];
   auto LZ = [](int par, int lj, int li) -> uint32_t {
     return kLz + (uint32_t)((par * kT + lj) * (kT + 1) + li);
   };
@@ -676,8 +712,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
   auto HY = [](int w, int li) -> uint32_t { return kHy + (uint32_t)((w & (kRing - 1)) * kT + li); };
   __shared__ int s_tile;
   if (threadIdx.x == 0) lds[kZero] = (V)0;
-  // wave-uniform (scalar) branch: the face wave is wave 4
-  const bool face_wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) == kThreads / 64;
+  // wave-uniform (scalar) branches: waves 0-3 compute, 4 reads the faces,
+  // 5 and 6 (sweeps only) load the record rings
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const bool face_wave = wave == kThreads / 64;
   const int Bz = S.K.Bz;
   const int nsteps = S.K.nsteps;
   for (;;) {
@@ -707,6 +745,63 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
       const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
       return base + (uint32_t)(sgn * (kc + off));
     };
+    if constexpr (!FAC) {
+      if (wave > kThreads / 64) {
+        // loader waves.  The 256 slots of step w are consecutive, starting
+        // at the slot of physical lane 0; steps past the end reload the
+        // last step (their ring slots are never read), so every loader
+        // issues the same number of loads for every step and its counter
+        // wait stays exact.
+        const bool cells = wave == kThreads / 64 + 1;
+        const int lane = threadIdx.x & 63;
+        auto block = [&](int w) -> uint32_t {
+          const int wc = w < nsteps ? w : nsteps - 1;
+          return (uint32_t)skew_slot(S.K, a, b, 0, 0, BWD ? nsteps - 1 - wc : wc);
+        };
+        auto issue = [&](int w) {
+          const uint32_t s0 = block(w);
+          const int r = w & (kR - 1);
+          if (cells) {
+            const char* g = reinterpret_cast<const char*>(S.cells) + (size_t)s0 * sizeof(CellT<V>);
+#pragma unroll
+            for (int k = 0; k < Ring::kCellOps; ++k)
+              __builtin_amdgcn_global_load_lds(
+                  g + k * 1024 + lane * 16,
+                  (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(&rc[r][0]) + k * 1024),
+                  16, 0, 0);
+          } else {
+            const char* g = reinterpret_cast<const char*>(S.in) + (size_t)s0 * sizeof(V);
+#pragma unroll
+            for (int k = 0; k < Ring::kRhsOps; ++k)
+              __builtin_amdgcn_global_load_lds(
+                  g + k * 1024 + lane * 16,
+                  (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(&rr[r][0]) + k * 1024),
+                  16, 0, 0);
+            const char* gk = reinterpret_cast<const char*>(S.code) + (size_t)s0 * 4;
+#pragma unroll
+            for (int k = 0; k < Ring::kCodeOps; ++k)
+              __builtin_amdgcn_global_load_lds(
+                  gk + k * 1024 + lane * 16,
+                  (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(&rk[r][0]) + k * 1024),
+                  16, 0, 0);
+          }
+        };
+        for (int t = 0; t < Ring::kAhead; ++t) issue(t);
+        for (int w = 0; w < nsteps; ++w) {
+          // steps w and w + 1 complete: the loads of the kAhead - 2 later
+          // steps may still be in flight
+          if (cells)
+            wait_vmcnt<Ring::kCellOps * (Ring::kAhead - 2)>();
+          else
+            wait_vmcnt<(Ring::kRhsOps + Ring::kCodeOps) * (Ring::kAhead - 2)>();
+          __builtin_amdgcn_s_barrier();
+          issue(w + Ring::kAhead);
+        }
+        wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+        continue;
+      }
+    }
     if (face_wave) {
       // lanes 0..15: x face of local row lj = lane (the pencil (0, lj));
       // lanes 16..31: y face of column li = lane - 16 (the pencil (li, 0));
@@ -773,14 +868,62 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
     const int off = li + lj;  // this pencil starts at step off
     const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
     const int lyj = lj > 0 ? lj - 1 : 0;
+    if constexpr (!FAC) {
+      // The step's records come from the ring, read one step early together
+      // with the LDS operand of each ordered term: the -x / -y neighbour's
+      // value of the previous step (an upstream face value at the tile's
+      // low faces), this pencil's previous value, or +0 for an absent term.
+      const int pos = BWD ? kThreads - 1 - (int)threadIdx.x : (int)threadIdx.x;
+      const int pres = BWD ? S_BPRES : S_FPRES;
+      const int srcb = BWD ? S_BSRC : S_FSRC;
+      CellT<V> c;
+      V rhs;
+      bool active;
+      uint32_t src[3];
+      auto take = [&](int w) {
+        const int r = w & (kR - 1);
+        c = rc[r][pos];
+        rhs = rr[r][pos];
+        const int kk = w - off;
+        active = kk >= 0 && kk < Bz;
+        const uint32_t code = active ? rk[r][pos] : 0u;
+        const uint32_t ax = li > 0 ? LZ((w - 1) & 1, lj, lxi) : HX(w, lj);
+        const uint32_t ay = lj > 0 ? LZ((w - 1) & 1, lyj, li) : HY(w, li);
+        const uint32_t az = LZ((w - 1) & 1, lj, li);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const uint32_t t = (code >> (srcb + 2 * q)) & 3u;
+          const uint32_t p = t == 0 ? ax : (t == 1 ? ay : az);
+          src[q] = (code & (1u << (pres + q))) ? p : kZero;
+        }
+      };
+      for (int w = 0; w < nsteps; ++w) {
+        __syncthreads();  // ring slots of steps w and w + 1 are complete
+        if (w == 0) take(0);
+        V z = (V)0;
+        if (active) {
+          const V v0 = lds[src[0]], v1 = lds[src[1]], v2 = lds[src[2]];
+          CellIn<V> ci;
+          ci.c = c;
+          ci.rhs = rhs;
+          z = sweep_cell_ordered<V>(ci, v0, v1, v2);
+        }
+        // every step writes the lane's slot of that step (0 on the padding
+        // slots), so no slot keeps the sentinel after the pass
+        store_coherent(at(S.out, base + (uint32_t)(sgn * w)), z);
+        lds[LZ(w & 1, lj, li)] = z;
+        if (w + 1 < nsteps) take(w + 1);
+      }
+      __syncthreads();
+    } else {
     V zprev = 0;
-    // Record prefetch ring: slot s holds the inputs of step w0 + s, read kCh
-    // steps ahead.  The step loop is unrolled by kCh and a slot is refilled
-    // only after its step has used it, so every load writes the registers it
-    // is consumed from and no wait on a load in flight is needed before its
-    // own step.  (The scheduling barriers keep the loads in consumption
-    // order: the counter wait before a slot's first use then leaves the
-    // loads of the later slots in flight.)
+    // Record prefetch ring of the factor pass: slot s holds the inputs of
+    // step w0 + s, read kCh steps ahead.  The step loop is unrolled by kCh
+    // and a slot is refilled only after its step has used it, so every load
+    // writes the registers it is consumed from and no wait on a load in
+    // flight is needed before its own step.  (The scheduling barriers keep
+    // the loads in consumption order: the counter wait before a slot's first
+    // use then leaves the loads of the later slots in flight.)
     In win[kCh];
 #pragma unroll
     for (int s = 0; s < kCh; ++s) {
@@ -797,45 +940,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
           const In ci = win[s];
           const uint32_t code = active ? code_of(ci) : 0u;
           const uint32_t idx = base + (uint32_t)(sgn * w);
-          // the LDS operand of each ordered term (before the barrier, off
-          // the step's critical path): the -x / -y neighbour's value of the
-          // previous step (an upstream face value at the tile's low faces),
-          // this pencil's previous value, or +0 for an absent term
-          uint32_t src[3] = {kZero, kZero, kZero};
-          if constexpr (KIND != K_FAC) {
-            const int pres = BWD ? S_BPRES : S_FPRES;
-            const int srcb = BWD ? S_BSRC : S_FSRC;
-            const uint32_t ax = li > 0 ? LZ((w - 1) & 1, lj, lxi) : HX(w, lj);
-            const uint32_t ay = lj > 0 ? LZ((w - 1) & 1, lyj, li) : HY(w, li);
-            const uint32_t az = LZ((w - 1) & 1, lj, li);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-              const uint32_t a = (code >> (srcb + 2 * q)) & 3u;
-              const uint32_t p = a == 0 ? ax : (a == 1 ? ay : az);
-              src[q] = (code & (1u << (pres + q))) ? p : kZero;
-              // materialized here, not sunk past the barrier
-              asm volatile("" : "+v"(src[q]));
-            }
-          }
           __syncthreads();
-          V z;
-          if constexpr (KIND == K_FAC) {
-            const V lx = lds[LZ((w - 1) & 1, lj, lxi)];
-            const V ly = lds[LZ((w - 1) & 1, lyj, li)];
-            const V fx = lds[HX(w, lj)];
-            const V fy = lds[HY(w, li)];
-            const V zx = li > 0 ? lx : fx;
-            const V zy = lj > 0 ? ly : fy;
-            z = 0.0;
-            if (code & S_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx);
-          } else {
-            // each term reads its neighbour value straight from LDS at the
-            // address chosen before the barrier
-            const V v0 = lds[src[0]], v1 = lds[src[1]], v2 = lds[src[2]];
-            z = active ? sweep_cell_ordered<V>(ci, v0, v1, v2) : (V)0;
-          }
-          // every step writes the lane's slot of that step (0 on the padding
-          // slots), so no slot keeps the sentinel after the pass
+          const V lx = lds[LZ((w - 1) & 1, lj, lxi)];
+          const V ly = lds[LZ((w - 1) & 1, lyj, li)];
+          const V fx = lds[HX(w, lj)];
+          const V fy = lds[HY(w, li)];
+          const V zx = li > 0 ? lx : fx;
+          const V zy = lj > 0 ? ly : fy;
+          V z = 0.0;
+          if (code & S_DOF) z = factor_cell(S, ci, code, zx, zy, zprev, idx);
           store_coherent(at(S.out, idx), z);
           zprev = z;
           lds[LZ(w & 1, lj, li)] = z;
@@ -845,6 +958,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<V> S) {
       }
     }
     __syncthreads();
+    }
   }
 }
 
@@ -965,7 +1079,7 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
   S.mdr = mdr;
   S.out = P.zf.p;
   S.ticket = tick + 3;
-  hipLaunchKernelGGL((k_sweep<K_FAC, double>), dim3(P.ntiles), dim3(kBlock), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_FAC, double>), dim3(P.ntiles), dim3(SweepBlock<K_FAC>::value), 0, st, S);
   WC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_backward_records, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
                      P.ncell, skew_of(P), P.scode.p, reinterpret_cast<const Cell*>(P.cf.p),
@@ -1006,7 +1120,7 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.out = zf;
   S.ticket = tick;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(kBlock), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_FWD>::value), 0, st, S);
   WC_LAUNCH_CHECK();
   hipEvent_t e1 = timer ? timer->mark(st) : nullptr;
   S.cells = reinterpret_cast<const CellT<V>*>(cb);
@@ -1014,7 +1128,7 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.out = zb;
   S.ticket = tick + 1;
   hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(kBlock), 0, st, S);
+  hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_BWD>::value), 0, st, S);
   WC_LAUNCH_CHECK();
   if (timer) {
     timer->add(KTimer::SWEEP_FWD, ndof, e0, e1);
