@@ -245,7 +245,12 @@ void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y);
 void sweep_narrow(hipStream_t st, SweepPlan& P);
 void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
                    int64_t ndof);
-void box_spmv32(hipStream_t st, const SweepPlan& P, const float* x, float* y);
+// y = A x with the block partials of x . y (nblocks blocks, the solver's
+// reduction layout)
+void box_spmv_dot(hipStream_t st, const SweepPlan& P, const double* x, double* y,
+                  double* partial, int nblocks);
+void box_spmv32_dot(hipStream_t st, const SweepPlan& P, const float* x, float* y,
+                    double* partial, int nblocks);
 void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const double* x,
                   float* r, double* partial, int nblocks);
 void box_gather(hipStream_t st, const SweepPlan& P, const double* dof, double* box);

@@ -625,10 +625,8 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   int it;
   for (it = 0; it < maxit; ++it) {
     hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
-    box_spmv(st, P, s, z);
+    box_spmv_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
     if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
-    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z, nc,
-                       w.partial.p);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
                        w.scal.p, (int)R_ALPHA);
     hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(kRedThreads), 0, st, x, r, s, z, nc,
@@ -729,10 +727,8 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     WC_HIP(hipMemcpyAsync(s, z, sizeof(float) * nc, hipMemcpyDeviceToDevice, st));
     while (total < maxit) {
       hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
-      box_spmv32(st, P, s, z);
+      box_spmv32_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
       if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
-      hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z, nc,
-                         w.partial.p);
       hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
                          w.scal.p, (int)R_ALPHA);
       hipLaunchKernelGGL(k_update_xr_mixed, dim3(nb), dim3(kRedThreads), 0, st, x, r, s, z,

@@ -170,7 +170,7 @@ struct BoxArgs {
   uint32_t* scode;  // sweep code (S_*)
   Cell* cf;      // forward sweep (lower terms), zeroed here, filled by the factor pass
   Cell* cb;      // backward sweep (upper terms), likewise
-  double* coef;  // kCoef entries of A per cell
+  double* coef;  // kCoef planes of A entries: coef[e * nslot + cell]
   FacIn* fin;    // factor pass inputs
 };
 
@@ -347,7 +347,7 @@ __global__ void k_fill_box(BoxArgs A) {
   const Cell zero = {0, 0, 0, 0};
   A.cf[t] = zero;
   A.cb[t] = zero;
-  for (int e = 0; e < kCoef; ++e) A.coef[kCoef * t + e] = cf[e];
+  for (int e = 0; e < kCoef; ++e) A.coef[e * A.nslot + t] = cf[e];
 }
 
 // DOF-ordered vector -> box (zero where the cell is not a DOF)
@@ -388,50 +388,97 @@ __device__ inline Nbr neighbours(const Skew& K, int64_t t) {
 
 // Row t of A x on box vectors: multiply(FixedSparseMatrix, x, y)
 // (SparseMatrix.h:298-322) with the row's entries added in ascending column
-// order, in FP64 whatever the storage type of A (CV) and x (V).  A present
-// entry always couples two DOFs, so every x read stays inside the box.
+// order, in FP64 whatever the storage type of A (CV) and x (V).  The entries
+// are stored in kCoef planes of nslot values, and an absent entry (and every
+// entry of a cell that is not a DOF) is stored as +0.  So the row adds all
+// seven products in order: an absent entry's product is +-0, and adding it
+// is an exact identity (a sum that starts at +0 never becomes -0).  Every
+// load is then independent of the code word, which only orders the terms;
+// the x index of an absent neighbour is clamped into the box.  Slots are
+// addressed with 32-bit indices (every plan holds fewer than 2^32 slots).
 template <class CV, class V>
-__device__ inline double box_row(const Skew& K, int64_t t, uint32_t c, const CV* coef,
-                                 const V* x) {
+__device__ inline double box_row(const Skew& K, uint32_t t, uint32_t c,
+                                 const CV* __restrict__ coef, uint32_t nslot,
+                                 const V* __restrict__ x) {
+  const uint32_t lane = t & (kThreads - 1);
+  const uint32_t li = lane & (kT - 1), lj = lane / kT;
+  const uint32_t dx = (uint32_t)(int32_t)K.dx, dy = (uint32_t)(int32_t)K.dy;
+  const uint32_t last = nslot - 1;
+  auto clampn = [&](uint32_t n) { return n < last ? n : last; };
+  const uint32_t n_lo[3] = {clampn(li > 0 ? t - kThreads - 1 : t + dx),
+                            clampn(lj > 0 ? t - kThreads - kT : t + dy),
+                            clampn(t - kThreads)};
+  const uint32_t n_up[3] = {clampn(li < kT - 1 ? t + kThreads + 1 : t - dx),
+                            clampn(lj < kT - 1 ? t + kThreads + kT : t - dy),
+                            clampn(t + kThreads)};
+  double a[kCoef], xv[kCoef];
+#pragma unroll
+  for (int e = 0; e < kCoef; ++e) a[e] = (double)coef[(uint32_t)e * nslot + t];
+  xv[0] = (double)x[t];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    xv[1 + d] = (double)x[n_lo[d]];
+    xv[4 + d] = (double)x[n_up[d]];
+  }
+  const int lor = (int)((c >> C_LORD) & 7), upr = (int)((c >> C_UORD) & 7);
   double acc = 0.0;
-  if (c & C_DOF) {
-    const CV* a = coef + kCoef * t;
-    const Nbr n = neighbours(K, t);
-    const int lo = (c >> C_LORD) & 7, up = (c >> C_UORD) & 7;
-    for (int q = 0; q < 3; ++q) {
-      const int ax = perm_axis(lo, q);
-      if (c & (1u << (C_ALO + ax))) acc += (double)a[1 + ax] * (double)x[n.lo[ax]];
-    }
-    if (c & C_DIAG) acc += (double)a[0] * (double)x[t];
-    for (int q = 0; q < 3; ++q) {
-      const int ax = perm_axis(up, q);
-      if (c & (1u << (C_AUP + ax))) acc += (double)a[4 + ax] * (double)x[n.up[ax]];
-    }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(lor, q);
+    const double ca = ax == 0 ? a[1] : (ax == 1 ? a[2] : a[3]);
+    const double cx = ax == 0 ? xv[1] : (ax == 1 ? xv[2] : xv[3]);
+    acc += ca * cx;
+  }
+  acc += a[0] * xv[0];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(upr, q);
+    const double ca = ax == 0 ? a[4] : (ax == 1 ? a[5] : a[6]);
+    const double cx = ax == 0 ? xv[4] : (ax == 1 ? xv[5] : xv[6]);
+    acc += ca * cx;
   }
   return acc;
 }
 
-// y = A x (FP64, or FP32 storage with FP64 row sums)
-template <class V>
-__global__ void k_box_spmv(int64_t nslot, Skew K, const uint32_t* code, const V* coef,
-                           const V* x, V* y) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nslot) return;
-  y[t] = (V)box_row(K, t, code[t], coef, x);
+// y = A x (FP64, or FP32 storage with FP64 row sums).  With DOT, also the
+// FP64 block partials of x . y, with the thread-to-slot mapping and the
+// reduction tree of the solver's dot-product kernel, so the sum is the one
+// that kernel would compute from the stored y.
+template <class V, bool DOT>
+__global__ __launch_bounds__(256) void k_box_spmv(uint32_t nslot, Skew K,
+                                                  const uint32_t* __restrict__ code,
+                                                  const V* __restrict__ coef,
+                                                  const V* __restrict__ x, V* __restrict__ y,
+                                                  double* partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < nslot; t += gridDim.x * 256u) {
+    const V v = (V)box_row(K, t, code[t], coef, nslot, x);
+    y[t] = v;
+    if (DOT) acc += (double)x[t] * (double)v;
+  }
+  if (DOT) {
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+  }
 }
 
 // The refinement residual of the mixed-precision solve, r = b - A x in FP64
 // (FP64 A, b and x), stored rounded to FP32, with |r|_inf block partials.
-__global__ __launch_bounds__(256) void k_box_residual(int64_t nslot, Skew K,
+__global__ __launch_bounds__(256) void k_box_residual(uint32_t nslot, Skew K,
                                                       const uint32_t* code,
                                                       const double* coef, const double* b,
                                                       const double* x, float* r,
                                                       double* partial) {
   __shared__ double sh[256];
   double m = 0.0;
-  for (int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x; t < nslot;
-       t += (int64_t)gridDim.x * 256) {
-    const double v = b[t] - box_row(K, t, code[t], coef, x);
+  for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < nslot; t += gridDim.x * 256u) {
+    const double v = b[t] - box_row(K, t, code[t], coef, nslot, x);
     r[t] = (float)v;
     m = fmax(m, fabs(v));
   }
@@ -653,7 +700,7 @@ __device__ inline double factor_cell(const SweepArgs<double>& S, const FacIn& f,
 // values of the next kFB steps, read one batch earlier (re-reading, with a
 // bounded wait, a value that was not final yet), publishes them into the
 // rings and reads the batch after.
-constexpr int kFB = 4;             // steps per face batch
+constexpr int kFB = 2;             // steps per face batch
 constexpr int kRing = 2 * kFB;     // ring slots (a power of two)
 
 // The forward and backward sweeps stage each step's records (factor
@@ -1146,21 +1193,33 @@ void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
   apply_impl<float>(st, P, r, P.cf32.p, P.cb32.p, P.zf32.p, P.zb32.p, timer, ndof);
 }
 
+static unsigned spmv_blocks(const SweepPlan& P) {
+  return (unsigned)std::min<int64_t>(1024, blocks(P.ncell, 256));
+}
+
 void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y) {
-  hipLaunchKernelGGL(k_box_spmv<double>, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
-                     P.ncell, skew_of(P), P.code.p, P.coef.p, x, y);
+  hipLaunchKernelGGL((k_box_spmv<double, false>), dim3(spmv_blocks(P)), dim3(256), 0, st,
+                     (uint32_t)P.ncell, skew_of(P), P.code.p, P.coef.p, x, y, nullptr);
   WC_LAUNCH_CHECK();
 }
 
-void box_spmv32(hipStream_t st, const SweepPlan& P, const float* x, float* y) {
-  hipLaunchKernelGGL(k_box_spmv<float>, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
-                     P.ncell, skew_of(P), P.code.p, P.coef32.p, x, y);
+void box_spmv_dot(hipStream_t st, const SweepPlan& P, const double* x, double* y,
+                  double* partial, int nblocks) {
+  hipLaunchKernelGGL((k_box_spmv<double, true>), dim3(nblocks), dim3(256), 0, st,
+                     (uint32_t)P.ncell, skew_of(P), P.code.p, P.coef.p, x, y, partial);
+  WC_LAUNCH_CHECK();
+}
+
+void box_spmv32_dot(hipStream_t st, const SweepPlan& P, const float* x, float* y,
+                    double* partial, int nblocks) {
+  hipLaunchKernelGGL((k_box_spmv<float, true>), dim3(nblocks), dim3(256), 0, st,
+                     (uint32_t)P.ncell, skew_of(P), P.code.p, P.coef32.p, x, y, partial);
   WC_LAUNCH_CHECK();
 }
 
 void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const double* x,
                   float* r, double* partial, int nblocks) {
-  hipLaunchKernelGGL(k_box_residual, dim3(nblocks), dim3(256), 0, st, P.ncell, skew_of(P),
+  hipLaunchKernelGGL(k_box_residual, dim3(nblocks), dim3(256), 0, st, (uint32_t)P.ncell, skew_of(P),
                      P.code.p, P.coef.p, b, x, r, partial);
   WC_LAUNCH_CHECK();
 }
