@@ -18,7 +18,7 @@
 namespace wc {
 namespace {
 
-constexpr int kP2GThreads = 256;
+constexpr int kP2GThreads = 256;  // the particle permutation in k_p2g assumes 256
 constexpr int kTile = 8;  // LDS nodes per axis: bucket (4) + halo 2 below, 2 above
 constexpr int kTileNodes = kTile * kTile * kTile;
 constexpr unsigned long long kPhiUntouched = 0xFFFFFFFFFFFFFFFFull;
@@ -33,28 +33,61 @@ __device__ inline int particle_bucket_axis(const Grid& g, int axis, R x) {
   return clampi(c, 0, g.nx[axis] - 1) >> 2;
 }
 
+// Runs of equal keys in consecutive lanes of a wave (the particles stay
+// nearly sorted from one substep to the next, so a wave's particles mostly
+// share a few buckets).  Each run's first lane claims space for the whole
+// run with one atomic; head is the run's first lane, len its length (valid in
+// the head lane), rank the lane's position in its run.
+struct WaveRun {
+  int head, len, rank;
+};
+__device__ inline WaveRun wave_run(uint32_t key) {
+  const int lane = (int)__lane_id();
+  const uint32_t prev = __shfl_up(key, 1u);
+  const bool is_head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(is_head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  WaveRun r;
+  r.head = 63 - __clzll(heads & upto);
+  const unsigned long long after = heads & ~upto;
+  const int next = after ? __ffsll(after) - 1 : 64;
+  r.len = next - lane;
+  r.rank = lane - r.head;
+  return r;
+}
+
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;  // lanes past the last particle
+
 template <class R>
-__global__ void k_sort_keys(Grid g, const R* x, int64_t n, int64_t cap,
-                            uint32_t* key, uint32_t* count) {
+__global__ __launch_bounds__(256) void k_sort_keys(Grid g, const R* x, int64_t n, int64_t cap,
+                                                   uint32_t* key, uint32_t* count) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int bi = particle_bucket_axis<R>(g, 0, x[p]);
-  const int bj = particle_bucket_axis<R>(g, 1, x[cap + p]);
-  const int bk = particle_bucket_axis<R>(g, 2, x[2 * cap + p]);
-  const uint32_t b = (uint32_t)bucket_index(g, bi, bj, bk);
-  key[p] = b;
-  atomicAdd(&count[b], 1u);
+  uint32_t b = kNoKey;
+  if (p < n) {
+    const int bi = particle_bucket_axis<R>(g, 0, x[p]);
+    const int bj = particle_bucket_axis<R>(g, 1, x[cap + p]);
+    const int bk = particle_bucket_axis<R>(g, 2, x[2 * cap + p]);
+    b = (uint32_t)bucket_index(g, bi, bj, bk);
+    key[p] = b;
+  }
+  const WaveRun r = wave_run(b);
+  if (b != kNoKey && r.rank == 0) atomicAdd(&count[b], (uint32_t)r.len);
 }
 
 template <class R>
-__global__ void k_sort_scatter(int64_t n, int64_t cap, const uint32_t* key,
-                               const uint32_t* start, uint32_t* cursor,
-                               const R* x, const R* v, const R* C, const uint32_t* pid,
-                               R* x2, R* v2, R* C2, uint32_t* pid2) {
+__global__ __launch_bounds__(256) void k_sort_scatter(int64_t n, int64_t cap, const uint32_t* key,
+                                                      const uint32_t* start, uint32_t* cursor,
+                                                      const R* x, const R* v, const R* C,
+                                                      const uint32_t* pid, R* x2, R* v2, R* C2,
+                                                      uint32_t* pid2) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t b = key[p];
-  const int64_t q = start[b] + atomicAdd(&cursor[b], 1u);
+  const uint32_t b = p < n ? key[p] : kNoKey;
+  const WaveRun r = wave_run(b);
+  uint32_t base = 0;
+  if (b != kNoKey && r.rank == 0) base = atomicAdd(&cursor[b], (uint32_t)r.len);
+  base = __shfl(base, r.head);
+  if (b == kNoKey) return;
+  const int64_t q = (int64_t)start[b] + base + r.rank;
   for (int d = 0; d < 3; ++d) {
     x2[d * cap + q] = x[d * cap + p];
     v2[d * cap + q] = v[d * cap + p];
@@ -115,7 +148,15 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   if (threadIdx.x < 27) stouch[threadIdx.x] = 0;
   __syncthreads();
 
-  for (uint32_t p = pbeg + threadIdx.x; p < pend; p += kP2GThreads) {
+  // The bucket's particles in chunks of 256; within a chunk thread t takes
+  // particle (t % 32) * 8 + t / 32 (a bijection of 0..255), so the lanes of a
+  // wave hold particles 8 apart: the bucket keeps its particles in their
+  // previous order, eight per cell when seeded, and lanes on different cells
+  // collide less on the same LDS nodes.
+  const uint32_t tperm = (threadIdx.x & 31) * 8 + (threadIdx.x >> 5);
+  for (uint32_t c0 = pbeg; c0 < pend; c0 += kP2GThreads) {
+    const uint32_t p = c0 + tperm;
+    if (p >= pend) continue;
     const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
     // u, v, w faces: TwoDScene.cpp:6700-6713
     for (int d = 0; d < 3; ++d) {
