@@ -99,6 +99,7 @@ struct SweepPlan {
   DevBuf<int32_t> pdof;             // box cell -> DOF or -1
   DevBuf<uint32_t> order, code;     // ticket order; per-cell coupling bits
   DevBuf<uint32_t> scode;           // per-cell sweep code (orders applied)
+  DevBuf<uint32_t> opf, opb;        // per-cell LDS operand offsets of each sweep
   DevBuf<double> cf, cb;            // 32-byte factor records (forward / backward)
   DevBuf<double> coef;              // 7 matrix entries per cell
   DevBuf<double> fin;               // 96-byte factor-pass inputs per cell

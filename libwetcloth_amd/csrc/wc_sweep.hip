@@ -168,6 +168,8 @@ struct BoxArgs {
   int32_t* pdof;
   uint32_t* code;   // per-axis coupling bits and orders (C_*)
   uint32_t* scode;  // sweep code (S_*)
+  uint32_t* opf;    // operand offsets of the forward / backward sweep
+  uint32_t* opb;
   Cell* cf;      // forward sweep (lower terms), zeroed here, filled by the factor pass
   Cell* cb;      // backward sweep (upper terms), likewise
   double* coef;  // kCoef planes of A entries: coef[e * nslot + cell]
@@ -221,6 +223,32 @@ __global__ __launch_bounds__(256) void k_dof_box(Grid g, const uint32_t* isdof, 
 // layout.  The row of a DOF holds its lower neighbours, its diagonal and its
 // upper neighbours; L(me, lower) sits in the lower neighbour's row and
 // L(upper, me) in this row (lv is indexed like val).
+// The sweeps' exchange planes: 4 planes (step % 4) of 17 x 17 values; cell
+// [1 + lj][1 + li] holds local pencil (li, lj), row 0 and column 0 hold the
+// upstream y and x faces, and [0][0] holds +0.
+constexpr int kPlane = (kT + 1) * (kT + 1);
+constexpr int kPlanes = 4;
+
+// One sweep's operands of a cell, for local pencil (li, lj) of that sweep:
+// per ordered term q, a signed 10-bit offset from the pencil's own position
+// in the previous step's plane to the term's operand: -1 for the -x
+// neighbour (or x face), -(kT + 1) for the -y neighbour (or y face), 0 for
+// the pencil's own previous value, and [0][0] (+0) for an absent term.
+__host__ __device__ inline uint32_t operand_word(uint32_t sc, int presb, int srcb, int li,
+                                                 int lj) {
+  const int pos = (lj + 1) * (kT + 1) + li + 1;
+  uint32_t word = 0;
+  for (int q = 0; q < 3; ++q) {
+    int d = -pos;
+    if (sc & (1u << (presb + q))) {
+      const uint32_t a = (sc >> (srcb + 2 * q)) & 3u;
+      d = a == 0 ? -1 : (a == 1 ? -(kT + 1) : 0);
+    }
+    word |= ((uint32_t)d & 0x3FFu) << (10 * q);
+  }
+  return word;
+}
+
 __global__ void k_fill_box(BoxArgs A) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= A.nslot) return;
@@ -341,6 +369,8 @@ __global__ void k_fill_box(BoxArgs A) {
   A.pdof[t] = me;
   A.code[t] = code;
   A.scode[t] = sc;
+  A.opf[t] = operand_word(sc, S_FPRES, S_FSRC, li, lj);
+  A.opb[t] = operand_word(sc, S_BPRES, S_BSRC, kT - 1 - li, kT - 1 - lj);
   fo.fcode = fco;
   fo.code = sc;
   A.fin[t] = fo;
@@ -745,9 +775,12 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
   // [parity][lj][li] (row stride kT + 1), the face rings [step % kRing][16]
   // for the upstream x face (indexed by lj) and y face (by li), and one +0,
   // the operand of an absent term.
+  // (The factor pass uses this layout; the sweeps use kPlanes exchange
+  // planes with halos in the same array.)
   constexpr uint32_t kLz = 0, kHx = 2 * kT * (kT + 1), kHy = kHx + kRing * kT,
                      kZero = kHy + kRing * kT;
-  __shared__ V lds[kZero + 1];
+  constexpr uint32_t kLdsN = FAC ? kZero + 1 : (uint32_t)(kPlanes * kPlane);
+  __shared__ V lds[kLdsN];
   // record rings of the sweeps: [step % kR][position of the slot in the step]
   __shared__ CellT<V> rc[kR][kThreads];
   __shared__ V rr[kR][kThreads];
@@ -758,7 +791,13 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
   auto HX = [](int w, int lj) -> uint32_t { return kHx + (uint32_t)((w & (kRing - 1)) * kT + lj); };
   auto HY = [](int w, int li) -> uint32_t { return kHy + (uint32_t)((w & (kRing - 1)) * kT + li); };
   __shared__ int s_tile;
-  if (threadIdx.x == 0) lds[kZero] = (V)0;
+  if constexpr (FAC) {
+    if (threadIdx.x == 0) lds[kZero] = (V)0;
+  } else {
+    if (threadIdx.x < kPlanes) lds[threadIdx.x * kPlane] = (V)0;  // the +0 cells
+  }
+  // plane of step w, its cell of local pencil (li, lj), and its halos
+  auto PL = [](int w) -> uint32_t { return (uint32_t)(w & (kPlanes - 1)) * kPlane; };
   // wave-uniform (scalar) branches: waves 0-3 compute, 4 reads the faces,
   // 5 and 6 (sweeps only) load the record rings
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -882,10 +921,15 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
           if (need && is_sentinel(v)) v = wait_value(at(S.out, face_addr(w)), S.abort_flag);
           if (!need) v = (V)0;
           if (pub) {
-            if (xf)
-              lds[HX(w, lj)] = v;
-            else
-              lds[HY(w, li)] = v;
+            if constexpr (FAC) {
+              if (xf)
+                lds[HX(w, lj)] = v;
+              else
+                lds[HY(w, li)] = v;
+            } else {
+              // the halo of the plane the step's operands are read from
+              lds[PL(w - 1) + (xf ? (uint32_t)(lj + 1) * (kT + 1) : (uint32_t)(li + 1))] = v;
+            }
           }
         }
       };
@@ -916,50 +960,48 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
     const int lxi = li > 0 ? li - 1 : 0;  // LDS neighbours (read unconditionally)
     const int lyj = lj > 0 ? lj - 1 : 0;
     if constexpr (!FAC) {
-      // The step's records come from the ring, read one step early together
-      // with the LDS operand of each ordered term: the -x / -y neighbour's
-      // value of the previous step (an upstream face value at the tile's
-      // low faces), this pencil's previous value, or +0 for an absent term.
+      // The step's records come from the ring, read one step early: the
+      // factor entries, the rhs and the operand word (three offsets from
+      // this pencil's position in the previous plane).  Padding slots hold
+      // zero records and operand words that all point at +0, so they compute
+      // +0 like any other slot.
       const int pos = BWD ? kThreads - 1 - (int)threadIdx.x : (int)threadIdx.x;
-      const int pres = BWD ? S_BPRES : S_FPRES;
-      const int srcb = BWD ? S_BSRC : S_FSRC;
+      const uint32_t own = (uint32_t)((lj + 1) * (kT + 1) + li + 1);
       CellT<V> c;
       V rhs;
-      bool active;
-      uint32_t src[3];
+      uint32_t op = 0;
       auto take = [&](int w) {
         const int r = w & (kR - 1);
         c = rc[r][pos];
         rhs = rr[r][pos];
-        const int kk = w - off;
-        active = kk >= 0 && kk < Bz;
-        const uint32_t code = active ? rk[r][pos] : 0u;
-        const uint32_t ax = li > 0 ? LZ((w - 1) & 1, lj, lxi) : HX(w, lj);
-        const uint32_t ay = lj > 0 ? LZ((w - 1) & 1, lyj, li) : HY(w, li);
-        const uint32_t az = LZ((w - 1) & 1, lj, li);
+        op = rk[r][pos];
+      };
+      uint32_t src[3];
+      auto sources = [&](int w) {
+        const uint32_t at0 = PL(w - 1) + own;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const uint32_t t = (code >> (srcb + 2 * q)) & 3u;
-          const uint32_t p = t == 0 ? ax : (t == 1 ? ay : az);
-          src[q] = (code & (1u << (pres + q))) ? p : kZero;
-        }
+        for (int q = 0; q < 3; ++q)
+          src[q] = at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 10 * q, 10);
       };
       for (int w = 0; w < nsteps; ++w) {
         __syncthreads();  // ring slots of steps w and w + 1 are complete
-        if (w == 0) take(0);
-        V z = (V)0;
-        if (active) {
-          const V v0 = lds[src[0]], v1 = lds[src[1]], v2 = lds[src[2]];
-          CellIn<V> ci;
-          ci.c = c;
-          ci.rhs = rhs;
-          z = sweep_cell_ordered<V>(ci, v0, v1, v2);
+        if (w == 0) {
+          take(0);
+          sources(0);
         }
-        // every step writes the lane's slot of that step (0 on the padding
+        const V v0 = lds[src[0]], v1 = lds[src[1]], v2 = lds[src[2]];
+        CellIn<V> ci;
+        ci.c = c;
+        ci.rhs = rhs;
+        const V z = sweep_cell_ordered<V>(ci, v0, v1, v2);
+        // every step writes the lane's slot of that step (+0 on the padding
         // slots), so no slot keeps the sentinel after the pass
         store_coherent(at(S.out, base + (uint32_t)(sgn * w)), z);
-        lds[LZ(w & 1, lj, li)] = z;
-        if (w + 1 < nsteps) take(w + 1);
+        lds[PL(w) + own] = z;
+        if (w + 1 < nsteps) {
+          take(w + 1);
+          sources(w + 1);
+        }
       }
       __syncthreads();
     } else {
@@ -1059,6 +1101,8 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.pdof.ensure(P.ncell);
   P.code.ensure(P.ncell);
   P.scode.ensure(P.ncell);
+  P.opf.ensure(P.ncell);
+  P.opb.ensure(P.ncell);
   P.cf.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.cb.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.coef.ensure(P.ncell * kCoef);
@@ -1094,6 +1138,8 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   A.pdof = P.pdof.p;
   A.code = P.code.p;
   A.scode = P.scode.p;
+  A.opf = P.opf.p;
+  A.opb = P.opb.p;
   A.cf = reinterpret_cast<Cell*>(P.cf.p);
   A.cb = reinterpret_cast<Cell*>(P.cb.p);
   A.coef = P.coef.p;
@@ -1165,6 +1211,7 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.cells = reinterpret_cast<const CellT<V>*>(cf);
   S.in = r;
   S.out = zf;
+  S.code = P.opf.p;  // the sweeps read operand words, not the code
   S.ticket = tick;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
   hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_FWD>::value), 0, st, S);
@@ -1173,6 +1220,7 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.cells = reinterpret_cast<const CellT<V>*>(cb);
   S.in = zf;
   S.out = zb;
+  S.code = P.opb.p;
   S.ticket = tick + 1;
   hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
   hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_BWD>::value), 0, st, S);
@@ -1246,6 +1294,8 @@ void SweepPlan::release() {
   pdof.release();
   code.release();
   scode.release();
+  opf.release();
+  opb.release();
   cf.release();
   cb.release();
   coef.release();
