@@ -150,6 +150,9 @@ struct PcgWork {
   int red_blocks = 0;
   double* host_scal = nullptr;  // pinned
   int* host_flag = nullptr;     // pinned
+  double* host_part = nullptr;  // pinned, red_blocks |r| partials
+  hipEvent_t res_event = nullptr;  // marks the residual copies of the box loops
+  DevBuf<double> partial2;      // |r| partials of the box loops
   void release();
 };
 

@@ -25,7 +25,7 @@ constexpr int kRedThreads = 256;
 constexpr int kMaxUpper = 16;  // upper entries per row the factor supports
 
 // scalar slots in PcgWork::scal
-enum { S_RHO = 0, S_ALPHA = 1, S_BETA = 2, S_RES = 3, S_DOT = 4, S_NSCAL = 8 };
+enum { S_RHO = 0, S_ALPHA = 1, S_BETA = 2, S_RES = 3, S_DOT = 4, S_RHO2 = 5, S_NSCAL = 8 };
 
 __device__ inline void row_range(const DevCsr& A, int64_t i, uint32_t& b,
                                  uint32_t& e) {
@@ -335,26 +335,6 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr(
   if (threadIdx.x == 0) partial[blockIdx.x] = mm;
 }
 
-// The inner iteration of the mixed-precision solve: the FP64 solution takes
-// s*alpha directly and the FP32 residual r -= z*alpha is formed in FP64
-// before rounding; |r|_inf block partials.
-__global__ __launch_bounds__(kRedThreads) void k_update_xr_mixed(
-    double* x, float* r, const float* s, const float* z, int64_t n,
-    const double* scal, double* partial) {
-  __shared__ double sh[kRedThreads];
-  const double alpha = scal[S_ALPHA];
-  double m = 0;
-  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kRedThreads) {
-    x[i] += (double)s[i] * alpha;
-    const float ri = (float)((double)r[i] - (double)z[i] * alpha);
-    r[i] = ri;
-    m = fmax(m, (double)fabsf(ri));
-  }
-  const double mm = block_max(m, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = mm;
-}
-
 // s = z + s*beta (add_scaled then swap, PCGSolver.h:302-303); FP32 vectors
 // are updated in FP64 and rounded
 template <class V>
@@ -362,6 +342,60 @@ __global__ void k_update_s(V* s, const V* z, int64_t n, const double* scal) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   s[i] = (V)((double)z[i] + (double)s[i] * scal[S_BETA]);
+}
+
+// The box loops' update kernels take alpha and beta straight from the block
+// partials of the inner products: every block repeats the final reduction
+// with the thread mapping and tree of k_dot_final, so each computes the same
+// value k_dot_final would (rho_old / s . z, and z . r / rho_old).
+
+// x += s alpha, r -= z alpha (FP64 r and z, or FP32 formed in FP64), |r|_inf
+// block partials into rpart.
+template <class V>
+__global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
+    double* x, V* r, const V* s, const V* z, int64_t n, const double* dpart, int nb,
+    const double* scal, int rho_slot, double* rpart) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) acc += dpart[i];
+  const double sz = block_sum(acc, sh);
+  const double alpha = scal[rho_slot] / sz;  // PCGSolver.h:291
+  const double nalpha = -alpha;
+  __syncthreads();  // every thread has read the sum before sh is reused
+  double m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads) {
+    if constexpr (sizeof(V) == sizeof(double)) {
+      x[i] += s[i] * alpha;
+      const double ri = r[i] + z[i] * nalpha;
+      r[i] = ri;
+      m = fmax(m, fabs(ri));
+    } else {
+      x[i] += (double)s[i] * alpha;
+      const V ri = (V)((double)r[i] - (double)z[i] * alpha);
+      r[i] = ri;
+      m = fmax(m, (double)fabsf(ri));
+    }
+  }
+  const double mm = block_max(m, sh);
+  if (threadIdx.x == 0) rpart[blockIdx.x] = mm;
+}
+
+// beta = (z . r) / rho_old (PCGSolver.h:301), s = z + s beta; the new rho is
+// written to the other rho slot, which no block reads here.
+template <class V>
+__global__ __launch_bounds__(kRedThreads) void k_update_s_fused(
+    V* s, const V* z, int64_t n, const double* dpart, int nb, double* scal, int rho_old,
+    int rho_new) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) acc += dpart[i];
+  const double zr = block_sum(acc, sh);
+  const double beta = zr / scal[rho_old];
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[rho_new] = zr;
+  for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads)
+    s[i] = (V)((double)z[i] + (double)s[i] * beta);
 }
 
 inline unsigned blocks_for(int64_t n, int t) {
@@ -436,6 +470,11 @@ void PcgWork::release() {
   host_scal = nullptr;
   if (host_flag) (void)hipHostFree(host_flag);
   host_flag = nullptr;
+  if (host_part) (void)hipHostFree(host_part);
+  host_part = nullptr;
+  if (res_event) (void)hipEventDestroy(res_event);
+  res_event = nullptr;
+  partial2.release();
 }
 
 static size_t slot_count(const DevCsr& A) { return (size_t)A.nnz_slots; }
@@ -454,6 +493,9 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
   w.x.ensure(n);
   w.red_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
   w.partial.ensure(w.red_blocks);
+  w.partial2.ensure(w.red_blocks);
+  if (!w.host_part) WC_HIP(hipHostMalloc((void**)&w.host_part, sizeof(double) * 1024));
+  if (!w.res_event) WC_HIP(hipEventCreateWithFlags(&w.res_event, hipEventDisableTiming));
   w.scal.ensure(S_NSCAL);
   if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
   if (!w.host_flag) WC_HIP(hipHostMalloc((void**)&w.host_flag, sizeof(int)));
@@ -593,9 +635,43 @@ static double read_scal(hipStream_t st, PcgWork& w, int slot) {
   return w.host_scal[slot];
 }
 
+// The residual of the last update: the max of its block partials, read back
+// with the sweeps' abort flag once the event recorded after their copies has
+// completed (work enqueued after the event may still be running).
+static double box_residual_max(PcgWork& w, const SweepPlan& P, hipEvent_t ev) {
+  WC_HIP(hipEventSynchronize(ev));
+  if (*w.host_flag)
+    throw HipError{hipErrorLaunchTimeOut,
+                   "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__,
+                   __LINE__};
+  double m = 0;
+  for (int i = 0; i < w.red_blocks; ++i) m = fmax(m, w.host_part[i]);
+  return m;
+}
+
+static void copy_residual(hipStream_t st, PcgWork& w, const SweepPlan& P, hipEvent_t ev) {
+  WC_HIP(hipMemcpyAsync(w.host_part, w.partial2.p, sizeof(double) * w.red_blocks,
+                        hipMemcpyDeviceToHost, st));
+  WC_HIP(hipMemcpyAsync(w.host_flag, P.progress.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+  WC_HIP(hipEventRecord(ev, st));
+}
+
+// Harvest the kernel timer once everything enqueued has completed.
+static void settle_timer(hipStream_t st, PcgWork& w) {
+  if (w.timer.on && !w.timer.pending.empty()) {
+    WC_HIP(hipStreamSynchronize(st));
+    w.timer.harvest();
+  }
+}
+
 // The PCG loop of PCGSolver::solve (PCGSolver.h:280-304) on box vectors: the
 // pipelined sweeps apply MIC(0) in place and the SpMV is the box stencil.
-// b is gathered into the box once and x scattered back once.
+// b is gathered into the box once and x scattered back once.  Each iteration
+// is the SpMV with the s . z partials, one update kernel (alpha, x, r and the
+// |r|_inf partials), then -- enqueued before the host has read the residual,
+// since they touch only s and the sweep outputs, which are dead once the
+// residual meets the tolerance -- the MIC(0) apply, the z . r partials and
+// the update of s with beta.
 static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double* b,
                          double* x_dof, double tolr, int maxit, double* residual_out,
                          int* iters_out) {
@@ -621,48 +697,40 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     return false;
   }
   WC_HIP(hipMemcpyAsync(s, z, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
-  const unsigned gb = blocks_for(nc, 256);
+  hipEvent_t ev = w.res_event;
+  int rho_slot = S_RHO;
   int it;
+  bool ok = false;
   for (it = 0; it < maxit; ++it) {
     hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
     box_spmv_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
     if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
-                       w.scal.p, (int)R_ALPHA);
-    hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(kRedThreads), 0, st, x, r, s, z, nc,
-                       w.scal.p, w.partial.p);
-    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p,
-                       nb, w.scal.p);
+    hipLaunchKernelGGL(k_update_xr_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
+                       z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.partial2.p);
     WC_LAUNCH_CHECK();
-    // the residual and the sweeps' abort flag in one round trip
-    WC_HIP(hipMemcpyAsync(w.host_scal + S_RES, w.scal.p + S_RES, sizeof(double),
-                          hipMemcpyDeviceToHost, st));
-    WC_HIP(hipMemcpyAsync(w.host_flag, P.progress.p + 2, sizeof(int), hipMemcpyDeviceToHost,
-                          st));
-    WC_HIP(hipStreamSynchronize(st));
-    if (!w.timer.pending.empty()) w.timer.harvest();
-    const double res = w.host_scal[S_RES];
-    *residual_out = res;
-    if (*w.host_flag)
-      throw HipError{hipErrorLaunchTimeOut,
-                     "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__,
-                     __LINE__};
-    if (res <= tolr) {
-      *iters_out = it + 1;
-      box_scatter(st, P, x, x_dof);
-      return true;
+    copy_residual(st, w, P, ev);
+    if (it + 1 < maxit) {
+      sweep_apply(st, P, r, tm, ndof);
+      hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+                         w.partial.p);
+      const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
+      hipLaunchKernelGGL(k_update_s_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
+                         nc, w.partial.p, nb, w.scal.p, rho_slot, rho_next);
+      WC_LAUNCH_CHECK();
+      rho_slot = rho_next;
     }
-    sweep_apply(st, P, r, tm, ndof);
-    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
-                       w.partial.p);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
-                       w.scal.p, (int)R_BETA);
-    hipLaunchKernelGGL(k_update_s<double>, dim3(gb), dim3(256), 0, st, s, z, nc, w.scal.p);
-    WC_LAUNCH_CHECK();
+    const double res = box_residual_max(w, P, ev);
+    *residual_out = res;
+    if (res <= tolr) {
+      ++it;
+      ok = true;
+      break;
+    }
   }
   *iters_out = it;
   box_scatter(st, P, x, x_dof);
-  return false;
+  settle_timer(st, w);
+  return ok;
 }
 
 // The mixed-precision form of the solve (no reference counterpart): PCG
@@ -689,8 +757,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
   KTimer* tm = w.timer.on ? &w.timer : nullptr;
   box_gather(st, P, b, P.bb.p);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
-  const unsigned gb = blocks_for(nc, 256);
-  double eta = 1.0;
+    double eta = 1.0;
   for (int d = 0; d < w.mixed_digits; ++d) eta *= 0.1;
   int total = 0;
   w.passes = 0;
@@ -705,10 +772,12 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     *iters_out = total;
     if (res <= tolr) {
       box_scatter(st, P, x, x_dof);
+      settle_timer(st, w);
       return true;
     }
     if (total >= maxit || w.passes > w.mixed_refinements || res != res) {
       box_scatter(st, P, x, x_dof);
+      settle_timer(st, w);
       return false;
     }
     ++w.passes;
@@ -722,40 +791,35 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     const double rho = read_scal(st, w, S_RHO);
     if (rho == 0 || rho != rho) {
       box_scatter(st, P, x, x_dof);
+      settle_timer(st, w);
       return false;
     }
     WC_HIP(hipMemcpyAsync(s, z, sizeof(float) * nc, hipMemcpyDeviceToDevice, st));
+    // inner PCG iterations, with the next apply enqueued before the host reads
+    // the residual (as in pcg_loop_box: it touches only s and the sweep
+    // outputs, which the next refinement pass recomputes)
+    int rho_slot = S_RHO;
     while (total < maxit) {
       hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
       box_spmv32_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
       if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
-      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
-                         w.scal.p, (int)R_ALPHA);
-      hipLaunchKernelGGL(k_update_xr_mixed, dim3(nb), dim3(kRedThreads), 0, st, x, r, s, z,
-                         nc, w.scal.p, w.partial.p);
-      hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p,
-                         nb, w.scal.p);
+      hipLaunchKernelGGL(k_update_xr_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, x, r,
+                         s, z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.partial2.p);
       WC_LAUNCH_CHECK();
-      WC_HIP(hipMemcpyAsync(w.host_scal + S_RES, w.scal.p + S_RES, sizeof(double),
-                            hipMemcpyDeviceToHost, st));
-      WC_HIP(hipMemcpyAsync(w.host_flag, P.progress.p + 2, sizeof(int),
-                            hipMemcpyDeviceToHost, st));
-      WC_HIP(hipStreamSynchronize(st));
-      if (!w.timer.pending.empty()) w.timer.harvest();
+      copy_residual(st, w, P, w.res_event);
       ++total;
-      if (*w.host_flag)
-        throw HipError{hipErrorLaunchTimeOut,
-                       "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__,
-                       __LINE__};
-      const double r32 = w.host_scal[S_RES];
+      if (total < maxit) {
+        sweep_apply32(st, P, r, tm, ndof);
+        hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
+                           w.partial.p);
+        const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
+        hipLaunchKernelGGL(k_update_s_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
+                           nc, w.partial.p, nb, w.scal.p, rho_slot, rho_next);
+        WC_LAUNCH_CHECK();
+        rho_slot = rho_next;
+      }
+      const double r32 = box_residual_max(w, P, w.res_event);
       if (r32 <= inner_tol || r32 != r32) break;
-      sweep_apply32(st, P, r, tm, ndof);
-      hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
-                         w.partial.p);
-      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
-                         w.scal.p, (int)R_BETA);
-      hipLaunchKernelGGL(k_update_s<float>, dim3(gb), dim3(256), 0, st, s, z, nc, w.scal.p);
-      WC_LAUNCH_CHECK();
     }
   }
 }
