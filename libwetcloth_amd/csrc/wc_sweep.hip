@@ -274,13 +274,23 @@ __global__ void k_fill_box(BoxArgs A) {
     if (A.isdof[n]) {
       me = (int32_t)A.dofscan[n];
       code |= C_DOF;
+      // this row and, per lower neighbour, its row: read once each (an ELL
+      // row always has 7 allocated entries; those past the length are unused)
       const int64_t row = 7 * (int64_t)me;
       const uint32_t len = A.rowlen[me];
-      for (uint32_t s = 0; s < len; ++s)
-        if (A.col[row + s] == (uint32_t)me) {
+      uint32_t mc[7];
+      double mv[7];
+#pragma unroll
+      for (int s = 0; s < 7; ++s) {
+        mc[s] = (uint32_t)s < len ? A.col[row + s] : 0xFFFFFFFFu;
+        mv[s] = A.val[row + s];
+      }
+#pragma unroll
+      for (int s = 0; s < 7; ++s)
+        if (mc[s] == (uint32_t)me) {
           code |= C_DIAG;
-          cf[0] = A.val[row + s];
-          f.adiag = A.val[row + s];
+          cf[0] = mv[s];
+          f.adiag = mv[s];
         }
       int64_t lkey[3], ukey[3];
       for (int d = 0; d < 3; ++d) {
@@ -298,22 +308,26 @@ __global__ void k_fill_box(BoxArgs A) {
             const uint32_t olen = A.rowlen[other];
             double kdiag = 0;
             int noth = 0;
-            for (uint32_t s = 0; s < olen; ++s) {
+#pragma unroll
+            for (int s = 0; s < 7; ++s) {
+              if ((uint32_t)s >= olen) continue;
               const uint32_t c = A.col[orow + s];
+              const double v = A.val[orow + s];
               if (c == (uint32_t)me) {
-                f.lo[d] = A.val[orow + s];
+                f.lo[d] = v;
                 code |= 1u << (C_LOWER + d);
               } else if (c == other) {
-                kdiag = A.val[orow + s];
+                kdiag = v;
               } else if (c > other && noth < 2) {
-                f.oth[d][noth++] = A.val[orow + s];  // ascending: rows are sorted
+                f.oth[d][noth++] = v;  // ascending: rows are sorted
               }
             }
             if (kdiag == 0) fcode |= 1u << (F_KNULL + d);
             fcode |= (uint32_t)noth << (F_NOTH + 2 * d);
-            for (uint32_t s = 0; s < len; ++s)
-              if (A.col[row + s] == other) {
-                cf[1 + d] = A.val[row + s];
+#pragma unroll
+            for (int s = 0; s < 7; ++s)
+              if (mc[s] == other) {
+                cf[1 + d] = mv[s];
                 code |= 1u << (C_ALO + d);
               }
           }
@@ -327,9 +341,10 @@ __global__ void k_fill_box(BoxArgs A) {
           if (A.isdof[nn]) {
             const uint32_t other = A.dofscan[nn];
             ukey[d] = other;
-            for (uint32_t s = 0; s < len; ++s)
-              if (A.col[row + s] == other) {
-                cf[4 + d] = A.val[row + s];
+#pragma unroll
+            for (int s = 0; s < 7; ++s)
+              if (mc[s] == other) {
+                cf[4 + d] = mv[s];
                 code |= (1u << (C_UPPER + d)) | (1u << (C_AUP + d));
               }
           }
