@@ -319,7 +319,20 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
     A.v[d * A.cap + p] = fv[d];
     nx[d] = px[d] + fv[d] * A.dt;
   }
+  // solidProjection moves the particle only if phi_now < 0.  The box distance
+  // is 1-Lipschitz, so every stencil node's phi (or the inactive value 3 dx)
+  // is at least min(3 dx, phi(px) - 2.6 dx), phi_ori, their convex
+  // combination, too, and phi_now >= phi_ori - |dp| with |gr| <= 1: a
+  // particle with phi(px) > 4 dx + |v|_1 dt and |v|_1 dt < 2 dx is never
+  // moved, and skips the evaluation.
+  bool near_wall = true;
   if (A.solid_projection) {
+    const R step = (fabs(fv[0]) + fabs(fv[1]) + fabs(fv[2])) * A.dt;
+    const R cdx = (R)g.dx;
+    near_wall = !(container_phi_r<R>(g, px[0], px[1], px[2]) > (R)4 * cdx + step &&
+                  step < (R)2 * cdx);
+  }
+  if (A.solid_projection && near_wall) {
     int base[3];
     R w[3][3];
     for (int a = 0; a < 3; ++a) {
