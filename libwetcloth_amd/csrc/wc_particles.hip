@@ -133,15 +133,17 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   const int bk = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
   const int o0 = bi * 4 - 2, o1 = bj * 4 - 2, o2 = bk * 4 - 2;
 
-  // [axis][mass, momentum, volume], summed in FP64 for either R
-  __shared__ double sm[3][3][kTileNodes];
+  // [axis][sum of weights, sum of weight * velocity], in FP64 for either R.
+  // Particle mass and volume are uniform, so a node's mass and volume are
+  // fm and fvol times its weight sum (TwoDScene.cpp:6700-6713 sums fm w and
+  // fvol w): two LDS atomics per node instead of three.
+  __shared__ double sm[3][2][kTileNodes];
   __shared__ unsigned long long sphi[kTileNodes];
   __shared__ int stouch[27];
   for (int i = threadIdx.x; i < kTileNodes; i += kP2GThreads) {
     for (int d = 0; d < 3; ++d) {
       sm[d][0][i] = 0.0;
       sm[d][1][i] = 0.0;
-      sm[d][2][i] = 0.0;
     }
     sphi[i] = kPhiUntouched;
   }
@@ -179,21 +181,17 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
             const R e1 = node_pos_r<R>(g, j, face_off_r<R>(d, 1)) - px[1];
             const R e2 = node_pos_r<R>(g, k, face_off_r<R>(d, 2)) - px[2];
             const R vel = vd + (b0 * e0 + b1 * e1 + b2 * e2);
-            const R dm = A.fm * ww;
-            const R dp = vel * A.fm * ww;
-            const R dv = A.fvol * ww;
             const int li = i - o0, lj = j - o1, lk = k - o2;
             if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
                 lk < kTile) {
               const int l = (lk * kTile + lj) * kTile + li;
-              atomicAdd(&sm[d][0][l], (double)dm);
-              atomicAdd(&sm[d][1][l], (double)dp);
-              atomicAdd(&sm[d][2][l], (double)dv);
+              atomicAdd(&sm[d][0][l], (double)ww);
+              atomicAdd(&sm[d][1][l], (double)(vel * ww));
             } else {
               const int64_t n = node_index(g, i, j, k);
-              unsafeAtomicAdd(&A.mass[d][n], (double)dm);
-              unsafeAtomicAdd(&A.mom[d][n], (double)dp);
-              unsafeAtomicAdd(&A.vol[d][n], (double)dv);
+              unsafeAtomicAdd(&A.mass[d][n], (double)A.fm * (double)ww);
+              unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * (double)(vel * ww));
+              unsafeAtomicAdd(&A.vol[d][n], (double)A.fvol * (double)ww);
             }
           }
     }
@@ -246,11 +244,11 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
     if (!in_range(g, i, j, k)) continue;
     const int64_t n = node_index(g, i, j, k);
     for (int d = 0; d < 3; ++d) {
-      const double m = sm[d][0][l];
-      if (m != 0.0) {
-        unsafeAtomicAdd(&A.mass[d][n], m);
-        unsafeAtomicAdd(&A.mom[d][n], sm[d][1][l]);
-        unsafeAtomicAdd(&A.vol[d][n], sm[d][2][l]);
+      const double sw = sm[d][0][l];
+      if (sw != 0.0) {
+        unsafeAtomicAdd(&A.mass[d][n], (double)A.fm * sw);
+        unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * sm[d][1][l]);
+        unsafeAtomicAdd(&A.vol[d][n], (double)A.fvol * sw);
       }
     }
     const unsigned long long key = sphi[l];
