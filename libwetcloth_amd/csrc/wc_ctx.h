@@ -110,6 +110,9 @@ struct SweepPlan {
   DevBuf<float> cf32, cb32, coef32, zf32, zb32, rb32, sb32;
   DevBuf<double> bb;
   DevBuf<int> progress, flags;
+  // The forward sweep's ticket counter is zero: the sweeps zero each
+  // other's counter, so only the first apply after the plan needs a clear.
+  bool ticks_zero = false;
   void release();
 };
 
@@ -259,6 +262,8 @@ void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const dou
                   float* r, double* partial, int nblocks);
 void box_gather(hipStream_t st, const SweepPlan& P, const double* dof, double* box);
 void box_scatter(hipStream_t st, const SweepPlan& P, const double* box, double* dof);
+// whether a sweep gave up a bounded wait since the last factor or check
+// (the flag is cleared by reading it)
 bool sweep_aborted(hipStream_t st, SweepPlan& P);
 
 // ---- wc_pcg.hip ----

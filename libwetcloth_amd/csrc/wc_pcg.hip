@@ -757,7 +757,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
   KTimer* tm = w.timer.on ? &w.timer : nullptr;
   box_gather(st, P, b, P.bb.p);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
-    double eta = 1.0;
+  double eta = 1.0;
   for (int d = 0; d < w.mixed_digits; ++d) eta *= 0.1;
   int total = 0;
   w.passes = 0;

@@ -591,7 +591,9 @@ struct SweepArgs {
   const uint32_t* code;
   const V* in;             // forward: r; backward: the forward result
   V* out;                  // the sweep result, or the factor's invdiag
+  V* reset;                // forward sweep: set to the sentinel step by step
   int* ticket;
+  int* next_ticket;        // sweeps: zeroed by the block that takes ticket 0
   int* abort_flag;
   // factor pass
   const FacIn* fin;
@@ -762,8 +764,9 @@ struct ApplyRing {
   static constexpr int kCellOps = kThreads * (int)sizeof(CellT<V>) / 1024;
   static constexpr int kRhsOps = kThreads * (int)sizeof(V) / 1024;
   static constexpr int kCodeOps = kThreads * 4 / 1024;
-  static_assert(kCellOps * kAhead <= 63 && (kRhsOps + kCodeOps) * kAhead <= 63,
-                "a loader's loads in flight fit its 6-bit load counter");
+  static constexpr int kStepOps = kRhsOps + kCodeOps;  // rhs loader, per step
+  static_assert(kCellOps * kAhead <= 63 && kStepOps * kAhead + kRhsOps * (kAhead - 1) <= 63,
+                "a loader's loads and stores in flight fit its 6-bit counter");
 };
 template <int KIND>
 struct SweepBlock {
@@ -823,6 +826,9 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
     if (threadIdx.x == 0) {
       const int t = atomicAdd(S.ticket, 1);
       s_tile = t < S.ntiles ? (int)S.order[t] : -1;
+      // the other pass's counter is not used while this one runs
+      if constexpr (!FAC)
+        if (t == 0) *S.next_ticket = 0;
     }
     __syncthreads();
     const int tile = __builtin_amdgcn_readfirstlane(s_tile);
@@ -887,16 +893,36 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
                   16, 0, 0);
           }
         };
+        // The forward pass's rhs loader also sets the step's slots of
+        // S.reset (the backward pass's output) to the sentinel, one 16-byte
+        // store per lane and KB.
+        // (the sentinel words are kept in four registers of their own, so a
+        // store never waits for an earlier one to have read its data)
+        typedef unsigned int U4 __attribute__((ext_vector_type(4)));
+        U4 ones = {~0u, ~0u, ~0u, ~0u};
+        asm volatile("" : "+v"(ones));
+        auto clear = [&](int w) {
+          char* g = reinterpret_cast<char*>(S.reset) + (size_t)block(w) * sizeof(V);
+#pragma unroll
+          for (int k = 0; k < Ring::kRhsOps; ++k)
+            *reinterpret_cast<U4*>(g + k * 1024 + lane * 16) = ones;
+        };
         for (int t = 0; t < Ring::kAhead; ++t) issue(t);
         for (int w = 0; w < nsteps; ++w) {
           // steps w and w + 1 complete: the loads of the kAhead - 2 later
-          // steps may still be in flight
+          // steps may still be in flight, and (rhs loader) the stores of
+          // the min(w, kAhead - 1) steps before w, issued after step w + 1's
+          // loads (the counter is in order; early steps wait for more)
           if (cells)
             wait_vmcnt<Ring::kCellOps * (Ring::kAhead - 2)>();
+          else if (w < Ring::kAhead - 1)
+            wait_vmcnt<Ring::kStepOps * (Ring::kAhead - 2)>();
           else
-            wait_vmcnt<(Ring::kRhsOps + Ring::kCodeOps) * (Ring::kAhead - 2)>();
+            wait_vmcnt<Ring::kStepOps * (Ring::kAhead - 2) +
+                       Ring::kRhsOps * (Ring::kAhead - 1)>();
           __builtin_amdgcn_s_barrier();
           issue(w + Ring::kAhead);
+          if (!cells && !BWD) clear(w);
         }
         wait_vmcnt<0>();
         __builtin_amdgcn_s_barrier();
@@ -1139,6 +1165,10 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   WC_HIP(hipMemcpyAsync(P.order.p, order.data(), sizeof(uint32_t) * P.ntiles,
                         hipMemcpyHostToDevice, st));
   P.progress.ensure(8);
+  // tickets and the abort flag start at zero (the abort flag is cleared
+  // again by each factor, not by the applies: once set it stays set)
+  WC_HIP(hipMemsetAsync(P.progress.p, 0, sizeof(int) * 8, st));
+  P.ticks_zero = true;
   BoxArgs A;
   A.g = g;
   for (int d = 0; d < 3; ++d) A.o[d] = P.o[d];
@@ -1187,6 +1217,7 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr) {
   S.mdr = mdr;
   S.out = P.zf.p;
   S.ticket = tick + 3;
+  P.ticks_zero = true;   // counters 0 and 1 are not touched by the factor
   hipLaunchKernelGGL((k_sweep<K_FAC, double>), dim3(P.ntiles), dim3(SweepBlock<K_FAC>::value), 0, st, S);
   WC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_backward_records, dim3(blocks(P.ncell, 256)), dim3(256), 0, st,
@@ -1218,16 +1249,22 @@ template <class V>
 static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, const V* cb,
                        V* zf, V* zb, KTimer* timer, int64_t ndof) {
   // two ticket counters and the abort flag; outputs start as the sentinel
+  // Two ticket counters; the outputs start as the sentinel.  Each pass zeroes
+  // the other pass's counter, and the forward pass's rhs loader sets every
+  // slot of zb to the sentinel as it goes, so an apply clears only zf (just
+  // before the forward pass, which leaves it cache-resident for the face
+  // reads) and, after the plan or the factor, the counters.
   int* tick = P.progress.p;
-  WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 8, st));
+  if (!P.ticks_zero) WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 2, st));
   WC_HIP(hipMemsetAsync(zf, 0xFF, sizeof(V) * P.ncell, st));
-  WC_HIP(hipMemsetAsync(zb, 0xFF, sizeof(V) * P.ncell, st));
   SweepArgs<V> S = sweep_args<V>(P);
   S.cells = reinterpret_cast<const CellT<V>*>(cf);
   S.in = r;
   S.out = zf;
+  S.reset = zb;
   S.code = P.opf.p;  // the sweeps read operand words, not the code
   S.ticket = tick;
+  S.next_ticket = tick + 1;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
   hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_FWD>::value), 0, st, S);
   WC_LAUNCH_CHECK();
@@ -1235,11 +1272,14 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.cells = reinterpret_cast<const CellT<V>*>(cb);
   S.in = zf;
   S.out = zb;
+  S.reset = nullptr;
   S.code = P.opb.p;
   S.ticket = tick + 1;
+  S.next_ticket = tick;
   hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
   hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_BWD>::value), 0, st, S);
   WC_LAUNCH_CHECK();
+  P.ticks_zero = true;
   if (timer) {
     timer->add(KTimer::SWEEP_FWD, ndof, e0, e1);
     timer->add(KTimer::SWEEP_BWD, ndof, e2, timer->mark(st));
@@ -1303,6 +1343,11 @@ bool sweep_aborted(hipStream_t st, SweepPlan& P) {
   int v = 0;
   WC_HIP(hipMemcpyAsync(&v, P.progress.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
   WC_HIP(hipStreamSynchronize(st));
+  if (v != 0) {
+    // reported once: the applies do not clear the flag themselves
+    WC_HIP(hipMemsetAsync(P.progress.p + 2, 0, sizeof(int), st));
+    WC_HIP(hipStreamSynchronize(st));
+  }
   return v != 0;
 }
 void SweepPlan::release() {
