@@ -152,10 +152,11 @@ struct PcgWork {
   bool levels_ready = false;
   int red_blocks = 0;
   double* host_scal = nullptr;  // pinned
-  int* host_flag = nullptr;     // pinned
-  double* host_part = nullptr;  // pinned, red_blocks |r| partials
-  hipEvent_t res_event = nullptr;  // marks the residual copies of the box loops
-  DevBuf<double> partial2;      // |r| partials of the box loops
+  int* host_flag = nullptr;     // mapped host memory: the sweeps' abort flag
+  double* host_part = nullptr;  // mapped host memory, red_blocks |r| partials
+  double* host_part_dev = nullptr;  // its device address
+  int* host_flag_dev = nullptr;     // device address of host_flag (mapped)
+  hipEvent_t res_event = nullptr;  // marks the update kernel of the box loops
   void release();
 };
 

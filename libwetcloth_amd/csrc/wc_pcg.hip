@@ -350,11 +350,12 @@ __global__ void k_update_s(V* s, const V* z, int64_t n, const double* scal) {
 // value k_dot_final would (rho_old / s . z, and z . r / rho_old).
 
 // x += s alpha, r -= z alpha (FP64 r and z, or FP32 formed in FP64), |r|_inf
-// block partials into rpart.
+// block partials into rpart and the sweeps' abort flag into flag_out (both
+// in mapped host memory, which the host reads once the kernel is complete).
 template <class V>
 __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
     double* x, V* r, const V* s, const V* z, int64_t n, const double* dpart, int nb,
-    const double* scal, int rho_slot, double* rpart) {
+    const double* scal, int rho_slot, double* rpart, const int* flag, int* flag_out) {
   __shared__ double sh[kRedThreads];
   double acc = 0;
   for (int i = threadIdx.x; i < nb; i += kRedThreads) acc += dpart[i];
@@ -378,7 +379,10 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
     }
   }
   const double mm = block_max(m, sh);
-  if (threadIdx.x == 0) rpart[blockIdx.x] = mm;
+  if (threadIdx.x == 0) {
+    rpart[blockIdx.x] = mm;
+    if (blockIdx.x == 0) *flag_out = *flag;
+  }
 }
 
 // beta = (z . r) / rho_old (PCGSolver.h:301), s = z + s beta; the new rho is
@@ -470,11 +474,12 @@ void PcgWork::release() {
   host_scal = nullptr;
   if (host_flag) (void)hipHostFree(host_flag);
   host_flag = nullptr;
+  host_flag_dev = nullptr;
   if (host_part) (void)hipHostFree(host_part);
   host_part = nullptr;
+  host_part_dev = nullptr;
   if (res_event) (void)hipEventDestroy(res_event);
   res_event = nullptr;
-  partial2.release();
 }
 
 static size_t slot_count(const DevCsr& A) { return (size_t)A.nnz_slots; }
@@ -493,12 +498,20 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
   w.x.ensure(n);
   w.red_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
   w.partial.ensure(w.red_blocks);
-  w.partial2.ensure(w.red_blocks);
-  if (!w.host_part) WC_HIP(hipHostMalloc((void**)&w.host_part, sizeof(double) * 1024));
+  // written by the update kernels of the box loops, read by the host
+  if (!w.host_part) {
+    WC_HIP(hipHostMalloc((void**)&w.host_part, sizeof(double) * 1024,
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    WC_HIP(hipHostGetDevicePointer((void**)&w.host_part_dev, w.host_part, 0));
+  }
   if (!w.res_event) WC_HIP(hipEventCreateWithFlags(&w.res_event, hipEventDisableTiming));
   w.scal.ensure(S_NSCAL);
   if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
-  if (!w.host_flag) WC_HIP(hipHostMalloc((void**)&w.host_flag, sizeof(int)));
+  if (!w.host_flag) {
+    WC_HIP(hipHostMalloc((void**)&w.host_flag, sizeof(int),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    WC_HIP(hipHostGetDevicePointer((void**)&w.host_flag_dev, w.host_flag, 0));
+  }
   w.level_in = level;
   w.level_span = level_span;
   w.levels_ready = false;
@@ -635,9 +648,10 @@ static double read_scal(hipStream_t st, PcgWork& w, int slot) {
   return w.host_scal[slot];
 }
 
-// The residual of the last update: the max of its block partials, read back
-// with the sweeps' abort flag once the event recorded after their copies has
-// completed (work enqueued after the event may still be running).
+// The residual of the last update: the max of its block partials, which
+// the update kernel wrote into mapped host memory with the sweeps' abort
+// flag, once the event recorded after it has completed (work enqueued after
+// the event may still be running).
 static double box_residual_max(PcgWork& w, const SweepPlan& P, hipEvent_t ev) {
   WC_HIP(hipEventSynchronize(ev));
   if (*w.host_flag)
@@ -649,12 +663,7 @@ static double box_residual_max(PcgWork& w, const SweepPlan& P, hipEvent_t ev) {
   return m;
 }
 
-static void copy_residual(hipStream_t st, PcgWork& w, const SweepPlan& P, hipEvent_t ev) {
-  WC_HIP(hipMemcpyAsync(w.host_part, w.partial2.p, sizeof(double) * w.red_blocks,
-                        hipMemcpyDeviceToHost, st));
-  WC_HIP(hipMemcpyAsync(w.host_flag, P.progress.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
-  WC_HIP(hipEventRecord(ev, st));
-}
+static void mark_residual(hipStream_t st, hipEvent_t ev) { WC_HIP(hipEventRecord(ev, st)); }
 
 // Harvest the kernel timer once everything enqueued has completed.
 static void settle_timer(hipStream_t st, PcgWork& w) {
@@ -706,9 +715,10 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     box_spmv_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
     if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
     hipLaunchKernelGGL(k_update_xr_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
-                       z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.partial2.p);
+                       z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
+                       P.progress.p + 2, w.host_flag_dev);
     WC_LAUNCH_CHECK();
-    copy_residual(st, w, P, ev);
+    mark_residual(st, ev);
     if (it + 1 < maxit) {
       sweep_apply(st, P, r, tm, ndof);
       hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
@@ -804,9 +814,10 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
       box_spmv32_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
       if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
       hipLaunchKernelGGL(k_update_xr_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, x, r,
-                         s, z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.partial2.p);
+                         s, z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
+                         P.progress.p + 2, w.host_flag_dev);
       WC_LAUNCH_CHECK();
-      copy_residual(st, w, P, w.res_event);
+      mark_residual(st, w.res_event);
       ++total;
       if (total < maxit) {
         sweep_apply32(st, P, r, tm, ndof);
