@@ -87,6 +87,30 @@ wc_status wc_num_particles(const wc_ctx* ctx, int64_t* n);
 wc_status wc_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3],
                         uint64_t seed, double sigma, int64_t* n_out);
 
+/* ---- on-disk formats of the reference's serializer (App/TwoDSceneSerializer.cpp) ----
+ * The particles are written in the caller's order (the order of
+ * wc_set_particles), as doubles; the files are written before the call
+ * returns (the reference writes from a detached thread). */
+
+/* TwoDSceneSerializer::serializePositionOnly (:161-170) with
+ * serialize_pos_subprog (:23-37): an int byte count and m_x (4 doubles per
+ * particle: x, y, z and theta = 0), then an int byte count and m_d_gauss,
+ * which is empty for a liquid-only scene (count 0).  WC_ERR_ARG if the byte
+ * count does not fit an int (the reference's int overflows there). */
+wc_status wc_serialize_position_only(wc_ctx* ctx, const char* path);
+/* TwoDSceneSerializer::loadPosOnly (:172-183): reads such a file back into
+ * the particle positions (velocities and affine matrices are kept).  The
+ * file must hold 4 doubles per particle of the context and an empty
+ * m_d_gauss block; anything else is WC_ERR_ARG (the reference reads the
+ * stored byte count into m_x unchecked). */
+wc_status wc_load_position_only(wc_ctx* ctx, const char* path);
+/* The fluid file of TwoDSceneSerializer::serializeScene (:135-159):
+ * serialize_subprog's fluid loop (:39-47) over updateFluid's data
+ * (:242-255), one line "v x y z r" per particle, each number as an
+ * iostream inserts a double at precision 8 (printf "%.8g"), r = the
+ * context's particle radius. */
+wc_status wc_serialize_fluid(wc_ctx* ctx, const char* path);
+
 /* ---- the substep, phase by phase, in Core/WetClothCore.cpp:77-300 order ---- */
 
 /* updateParticleBoundingBox + rebucketizeParticles (TwoDScene.cpp:1588-1708):

@@ -98,6 +98,8 @@ def lib():
         L.wc_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
         L.wc_kernel_stats.argtypes = [vp, C.c_int32, _dp, _i64p, _i64p]
         L.wc_refinement_passes.argtypes = [vp, _i32p]
+        for nm in ("wc_serialize_position_only", "wc_load_position_only", "wc_serialize_fluid"):
+            getattr(L, nm).argtypes = [vp, C.c_char_p]
         L.wc_apply_preconditioner.argtypes = [vp, _dp, _dp]
         L.wc_system_multiply.argtypes = [vp, _dp, _dp]
         L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
@@ -213,6 +215,19 @@ class Scene:
         _check(lib().wc_seed_block(self.h, _ptr(lo), _ptr(hi), C.c_uint64(seed), sigma,
                                    C.byref(n)), self.h)
         return n.value
+
+    # ---- the reference serializer's particle files (TwoDSceneSerializer.cpp) ----
+    def serializePositionOnly(self, path):
+        """TwoDSceneSerializer::serializePositionOnly (:161-170, :23-37)."""
+        _check(lib().wc_serialize_position_only(self.h, os.fsencode(path)), self.h)
+
+    def loadPosOnly(self, path):
+        """TwoDSceneSerializer::loadPosOnly (:172-183): positions only."""
+        _check(lib().wc_load_position_only(self.h, os.fsencode(path)), self.h)
+
+    def serializeFluid(self, path):
+        """The fluid file of TwoDSceneSerializer::serializeScene (:39-47, :242-255)."""
+        _check(lib().wc_serialize_fluid(self.h, os.fsencode(path)), self.h)
 
     # ---- substep phases (reference names) ----
     def rebucketizeParticles(self):

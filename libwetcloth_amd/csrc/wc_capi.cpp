@@ -4,8 +4,13 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <stdio.h>
+
 #include <algorithm>
+#include <charconv>
+#include <climits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "wc_ctx.h"
@@ -555,6 +560,135 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
   }
   return fail(c, WC_ERR_ARG, "unknown option " + n);
 }
+
+}  // extern "C"
+
+namespace {
+
+// ---- the reference serializer's particle files (App/TwoDSceneSerializer.cpp) ----
+
+// "v x y z r\n" for particles [p0, p1): each number as std::ostream inserts
+// a double at setprecision(8) with the default float field, which is printf
+// "%.8g" ([facet.num.put.virtuals]); std::to_chars with chars_format::general
+// and precision 8 is that conversion.
+void format_fluid_lines(const double* x4, double r, int64_t p0, int64_t p1, std::string& out) {
+  char buf[128];
+  for (int64_t p = p0; p < p1; ++p) {
+    char* e = buf;
+    *e++ = 'v';
+    const double v[4] = {x4[4 * p], x4[4 * p + 1], x4[4 * p + 2], r};
+    for (int k = 0; k < 4; ++k) {
+      *e++ = ' ';
+      e = std::to_chars(e, buf + sizeof(buf) - 2, v[k], std::chars_format::general, 8).ptr;
+    }
+    *e++ = '\n';
+    out.append(buf, (size_t)(e - buf));
+  }
+}
+
+struct File {
+  FILE* f = nullptr;
+  ~File() {
+    if (f) fclose(f);
+  }
+};
+
+bool write_all(FILE* f, const void* p, size_t n) { return n == 0 || fwrite(p, 1, n, f) == n; }
+
+}  // namespace
+
+extern "C" {
+
+wc_status wc_serialize_position_only(wc_ctx* c, const char* path) {
+  if (!ready(c) || !path) return fail(c, WC_ERR_ARG, "bad argument");
+  const int64_t n = c->np;
+  const int64_t bytes = n * 4 * (int64_t)sizeof(double);
+  if (bytes > INT_MAX) return fail(c, WC_ERR_ARG, "position block exceeds the format's int byte count");
+  std::vector<double> x4;
+  WC_TRY(c, {
+    x4.resize((size_t)(4 * n));
+    wc::particles_get_soa(c, x4.data(), nullptr, nullptr, true);
+  })
+  File F;
+  F.f = fopen(path, "wb");
+  if (!F.f) return fail(c, WC_ERR_ARG, std::string("cannot open ") + path);
+  const int b0 = (int)bytes, b1 = 0;  // m_d_gauss is 0 x 3 without elastic bodies
+  if (!write_all(F.f, &b0, sizeof(int)) || !write_all(F.f, x4.data(), (size_t)bytes) ||
+      !write_all(F.f, &b1, sizeof(int)) || fclose(F.f) != 0) {
+    F.f = nullptr;
+    return fail(c, WC_ERR_ARG, std::string("write failed: ") + path);
+  }
+  F.f = nullptr;
+  return WC_OK;
+}
+
+wc_status wc_load_position_only(wc_ctx* c, const char* path) {
+  if (!ready(c) || !path) return fail(c, WC_ERR_ARG, "bad argument");
+  const int64_t n = c->np;
+  File F;
+  F.f = fopen(path, "rb");
+  if (!F.f) return fail(c, WC_ERR_ARG, std::string("cannot open ") + path);
+  int b0 = -1, b1 = -1;
+  if (fread(&b0, sizeof(int), 1, F.f) != 1 || (int64_t)b0 != n * 4 * (int64_t)sizeof(double))
+    return fail(c, WC_ERR_ARG, "position block size does not match the context's particles");
+  std::vector<double> x4, v4, fB;
+  try {
+    x4.resize((size_t)(4 * n));
+  } catch (const std::bad_alloc&) {
+    return fail(c, WC_ERR_NOMEM, "host allocation failed");
+  }
+  if ((n > 0 && fread(x4.data(), (size_t)b0, 1, F.f) != 1) || fread(&b1, sizeof(int), 1, F.f) != 1)
+    return fail(c, WC_ERR_ARG, "truncated position file");
+  if (b1 != 0)
+    return fail(c, WC_ERR_ARG, "the file holds Gauss-point data (elastic bodies), which a liquid context has none of");
+  WC_TRY(c, {
+    v4.resize((size_t)(4 * n));
+    fB.resize((size_t)(9 * n));
+    wc::particles_get_soa(c, nullptr, v4.data(), fB.data(), true);
+    wc::particles_set_soa(c, n, x4.data(), v4.data(), fB.data(), true);
+  })
+  return WC_OK;
+}
+
+wc_status wc_serialize_fluid(wc_ctx* c, const char* path) {
+  if (!ready(c) || !path) return fail(c, WC_ERR_ARG, "bad argument");
+  const int64_t n = c->np;
+  std::vector<double> x4;
+  WC_TRY(c, {
+    x4.resize((size_t)(4 * n));
+    wc::particles_get_soa(c, x4.data(), nullptr, nullptr, true);
+  })
+  File F;
+  F.f = fopen(path, "wb");
+  if (!F.f) return fail(c, WC_ERR_ARG, std::string("cannot open ") + path);
+  // formatted in parallel in chunks, written in order
+  const double r = c->desc.particle_radius;
+  const int64_t chunk = 1 << 18;
+  const int64_t nchunk = (n + chunk - 1) / chunk;
+  const int nthr = (int)std::max<int64_t>(1, std::min<int64_t>(nchunk, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+  bool ok = true;
+  try {
+    for (int64_t c0 = 0; c0 < nchunk && ok; c0 += nthr) {
+      const int m = (int)std::min<int64_t>(nthr, nchunk - c0);
+      std::vector<std::string> text((size_t)m);
+      std::vector<std::thread> th;
+      for (int t = 0; t < m; ++t)
+        th.emplace_back([&, t] {
+          const int64_t p0 = (c0 + t) * chunk, p1 = std::min(n, p0 + chunk);
+          format_fluid_lines(x4.data(), r, p0, p1, text[(size_t)t]);
+        });
+      for (auto& t : th) t.join();
+      for (int t = 0; t < m && ok; ++t) ok = write_all(F.f, text[(size_t)t].data(), text[(size_t)t].size());
+    }
+  } catch (const std::exception&) {
+    return fail(c, WC_ERR_NOMEM, "formatting the fluid file failed");
+  }
+  const bool closed = fclose(F.f) == 0;
+  F.f = nullptr;
+  if (!ok || !closed) return fail(c, WC_ERR_ARG, std::string("write failed: ") + path);
+  return WC_OK;
+}
+
 
 wc_status wc_refinement_passes(const wc_ctx* c, int32_t* passes) {
   if (!c || !passes) return WC_ERR_ARG;

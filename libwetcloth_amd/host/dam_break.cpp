@@ -14,6 +14,7 @@
 int main(int argc, char** argv) {
   const int n = argc > 1 ? std::atoi(argv[1]) : 32;
   const int steps = argc > 2 ? std::atoi(argv[2]) : 5;
+  const char* outdir = argc > 3 ? argv[3] : nullptr;  // optional: write the particle files
   try {
     // 1. PCGSolver<double> on a 2-D Poisson matrix with a known solution
     const int m = 40, N = m * m;
@@ -67,6 +68,12 @@ int main(int argc, char** argv) {
     for (int s = 0; s < steps; ++s) all_ok &= core.stepSystem(1.0 / 480);
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (outdir) {
+      // the reference's output files of the frame (TwoDSceneSerializer)
+      wcamd::TwoDSceneSerializer ser;
+      ser.serializeFluid(scene, std::string(outdir) + "/fluid.obj");
+      ser.serializePositionOnly(scene, std::string(outdir) + "/pos.bin");
+    }
     std::vector<double> x4, v4, fB;
     scene.getParticles(x4, v4, fB);
     double vy = 0.0;

@@ -10,6 +10,8 @@
 //  * wcamd::TwoDScene / LinearizedImplicitEuler / WetClothCore: the phase calls
 //    of one liquid substep (Core/WetClothCore.cpp:77-300) on a scene whose
 //    particles and grids stay resident on the GPU.
+//  * wcamd::TwoDSceneSerializer: the reference's particle files
+//    (App/TwoDSceneSerializer.cpp) written from that scene.
 //
 // Errors of the library (bad arguments, HIP failures) throw wcamd::Error; the
 // reference has no exceptions because it has no failure modes of this kind.
@@ -120,6 +122,25 @@ class LinearizedImplicitEuler {
   int maxiters_;
   double residual_ = 0.0;
   int32_t iterations_ = 0;
+};
+
+// TwoDSceneSerializer's particle files (App/TwoDSceneSerializer.cpp) for a
+// liquid scene: serializePositionOnly (:161-170), loadPosOnly (:172-183, here
+// from a file name rather than an open stream) and the fluid file of
+// serializeScene (:135-159; the cloth, hair, boundary and spring files of a
+// liquid-only scene are not written).  Files are complete when the call
+// returns (the reference writes from a detached thread).
+class TwoDSceneSerializer {
+ public:
+  void serializePositionOnly(TwoDScene& s, const std::string& fn_pos) {
+    check(wc_serialize_position_only(s.handle(), fn_pos.c_str()), s.handle());
+  }
+  void loadPosOnly(TwoDScene& s, const std::string& fn_pos) {
+    check(wc_load_position_only(s.handle(), fn_pos.c_str()), s.handle());
+  }
+  void serializeFluid(TwoDScene& s, const std::string& fn_fluid) {
+    check(wc_serialize_fluid(s.handle(), fn_fluid.c_str()), s.handle());
+  }
 };
 
 // WetClothCore::stepSystem for liquid scenes: one substep per call

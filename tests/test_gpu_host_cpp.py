@@ -14,11 +14,11 @@ EXE = os.path.join(os.path.dirname(os.path.dirname(__file__)), "libwetcloth_amd"
                    "dam_break")
 
 
-def test_cpp_host_mirror(oracle):
+def test_cpp_host_mirror(oracle, tmp_path):
     assert os.path.exists(EXE), "build() compiles the host example"
     n, steps = 24, 5
-    out = subprocess.run([EXE, str(n), str(steps)], capture_output=True, text=True,
-                         timeout=120)
+    out = subprocess.run([EXE, str(n), str(steps), str(tmp_path)], capture_output=True,
+                         text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout.strip().splitlines()[-1])
     assert r["pcg_ok"] and r["pcg_max_error"] < 1e-9
@@ -28,5 +28,16 @@ def test_cpp_host_mirror(oracle):
     s = oracle.OracleScene((n, n, n), 1.0 / n, x, v)
     for _ in range(steps):
         s.substep(1.0 / 480, 1e-8, 1000)
-    _, vc, _ = s.particles()
+    xc, vc, _ = s.particles()
     assert abs(r["mean_vy"] - vc[1].mean()) < 1e-9 * abs(vc[1].mean())
+    # the serializer's files of the final state (wcamd::TwoDSceneSerializer):
+    # the two files agree with each other byte for byte through the oracle's
+    # restatement, and hold the oracle's positions
+    from oracle import serializer as OS
+    x4, gauss = OS.parse_position_only((tmp_path / "pos.bin").read_bytes())
+    assert gauss.size == 0 and x4.size == 4 * x.shape[1]
+    assert np.all(x4[3::4] == 0.0)
+    assert np.abs(x4.reshape(-1, 4)[:, :3].T - xc).max() < 1e-9
+    dx = 1.0 / n
+    radius = float(np.cbrt(3.0 * (dx * dx * dx / 8.0) / (4.0 * np.pi)))  # as dam_break.cpp
+    assert (tmp_path / "fluid.obj").read_bytes() == OS.fluid_text(x4, radius).encode()
