@@ -273,16 +273,22 @@ struct G2PArgs {
   int64_t n, cap;
   const double* u[3];
   const uint8_t* act;
+  const uint32_t* bstart;  // bucket ranges of the sorted particles (k_g2p_bucket)
   R dt, flip, invD;
   int solid_projection;
 };
 
 // mapNodeParticlesAPIC fluid branch (TwoDScene.cpp:6958-7019), advectScene
 // (SceneStepper.cpp:30-31) and solidProjection (TwoDScene.cpp:1782-1821).
-template <class R>
-__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= A.n) return;
+//
+// fetch(d, i, j, k, u) gives the face-d velocity of node (i, j, k) and
+// returns true, or returns false for a node outside the active buckets
+// (whose term the reference skips).  A fetch may instead give 0 for such a
+// node: the weight is >= 0, so the term adds +0 or -0, which leaves every
+// sum unchanged (a sum that starts at +0 never becomes -0).
+template <class R, class Fetch>
+__device__ __attribute__((always_inline)) inline void g2p_particle(const G2PArgs<R>& A, int64_t p,
+                                                                   Fetch fetch) {
   const Grid& g = A.g;
   const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
   R fv[3] = {0, 0, 0};
@@ -294,14 +300,13 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
       base[a] = stencil_base_r<R>(g, a, px[a], face_off_r<R>(d, a));
       stencil_weights_r<R>(g, px[a], base[a], face_off_r<R>(d, a), w[a]);
     }
-    const double* ud = A.u[d];
     for (int c2 = 0; c2 < 3; ++c2)
       for (int c1 = 0; c1 < 3; ++c1)
         for (int c0 = 0; c0 < 3; ++c0) {
           const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
-          if (!node_active(g, A.act, i, j, k)) continue;
+          R fnv;
+          if (!fetch(d, i, j, k, fnv)) continue;
           const R ww = w[0][c0] * w[1][c1] * w[2][c2];
-          const R fnv = (R)ud[node_index(g, i, j, k)];
           fv[d] += fnv * ww;
           const R e0 = node_pos_r<R>(g, i, face_off_r<R>(d, 0)) - px[0];
           const R e1 = node_pos_r<R>(g, j, face_off_r<R>(d, 1)) - px[1];
@@ -367,6 +372,57 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
     }
   }
   for (int d = 0; d < 3; ++d) A.x[d * A.cap + p] = nx[d];
+}
+
+// One thread per particle, gathers from global memory.
+template <class R>
+__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= A.n) return;
+  g2p_particle<R>(A, p, [&](int d, int i, int j, int k, R& u) {
+    if (!node_active(A.g, A.act, i, j, k)) return false;
+    u = (R)A.u[d][node_index(A.g, i, j, k)];
+    return true;
+  });
+}
+
+// The same per particle, with particles sorted by bucket: one workgroup per
+// bucket stages the 8^3 node region of its particles' stencils (the region
+// of k_p2g; inactive nodes hold 0) in LDS once, and each particle gathers
+// from there (a node outside the region, which only a particle outside its
+// bucket reaches, is read from global memory as above).
+template <class R>
+__global__ __launch_bounds__(256) void k_g2p_bucket(G2PArgs<R> A) {
+  const int64_t b = blockIdx.x;
+  const uint32_t pbeg = A.bstart[b], pend = A.bstart[b + 1];
+  if (pbeg == pend) return;
+  const Grid& g = A.g;
+  const int bi = (int)(b % g.nb[0]);
+  const int bj = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bk = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  const int o0 = bi * 4 - 2, o1 = bj * 4 - 2, o2 = bk * 4 - 2;
+  __shared__ double su[3][kTileNodes];
+  for (int l = threadIdx.x; l < kTileNodes; l += 256) {
+    const int i = o0 + (l % kTile), j = o1 + ((l / kTile) % kTile), k = o2 + l / (kTile * kTile);
+    const bool on = node_active(g, A.act, i, j, k);
+    const int64_t n = on ? node_index(g, i, j, k) : 0;
+    for (int d = 0; d < 3; ++d) su[d][l] = on ? A.u[d][n] : 0.0;
+  }
+  __syncthreads();
+  for (uint32_t c0 = pbeg; c0 < pend; c0 += 256) {
+    const uint32_t p = c0 + threadIdx.x;
+    if (p >= pend) continue;
+    g2p_particle<R>(A, p, [&](int d, int i, int j, int k, R& u) {
+      const int li = i - o0, lj = j - o1, lk = k - o2;
+      if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile && lk < kTile) {
+        u = (R)su[d][(lk * kTile + lj) * kTile + li];
+        return true;
+      }
+      if (!node_active(g, A.act, i, j, k)) return false;
+      u = (R)A.u[d][node_index(g, i, j, k)];
+      return true;
+    });
+  }
 }
 
 // ---- seeding ----
@@ -646,8 +702,14 @@ void g2p_launch(wc_ctx* ctx, Planes<R> P, double dt) {
   A.flip = (R)ctx->desc.flip_coeff;
   A.invD = (R)(4.0 / (ctx->g.dx * ctx->g.dx));
   A.solid_projection = ctx->desc.solid_projection;
-  hipLaunchKernelGGL(k_g2p_advect<R>, dim3(blocks(ctx->np, 256)), dim3(256), 0,
-                     ctx->stream, A);
+  A.bstart = ctx->bstart.p;
+  // with bstart describing the particle arrays, one workgroup per bucket
+  if (ctx->sorted)
+    hipLaunchKernelGGL(k_g2p_bucket<R>, dim3((unsigned)ctx->g.nbucket), dim3(256), 0,
+                       ctx->stream, A);
+  else
+    hipLaunchKernelGGL(k_g2p_advect<R>, dim3(blocks(ctx->np, 256)), dim3(256), 0,
+                       ctx->stream, A);
   WC_LAUNCH_CHECK();
 }
 
