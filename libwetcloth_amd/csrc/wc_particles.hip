@@ -286,13 +286,17 @@ struct G2PArgs {
 // (whose term the reference skips).  A fetch may instead give 0 for such a
 // node: the weight is >= 0, so the term adds +0 or -0, which leaves every
 // sum unchanged (a sum that starts at +0 never becomes -0).
-template <class R, class Fetch>
+// region(d, base) may return an LDS array holding face d's values at
+// [(c2 * kTile + c1) * kTile + c0] for the 27 stencil nodes base + (c0, c1,
+// c2) (then no node is fetched one by one), or null.
+template <class R, class Fetch, class Region>
 __device__ __attribute__((always_inline)) inline void g2p_particle(const G2PArgs<R>& A, int64_t p,
-                                                                   Fetch fetch) {
+                                                                   Fetch fetch, Region region) {
   const Grid& g = A.g;
   const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
   R fv[3] = {0, 0, 0};
   R B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     int base[3];
     R w[3][3];
@@ -300,21 +304,38 @@ __device__ __attribute__((always_inline)) inline void g2p_particle(const G2PArgs
       base[a] = stencil_base_r<R>(g, a, px[a], face_off_r<R>(d, a));
       stencil_weights_r<R>(g, px[a], base[a], face_off_r<R>(d, a), w[a]);
     }
-    for (int c2 = 0; c2 < 3; ++c2)
-      for (int c1 = 0; c1 < 3; ++c1)
-        for (int c0 = 0; c0 < 3; ++c0) {
-          const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
-          R fnv;
-          if (!fetch(d, i, j, k, fnv)) continue;
-          const R ww = w[0][c0] * w[1][c1] * w[2][c2];
-          fv[d] += fnv * ww;
-          const R e0 = node_pos_r<R>(g, i, face_off_r<R>(d, 0)) - px[0];
-          const R e1 = node_pos_r<R>(g, j, face_off_r<R>(d, 1)) - px[1];
-          const R e2 = node_pos_r<R>(g, k, face_off_r<R>(d, 2)) - px[2];
-          B[d * 3 + 0] += fnv * ww * e0 * A.invD;
-          B[d * 3 + 1] += fnv * ww * e1 * A.invD;
-          B[d * 3 + 2] += fnv * ww * e2 * A.invD;
-        }
+    // node offsets from the particle along each axis (the same operations
+    // for every node that shares the coordinate)
+    R e[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) e[a][c] = node_pos_r<R>(g, base[a] + c, face_off_r<R>(d, a)) - px[a];
+    auto add = [&](int c0, int c1, int c2, R fnv) {
+      const R ww = w[0][c0] * w[1][c1] * w[2][c2];
+      fv[d] += fnv * ww;
+      B[d * 3 + 0] += fnv * ww * e[0][c0] * A.invD;
+      B[d * 3 + 1] += fnv * ww * e[1][c1] * A.invD;
+      B[d * 3 + 2] += fnv * ww * e[2][c2] * A.invD;
+    };
+    if (const double* reg = region(d, base)) {
+#pragma unroll
+      for (int c2 = 0; c2 < 3; ++c2)
+#pragma unroll
+        for (int c1 = 0; c1 < 3; ++c1)
+#pragma unroll
+          for (int c0 = 0; c0 < 3; ++c0) add(c0, c1, c2, (R)reg[(c2 * kTile + c1) * kTile + c0]);
+    } else {
+#pragma unroll
+      for (int c2 = 0; c2 < 3; ++c2)
+#pragma unroll
+        for (int c1 = 0; c1 < 3; ++c1)
+#pragma unroll
+          for (int c0 = 0; c0 < 3; ++c0) {
+            R fnv;
+            if (fetch(d, base[0] + c0, base[1] + c1, base[2] + c2, fnv)) add(c0, c1, c2, fnv);
+          }
+    }
   }
   for (int t = 0; t < 9; ++t) A.C[t * A.cap + p] = B[t] * A.flip;
   R nx[3];
@@ -379,11 +400,14 @@ template <class R>
 __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= A.n) return;
-  g2p_particle<R>(A, p, [&](int d, int i, int j, int k, R& u) {
-    if (!node_active(A.g, A.act, i, j, k)) return false;
-    u = (R)A.u[d][node_index(A.g, i, j, k)];
-    return true;
-  });
+  g2p_particle<R>(
+      A, p,
+      [&](int d, int i, int j, int k, R& u) {
+        if (!node_active(A.g, A.act, i, j, k)) return false;
+        u = (R)A.u[d][node_index(A.g, i, j, k)];
+        return true;
+      },
+      [](int, const int*) -> const double* { return nullptr; });
 }
 
 // The same per particle, with particles sorted by bucket: one workgroup per
@@ -412,16 +436,27 @@ __global__ __launch_bounds__(256) void k_g2p_bucket(G2PArgs<R> A) {
   for (uint32_t c0 = pbeg; c0 < pend; c0 += 256) {
     const uint32_t p = c0 + threadIdx.x;
     if (p >= pend) continue;
-    g2p_particle<R>(A, p, [&](int d, int i, int j, int k, R& u) {
-      const int li = i - o0, lj = j - o1, lk = k - o2;
-      if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile && lk < kTile) {
-        u = (R)su[d][(lk * kTile + lj) * kTile + li];
-        return true;
-      }
-      if (!node_active(g, A.act, i, j, k)) return false;
-      u = (R)A.u[d][node_index(g, i, j, k)];
-      return true;
-    });
+    g2p_particle<R>(
+        A, p,
+        [&](int d, int i, int j, int k, R& u) {
+          const int li = i - o0, lj = j - o1, lk = k - o2;
+          if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile && lk < kTile) {
+            u = (R)su[d][(lk * kTile + lj) * kTile + li];
+            return true;
+          }
+          if (!node_active(g, A.act, i, j, k)) return false;
+          u = (R)A.u[d][node_index(g, i, j, k)];
+          return true;
+        },
+        // the whole stencil inside the region (always, for a particle in
+        // its bucket): one check per face instead of one per node
+        [&](int d, const int* base) -> const double* {
+          const int l0 = base[0] - o0, l1 = base[1] - o1, l2 = base[2] - o2;
+          const bool in = (unsigned)l0 <= (unsigned)(kTile - 3) &&
+                          (unsigned)l1 <= (unsigned)(kTile - 3) &&
+                          (unsigned)l2 <= (unsigned)(kTile - 3);
+          return in ? &su[d][(l2 * kTile + l1) * kTile + l0] : nullptr;
+        });
   }
 }
 
