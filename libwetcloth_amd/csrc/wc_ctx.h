@@ -103,6 +103,7 @@ struct SweepPlan {
   DevBuf<double> cf, cb;            // 32-byte factor records (forward / backward)
   DevBuf<double> coef;              // 7 matrix entries per cell
   DevBuf<double> fin;               // 96-byte factor-pass inputs per cell
+  DevBuf<double> rsum;              // 80-byte row summaries per cell (plan fill only)
   DevBuf<double> zf, zb;            // sweep outputs (zb = preconditioned vector)
   DevBuf<double> rb, sb, xb;        // PCG vectors r, s, x in box layout
   // mixed-precision solve: FP32 copies of the records and entries, FP32

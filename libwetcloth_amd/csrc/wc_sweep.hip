@@ -249,7 +249,86 @@ __host__ __device__ inline uint32_t operand_word(uint32_t sc, int presb, int src
   return word;
 }
 
-__global__ void k_fill_box(BoxArgs A) {
+// One DOF's row of the assembled matrix, summarised for the fill: written
+// by k_row_sums (in node order, so the ELL rows are read coalesced) at the
+// DOF's slot of the skewed layout, and read by k_fill_box at that slot and
+// at the slots of the cell's six neighbours (fixed offsets within a tile).
+// A slot without a DOF holds zeros (id1 = 0).
+struct __attribute__((aligned(16))) RowSum {
+  uint32_t id1;       // DOF id + 1, or 0
+  uint32_t bits;      // R_DIAG, R_LO + d, R_UP + d
+  double diag;        // A(me, me)
+  double lo[3];       // A(me, me - e_d)
+  double up[3];       // A(me, me + e_d)
+  uint32_t upcol[3];  // the column (DOF id) of up[d]
+  uint32_t pad;
+};
+static_assert(sizeof(RowSum) == 80, "RowSum is five dwordx4");
+constexpr uint32_t R_DIAG = 1u;
+constexpr int R_LO = 1;  // bits 1..3: the row holds A(me, me - e_d)
+constexpr int R_UP = 4;  // bits 4..6: the row holds A(me, me + e_d)
+
+__global__ __launch_bounds__(256) void k_row_sums(BoxArgs A, RowSum* S) {
+  const Grid& g = A.g;
+  for (int64_t n = blockIdx.x * 256ll + threadIdx.x; n < g.nnode; n += (int64_t)gridDim.x * 256) {
+    if (!A.isdof[n]) continue;
+    int c[3];
+    node_coords(g, n, c[0], c[1], c[2]);
+    const int x = c[0] - A.o[0], y = c[1] - A.o[1], z = c[2] - A.o[2];
+    const int li = x % kT, lj = y % kT;
+    const int64_t t = skew_slot(A.K, x / kT, y / kT, li, lj, z + li + lj);
+    const uint32_t me = A.dofscan[n];
+    const int64_t row = 7 * (int64_t)me;
+    const uint32_t len = A.rowlen[me];
+    uint32_t mc[7];
+    double mv[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+      mc[s] = (uint32_t)s < len ? A.col[row + s] : 0xFFFFFFFFu;
+      mv[s] = A.val[row + s];
+    }
+    RowSum r;
+    r.id1 = me + 1;
+    r.bits = 0;
+    r.diag = 0;
+    r.pad = 0;
+#pragma unroll
+    for (int s = 0; s < 7; ++s)
+      if (mc[s] == me) {
+        r.bits |= R_DIAG;
+        r.diag = mv[s];
+      }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      r.lo[d] = r.up[d] = 0;
+      r.upcol[d] = 0;
+#pragma unroll
+      for (int dir = 0; dir < 2; ++dir) {
+        int q[3] = {c[0], c[1], c[2]};
+        q[d] += dir ? 1 : -1;
+        if (!in_range(g, q[0], q[1], q[2])) continue;
+        const int64_t nn = node_index(g, q[0], q[1], q[2]);
+        if (!A.isdof[nn]) continue;
+        const uint32_t other = A.dofscan[nn];
+#pragma unroll
+        for (int s = 0; s < 7; ++s)
+          if (mc[s] == other) {
+            if (dir) {
+              r.up[d] = mv[s];
+              r.upcol[d] = other;
+              r.bits |= 1u << (R_UP + d);
+            } else {
+              r.lo[d] = mv[s];
+              r.bits |= 1u << (R_LO + d);
+            }
+          }
+      }
+    }
+    S[t] = r;
+  }
+}
+
+__global__ void k_fill_box(BoxArgs A, const RowSum* S) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= A.nslot) return;
   const int lane = (int)(t % kThreads);
@@ -258,101 +337,93 @@ __global__ void k_fill_box(BoxArgs A) {
   const int tile = (int)(q / A.K.nsteps);
   const int li = lane % kT, lj = lane / kT;
   const int z = w - li - lj;
-  const int x = (tile % A.K.ntile_a) * kT + li;
-  const int y = (tile / A.K.ntile_a) * kT + lj;
-  const int i = A.o[0] + x, j = A.o[1] + y, k = A.o[2] + z;
-  const bool inbox = z >= 0 && z < A.K.Bz;
-  const Grid& g = A.g;
+  const int ta = tile % A.K.ntile_a, tb = tile / A.K.ntile_a;
   uint32_t code = 0, fcode = 0;
   int32_t me = -1;
   double cf[kCoef] = {0, 0, 0, 0, 0, 0, 0};
   FacIn f;
   f.adiag = 0;
   for (int d = 0; d < 3; ++d) f.lo[d] = f.oth[d][0] = f.oth[d][1] = 0;
-  if (inbox && in_range(g, i, j, k)) {
-    const int64_t n = node_index(g, i, j, k);
-    if (A.isdof[n]) {
-      me = (int32_t)A.dofscan[n];
-      code |= C_DOF;
-      // this row and, per lower neighbour, its row: read once each (an ELL
-      // row always has 7 allocated entries; those past the length are unused)
-      const int64_t row = 7 * (int64_t)me;
-      const uint32_t len = A.rowlen[me];
-      uint32_t mc[7];
-      double mv[7];
-#pragma unroll
-      for (int s = 0; s < 7; ++s) {
-        mc[s] = (uint32_t)s < len ? A.col[row + s] : 0xFFFFFFFFu;
-        mv[s] = A.val[row + s];
-      }
-#pragma unroll
-      for (int s = 0; s < 7; ++s)
-        if (mc[s] == (uint32_t)me) {
-          code |= C_DIAG;
-          cf[0] = mv[s];
-          f.adiag = mv[s];
-        }
-      int64_t lkey[3], ukey[3];
-      for (int d = 0; d < 3; ++d) {
-        lkey[d] = ukey[d] = 0x7FFFFFFFFFFFll;
-        // lower neighbour k_d: its row holds (k_d, me) -> column k_d of L;
-        // this row holds A(me, k_d) for the SpMV
-        int q[3] = {i, j, k};
-        q[d] -= 1;
-        if (in_range(g, q[0], q[1], q[2])) {
-          const int64_t nn = node_index(g, q[0], q[1], q[2]);
-          if (A.isdof[nn]) {
-            const uint32_t other = A.dofscan[nn];
-            lkey[d] = other;
-            const int64_t orow = 7 * (int64_t)other;
-            const uint32_t olen = A.rowlen[other];
-            double kdiag = 0;
-            int noth = 0;
-#pragma unroll
-            for (int s = 0; s < 7; ++s) {
-              if ((uint32_t)s >= olen) continue;
-              const uint32_t c = A.col[orow + s];
-              const double v = A.val[orow + s];
-              if (c == (uint32_t)me) {
-                f.lo[d] = v;
-                code |= 1u << (C_LOWER + d);
-              } else if (c == other) {
-                kdiag = v;
-              } else if (c > other && noth < 2) {
-                f.oth[d][noth++] = v;  // ascending: rows are sorted
-              }
-            }
-            if (kdiag == 0) fcode |= 1u << (F_KNULL + d);
-            fcode |= (uint32_t)noth << (F_NOTH + 2 * d);
-#pragma unroll
-            for (int s = 0; s < 7; ++s)
-              if (mc[s] == other) {
-                cf[1 + d] = mv[s];
-                code |= 1u << (C_ALO + d);
-              }
-          }
-        }
-        // upper neighbour: this row holds A(me, other); L(other, me) is
-        // written here by the factor pass of the neighbour
-        int u[3] = {i, j, k};
-        u[d] += 1;
-        if (in_range(g, u[0], u[1], u[2])) {
-          const int64_t nn = node_index(g, u[0], u[1], u[2]);
-          if (A.isdof[nn]) {
-            const uint32_t other = A.dofscan[nn];
-            ukey[d] = other;
-#pragma unroll
-            for (int s = 0; s < 7; ++s)
-              if (mc[s] == other) {
-                cf[4 + d] = mv[s];
-                code |= (1u << (C_UPPER + d)) | (1u << (C_AUP + d));
-              }
-          }
-        }
-      }
-      code |= (uint32_t)perm_of(lkey[0], lkey[1], lkey[2]) << C_LORD;
-      code |= (uint32_t)perm_of(ukey[0], ukey[1], ukey[2]) << C_UORD;
+  const RowSum own = S[t];  // padding slots hold zeros
+  if (own.id1) {
+    me = (int32_t)(own.id1 - 1);
+    code |= C_DOF;
+    if (own.bits & R_DIAG) {
+      code |= C_DIAG;
+      cf[0] = own.diag;
+      f.adiag = own.diag;
     }
+    int64_t lkey[3], ukey[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lkey[d] = ukey[d] = 0x7FFFFFFFFFFFll;
+      // the slots of the -e_d and +e_d neighbours (-1 outside the box, where
+      // no cell is a DOF): the previous / next step of the lane one to the
+      // left (below), or across the tile face
+      int64_t tl = -1, tu = -1;
+      if (d == 0) {
+        tl = li > 0 ? t - kThreads - 1 : (ta > 0 ? t + A.K.dx : -1);
+        tu = li < kT - 1 ? t + kThreads + 1 : (ta + 1 < A.K.ntile_a ? t - A.K.dx : -1);
+      } else if (d == 1) {
+        tl = lj > 0 ? t - kThreads - kT : (tb > 0 ? t + A.K.dy : -1);
+        tu = lj < kT - 1 ? t + kThreads + kT : (tb + 1 < A.K.ntile_b ? t - A.K.dy : -1);
+      } else {
+        tl = z > 0 ? t - kThreads : -1;
+        tu = z + 1 < A.K.Bz ? t + kThreads : -1;
+      }
+      // lower neighbour k_d: its row holds (k_d, me) -> column k_d of L;
+      // this row holds A(me, k_d) for the SpMV
+      if (tl >= 0) {
+        const RowSum L = S[tl];
+        if (L.id1) {
+          const uint32_t other = L.id1 - 1;
+          lkey[d] = other;
+          if (L.bits & (1u << (R_UP + d))) {  // its +e_d entry is (k_d, me)
+            f.lo[d] = L.up[d];
+            code |= 1u << (C_LOWER + d);
+          }
+          const double kdiag = (L.bits & R_DIAG) ? L.diag : 0.0;
+          // row k_d's other entries above its diagonal, ascending column
+          uint32_t oc[2];
+          double ov[2];
+          int noth = 0;
+#pragma unroll
+          for (int e = 0; e < 3; ++e)
+            if (e != d && (L.bits & (1u << (R_UP + e)))) {
+              oc[noth] = L.upcol[e];
+              ov[noth] = L.up[e];
+              ++noth;
+            }
+          if (noth == 2 && oc[1] < oc[0]) {
+            f.oth[d][0] = ov[1];
+            f.oth[d][1] = ov[0];
+          } else {
+            if (noth > 0) f.oth[d][0] = ov[0];
+            if (noth > 1) f.oth[d][1] = ov[1];
+          }
+          if (kdiag == 0) fcode |= 1u << (F_KNULL + d);
+          fcode |= (uint32_t)noth << (F_NOTH + 2 * d);
+          if (own.bits & (1u << (R_LO + d))) {
+            cf[1 + d] = own.lo[d];
+            code |= 1u << (C_ALO + d);
+          }
+        }
+      }
+      // upper neighbour: this row holds A(me, other); L(other, me) is
+      // written here by the factor pass of the neighbour
+      if (tu >= 0) {
+        const uint32_t uid1 = S[tu].id1;
+        if (uid1) {
+          ukey[d] = uid1 - 1;
+          if (own.bits & (1u << (R_UP + d))) {
+            cf[4 + d] = own.up[d];
+            code |= (1u << (C_UPPER + d)) | (1u << (C_AUP + d));
+          }
+        }
+      }
+    }
+    code |= (uint32_t)perm_of(lkey[0], lkey[1], lkey[2]) << C_LORD;
+    code |= (uint32_t)perm_of(ukey[0], ukey[1], ukey[2]) << C_UORD;
   }
   // the sweeps' and the factor's term order
   uint32_t sc = 0;
@@ -1189,7 +1260,15 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   A.cb = reinterpret_cast<Cell*>(P.cb.p);
   A.coef = P.coef.p;
   A.fin = reinterpret_cast<FacIn*>(P.fin.p);
-  hipLaunchKernelGGL(k_fill_box, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, A);
+  // row summaries at the DOFs' slots (zeros elsewhere), then the fill
+  P.rsum.ensure(P.ncell * sizeof(RowSum) / sizeof(double));
+  RowSum* rs = reinterpret_cast<RowSum*>(P.rsum.p);
+  WC_HIP(hipMemsetAsync(rs, 0, sizeof(RowSum) * P.ncell, st));
+  hipLaunchKernelGGL(k_row_sums, dim3((unsigned)std::min<int64_t>(4096, blocks(g.nnode, 256))),
+                     dim3(256), 0, st, A, rs);
+  WC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fill_box, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, A,
+                     (const RowSum*)rs);
   WC_LAUNCH_CHECK();
   WC_HIP(hipStreamSynchronize(st));  // 'order' must outlive its copy
   P.valid = true;
@@ -1360,6 +1439,7 @@ void SweepPlan::release() {
   cb.release();
   coef.release();
   fin.release();
+  rsum.release();
   rb.release();
   sb.release();
   xb.release();
