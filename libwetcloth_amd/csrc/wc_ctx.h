@@ -111,6 +111,7 @@ struct SweepPlan {
   DevBuf<float> cf32, cb32, coef32, zf32, zb32, rb32, sb32;
   DevBuf<double> bb;
   DevBuf<int> progress, flags;
+  DevBuf<double> tdot;              // per-tile z . r partials of the last apply
   // The forward sweep's ticket counter is zero: the sweeps zero each
   // other's counter, so only the first apply after the plan needs a clear.
   bool ticks_zero = false;
@@ -244,7 +245,8 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
                    int64_t ndof);
 // MIC(0) factor of the prepared box (pipelined), into the sweep records
 void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr);
-// MIC(0) apply on box vectors: r (box) -> P.zb
+// MIC(0) apply on box vectors: r (box) -> P.zb; the backward sweep also
+// leaves the per-tile partials of z . r in P.tdot (P.ntiles of them)
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
                  int64_t ndof);
 void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y);

@@ -679,8 +679,9 @@ static void settle_timer(hipStream_t st, PcgWork& w) {
 // is the SpMV with the s . z partials, one update kernel (alpha, x, r and the
 // |r|_inf partials), then -- enqueued before the host has read the residual,
 // since they touch only s and the sweep outputs, which are dead once the
-// residual meets the tolerance -- the MIC(0) apply, the z . r partials and
-// the update of s with beta.
+// residual meets the tolerance -- the MIC(0) apply (whose backward sweep
+// also leaves the z . r partials, one per tile) and the update of s with
+// beta.
 static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double* b,
                          double* x_dof, double tolr, int maxit, double* residual_out,
                          int* iters_out) {
@@ -694,10 +695,8 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   KTimer* tm = w.timer.on ? &w.timer : nullptr;
   box_gather(st, P, b, r);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
-  sweep_apply(st, P, r, tm, ndof);
-  hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
-                     w.partial.p);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+  sweep_apply(st, P, r, tm, ndof);  // z and the z . r tile partials
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ntiles,
                      w.scal.p, (int)R_SET_RHO);
   WC_LAUNCH_CHECK();
   const double rho = read_scal(st, w, S_RHO);
@@ -721,11 +720,9 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     mark_residual(st, ev);
     if (it + 1 < maxit) {
       sweep_apply(st, P, r, tm, ndof);
-      hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
-                         w.partial.p);
       const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
       hipLaunchKernelGGL(k_update_s_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
-                         nc, w.partial.p, nb, w.scal.p, rho_slot, rho_next);
+                         nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);
       WC_LAUNCH_CHECK();
       rho_slot = rho_next;
     }
@@ -792,10 +789,8 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     }
     ++w.passes;
     const double inner_tol = std::max(tolr, eta * res);
-    sweep_apply32(st, P, r, tm, ndof);
-    hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
-                       w.partial.p);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+    sweep_apply32(st, P, r, tm, ndof);  // z and the z . r tile partials
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ntiles,
                        w.scal.p, (int)R_SET_RHO);
     WC_LAUNCH_CHECK();
     const double rho = read_scal(st, w, S_RHO);
@@ -821,11 +816,9 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
       ++total;
       if (total < maxit) {
         sweep_apply32(st, P, r, tm, ndof);
-        hipLaunchKernelGGL(k_dot_partial<float>, dim3(nb), dim3(kRedThreads), 0, st, z, r, nc,
-                           w.partial.p);
         const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
         hipLaunchKernelGGL(k_update_s_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
-                           nc, w.partial.p, nb, w.scal.p, rho_slot, rho_next);
+                           nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);
         WC_LAUNCH_CHECK();
         rho_slot = rho_next;
       }

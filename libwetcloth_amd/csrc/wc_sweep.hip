@@ -663,6 +663,8 @@ struct SweepArgs {
   const V* in;             // forward: r; backward: the forward result
   V* out;                  // the sweep result, or the factor's invdiag
   V* reset;                // forward sweep: set to the sentinel step by step
+  const V* dot_r;          // backward sweep: r (the forward sweep's input) ...
+  double* dot_out;         // ... and its per-tile partials of z . r
   int* ticket;
   int* next_ticket;        // sweeps: zeroed by the block that takes ticket 0
   int* abort_flag;
@@ -835,8 +837,12 @@ struct ApplyRing {
   static constexpr int kCellOps = kThreads * (int)sizeof(CellT<V>) / 1024;
   static constexpr int kRhsOps = kThreads * (int)sizeof(V) / 1024;
   static constexpr int kCodeOps = kThreads * 4 / 1024;
-  static constexpr int kStepOps = kRhsOps + kCodeOps;  // rhs loader, per step
-  static_assert(kCellOps * kAhead <= 63 && kStepOps * kAhead + kRhsOps * (kAhead - 1) <= 63,
+  // rhs loader, per step: rhs and code loads, and then (forward) the
+  // sentinel stores or (backward) the loads of r for the z . r partials
+  static constexpr int kStepOps = kRhsOps + kCodeOps;
+  static constexpr int kBwdOps = kStepOps + kRhsOps;
+  static_assert(kCellOps * kAhead <= 63 && kStepOps * kAhead + kRhsOps * (kAhead - 1) <= 63 &&
+                    kBwdOps * kAhead <= 63,
                 "a loader's loads and stores in flight fit its 6-bit counter");
 };
 template <int KIND>
@@ -874,6 +880,9 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
   __shared__ CellT<V> rc[kR][kThreads];
   __shared__ V rr[kR][kThreads];
   __shared__ uint32_t rk[kR][kThreads];
+  // backward sweep: r of each step, for the z . r partials
+  __shared__ V rq[BWD ? kR : 1][kThreads];
+  __shared__ double sdot[kThreads / 64];
   auto LZ = [](int par, int lj, int li) -> uint32_t {
     return kLz + (uint32_t)((par * kT + lj) * (kT + 1) + li);
   };
@@ -962,6 +971,15 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
                   gk + k * 1024 + lane * 16,
                   (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(&rk[r][0]) + k * 1024),
                   16, 0, 0);
+            if constexpr (BWD) {
+              const char* gq = reinterpret_cast<const char*>(S.dot_r) + (size_t)s0 * sizeof(V);
+#pragma unroll
+              for (int k = 0; k < Ring::kRhsOps; ++k)
+                __builtin_amdgcn_global_load_lds(
+                    gq + k * 1024 + lane * 16,
+                    (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(&rq[r][0]) + k * 1024),
+                    16, 0, 0);
+            }
           }
         };
         // The forward pass's rhs loader also sets the step's slots of
@@ -986,6 +1004,8 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
           // loads (the counter is in order; early steps wait for more)
           if (cells)
             wait_vmcnt<Ring::kCellOps * (Ring::kAhead - 2)>();
+          else if (BWD)
+            wait_vmcnt<Ring::kBwdOps * (Ring::kAhead - 2)>();
           else if (w < Ring::kAhead - 1)
             wait_vmcnt<Ring::kStepOps * (Ring::kAhead - 2)>();
           else
@@ -1080,14 +1100,18 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
       const int pos = BWD ? kThreads - 1 - (int)threadIdx.x : (int)threadIdx.x;
       const uint32_t own = (uint32_t)((lj + 1) * (kT + 1) + li + 1);
       CellT<V> c;
-      V rhs;
+      V rhs, rv = 0;
       uint32_t op = 0;
       auto take = [&](int w) {
         const int r = w & (kR - 1);
         c = rc[r][pos];
         rhs = rr[r][pos];
         op = rk[r][pos];
+        if constexpr (BWD) rv = rq[r][pos];
       };
+      // backward sweep: this lane's z . r over its pencil, in FP64 (padding
+      // slots have z = r = 0 and add +0)
+      double dot = 0.0;
       uint32_t src[3];
       auto sources = [&](int w) {
         const uint32_t at0 = PL(w - 1) + own;
@@ -1110,12 +1134,28 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
         // slots), so no slot keeps the sentinel after the pass
         store_coherent(at(S.out, base + (uint32_t)(sgn * w)), z);
         lds[PL(w) + own] = z;
+        if constexpr (BWD) dot += (double)z * (double)rv;
         if (w + 1 < nsteps) {
           take(w + 1);
           sources(w + 1);
         }
       }
+      if constexpr (BWD) {
+        // the tile's partial: a fixed tree over the 64 lanes of each wave,
+        // then the four waves in order
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
+        if ((threadIdx.x & 63) == 0) sdot[threadIdx.x >> 6] = dot;
+      }
       __syncthreads();
+      if constexpr (BWD) {
+        if (threadIdx.x == 0) {
+          double t = sdot[0];
+#pragma unroll
+          for (int q = 1; q < kThreads / 64; ++q) t += sdot[q];
+          S.dot_out[b * S.K.ntile_a + a] = t;
+        }
+      }
     } else {
     V zprev = 0;
     // Record prefetch ring of the factor pass: slot s holds the inputs of
@@ -1236,6 +1276,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   WC_HIP(hipMemcpyAsync(P.order.p, order.data(), sizeof(uint32_t) * P.ntiles,
                         hipMemcpyHostToDevice, st));
   P.progress.ensure(8);
+  P.tdot.ensure(P.ntiles);
   // tickets and the abort flag start at zero (the abort flag is cleared
   // again by each factor, not by the applies: once set it stays set)
   WC_HIP(hipMemsetAsync(P.progress.p, 0, sizeof(int) * 8, st));
@@ -1352,6 +1393,8 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.in = zf;
   S.out = zb;
   S.reset = nullptr;
+  S.dot_r = r;
+  S.dot_out = P.tdot.p;
   S.code = P.opb.p;
   S.ticket = tick + 1;
   S.next_ticket = tick;
@@ -1455,6 +1498,7 @@ void SweepPlan::release() {
   bb.release();
   order.release();
   progress.release();
+  tdot.release();
   flags.release();
   valid = false;
 }
