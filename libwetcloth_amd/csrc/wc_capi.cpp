@@ -699,7 +699,17 @@ wc_status wc_refinement_passes(const wc_ctx* c, int32_t* passes) {
 wc_status wc_kernel_stats(const wc_ctx* c, int32_t kernel, double* ms, int64_t* launches,
                           int64_t* units) {
   if (!c || kernel < 0 || kernel >= wc::KTimer::COUNT) return WC_ERR_ARG;
-  const wc::KTimer& t = c->pcg.timer;
+  // launches still pending in the timer are harvested first (the solves do
+  // not wait for them); the statistics are the context's mutable state
+  wc::KTimer& t = const_cast<wc_ctx*>(c)->pcg.timer;
+  if (!t.pending.empty()) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return WC_ERR_HIP;
+    try {
+      t.harvest();
+    } catch (const wc::HipError&) {
+      return WC_ERR_HIP;
+    }
+  }
   if (ms) *ms = t.ms[kernel];
   if (launches) *launches = t.launches[kernel];
   if (units) *units = t.units[kernel];

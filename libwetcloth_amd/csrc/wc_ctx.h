@@ -32,12 +32,17 @@ template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t cap = 0;  // elements
+  // A buffer that has to grow again (the DOF count and the sweep plan vary
+  // from substep to substep) gets 25 % headroom, so that the free and
+  // allocation, which stall the device, are rare.
   void ensure(size_t n) {
     if (n <= cap && p) return;
+    const bool regrow = p != nullptr;
     if (p) WC_HIP(hipFree(p));
     p = nullptr;
     cap = 0;
     size_t c = n ? n : 1;
+    if (regrow) c += c / 4;
     WC_HIP(hipMalloc((void**)&p, c * sizeof(T)));
     cap = c;
   }
@@ -82,6 +87,7 @@ struct KTimer {
     pending.push_back(Rec{kind, units, a, b});
   }
   void harvest();  // the stream has been synchronized: accumulate pending
+  void harvest_ready();  // accumulate the pending pairs that have completed
   void reset();
   void release();
 };

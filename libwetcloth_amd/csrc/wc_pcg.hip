@@ -433,6 +433,21 @@ void KTimer::harvest() {
   pending.clear();
 }
 
+void KTimer::harvest_ready() {
+  // the pairs were recorded on one stream, so they complete in order
+  size_t k = 0;
+  for (; k < pending.size(); ++k) {
+    const hipError_t q = hipEventQuery(pending[k].b);
+    if (q == hipErrorNotReady) break;
+    WC_HIP(q);
+  }
+  if (k == 0) return;
+  std::vector<Rec> rest(pending.begin() + (ptrdiff_t)k, pending.end());
+  pending.resize(k);
+  harvest();
+  pending.swap(rest);
+}
+
 void KTimer::reset() {
   for (const Rec& r : pending) {
     spare.push_back(r.a);
@@ -665,12 +680,11 @@ static double box_residual_max(PcgWork& w, const SweepPlan& P, hipEvent_t ev) {
 
 static void mark_residual(hipStream_t st, hipEvent_t ev) { WC_HIP(hipEventRecord(ev, st)); }
 
-// Harvest the kernel timer once everything enqueued has completed.
-static void settle_timer(hipStream_t st, PcgWork& w) {
-  if (w.timer.on && !w.timer.pending.empty()) {
-    WC_HIP(hipStreamSynchronize(st));
-    w.timer.harvest();
-  }
+// Harvest the kernel timer's completed launches without waiting (the rest
+// are harvested later, at the latest when the statistics are read), so the
+// solve ends without draining the stream.
+static void settle_timer(hipStream_t, PcgWork& w) {
+  if (w.timer.on && !w.timer.pending.empty()) w.timer.harvest_ready();
 }
 
 // The PCG loop of PCGSolver::solve (PCGSolver.h:280-304) on box vectors: the
@@ -774,16 +788,17 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
                        w.scal.p);
     WC_LAUNCH_CHECK();
+    // the solution so far, in case this pass is the last (enqueued before
+    // the read-back, so the device is busy while the host wakes up)
+    box_scatter(st, P, x, x_dof);
     const double res = read_scal(st, w, S_RES);
     *residual_out = res;
     *iters_out = total;
     if (res <= tolr) {
-      box_scatter(st, P, x, x_dof);
       settle_timer(st, w);
       return true;
     }
     if (total >= maxit || w.passes > w.mixed_refinements || res != res) {
-      box_scatter(st, P, x, x_dof);
       settle_timer(st, w);
       return false;
     }
