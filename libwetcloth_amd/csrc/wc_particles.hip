@@ -82,18 +82,31 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int64_t n, int64_t cap, co
                                                       uint32_t* pid2) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const uint32_t b = p < n ? key[p] : kNoKey;
+  // the particle's state is read before the run takes its space, so the
+  // loads are in flight during the atomic's round trip
+  R xs[3], vs[3], cs[9];
+  uint32_t id = 0, sb = 0;
+  if (b != kNoKey) {
+    sb = start[b];
+    for (int d = 0; d < 3; ++d) {
+      xs[d] = x[d * cap + p];
+      vs[d] = v[d * cap + p];
+    }
+    for (int t = 0; t < 9; ++t) cs[t] = C[t * cap + p];
+    id = pid[p];
+  }
   const WaveRun r = wave_run(b);
   uint32_t base = 0;
   if (b != kNoKey && r.rank == 0) base = atomicAdd(&cursor[b], (uint32_t)r.len);
   base = __shfl(base, r.head);
   if (b == kNoKey) return;
-  const int64_t q = (int64_t)start[b] + base + r.rank;
+  const int64_t q = (int64_t)sb + base + r.rank;
   for (int d = 0; d < 3; ++d) {
-    x2[d * cap + q] = x[d * cap + p];
-    v2[d * cap + q] = v[d * cap + p];
+    x2[d * cap + q] = xs[d];
+    v2[d * cap + q] = vs[d];
   }
-  for (int t = 0; t < 9; ++t) C2[t * cap + q] = C[t * cap + p];
-  pid2[q] = pid[p];
+  for (int t = 0; t < 9; ++t) C2[t * cap + q] = cs[t];
+  pid2[q] = id;
 }
 
 template <class R>
