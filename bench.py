@@ -12,7 +12,7 @@ form (the reported value) and in the plain FP64 form (reported beside it,
 with the ratio and the positions' difference after the same steps).
 
 Prints ONE JSON line (see the keys below).  --gpus N > 1 exits with a message:
-this build runs on a single GPU.
+the library runs the substep on a single GPU.
 """
 from __future__ import annotations
 
@@ -177,7 +177,7 @@ def main():
         copy_calibration()
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "the substep runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import numpy as np
