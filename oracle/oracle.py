@@ -116,6 +116,20 @@ def lib():
         L.wco_pcg_solve.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
                                     C.c_int, C.c_double, C.c_double, _dp,
                                     C.POINTER(C.c_int)]
+        _i32p_ = C.POINTER(C.c_int32)
+        L.wco_amg_levels.restype = C.c_int
+        L.wco_amg_levels.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                     C.c_int, C.POINTER(C.c_int64), C.c_int]
+        L.wco_amg_coarse.restype = C.c_int64
+        L.wco_amg_coarse.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                     C.c_int, _u32p, _u32p, _u32p, _dp, C.c_int64]
+        L.wco_amg_rbgs.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, _dp, _dp, C.c_int]
+        L.wco_amg_vcycle.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                     C.c_int, _dp, _dp]
+        L.wco_amgpcg_solve.restype = C.c_int
+        L.wco_amgpcg_solve.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                       C.c_int, _dp, _dp, C.c_double, C.c_int, _dp,
+                                       C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -347,6 +361,71 @@ def pcg(rs, col, val, b, tol=1e-8, maxit=500, tau=0.97, mdr=0.25):
     it = C.c_int(0)
     ok = lib().wco_pcg_solve(n, ptr(rs, _u32p), ptr(col, _u32p), ptr(val), ptr(b), ptr(x),
                              tol, maxit, tau, mdr, C.byref(res), C.byref(it))
+    return bool(ok), x, it.value, res.value
+
+
+# ------------------------------------------------------------ AMG-PCG
+
+def _amg_args(rs, col, val, ijk):
+    rs = np.ascontiguousarray(rs, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32)
+    return rs, col, val, ijk
+
+
+def amg_levels(rs, col, val, ijk, dims):
+    rs, col, val, ijk = _amg_args(rs, col, val, ijk)
+    sizes = np.zeros(32, dtype=np.int64)
+    nl = lib().wco_amg_levels(len(rs) - 1, ptr(rs, _u32p), ptr(col, _u32p), ptr(val),
+                              ptr(ijk, C.POINTER(C.c_int32)), *dims,
+                              sizes.ctypes.data_as(C.POINTER(C.c_int64)), 32)
+    return list(sizes[:nl])
+
+
+def amg_coarse(rs, col, val, ijk, dims):
+    """One coarsening step: (parent, coarse rowstart, col, val)."""
+    rs, col, val, ijk = _amg_args(rs, col, val, ijk)
+    n = len(rs) - 1
+    cap = len(col)
+    parent = np.zeros(n, dtype=np.uint32)
+    crs = np.zeros(n + 1, dtype=np.uint32)
+    ccol = np.zeros(cap, dtype=np.uint32)
+    cval = np.zeros(cap)
+    nc = lib().wco_amg_coarse(n, ptr(rs, _u32p), ptr(col, _u32p), ptr(val),
+                              ptr(ijk, C.POINTER(C.c_int32)), *dims, ptr(parent, _u32p),
+                              ptr(crs, _u32p), ptr(ccol, _u32p), ptr(cval), cap)
+    nnz = int(crs[nc])
+    return parent, crs[:nc + 1].copy(), ccol[:nnz].copy(), cval[:nnz].copy()
+
+
+def amg_rbgs(rs, col, val, ijk, b, x, iters):
+    rs, col, val, ijk = _amg_args(rs, col, val, ijk)
+    x = np.array(x, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    lib().wco_amg_rbgs(len(rs) - 1, ptr(rs, _u32p), ptr(col, _u32p), ptr(val),
+                       ptr(ijk, C.POINTER(C.c_int32)), ptr(b), ptr(x), iters)
+    return x
+
+
+def amg_vcycle(rs, col, val, ijk, dims, b):
+    rs, col, val, ijk = _amg_args(rs, col, val, ijk)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    z = np.zeros(len(rs) - 1)
+    lib().wco_amg_vcycle(len(rs) - 1, ptr(rs, _u32p), ptr(col, _u32p), ptr(val),
+                         ptr(ijk, C.POINTER(C.c_int32)), *dims, ptr(b), ptr(z))
+    return z
+
+
+def amgpcg(rs, col, val, ijk, dims, b, tol=1e-8, maxit=500):
+    rs, col, val, ijk = _amg_args(rs, col, val, ijk)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(rs) - 1)
+    res = C.c_double(0)
+    it = C.c_int(0)
+    ok = lib().wco_amgpcg_solve(len(rs) - 1, ptr(rs, _u32p), ptr(col, _u32p), ptr(val),
+                                ptr(ijk, C.POINTER(C.c_int32)), *dims, ptr(b), ptr(x), tol,
+                                maxit, C.byref(res), C.byref(it))
     return bool(ok), x, it.value, res.value
 
 

@@ -167,4 +167,27 @@ int wco_pcg_solve(int64_t n, const uint32_t* rowstart, const uint32_t* col,
 #ifdef __cplusplus
 }
 #endif
+/* ---- AMG-preconditioned PCG (wc_oracle_amg.c; AlgebraicMultigrid.h,
+ * GeometricLevelGen.h).  ijk: the DOFs' cell coordinates (3 per DOF) in an
+ * ni x nj x nk grid, as AMGPCGSolveSparse takes them. ---- */
+/* number of levels; sizes[l] = unknowns of level l (up to max_levels) */
+int wco_amg_levels(int64_t n, const uint32_t* rs, const uint32_t* col, const double* val,
+                   const int32_t* ijk, int ni, int nj, int nk, int64_t* sizes, int max_levels);
+/* one coarsening step: parent[] of every DOF and the coarse CSR (when its
+ * nnz fits cap); returns the number of coarse DOFs */
+int64_t wco_amg_coarse(int64_t n, const uint32_t* rs, const uint32_t* col, const double* val,
+                       const int32_t* ijk, int ni, int nj, int nk, uint32_t* parent,
+                       uint32_t* crs, uint32_t* ccol, double* cval, int64_t cap);
+/* iters red-black Gauss-Seidel sweeps on x */
+void wco_amg_rbgs(int64_t n, const uint32_t* rs, const uint32_t* col, const double* val,
+                  const int32_t* ijk, const double* b, double* x, int iters);
+/* z = one V-cycle applied to b from z = 0 (amgPrecondCompressed) */
+void wco_amg_vcycle(int64_t n, const uint32_t* rs, const uint32_t* col, const double* val,
+                    const int32_t* ijk, int ni, int nj, int nk, const double* b, double* z);
+/* AMGPCGSolveSparse: returns 1 on convergence */
+int wco_amgpcg_solve(int64_t n, const uint32_t* rs, const uint32_t* col, const double* val,
+                     const int32_t* ijk, int ni, int nj, int nk, const double* rhs,
+                     double* result, double tol_factor, int maxit, double* residual_out,
+                     int* iters_out);
+
 #endif
