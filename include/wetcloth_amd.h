@@ -226,6 +226,27 @@ wc_status wc_mic0_factor_csr(int64_t n, const uint32_t* rowstart,
 wc_status wc_mic0_apply_csr(int64_t n, const uint32_t* rowstart,
                             const uint32_t* col, const double* invdiag,
                             const double* lv, const double* r, double* z);
+/* AMGPCGSolveSparse(matrix, rhs, result, Dof_ijk, tolerance_factor,
+ * max_iterations, residual_out, iterations_out, ni, nj, nk)
+ * (AlgebraicMultigrid.h:247-359), the solve Pressure.cpp:797 calls: PCG
+ * preconditioned by one V-cycle of the aggregation AMG of
+ * GeometricLevelGen.h:186-249 (2x2x2 blocks of the DOFs' cells, Galerkin
+ * coarse operators 0.5 R A P, red-black Gauss-Seidel 4 sweeps per level and
+ * 200 on the coarsest, coarsening while a level holds more than 4096 DOFs).
+ * dof_ijk: 3 int32 per DOF, its cell (i, j, k) with 0 <= i < ni etc.  The
+ * rows must be sorted and the system 7-point (at most 7 coarse entries a
+ * row; WC_ERR_HIP otherwise).  x = 0 start, |r|_inf <= tol |b|_inf;
+ * WC_NOT_CONVERGED after max_iterations or when rho is 0 or NaN. */
+wc_status wc_amgpcg_solve_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                              const double* val, const int32_t* dof_ijk, int32_t ni,
+                              int32_t nj, int32_t nk, const double* rhs, double* result,
+                              double tol, int32_t maxit, double* residual_out,
+                              int32_t* iters_out);
+/* amgPrecondCompressed (AlgebraicMultigrid.h:234-245): z = one V-cycle
+ * applied to r from z = 0, on the hierarchy wc_amgpcg_solve_csr builds. */
+wc_status wc_amg_apply_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                           const double* val, const int32_t* dof_ijk, int32_t ni, int32_t nj,
+                           int32_t nk, const double* r, double* z);
 /* multiply(FixedSparseMatrix, x, result) (SparseMatrix.h:298-322). */
 wc_status wc_spmv_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
                       const double* val, const double* x, double* y);

@@ -108,6 +108,12 @@ def lib():
                                          _dp, _dp]
         L.wc_mic0_apply_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp]
         L.wc_spmv_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
+        _i32p_ = C.POINTER(C.c_int32)
+        L.wc_amgpcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int32,
+                                          C.c_int32, C.c_int32, _dp, _dp, C.c_double,
+                                          C.c_int32, _dp, _i32p]
+        L.wc_amg_apply_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int32,
+                                       C.c_int32, C.c_int32, _dp, _dp]
         _lib = L
     return _lib
 
@@ -385,3 +391,33 @@ def spmv_csr(rs, col, val, x):
                              _ptr(np.ascontiguousarray(val, dtype=np.float64)),
                              _ptr(np.ascontiguousarray(x, dtype=np.float64)), _ptr(y)))
     return y
+
+
+def amgpcg_solve_csr(rs, col, val, dof_ijk, dims, rhs, tol=1e-8, maxit=500):
+    """AMGPCGSolveSparse (AlgebraicMultigrid.h:247-359) on a host CSR:
+    (converged, x, iterations, residual)."""
+    rs, col = _csr(rs, col)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    ijk = np.ascontiguousarray(dof_ijk, dtype=np.int32)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    n = len(rs) - 1
+    x = np.zeros(n)
+    res = C.c_double(0)
+    it = C.c_int32(0)
+    st = lib().wc_amgpcg_solve_csr(n, _ptr(rs, _u32p), _ptr(col, _u32p), _ptr(val),
+                                   _ptr(ijk, C.POINTER(C.c_int32)), *dims, _ptr(rhs), _ptr(x),
+                                   tol, maxit, C.byref(res), C.byref(it))
+    _check(st, None, allow_nc=True)
+    return st == WC_OK, x, it.value, res.value
+
+
+def amg_apply_csr(rs, col, val, dof_ijk, dims, r):
+    """amgPrecondCompressed: z = one V-cycle applied to r from z = 0."""
+    rs, col = _csr(rs, col)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    ijk = np.ascontiguousarray(dof_ijk, dtype=np.int32)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros(len(rs) - 1)
+    _check(lib().wc_amg_apply_csr(len(rs) - 1, _ptr(rs, _u32p), _ptr(col, _u32p), _ptr(val),
+                                  _ptr(ijk, C.POINTER(C.c_int32)), *dims, _ptr(r), _ptr(z)))
+    return z

@@ -822,6 +822,103 @@ wc_status wc_pcg_solve_csr(int64_t n, const uint32_t* rowstart,
   return ok ? WC_OK : WC_NOT_CONVERGED;
 }
 
+}  // extern "C"
+
+namespace {
+wc_status check_ijk(int64_t n, const int32_t* ijk, int32_t ni, int32_t nj, int32_t nk) {
+  if (n > 0 && !ijk) return WC_ERR_ARG;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return n > 0 ? WC_ERR_ARG : WC_OK;
+  if ((int64_t)ni * nj * nk >= 0x7FFFFFFFll) return WC_ERR_ARG;
+  for (int64_t f = 0; f < n; ++f)
+    if (ijk[3 * f] < 0 || ijk[3 * f] >= ni || ijk[3 * f + 1] < 0 || ijk[3 * f + 1] >= nj ||
+        ijk[3 * f + 2] < 0 || ijk[3 * f + 2] >= nk)
+      return WC_ERR_ARG;
+  return WC_OK;
+}
+
+// the AMG hierarchy of an uploaded system: run(...) gets it built
+template <class F>
+wc_status with_amg(int64_t n, const uint32_t* rowstart, const uint32_t* col, const double* val,
+                   const int32_t* ijk, int32_t ni, int32_t nj, int32_t nk, F run) {
+  if (check_csr(n, rowstart, col) != WC_OK || (n > 0 && !val))
+    return fail(nullptr, WC_ERR_ARG, "invalid CSR matrix (rows must be sorted, columns < n)");
+  if (check_ijk(n, ijk, ni, nj, nk) != WC_OK)
+    return fail(nullptr, WC_ERR_ARG, "invalid DOF cells (need 0 <= i < ni, 0 <= j < nj, 0 <= k < nk)");
+  try {
+    FreeSession S;
+    wc::AmgHier H;
+    wc::DevBuf<int32_t> dijk;
+    if (n > 0) {
+      S.csr.up(S.st, n, rowstart, col, val);
+      dijk.ensure(3 * (size_t)n);
+      WC_HIP(hipMemcpyAsync(dijk.p, ijk, sizeof(int32_t) * 3 * (size_t)n, hipMemcpyHostToDevice,
+                            S.st));
+      wc::amg_build(S.st, H, S.csr.A, dijk.p, ni, nj, nk);
+    }
+    const wc_status st = run(S, H);
+    WC_HIP(hipStreamSynchronize(S.st));
+    H.release();
+    dijk.release();
+    return st;
+  } catch (const HipError& e) {
+    return hip_fail(nullptr, e);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+wc_status wc_amgpcg_solve_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                              const double* val, const int32_t* dof_ijk, int32_t ni,
+                              int32_t nj, int32_t nk, const double* rhs, double* result,
+                              double tol, int32_t maxit, double* residual_out,
+                              int32_t* iters_out) {
+  if (n > 0 && (!rhs || !result)) return fail(nullptr, WC_ERR_ARG, "bad argument");
+  if (maxit < 0) return fail(nullptr, WC_ERR_ARG, "maxit < 0");
+  double res = 0;
+  int it = 0;
+  bool ok = true;
+  const wc_status st = with_amg(n, rowstart, col, val, dof_ijk, ni, nj, nk,
+                                [&](FreeSession& S, wc::AmgHier& H) -> wc_status {
+    if (n == 0) return WC_OK;
+    wc::DevBuf<double> b, x;
+    b.ensure(n);
+    x.ensure(n);
+    WC_HIP(hipMemcpyAsync(b.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, S.st));
+    wc::pcg_prepare(S.st, S.work, S.csr.A, nullptr, 0);
+    ok = wc::amgpcg_solve(S.st, S.work, H, S.csr.A, b.p, x.p, tol, maxit, &res, &it);
+    WC_HIP(hipMemcpyAsync(result, x.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
+    WC_HIP(hipStreamSynchronize(S.st));
+    b.release();
+    x.release();
+    return WC_OK;
+  });
+  if (st != WC_OK) return st;
+  if (residual_out) *residual_out = res;
+  if (iters_out) *iters_out = it;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_amg_apply_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                           const double* val, const int32_t* dof_ijk, int32_t ni, int32_t nj,
+                           int32_t nk, const double* r, double* z) {
+  if (n > 0 && (!r || !z)) return fail(nullptr, WC_ERR_ARG, "bad argument");
+  return with_amg(n, rowstart, col, val, dof_ijk, ni, nj, nk,
+                  [&](FreeSession& S, wc::AmgHier& H) -> wc_status {
+    if (n == 0) return WC_OK;
+    wc::DevBuf<double> dr, dz;
+    dr.ensure(n);
+    dz.ensure(n);
+    WC_HIP(hipMemcpyAsync(dr.p, r, sizeof(double) * n, hipMemcpyHostToDevice, S.st));
+    wc::amg_vcycle(S.st, H, dr.p, dz.p);
+    WC_HIP(hipMemcpyAsync(z, dz.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
+    WC_HIP(hipStreamSynchronize(S.st));
+    dr.release();
+    dz.release();
+    return WC_OK;
+  });
+}
+
 wc_status wc_mic0_factor_csr(int64_t n, const uint32_t* rowstart,
                              const uint32_t* col, const double* val, double tau,
                              double mdr, double* invdiag, double* lv) {

@@ -168,6 +168,39 @@ struct PcgWork {
   void release();
 };
 
+// The AMG preconditioner of AMGPCGSolveSparse (AlgebraicMultigrid.h,
+// GeometricLevelGen.h) on the device (wc_amg.hip).
+struct AmgLevel {
+  int64_t n = 0;
+  int ni = 0, nj = 0, nk = 0;   // grid of the DOFs' cell coordinates
+  DevCsr A;                     // level 0: the caller's rows; coarser: 7 slots per row
+  DevBuf<uint32_t> col, len;    // coarse rows (7 slots each, len used)
+  DevBuf<double> val;
+  DevBuf<int32_t> ijk;          // 3 per DOF
+  DevBuf<uint8_t> red;          // (i + j + k) odd: smoothed first
+  DevBuf<uint32_t> parent;      // the coarse DOF of each DOF (every level but the last)
+  DevBuf<uint32_t> kids;        // 8 per DOF: its fine children ascending, ~0u after them
+                                // (every level but the first)
+  DevBuf<double> x, b, r;       // V-cycle vectors (x, b of level 0 are the caller's)
+};
+constexpr int kAmgMaxLevels = 16;
+struct AmgHier {
+  AmgLevel L[kAmgMaxLevels];
+  int nl = 0;
+  DevBuf<uint32_t> cmin, flag, scan_tmp;
+  DevBuf<int> err;
+  void release();
+};
+// builds the levels of A (level 0 rows as given, DOF cell coordinates ijk
+// on the device, 3 per DOF, in an ni x nj x nk grid)
+void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, int ni,
+               int nj, int nk);
+// z = one V-cycle applied to b from z = 0 (amgPrecondCompressed)
+void amg_vcycle(hipStream_t st, AmgHier& H, const double* b, double* z);
+// AMGPCGSolveSparse (AlgebraicMultigrid.h:247-359) on a built hierarchy
+bool amgpcg_solve(hipStream_t st, PcgWork& w, AmgHier& H, const DevCsr& A, const double* b,
+                  double* x, double tol, int maxit, double* residual_out, int* iters_out);
+
 struct PhaseEvents {
   hipEvent_t ev[8] = {};
   bool have = false;
@@ -224,6 +257,7 @@ namespace wc {
 // ---- wc_scan.hip ----
 // In-place exclusive scan of n uint32 values; returns the total.
 uint32_t exclusive_scan(wc_ctx* ctx, uint32_t* data, int64_t n);
+uint32_t exclusive_scan(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* data, int64_t n);
 
 // ---- wc_particles.hip ----
 void particles_sort(wc_ctx* ctx);

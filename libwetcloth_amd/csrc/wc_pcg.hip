@@ -935,4 +935,76 @@ bool pcg_solve(hipStream_t st, PcgWork& w, const DevCsr& A, const double* b,
   return false;
 }
 
+// AMGPCGSolveSparse (AlgebraicMultigrid.h:247-359): x = 0, r = b; the
+// tolerance is tol |b|_inf (no lower clamp, unlike PCGSolver); z = one
+// V-cycle of r; false with 0 iterations if rho = z . r is 0 or NaN; then the
+// PCG loop with the V-cycle as the preconditioner.
+bool amgpcg_solve(hipStream_t st, PcgWork& w, AmgHier& H, const DevCsr& A, const double* b,
+                  double* x, double tol, int maxit, double* residual_out, int* iters_out) {
+  const int64_t n = A.n;
+  if (n == 0) {
+    *residual_out = 0;
+    *iters_out = 0;
+    return true;
+  }
+  const int nb = w.red_blocks;
+  double* r = w.r.p;
+  double* z = w.z.p;
+  double* s = w.s.p;
+  WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+  WC_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_absmax_partial, dim3(nb), dim3(kRedThreads), 0, st, r, n, w.partial.p);
+  hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                     w.scal.p);
+  WC_LAUNCH_CHECK();
+  double res = read_scal(st, w, S_RES);
+  *residual_out = res;
+  if (res == 0) {
+    *iters_out = 0;
+    return true;
+  }
+  const double tolr = tol * res;
+  amg_vcycle(st, H, r, z);
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
+                     w.partial.p);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb, w.scal.p,
+                     (int)R_SET_RHO);
+  WC_LAUNCH_CHECK();
+  const double rho = read_scal(st, w, S_RHO);
+  if (rho == 0 || rho != rho) {
+    *iters_out = 0;
+    return false;
+  }
+  WC_HIP(hipMemcpyAsync(s, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  const unsigned gb = blocks_for(n, 256);
+  int it;
+  for (it = 0; it < maxit; ++it) {
+    hipLaunchKernelGGL(k_spmv, dim3(gb), dim3(256), 0, st, A, s, z);
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z, n,
+                       w.partial.p);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                       w.scal.p, (int)R_ALPHA);
+    hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(kRedThreads), 0, st, x, r, s, z, n,
+                       w.scal.p, w.partial.p);
+    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                       w.scal.p);
+    WC_LAUNCH_CHECK();
+    res = read_scal(st, w, S_RES);
+    *residual_out = res;
+    if (res <= tolr) {
+      *iters_out = it + 1;
+      return true;
+    }
+    amg_vcycle(st, H, r, z);
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, z, r, n,
+                       w.partial.p);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
+                       w.scal.p, (int)R_BETA);
+    hipLaunchKernelGGL(k_update_s<double>, dim3(gb), dim3(256), 0, st, s, z, n, w.scal.p);
+    WC_LAUNCH_CHECK();
+  }
+  *iters_out = it;
+  return false;
+}
+
 }  // namespace wc

@@ -77,19 +77,22 @@ void scan_rec(hipStream_t st, uint32_t* data, int64_t n, uint32_t* scratch,
 
 }  // namespace
 
-uint32_t exclusive_scan(wc_ctx* ctx, uint32_t* data, int64_t n) {
+uint32_t exclusive_scan(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* data, int64_t n) {
   if (n <= 0) return 0;
   // scratch: block sums of every recursion level plus the total
   int64_t need = 2;
   for (int64_t m = n; m > 1; m = (m + kScanTile - 1) / kScanTile) need += m / kScanTile + 2;
-  ctx->scan_tmp.ensure((size_t)need + 8);
-  uint32_t* total_dev = ctx->scan_tmp.p;
-  scan_rec(ctx->stream, data, n, ctx->scan_tmp.p + 2, total_dev);
+  scratch.ensure((size_t)need + 8);
+  uint32_t* total_dev = scratch.p;
+  scan_rec(st, data, n, scratch.p + 2, total_dev);
   uint32_t total = 0;
-  WC_HIP(hipMemcpyAsync(&total, total_dev, sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, ctx->stream));
-  WC_HIP(hipStreamSynchronize(ctx->stream));
+  WC_HIP(hipMemcpyAsync(&total, total_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  WC_HIP(hipStreamSynchronize(st));
   return total;
+}
+
+uint32_t exclusive_scan(wc_ctx* ctx, uint32_t* data, int64_t n) {
+  return exclusive_scan(ctx->stream, ctx->scan_tmp, data, n);
 }
 
 }  // namespace wc
