@@ -124,7 +124,8 @@ wc_status wc_map_particle_nodes(wc_ctx* ctx);
 wc_status wc_step_velocity(wc_ctx* ctx, double dt);
 /* LinearizedImplicitEuler::projectFine -> pressure::solveNodePressure
  * (Pressure.cpp:285-830) with the MIC(0) PCGSolver<double>::solve
- * (PCGSolver/PCGSolver.h:260) at the solve call site (Pressure.cpp:797). */
+ * (PCGSolver/PCGSolver.h:260) at the solve call site (Pressure.cpp:797), or
+ * with the reference's AMGPCGSolveSparse there when the "solver" option is 1. */
 wc_status wc_project(wc_ctx* ctx, double dt, double tol, int32_t maxit,
                      double* residual_out, int32_t* iters_out);
 /* applyPressureDragFluid (LinearizedImplicitEuler.cpp:2435), acceptVelocity
@@ -185,7 +186,14 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
  *   meets the solve's tolerance).
  *   "mixed_refinements" (default 2): WC_MIXED contexts; refinement passes
  *   allowed after the first (the solve reports not converged if the FP64
- *   residual still misses the tolerance after them). */
+ *   residual still misses the tolerance after them).
+ *   "solver" (default 0): 0 solves the pressure system with MIC(0)-PCG
+ *   (PCGSolver.h:260-324); 1 with AMG-preconditioned PCG in FP64 whatever
+ *   the context's precision (AMGPCGSolveSparse, AlgebraicMultigrid.h:263-372,
+ *   the reference's liquid solver at Pressure.cpp:797), on the DOFs' cells
+ *   bucket handle * 4 + node in the bucket on the context's bucket grid
+ *   (Pressure.cpp:316-318, 433-436).  With 1, wc_apply_preconditioner has
+ *   no factor and fails. */
 wc_status wc_set_option(wc_ctx* ctx, const char* name, int64_t value);
 
 /* Refinement passes of the context's last mixed-precision solve (0 for a

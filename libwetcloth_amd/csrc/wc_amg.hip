@@ -49,9 +49,11 @@ __device__ inline void rows(const DevCsr& A, int64_t i, uint32_t& b, uint32_t& e
   }
 }
 
-__device__ inline int64_t cell_key(const int32_t* ijk, int64_t f, int nni, int nnj) {
-  return (int64_t)(ijk[3 * f] / 2) + (int64_t)(ijk[3 * f + 1] / 2) * nni +
-         (int64_t)(ijk[3 * f + 2] / 2) * nni * nnj;
+// the coarse cell of DOF f, or -1 if its cell lies outside the grid
+__device__ inline int64_t cell_key(const int32_t* ijk, int64_t f, int nni, int nnj, int nnk) {
+  const int32_t i = ijk[3 * f], j = ijk[3 * f + 1], k = ijk[3 * f + 2];
+  if (i < 0 || j < 0 || k < 0 || i / 2 >= nni || j / 2 >= nnj || k / 2 >= nnk) return -1;
+  return (int64_t)(i / 2) + (int64_t)(j / 2) * nni + (int64_t)(k / 2) * nni * nnj;
 }
 
 // GeometricLevelGen.h:173-184
@@ -61,27 +63,39 @@ __global__ void k_amg_red(int64_t n, const int32_t* ijk, uint8_t* red) {
   red[i] = ((ijk[3 * i] + ijk[3 * i + 1] + ijk[3 * i + 2]) % 2) == 1;
 }
 
-__global__ void k_amg_cmin(int64_t n, const int32_t* ijk, int nni, int nnj, uint32_t* cmin) {
+__global__ void k_amg_cmin(int64_t n, const int32_t* ijk, int nni, int nnj, int nnk,
+                           uint32_t* cmin, int* err) {
   const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f >= n) return;
-  atomicMin(&cmin[cell_key(ijk, f, nni, nnj)], (uint32_t)f);
+  const int64_t key = cell_key(ijk, f, nni, nnj, nnk);
+  if (key < 0) {
+    atomicOr(err, 2);  // a DOF outside the grid (the build then fails)
+    return;
+  }
+  atomicMin(&cmin[key], (uint32_t)f);
 }
 
-__global__ void k_amg_first(int64_t n, const int32_t* ijk, int nni, int nnj,
+__global__ void k_amg_first(int64_t n, const int32_t* ijk, int nni, int nnj, int nnk,
                             const uint32_t* cmin, uint32_t* flag) {
   const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f >= n) return;
-  flag[f] = cmin[cell_key(ijk, f, nni, nnj)] == (uint32_t)f ? 1u : 0u;
+  const int64_t key = cell_key(ijk, f, nni, nnj, nnk);
+  flag[f] = key >= 0 && cmin[key] == (uint32_t)f ? 1u : 0u;
 }
 
 // parent of each fine DOF, coordinates of each coarse DOF, and each coarse
 // DOF's children by their position in the block (GeometricLevelGen.h:138-156)
-__global__ void k_amg_parent(int64_t n, const int32_t* ijk, int nni, int nnj,
+__global__ void k_amg_parent(int64_t n, const int32_t* ijk, int nni, int nnj, int nnk,
                              const uint32_t* cmin, const uint32_t* first, uint32_t* parent,
                              int32_t* cijk, uint32_t* kids) {
   const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f >= n) return;
-  const uint32_t m = cmin[cell_key(ijk, f, nni, nnj)];
+  const int64_t key = cell_key(ijk, f, nni, nnj, nnk);
+  if (key < 0) {
+    parent[f] = 0;
+    return;
+  }
+  const uint32_t m = cmin[key];
   const uint32_t I = first[m];
   parent[f] = I;
   const int i = ijk[3 * f], j = ijk[3 * f + 1], k = ijk[3 * f + 2];
@@ -306,13 +320,18 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
     H.cmin.ensure((size_t)ncell);
     WC_HIP(hipMemsetAsync(H.cmin.p, 0xFF, sizeof(uint32_t) * (size_t)ncell, st));
     hipLaunchKernelGGL(k_amg_cmin, dim3(nblk(F.n, 256)), dim3(256), 0, st, F.n, F.ijk.p, nni,
-                       nnj, H.cmin.p);
+                       nnj, nnk, H.cmin.p, H.err.p);
     H.flag.ensure((size_t)F.n + 1);
     hipLaunchKernelGGL(k_amg_first, dim3(nblk(F.n, 256)), dim3(256), 0, st, F.n, F.ijk.p, nni,
-                       nnj, H.cmin.p, H.flag.p);
+                       nnj, nnk, H.cmin.p, H.flag.p);
     WC_LAUNCH_CHECK();
     WC_HIP(hipMemsetAsync(H.flag.p + F.n, 0, sizeof(uint32_t), st));
-    const int64_t nc = exclusive_scan(st, H.scan_tmp, H.flag.p, F.n + 1);
+    const int64_t nc = exclusive_scan(st, H.scan_tmp, H.flag.p, F.n + 1);  // synchronizes
+    int ferr = 0;
+    WC_HIP(hipMemcpy(&ferr, H.err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (ferr)
+      throw HipError{hipErrorInvalidValue, "AMG levels: a DOF cell lies outside the grid",
+                     __FILE__, __LINE__};
     if (nc >= F.n) break;  // no reduction (the reference would not stop)
     F.parent.ensure((size_t)F.n);
     C.n = nc;
@@ -323,7 +342,7 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
     C.kids.ensure(8 * (size_t)nc);
     WC_HIP(hipMemsetAsync(C.kids.p, 0xFF, sizeof(uint32_t) * 8 * (size_t)nc, st));
     hipLaunchKernelGGL(k_amg_parent, dim3(nblk(F.n, 256)), dim3(256), 0, st, F.n, F.ijk.p, nni,
-                       nnj, H.cmin.p, H.flag.p, F.parent.p, C.ijk.p, C.kids.p);
+                       nnj, nnk, H.cmin.p, H.flag.p, F.parent.p, C.ijk.p, C.kids.p);
     C.col.ensure(7 * (size_t)nc);
     C.val.ensure(7 * (size_t)nc);
     C.len.ensure((size_t)nc);

@@ -130,6 +130,8 @@ void release_all(wc_ctx* c) {
   c->p.release();
   c->dof_level.release();
   c->pcg.release();
+  c->amg.release();
+  c->dof_ijk.release();
   c->scan_tmp.release();
 }
 
@@ -160,8 +162,21 @@ bool solve_system(wc_ctx* c, double tol, int maxit, double* res, int* it) {
   gs.col = c->col.p;
   wc::pcg_prepare(c->stream, c->pcg, A, c->dof_level.p,
                   c->level_max - c->level_min + 1);
-  const bool ok = wc::pcg_solve(c->stream, c->pcg, A, c->b.p, c->p.p, tol, maxit,
-                                0.97, 0.25, res, it);
+  bool ok;
+  if (c->solver == 1 && A.n > 0) {
+    // AMGPCGSolveSparse (Pressure.cpp:797) on the DOFs' cells, bucket
+    // handle * 4 + node in the bucket, and the bucket grid's node counts
+    // (Pressure.cpp:316-318, 433-436)
+    c->pcg.sweep.valid = false;
+    c->pcg.factored = false;
+    c->dof_ijk.ensure(3 * (size_t)A.n);
+    wc::grid_dof_cells(c, c->dof_ijk.p);
+    const wc::Grid& g = c->g;
+    wc::amg_build(c->stream, c->amg, A, c->dof_ijk.p, g.nx[0], g.nx[1], g.nx[2]);
+    ok = wc::amgpcg_solve(c->stream, c->pcg, c->amg, A, c->b.p, c->p.p, tol, maxit, res, it);
+  } else {
+    ok = wc::pcg_solve(c->stream, c->pcg, A, c->b.p, c->p.p, tol, maxit, 0.97, 0.25, res, it);
+  }
   wc::grid_scatter_pressure(c);
   c->last_residual = *res;
   c->last_iters = *it;
@@ -535,6 +550,11 @@ wc_status wc_system_multiply(wc_ctx* c, const double* x, double* y) {
 wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
   if (!c || !name) return fail(c, WC_ERR_ARG, "bad argument");
   const std::string n(name);
+  if (n == "solver") {
+    if (value != 0 && value != 1) return fail(c, WC_ERR_ARG, "solver must be 0 (MIC(0)) or 1 (AMG)");
+    c->solver = (int)value;
+    return WC_OK;
+  }
   if (n == "pipelined_sweeps") {
     c->pcg.use_sweeps = value != 0;
     return WC_OK;

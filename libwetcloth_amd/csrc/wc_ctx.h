@@ -248,6 +248,12 @@ struct wc_ctx {
 
   double last_residual = 0.0;
   int last_iters = 0;
+  // pressure solver: 0 = MIC(0) PCGSolver, 1 = AMGPCGSolveSparse (the
+  // reference's call at Pressure.cpp:797), with its hierarchy and the DOFs'
+  // cell coordinates
+  int solver = 0;
+  wc::AmgHier amg;
+  wc::DevBuf<int32_t> dof_ijk;
   double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   wc::PhaseEvents events;
 };
@@ -276,6 +282,9 @@ void grid_nodes(wc_ctx* ctx);
 void grid_step_velocity(wc_ctx* ctx, double dt);
 int64_t grid_assemble(wc_ctx* ctx, double dt);
 void grid_scatter_pressure(wc_ctx* ctx);
+// each DOF's node (i, j, k) on the dense bucket grid, bucket handle * 4 plus
+// the node in the bucket (Pressure.cpp:433-436), 3 per DOF, into ijk
+void grid_dof_cells(wc_ctx* ctx, int32_t* ijk);
 void grid_apply_pressure(wc_ctx* ctx, double dt);
 
 // ---- wc_sweep.hip ----

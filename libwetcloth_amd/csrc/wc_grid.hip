@@ -504,6 +504,27 @@ int64_t grid_assemble(wc_ctx* ctx, double dt) {
   return ndof;
 }
 
+namespace {
+__global__ void k_dof_cells(Grid g, const uint32_t* isdof, const uint32_t* dofscan,
+                            int32_t* ijk) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= g.nnode || !isdof[n]) return;
+  int i, j, k;
+  node_coords(g, n, i, j, k);
+  const int64_t d = dofscan[n];
+  ijk[3 * d] = i;
+  ijk[3 * d + 1] = j;
+  ijk[3 * d + 2] = k;
+}
+}  // namespace
+
+void grid_dof_cells(wc_ctx* ctx, int32_t* ijk) {
+  const Grid& g = ctx->g;
+  hipLaunchKernelGGL(k_dof_cells, dim3(blocks(g.nnode, 256)), dim3(256), 0, ctx->stream, g,
+                     ctx->isdof.p, ctx->dofscan.p, ijk);
+  WC_LAUNCH_CHECK();
+}
+
 void grid_scatter_pressure(wc_ctx* ctx) {
   const Grid& g = ctx->g;
   if (ctx->ndof == 0) {

@@ -53,3 +53,53 @@ def test_amg_rejects_bad_input(wc, oracle):
     # a zero right-hand side returns at once
     ok, x, it, res = wc.amgpcg_solve_csr(rs, col, val, ijk, nx, np.zeros(len(b)))
     assert ok and it == 0 and not x.any()
+
+
+def test_context_solver_option(oracle, wc):
+    """set_option("solver", 1): wc_project solves the context's own pressure
+    system with AMG-PCG; the solution is the oracle's AMG-PCG on the
+    downloaded system and the DOFs' cells (bucket handle * 4 + node in the
+    bucket: the dense node coordinates, as Pressure.cpp:433-436 on the
+    reference's bucket grid), and whole substeps converge with it."""
+    n, dt = 24, 1.0 / 480
+    g, x, v = oracle.dam_break(n, seed=2, sigma=0.5)
+    cpu = oracle.OracleScene((n, n, n), 1.0 / n, x, v)
+    s = wc.Scene((n, n, n), 1.0 / n)
+    s.set_particles(x, v)
+    with pytest.raises(wc.WetclothError):
+        s.set_option("solver", 2)
+    s.set_option("solver", 1)
+    s.rebucketizeParticles()
+    s.mapParticleNodesAPIC()
+    s.stepVelocity(dt)
+    ok, it, res = s.projectFine(dt, tol=1e-10, maxit=300)
+    assert ok and it > 0
+    rs, col, val, b, p = s.system()
+    assert res <= 1e-10 * np.abs(b).max()
+    nx = tuple(int(t) for t in cpu.grid.nx)
+    assert s.grid_info()[0] == nx
+    dof = s.field("dof")
+    ii, jj, kk = np.meshgrid(np.arange(nx[0]), np.arange(nx[1]), np.arange(nx[2]),
+                             indexing="ij")
+    d = dof[oracle.node_index_array(cpu.grid, ii.ravel(), jj.ravel(), kk.ravel())]
+    m = d >= 0
+    ijk = np.zeros((len(b), 3), dtype=np.int32)
+    ijk[d[m]] = np.stack([ii.ravel()[m], jj.ravel()[m], kk.ravel()[m]], axis=1)
+    dims = nx
+    ok_r, x_r, it_r, _ = oracle.amgpcg(rs, col, val, ijk, dims, b, tol=1e-10, maxit=300)
+    assert ok_r and abs(it - it_r) <= 1
+    assert np.abs(p - x_r).max() <= 1e-6 * np.abs(x_r).max()
+    # the V-cycle on those cells is the oracle's, bit for bit
+    r = np.random.default_rng(7).standard_normal(len(b))
+    assert np.array_equal(wc.amg_apply_csr(rs, col, val, ijk, dims, r),
+                          oracle.amg_vcycle(rs, col, val, ijk, dims, r))
+    s.applyPressureDragFluid(dt)
+    s.mapNodeParticlesAPIC(dt)
+    for _ in range(3):
+        ok, it, res = s.step(dt, tol=1e-8, maxit=300)
+        assert ok
+    # and back to MIC(0) on the same context
+    s.set_option("solver", 0)
+    ok, it, res = s.step(dt, tol=1e-8, maxit=1000)
+    assert ok
+    s.close()
