@@ -9,7 +9,9 @@ input resident in HBM.
 
 The same workload runs twice from the same seeds: in the mixed-precision
 form (the reported value) and in the plain FP64 form (reported beside it,
-with the ratio and the positions' difference after the same steps).
+with the ratio and the positions' difference after the same steps).  A third
+run solves the pressure with FP64 AMG-PCG instead, the reference's as-shipped
+solver, for comparison (fp64_amg; --no-amg skips it).
 
 Prints ONE JSON line (see the keys below).  --gpus N > 1 exits with a message:
 the library runs the substep on a single GPU.
@@ -37,6 +39,9 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--maxit", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-amg", action="store_true",
+                    help="skip the FP64 AMG-PCG run of the workload (the reference's "
+                         "as-shipped pressure solver, reported under fp64_amg)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (the known-size "
@@ -139,11 +144,13 @@ def cpu_baseline(n_particles_gpu: int, steps: int) -> dict:
     }
 
 
-def run(W, precision, n, dt, a):
+def run(W, precision, n, dt, a, solver=0):
     """Seed the workload in one precision, step it warmup + steps times and
     time the last steps (the library's stream is synchronized by every step:
-    the solver reads its residual back each iteration)."""
+    the solver reads its residual back each iteration).  solver 1 solves the
+    pressure with AMG-PCG instead of MIC(0)-PCG."""
     scene = W.Scene((n, n, n), 1.0 / n, precision=precision)
+    scene.set_option("solver", solver)
     npart = scene.seed_block((0.0, 0.0, 0.0), (0.5, 0.6, 1.0), seed=4, sigma=1.0)
     iters, passes = [], []
     for _ in range(a.warmup):
@@ -225,6 +232,11 @@ def main():
     }
     out["roofline"] = roofline(m["kernel_stats"], 4, workload)
     out["fp64"]["roofline"] = roofline(f["kernel_stats"], 8, workload)
+    if not a.no_amg:
+        amg, g = run(W, W.WC_FP64, n, dt, a, solver=1)
+        amg.close()
+        out["fp64_amg"] = {k: g[k] for k in ("value", "ms_per_step", "pcg_iterations",
+                                             "phase_ms_per_step")}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(npart, a.cpu_steps)
     print(json.dumps(out))

@@ -1,11 +1,13 @@
 // Drives the C++ host mirror the way a reference user would: a
 // robertbridson::PCGSolver<double> solve on a matrix assembled with
-// SparseMatrix::add_to_element, then liquid substeps through WetClothCore.
-// Prints one JSON line.  Usage: dam_break <n> <steps>
+// SparseMatrix::add_to_element, AMGPCGSolveSparse on a 3-D one with the
+// DOFs' cells, then liquid substeps through WetClothCore.
+// Prints one JSON line.  Usage: dam_break <n> <steps> [outdir]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
 #include <random>
 #include <vector>
 
@@ -45,6 +47,39 @@ int main(int argc, char** argv) {
       nrm = std::fmax(nrm, std::fabs(xt[r]));
     }
 
+    // 1b. AMGPCGSolveSparse on a 3-D 7-point Poisson matrix (20^3 cells, two
+    // AMG levels) with a known solution
+    const int q = 20, Q = q * q * q;
+    robertbridson::SparseMatrix<double> A3(Q);
+    std::vector<std::array<int, 3>> ijk(Q);
+    for (int k = 0; k < q; ++k)
+      for (int j = 0; j < q; ++j)
+        for (int i = 0; i < q; ++i) {
+          const int r = (k * q + j) * q + i;
+          ijk[r] = {i, j, k};
+          A3.add_to_element(r, r, 6.0 + 0.01);
+          if (i > 0) A3.add_to_element(r, r - 1, -1.0);
+          if (i + 1 < q) A3.add_to_element(r, r + 1, -1.0);
+          if (j > 0) A3.add_to_element(r, r - q, -1.0);
+          if (j + 1 < q) A3.add_to_element(r, r + q, -1.0);
+          if (k > 0) A3.add_to_element(r, r - q * q, -1.0);
+          if (k + 1 < q) A3.add_to_element(r, r + q * q, -1.0);
+        }
+    std::vector<double> xt3(Q), b3(Q, 0.0), x3;
+    for (auto& v : xt3) v = nd(rng);
+    for (int r = 0; r < Q; ++r)
+      for (size_t k = 0; k < A3.index[r].size(); ++k)
+        b3[r] += A3.value[r][k] * xt3[A3.index[r][k]];
+    double amg_res = 0.0;
+    int amg_it = 0;
+    const bool amg_ok =
+        AMGPCGSolveSparse(A3, b3, x3, ijk, 1e-12, 100, amg_res, amg_it, q, q, q);
+    double amg_err = 0.0, amg_nrm = 0.0;
+    for (int r = 0; r < Q; ++r) {
+      amg_err = std::fmax(amg_err, std::fabs(x3[r] - xt3[r]));
+      amg_nrm = std::fmax(amg_nrm, std::fabs(xt3[r]));
+    }
+
     // 2. liquid substeps
     wc_desc d{};
     d.ni = d.nj = d.nk = n;
@@ -80,9 +115,11 @@ int main(int argc, char** argv) {
     for (int64_t p = 0; p < np; ++p) vy += v4[4 * p + 1];
     std::printf(
         "{\"pcg_ok\": %s, \"pcg_iterations\": %d, \"pcg_residual\": %.17g, "
-        "\"pcg_max_error\": %.17g, \"particles\": %lld, \"steps_ok\": %s, "
+        "\"pcg_max_error\": %.17g, \"amg_ok\": %s, \"amg_iterations\": %d, "
+        "\"amg_max_error\": %.17g, \"particles\": %lld, \"steps_ok\": %s, "
         "\"last_iterations\": %d, \"mean_vy\": %.17g, \"seconds\": %.6f}\n",
-        ok ? "true" : "false", iterations, residual, err / nrm, (long long)np,
+        ok ? "true" : "false", iterations, residual, err / nrm, amg_ok ? "true" : "false",
+        amg_it, amg_err / amg_nrm, (long long)np,
         all_ok ? "true" : "false", stepper.lastIterations(), vy / (double)np, secs);
   } catch (const wcamd::Error& e) {
     std::fprintf(stderr, "%s\n", e.what());

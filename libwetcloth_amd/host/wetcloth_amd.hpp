@@ -277,3 +277,38 @@ struct PCGSolver {
 };
 
 }  // namespace robertbridson
+
+// AMGPCGSolveSparse (AlgebraicMultigrid.h:247-251), the solve Pressure.cpp:797
+// calls, with its signature: PCG preconditioned by the aggregation AMG
+// V-cycle, run on the GPU.  Dof_ijk[d] is DOF d's cell (any type with
+// operator[] over three ints, such as Eigen's Vector3i); 0 <= i < ni etc.
+// Returns false when the solve does not converge; throws wcamd::Error on a
+// bad argument or a device error.
+template <class T, class IJK>
+bool AMGPCGSolveSparse(const robertbridson::SparseMatrix<T>& matrix, const std::vector<T>& rhs,
+                       std::vector<T>& result, std::vector<IJK>& Dof_ijk, T tolerance_factor,
+                       int max_iterations, T& residual_out, int& iterations_out, int ni, int nj,
+                       int nk) {
+  static_assert(sizeof(T) == 8, "the GPU solve runs in double");
+  const unsigned int n = matrix.n;
+  std::vector<uint32_t> rowstart(n + 1, 0), colindex;
+  std::vector<double> values;
+  std::vector<int32_t> ijk(3 * (size_t)n);
+  for (unsigned int i = 0; i < n; ++i) {
+    rowstart[i + 1] = rowstart[i] + (uint32_t)matrix.index[i].size();
+    colindex.insert(colindex.end(), matrix.index[i].begin(), matrix.index[i].end());
+    values.insert(values.end(), matrix.value[i].begin(), matrix.value[i].end());
+    for (int d = 0; d < 3; ++d) ijk[3 * (size_t)i + d] = (int32_t)Dof_ijk[i][d];
+  }
+  result.assign(n, 0);
+  double res = 0.0;
+  int32_t it = 0;
+  const wc_status st =
+      wc_amgpcg_solve_csr(n, rowstart.data(), colindex.data(), values.data(), ijk.data(), ni, nj,
+                          nk, rhs.data(), result.data(), tolerance_factor, max_iterations, &res,
+                          &it);
+  wcamd::check(st, nullptr);
+  residual_out = res;
+  iterations_out = it;
+  return st == WC_OK;
+}

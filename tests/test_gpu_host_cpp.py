@@ -1,6 +1,7 @@
 """The C++ host mirror (libwetcloth_amd/host/wetcloth_amd.hpp) driven from a
-compiled program: PCGSolver<double>::solve on a manufactured system and
-WetClothCore substeps of the dam break, checked against the oracle."""
+compiled program: PCGSolver<double>::solve and AMGPCGSolveSparse on
+manufactured systems and WetClothCore substeps of the dam break, checked
+against the oracle."""
 import json
 import os
 import subprocess
@@ -22,6 +23,7 @@ def test_cpp_host_mirror(oracle, tmp_path):
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout.strip().splitlines()[-1])
     assert r["pcg_ok"] and r["pcg_max_error"] < 1e-9
+    assert r["amg_ok"] and r["amg_max_error"] < 1e-9 and 0 < r["amg_iterations"] < 30
     assert r["steps_ok"]
     g, x, v = oracle.dam_break(n)
     assert r["particles"] == x.shape[1]
