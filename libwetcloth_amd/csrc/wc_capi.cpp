@@ -169,6 +169,7 @@ bool solve_system(wc_ctx* c, double tol, int maxit, double* res, int* it) {
     // (Pressure.cpp:316-318, 433-436)
     c->pcg.sweep.valid = false;
     c->pcg.factored = false;
+    c->pcg.passes = 0;  // FP64 throughout: no refinement passes
     c->dof_ijk.ensure(3 * (size_t)A.n);
     wc::grid_dof_cells(c, c->dof_ijk.p);
     const wc::Grid& g = c->g;

@@ -103,3 +103,26 @@ def test_context_solver_option(oracle, wc):
     ok, it, res = s.step(dt, tol=1e-8, maxit=1000)
     assert ok
     s.close()
+
+
+def test_solver_option_in_a_mixed_context(oracle, wc):
+    """A WC_MIXED context with "solver" = 1 solves its FP64 system with
+    AMG-PCG to the FP64 criterion; no refinement passes are reported."""
+    n, dt = 24, 1.0 / 480
+    s = wc.Scene((n, n, n), 1.0 / n, precision=wc.WC_MIXED)
+    s.seed_block((0.0, 0.0, 0.0), (0.4, 0.6, 1.0), seed=3, sigma=0.3)
+    ok, it, res = s.step(dt, tol=1e-8, maxit=1000)  # MIC(0) with refinement first
+    assert ok and s.refinement_passes() >= 1
+    s.set_option("solver", 1)
+    for _ in range(3):
+        ok, it, res = s.step(dt, tol=1e-8, maxit=300)
+        assert ok and 0 < it < 30
+    assert s.refinement_passes() == 0
+    s.rebucketizeParticles()
+    s.mapParticleNodesAPIC()
+    s.stepVelocity(dt)
+    ok, it, res = s.projectFine(dt, tol=1e-8, maxit=300)
+    rs, col, val, b, p = s.system()
+    r = b - oracle.spmv(rs, col, val, p)
+    assert ok and np.abs(r).max() <= 1e-8 * np.abs(b).max()
+    s.close()
