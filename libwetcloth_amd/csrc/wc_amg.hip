@@ -36,6 +36,8 @@ namespace {
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kCoarsestThreads = 1024;
 constexpr int kCoarsestMax = 4096;  // DOFs the single-workgroup coarse solve holds in LDS
+constexpr int kCoarsestRows = kCoarsestMax / kCoarsestThreads;  // rows per thread
+constexpr int kSlots = 7;           // entries of a colour-ordered row
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -232,6 +234,138 @@ __global__ __launch_bounds__(kCoarsestThreads) void k_amg_coarsest(DevCsr A, con
   for (int i = threadIdx.x; i < n; i += kCoarsestThreads) x[i] = xs[i];
 }
 
+// ---- the colour-ordered rows ----
+
+__global__ void k_amg_redflag(int64_t n, const uint8_t* red, uint32_t* flag) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = red[i];
+}
+
+// row i to position p (red rows first); its entries to the planes
+__global__ void k_amg_compact(DevCsr A, const uint8_t* red, const uint32_t* scan, int64_t nred,
+                              uint32_t* crow, uint32_t* ccol, double* cval, uint8_t* clen,
+                              int* err) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n = A.n;
+  if (i >= n) return;
+  const int64_t p = red[i] ? (int64_t)scan[i] : nred + (i - (int64_t)scan[i]);
+  uint32_t rb, re;
+  rows(A, i, rb, re);
+  if (re - rb > (uint32_t)kSlots) {
+    atomicOr(err, 4);  // a longer row: the generic kernels serve this hierarchy
+    return;
+  }
+  crow[p] = (uint32_t)i;
+  clen[p] = (uint8_t)(re - rb);
+  for (uint32_t s = rb; s < re; ++s) {
+    ccol[(int64_t)(s - rb) * n + p] = A.col[s];
+    cval[(int64_t)(s - rb) * n + p] = A.val[s];
+  }
+}
+
+// rbgs_row on a colour-ordered row: the same operations in the same order
+__device__ inline double rbgs_crow(const uint32_t* ccol, const double* cval, int len, int64_t n,
+                                   int64_t p, uint32_t i, double bi, const double* x) {
+  double sum = 0, diag = 0;
+  for (int s = 0; s < len; ++s) {
+    const uint32_t c = ccol[(int64_t)s * n + p];
+    const double v = cval[(int64_t)s * n + p];
+    if (c != i && c < (uint64_t)n)
+      sum += v * x[c];
+    else
+      diag = v;
+  }
+  return diag != 0 ? (bi - sum) / diag : 0.0;
+}
+
+// one colour: positions [p0, p1)
+__global__ void k_amg_rbgs_c(int64_t n, int64_t p0, int64_t p1, const uint32_t* crow,
+                             const uint32_t* ccol, const double* cval, const uint8_t* clen,
+                             const double* b, double* x) {
+  const int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  const uint32_t i = crow[p];
+  x[i] = rbgs_crow(ccol, cval, clen[p], n, p, i, b[i], x);
+}
+
+// r = b - A x on the colour-ordered rows (as k_amg_resid)
+__global__ void k_amg_resid_c(int64_t n, const uint32_t* crow, const uint32_t* ccol,
+                              const double* cval, const uint8_t* clen, const double* b,
+                              const double* x, double* r) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t i = crow[p];
+  const int len = clen[p];
+  double v = b[i];
+  for (int s = 0; s < len; ++s) v -= cval[(int64_t)s * n + p] * x[ccol[(int64_t)s * n + p]];
+  r[i] = v;
+}
+
+// y = A x on the colour-ordered rows (multiply, SparseMatrix.h:298-322)
+__global__ void k_amg_spmv_c(int64_t n, const uint32_t* crow, const uint32_t* ccol,
+                             const double* cval, const uint8_t* clen, const double* x,
+                             double* y) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int len = clen[p];
+  double acc = 0;
+  for (int s = 0; s < len; ++s) acc += cval[(int64_t)s * n + p] * x[ccol[(int64_t)s * n + p]];
+  y[crow[p]] = acc;
+}
+
+// the coarsest level's sweeps with each thread's rows in registers: thread t
+// owns positions t + q * 1024; x in LDS
+__global__ __launch_bounds__(kCoarsestThreads) void k_amg_coarsest_c(
+    int n, int nred, const uint32_t* crow, const uint32_t* ccol, const double* cval,
+    const uint8_t* clen, const double* b, double* x, int iters) {
+  __shared__ double xs[kCoarsestMax];
+  uint32_t row[kCoarsestRows];
+  int len[kCoarsestRows];
+  double bi[kCoarsestRows];
+  uint32_t col[kCoarsestRows][kSlots];
+  double val[kCoarsestRows][kSlots];
+#pragma unroll
+  for (int q = 0; q < kCoarsestRows; ++q) {
+    const int p = (int)threadIdx.x + q * kCoarsestThreads;
+    len[q] = 0;
+    row[q] = 0;
+    bi[q] = 0;
+    if (p < n) {
+      row[q] = crow[p];
+      len[q] = clen[p];
+      bi[q] = b[row[q]];
+    }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      col[q][s] = s < len[q] ? ccol[(int64_t)s * n + p] : 0u;
+      val[q][s] = s < len[q] ? cval[(int64_t)s * n + p] : 0.0;
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += kCoarsestThreads) xs[i] = x[i];
+  __syncthreads();
+  for (int it = 0; it < iters; ++it)
+    for (int color = 1; color >= 0; --color) {
+#pragma unroll
+      for (int q = 0; q < kCoarsestRows; ++q) {
+        const int p = (int)threadIdx.x + q * kCoarsestThreads;
+        if (p < n && (p < nred) == (color == 1)) {
+          double sum = 0, diag = 0;
+#pragma unroll
+          for (int s = 0; s < kSlots; ++s)
+            if (s < len[q]) {
+              if (col[q][s] != row[q] && col[q][s] < (uint32_t)n)
+                sum += val[q][s] * xs[col[q][s]];
+              else
+                diag = val[q][s];
+            }
+          xs[row[q]] = diag != 0 ? (bi[q] - sum) / diag : 0.0;
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < n; i += kCoarsestThreads) x[i] = xs[i];
+}
+
 // r = b - A x, subtracting the row's products one by one (SparseMatrix.h:404-418)
 __global__ void k_amg_resid(DevCsr A, const double* b, const double* x, double* r) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -265,11 +399,20 @@ __global__ void k_amg_prolong(int64_t n, const uint32_t* parent, const double* x
   x[f] += 1.0 * (0.0 + 1.0 * xc[parent[f]]);
 }
 
-void smooth(hipStream_t st, AmgLevel& L, const double* b, double* x, int iters) {
+void smooth(hipStream_t st, const AmgHier& H, const AmgLevel& L, const double* b, double* x,
+            int iters) {
   for (int it = 0; it < iters; ++it)
-    for (int color = 1; color >= 0; --color)
-      hipLaunchKernelGGL(k_amg_rbgs, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.A, L.red.p,
-                         (uint8_t)color, b, x);
+    for (int color = 1; color >= 0; --color) {
+      if (H.compact) {
+        const int64_t p0 = color ? 0 : L.nred, p1 = color ? L.nred : L.n;
+        if (p1 > p0)
+          hipLaunchKernelGGL(k_amg_rbgs_c, dim3(nblk(p1 - p0, 256)), dim3(256), 0, st, L.n, p0,
+                             p1, L.crow.p, L.ccol.p, L.cval.p, L.clen.p, b, x);
+      } else {
+        hipLaunchKernelGGL(k_amg_rbgs, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.A, L.red.p,
+                           (uint8_t)color, b, x);
+      }
+    }
   WC_LAUNCH_CHECK();
 }
 
@@ -288,9 +431,15 @@ void AmgHier::release() {
     L.x.release();
     L.b.release();
     L.r.release();
+    L.crow.release();
+    L.ccol.release();
+    L.cval.release();
+    L.clen.release();
     L.n = 0;
+    L.nred = 0;
   }
   nl = 0;
+  compact = false;
   cmin.release();
   flag.release();
   scan_tmp.release();
@@ -368,14 +517,38 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
       L.b.ensure((size_t)L.n);
     }
     if (l + 1 < H.nl) L.r.ensure((size_t)L.n);
+    // colour-ordered rows: red positions by a scan of the red flags
+    H.flag.ensure((size_t)L.n + 1);
+    hipLaunchKernelGGL(k_amg_redflag, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.n, L.red.p,
+                       H.flag.p);
+    WC_LAUNCH_CHECK();
+    WC_HIP(hipMemsetAsync(H.flag.p + L.n, 0, sizeof(uint32_t), st));
+    L.nred = exclusive_scan(st, H.scan_tmp, H.flag.p, L.n + 1);
+    L.crow.ensure((size_t)L.n);
+    L.clen.ensure((size_t)L.n);
+    L.ccol.ensure((size_t)kSlots * L.n);
+    L.cval.ensure((size_t)kSlots * L.n);
+    hipLaunchKernelGGL(k_amg_compact, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.A, L.red.p,
+                       H.flag.p, L.nred, L.crow.p, L.ccol.p, L.cval.p, L.clen.p, H.err.p);
+    WC_LAUNCH_CHECK();
   }
-  WC_LAUNCH_CHECK();
   int herr = 0;
   WC_HIP(hipMemcpyAsync(&herr, H.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
   WC_HIP(hipStreamSynchronize(st));
-  if (herr)
+  if (herr & 1)
     throw HipError{hipErrorInvalidValue, "AMG levels: a coarse row has more than 7 entries",
                    __FILE__, __LINE__};
+  H.compact = (herr & 4) == 0;
+}
+
+bool amg_spmv(hipStream_t st, AmgHier& H, const double* x, double* y) {
+  if (!H.compact || H.nl < 1) return false;
+  const AmgLevel& L = H.L[0];
+  if (L.n > 0)
+    hipLaunchKernelGGL(k_amg_spmv_c, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.n, L.crow.p,
+                       L.ccol.p, L.cval.p, L.clen.p, x, y);
+  WC_LAUNCH_CHECK();
+  return true;
 }
 
 // AlgebraicMultigrid.h:179-245
@@ -390,28 +563,36 @@ void amg_vcycle(hipStream_t st, AmgHier& H, const double* b, double* z) {
   }
   for (int l = 0; l + 1 < nl; ++l) {
     AmgLevel& L = H.L[l];
-    smooth(st, L, bl[l], xl[l], 4);
-    hipLaunchKernelGGL(k_amg_resid, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.A, bl[l], xl[l],
-                       L.r.p);
+    smooth(st, H, L, bl[l], xl[l], 4);
+    if (H.compact)
+      hipLaunchKernelGGL(k_amg_resid_c, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.n, L.crow.p,
+                         L.ccol.p, L.cval.p, L.clen.p, bl[l], xl[l], L.r.p);
+    else
+      hipLaunchKernelGGL(k_amg_resid, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.A, bl[l], xl[l],
+                         L.r.p);
     hipLaunchKernelGGL(k_amg_restrict, dim3(nblk(H.L[l + 1].n, 256)), dim3(256), 0, st,
                        H.L[l + 1].n, H.L[l + 1].kids.p, L.r.p, H.L[l + 1].b.p);
     WC_LAUNCH_CHECK();
   }
   AmgLevel& T = H.L[nl - 1];
   if (T.n <= kCoarsestMax) {
-    if (T.n > 0)
+    if (T.n > 0 && H.compact)
+      hipLaunchKernelGGL(k_amg_coarsest_c, dim3(1), dim3(kCoarsestThreads), 0, st, (int)T.n,
+                         (int)T.nred, T.crow.p, T.ccol.p, T.cval.p, T.clen.p, bl[nl - 1],
+                         xl[nl - 1], 200);
+    else if (T.n > 0)
       hipLaunchKernelGGL(k_amg_coarsest, dim3(1), dim3(kCoarsestThreads), 0, st, T.A, T.red.p,
                          bl[nl - 1], xl[nl - 1], 200);
     WC_LAUNCH_CHECK();
   } else {
-    smooth(st, T, bl[nl - 1], xl[nl - 1], 200);
+    smooth(st, H, T, bl[nl - 1], xl[nl - 1], 200);
   }
   for (int l = nl - 2; l >= 0; --l) {
     AmgLevel& L = H.L[l];
     hipLaunchKernelGGL(k_amg_prolong, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.n, L.parent.p,
                        xl[l + 1], xl[l]);
     WC_LAUNCH_CHECK();
-    smooth(st, L, bl[l], xl[l], 4);
+    smooth(st, H, L, bl[l], xl[l], 4);
   }
 }
 

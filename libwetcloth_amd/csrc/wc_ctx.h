@@ -182,11 +182,20 @@ struct AmgLevel {
   DevBuf<uint32_t> kids;        // 8 per DOF: its fine children ascending, ~0u after them
                                 // (every level but the first)
   DevBuf<double> x, b, r;       // V-cycle vectors (x, b of level 0 are the caller's)
+  // the rows again, ordered by colour (red rows ascending, then black rows
+  // ascending) with each row's entries in their order as 7 planes of n, so a
+  // colour's smoothing reads only its own rows, coalesced
+  int64_t nred = 0;
+  DevBuf<uint32_t> crow;        // compacted position -> row
+  DevBuf<uint32_t> ccol;        // [s * n + p]
+  DevBuf<double> cval;
+  DevBuf<uint8_t> clen;
 };
 constexpr int kAmgMaxLevels = 16;
 struct AmgHier {
   AmgLevel L[kAmgMaxLevels];
   int nl = 0;
+  bool compact = false;         // every level has its colour-ordered rows
   DevBuf<uint32_t> cmin, flag, scan_tmp;
   DevBuf<int> err;
   void release();
@@ -197,6 +206,9 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
                int nj, int nk);
 // z = one V-cycle applied to b from z = 0 (amgPrecondCompressed)
 void amg_vcycle(hipStream_t st, AmgHier& H, const double* b, double* z);
+// y = A x on level 0's colour-ordered rows (the products added in the row's
+// order from 0, as multiply); false when the hierarchy has none
+bool amg_spmv(hipStream_t st, AmgHier& H, const double* x, double* y);
 // AMGPCGSolveSparse (AlgebraicMultigrid.h:247-359) on a built hierarchy
 bool amgpcg_solve(hipStream_t st, PcgWork& w, AmgHier& H, const DevCsr& A, const double* b,
                   double* x, double tol, int maxit, double* residual_out, int* iters_out);

@@ -979,7 +979,8 @@ bool amgpcg_solve(hipStream_t st, PcgWork& w, AmgHier& H, const DevCsr& A, const
   const unsigned gb = blocks_for(n, 256);
   int it;
   for (it = 0; it < maxit; ++it) {
-    hipLaunchKernelGGL(k_spmv, dim3(gb), dim3(256), 0, st, A, s, z);
+    if (!amg_spmv(st, H, s, z))
+      hipLaunchKernelGGL(k_spmv, dim3(gb), dim3(256), 0, st, A, s, z);
     hipLaunchKernelGGL(k_dot_partial<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z, n,
                        w.partial.p);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, w.partial.p, nb,
