@@ -242,8 +242,11 @@ wc_status wc_mic0_apply_csr(int64_t n, const uint32_t* rowstart,
  * coarse operators 0.5 R A P, red-black Gauss-Seidel 4 sweeps per level and
  * 200 on the coarsest, coarsening while a level holds more than 4096 DOFs).
  * dof_ijk: 3 int32 per DOF, its cell (i, j, k) with 0 <= i < ni etc.  The
- * rows must be sorted and the system 7-point (at most 7 coarse entries a
- * row; WC_ERR_HIP otherwise).  x = 0 start, |r|_inf <= tol |b|_inf;
+ * rows must be sorted and every coarse row may hold at most 7 entries
+ * (WC_ERR_HIP otherwise; a 7-point system always meets this).  Rows of one
+ * colour (parity of i + j + k) must not couple: the reference smooths each
+ * colour in a parallel loop (AlgebraicMultigrid.h:123, 150), whose result is
+ * defined only then, and so do these kernels.  x = 0 start, |r|_inf <= tol |b|_inf;
  * WC_NOT_CONVERGED after max_iterations or when rho is 0 or NaN. */
 wc_status wc_amgpcg_solve_csr(int64_t n, const uint32_t* rowstart, const uint32_t* col,
                               const double* val, const int32_t* dof_ijk, int32_t ni,

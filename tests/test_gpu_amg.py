@@ -126,3 +126,45 @@ def test_solver_option_in_a_mixed_context(oracle, wc):
     r = b - oracle.spmv(rs, col, val, p)
     assert ok and np.abs(r).max() <= 1e-8 * np.abs(b).max()
     s.close()
+
+
+def block_coupled(rs, col, val, ijk):
+    """The system plus a -0.1 coupling between opposite corners of each
+    2x2x2 block (0.1 more on both diagonals): rows of up to 8 entries, coarse
+    rows still 7-point.  Opposite corners differ in all three coordinates,
+    so they have different colours: the reference smooths each colour in a
+    parallel loop (AlgebraicMultigrid.h:123, 150), which is deterministic
+    only when rows of one colour do not couple."""
+    import scipy.sparse as sp
+    n = len(rs) - 1
+    A = sp.csr_matrix((val, col.astype(np.int64), rs.astype(np.int64)), shape=(n, n)).tolil()
+    where = {tuple(c): d for d, c in enumerate(ijk.tolist())}
+    for d, (i, j, k) in enumerate(ijk.tolist()):
+        for di, dj, dk in ((1, 1, 1), (1, 1, -1), (1, -1, 1), (1, -1, -1)):
+            e = where.get((i + di, j + dj, k + dk))
+            if e is None or ((i // 2, j // 2, k // 2) != ((i + di) // 2, (j + dj) // 2,
+                                                          (k + dk) // 2)):
+                continue
+            A[d, e] = -0.1
+            A[e, d] = -0.1
+            A[d, d] += 0.1
+            A[e, e] += 0.1
+    A = A.tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data.copy()
+
+
+def test_rows_longer_than_seven(oracle, wc):
+    """Rows of more than 7 entries take the generic kernels (the
+    colour-ordered copies hold 7): still bitwise the oracle's V-cycle, and
+    the solve matches it."""
+    rs0, col0, val0, b, ijk, nx = dam_system(oracle, 24)
+    rs, col, val = block_coupled(rs0, col0, val0, ijk)
+    assert np.diff(rs).max() > 7
+    r = np.random.default_rng(5).standard_normal(len(b))
+    assert np.array_equal(wc.amg_apply_csr(rs, col, val, ijk, nx, r),
+                          oracle.amg_vcycle(rs, col, val, ijk, nx, r))
+    ok_r, x_r, it_r, _ = oracle.amgpcg(rs, col, val, ijk, nx, b, tol=1e-10, maxit=300)
+    ok, x, it, res = wc.amgpcg_solve_csr(rs, col, val, ijk, nx, b, tol=1e-10, maxit=300)
+    assert ok_r and ok and abs(it - it_r) <= 1
+    assert np.abs(x - x_r).max() <= 1e-6 * np.abs(x_r).max()
