@@ -179,8 +179,9 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
  *   wavefront kernels; 0 selects one launch per i+j+k level.  Both give
  *   bitwise the same result.
  *   "kernel_timing" (default 0): bracket the solver's sweep and SpMV launches
- *   with HIP events on the context's stream; setting it (to 0 or 1) clears
- *   the counters read by wc_kernel_stats.
+ *   with HIP events on the context's stream; n > 1 brackets those of every
+ *   n-th PCG iteration only (each pair of events costs a little time on the
+ *   stream).  Setting it clears the counters read by wc_kernel_stats.
  *   "mixed_digits" (default 6, 1..15): WC_MIXED contexts; each refinement
  *   pass iterates until its FP32 residual has dropped by 10^-digits (or
  *   meets the solve's tolerance).

@@ -576,6 +576,8 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
       WC_HIP(hipStreamSynchronize(c->stream));
       c->pcg.timer.reset();
       c->pcg.timer.on = value != 0;
+      c->pcg.timer.stride = value > 1 ? (int)std::min<int64_t>(value, 1 << 20) : 1;
+      c->pcg.timer.tick = 0;
     })
     return WC_OK;
   }

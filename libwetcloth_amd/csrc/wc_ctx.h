@@ -72,6 +72,10 @@ struct DevCsr {
 struct KTimer {
   enum Kind { SWEEP_FWD = 0, SWEEP_BWD = 1, SPMV = 2, COUNT = 3 };
   bool on = false;
+  int stride = 1;    // time the launches of every stride-th PCG iteration
+  int64_t tick = 0;
+  // this timer if the next PCG iteration is one to time, else null
+  KTimer* sample() { return on && tick++ % stride == 0 ? this : nullptr; }
   double ms[COUNT] = {};
   int64_t launches[COUNT] = {};
   int64_t units[COUNT] = {};

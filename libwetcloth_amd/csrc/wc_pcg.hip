@@ -709,7 +709,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   KTimer* tm = w.timer.on ? &w.timer : nullptr;
   box_gather(st, P, b, r);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
-  sweep_apply(st, P, r, tm, ndof);  // z and the z . r tile partials
+  sweep_apply(st, P, r, tm ? tm->sample() : nullptr, ndof);  // z and the z . r tile partials
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ntiles,
                      w.scal.p, (int)R_SET_RHO);
   WC_LAUNCH_CHECK();
@@ -724,16 +724,17 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   int it;
   bool ok = false;
   for (it = 0; it < maxit; ++it) {
-    hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
+    KTimer* ti = tm ? tm->sample() : nullptr;
+    hipEvent_t e0 = ti ? ti->mark(st) : nullptr;
     box_spmv_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
-    if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
+    if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
     hipLaunchKernelGGL(k_update_xr_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
                        z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
                        P.progress.p + 2, w.host_flag_dev);
     WC_LAUNCH_CHECK();
     mark_residual(st, ev);
     if (it + 1 < maxit) {
-      sweep_apply(st, P, r, tm, ndof);
+      sweep_apply(st, P, r, ti, ndof);
       const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
       hipLaunchKernelGGL(k_update_s_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
                          nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);
@@ -804,7 +805,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     }
     ++w.passes;
     const double inner_tol = std::max(tolr, eta * res);
-    sweep_apply32(st, P, r, tm, ndof);  // z and the z . r tile partials
+    sweep_apply32(st, P, r, tm ? tm->sample() : nullptr, ndof);  // z and the z . r partials
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ntiles,
                        w.scal.p, (int)R_SET_RHO);
     WC_LAUNCH_CHECK();
@@ -820,9 +821,10 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     // outputs, which the next refinement pass recomputes)
     int rho_slot = S_RHO;
     while (total < maxit) {
-      hipEvent_t e0 = tm ? tm->mark(st) : nullptr;
+      KTimer* ti = tm ? tm->sample() : nullptr;
+      hipEvent_t e0 = ti ? ti->mark(st) : nullptr;
       box_spmv32_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
-      if (e0) tm->add(KTimer::SPMV, ndof, e0, tm->mark(st));
+      if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
       hipLaunchKernelGGL(k_update_xr_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, x, r,
                          s, z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
                          P.progress.p + 2, w.host_flag_dev);
@@ -830,7 +832,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
       mark_residual(st, w.res_event);
       ++total;
       if (total < maxit) {
-        sweep_apply32(st, P, r, tm, ndof);
+        sweep_apply32(st, P, r, ti, ndof);
         const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
         hipLaunchKernelGGL(k_update_s_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
                            nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);

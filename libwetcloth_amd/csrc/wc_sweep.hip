@@ -1365,18 +1365,29 @@ void sweep_narrow(hipStream_t st, SweepPlan& P) {
   WC_LAUNCH_CHECK();
 }
 
+// all-ones (the sentinel) over n 16-byte words
+__global__ void k_fill_sentinel(uint4* p, int64_t n) {
+  const uint4 ones = make_uint4(~0u, ~0u, ~0u, ~0u);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = ones;
+}
+
 template <class V>
 static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, const V* cb,
                        V* zf, V* zb, KTimer* timer, int64_t ndof) {
-  // two ticket counters and the abort flag; outputs start as the sentinel
   // Two ticket counters; the outputs start as the sentinel.  Each pass zeroes
   // the other pass's counter, and the forward pass's rhs loader sets every
   // slot of zb to the sentinel as it goes, so an apply clears only zf (just
   // before the forward pass, which leaves it cache-resident for the face
-  // reads) and, after the plan or the factor, the counters.
+  // reads) and, after the plan or the factor, the counters.  The clear is a
+  // kernel of its own: a runtime fill waited ~15 us longer to start.
   int* tick = P.progress.p;
   if (!P.ticks_zero) WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 2, st));
-  WC_HIP(hipMemsetAsync(zf, 0xFF, sizeof(V) * P.ncell, st));
+  static_assert(kThreads % 16 == 0, "slots come in whole 16-byte words");
+  const int64_t words = (int64_t)sizeof(V) * P.ncell / 16;
+  hipLaunchKernelGGL(k_fill_sentinel, dim3((unsigned)std::min<int64_t>((words + 255) / 256, 4096)),
+                     dim3(256), 0, st, reinterpret_cast<uint4*>(zf), words);
   SweepArgs<V> S = sweep_args<V>(P);
   S.cells = reinterpret_cast<const CellT<V>*>(cf);
   S.in = r;

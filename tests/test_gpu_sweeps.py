@@ -98,3 +98,14 @@ def test_kernel_timing_counts_launches(oracle, wc):
     assert nl == it and units == it * ndof and ms > 0
     s.set_option("kernel_timing", 0)  # clears the counters
     assert all(v == (0.0, 0, 0) for v in s.kernel_stats().values())
+    # kernel_timing n > 1: the apply before the loop, then the launches of
+    # every n-th iteration
+    s.set_option("kernel_timing", 4)
+    ok, it, res = s.projectFine(1.0 / 480, 1e-8, 2000)
+    st = s.kernel_stats()
+    assert ok
+    for k in ("sweep_fwd", "sweep_bwd"):
+        ms, nl, units = st[k]
+        assert nl == 1 + it // 4 and units == nl * ndof and ms > 0
+    ms, nl, units = st["spmv"]
+    assert nl == it // 4 and units == nl * ndof
