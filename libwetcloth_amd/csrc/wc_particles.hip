@@ -21,6 +21,11 @@ namespace {
 constexpr int kP2GThreads = 256;  // the particle permutation in k_p2g assumes 256
 constexpr int kTile = 8;  // LDS nodes per axis: bucket (4) + halo 2 below, 2 above
 constexpr int kTileNodes = kTile * kTile * kTile;
+// P2G's LDS node layout: k planes 68 doubles apart, so the FP64 sums of the
+// 64 cells of a bucket fall on more distinct banks than at a stride of 64
+constexpr int kPlane = 68;
+constexpr int kLdsNodes = kTile * kPlane;
+__device__ inline int p2g_lds(int li, int lj, int lk) { return lk * kPlane + lj * kTile + li; }
 constexpr unsigned long long kPhiUntouched = 0xFFFFFFFFFFFFFFFFull;
 
 __device__ inline int clampi(int a, int lo, int hi) {
@@ -150,10 +155,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   // Particle mass and volume are uniform, so a node's mass and volume are
   // fm and fvol times its weight sum (TwoDScene.cpp:6700-6713 sums fm w and
   // fvol w): two LDS atomics per node instead of three.
-  __shared__ double sm[3][2][kTileNodes];
-  __shared__ unsigned long long sphi[kTileNodes];
+  __shared__ double sm[3][2][kLdsNodes];
+  __shared__ unsigned long long sphi[kLdsNodes];
   __shared__ int stouch[27];
-  for (int i = threadIdx.x; i < kTileNodes; i += kP2GThreads) {
+  for (int i = threadIdx.x; i < kLdsNodes; i += kP2GThreads) {
     for (int d = 0; d < 3; ++d) {
       sm[d][0][i] = 0.0;
       sm[d][1][i] = 0.0;
@@ -163,14 +168,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   if (threadIdx.x < 27) stouch[threadIdx.x] = 0;
   __syncthreads();
 
-  // The bucket's particles in chunks of 256; within a chunk thread t takes
-  // particle (t % 32) * 8 + t / 32 (a bijection of 0..255), so the lanes of a
-  // wave hold particles 8 apart: the bucket keeps its particles in their
-  // previous order, eight per cell when seeded, and lanes on different cells
-  // collide less on the same LDS nodes.
-  const uint32_t tperm = (threadIdx.x & 31) * 8 + (threadIdx.x >> 5);
-  for (uint32_t c0 = pbeg; c0 < pend; c0 += kP2GThreads) {
-    const uint32_t p = c0 + tperm;
+  // The bucket's particles in chunks of 512, two per thread: in pass q
+  // thread t takes particle 8 (t % 64) + t / 64 + 4 q (a bijection of
+  // 0..511), so the 64 lanes of a wave hold particles 8 apart: the bucket
+  // keeps its particles in their previous order, eight per cell when seeded,
+  // so the lanes sit in 64 different cells and collide less on the same LDS
+  // nodes.
+  const uint32_t tperm = (threadIdx.x & 63) * 8 + (threadIdx.x >> 6);
+#pragma unroll 1
+  for (uint32_t c0 = pbeg; c0 < pend; c0 += 2 * kP2GThreads)
+#pragma unroll 1
+  for (uint32_t q = 0; q < 2; ++q) {
+    const uint32_t p = c0 + tperm + 4 * q;
     if (p >= pend) continue;
     const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
     // u, v, w faces: TwoDScene.cpp:6700-6713
@@ -197,7 +206,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
             const int li = i - o0, lj = j - o1, lk = k - o2;
             if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
                 lk < kTile) {
-              const int l = (lk * kTile + lj) * kTile + li;
+              const int l = p2g_lds(li, lj, lk);
               atomicAdd(&sm[d][0][l], (double)ww);
               atomicAdd(&sm[d][1][l], (double)(vel * ww));
             } else {
@@ -224,7 +233,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
             const int li = i - o0, lj = j - o1, lk = k - o2;
             if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
                 lk < kTile)
-              atomicMin(&sphi[(lk * kTile + lj) * kTile + li], key);
+              atomicMin(&sphi[p2g_lds(li, lj, lk)], key);
             else
               atomicMin(&A.phikey[node_index(g, i, j, k)], key);
           }
@@ -251,11 +260,12 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   }
   __syncthreads();
 
-  for (int l = threadIdx.x; l < kTileNodes; l += kP2GThreads) {
-    const int i = o0 + (l % kTile), j = o1 + ((l / kTile) % kTile),
-              k = o2 + l / (kTile * kTile);
+  for (int t = threadIdx.x; t < kTileNodes; t += kP2GThreads) {
+    const int li = t % kTile, lj = (t / kTile) % kTile, lk = t / (kTile * kTile);
+    const int i = o0 + li, j = o1 + lj, k = o2 + lk;
     if (!in_range(g, i, j, k)) continue;
     const int64_t n = node_index(g, i, j, k);
+    const int l = p2g_lds(li, lj, lk);
     for (int d = 0; d < 3; ++d) {
       const double sw = sm[d][0][l];
       if (sw != 0.0) {
