@@ -241,13 +241,15 @@ __global__ void k_amg_redflag(int64_t n, const uint8_t* red, uint32_t* flag) {
   if (i < n) flag[i] = red[i];
 }
 
-// row i to position p (red rows first); its entries to the planes
-__global__ void k_amg_compact(DevCsr A, const uint8_t* red, const uint32_t* scan, int64_t nred,
+// row i to position p (red rows first); its entries to the planes.  scan
+// holds the exclusive scan of the red flags, and scan[n] their count.
+__global__ void k_amg_compact(DevCsr A, const uint8_t* red, const uint32_t* scan,
                               uint32_t* crow, uint32_t* ccol, double* cval, uint8_t* clen,
                               int* err) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t n = A.n;
   if (i >= n) return;
+  const int64_t nred = scan[n];
   const int64_t p = red[i] ? (int64_t)scan[i] : nred + (i - (int64_t)scan[i]);
   uint32_t rb, re;
   rows(A, i, rb, re);
@@ -443,6 +445,7 @@ void AmgHier::release() {
   cmin.release();
   flag.release();
   scan_tmp.release();
+  nred.release();
   err.release();
 }
 
@@ -460,6 +463,7 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
   WC_HIP(hipMemcpyAsync(L0.ijk.p, ijk, sizeof(int32_t) * 3 * (size_t)A.n,
                         hipMemcpyDeviceToDevice, st));
   H.err.ensure(1);
+  H.nred.ensure(kAmgMaxLevels);
   WC_HIP(hipMemsetAsync(H.err.p, 0, sizeof(int), st));
   while (H.L[H.nl - 1].n > 4096 && H.nl < kAmgMaxLevels) {
     AmgLevel& F = H.L[H.nl - 1];
@@ -475,13 +479,10 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
                        nnj, nnk, H.cmin.p, H.flag.p);
     WC_LAUNCH_CHECK();
     WC_HIP(hipMemsetAsync(H.flag.p + F.n, 0, sizeof(uint32_t), st));
+    // (a DOF outside the grid sets err and is skipped by every kernel; the
+    // build fails at its end)
     const int64_t nc = exclusive_scan(st, H.scan_tmp, H.flag.p, F.n + 1);  // synchronizes
-    int ferr = 0;
-    WC_HIP(hipMemcpy(&ferr, H.err.p, sizeof(int), hipMemcpyDeviceToHost));
-    if (ferr)
-      throw HipError{hipErrorInvalidValue, "AMG levels: a DOF cell lies outside the grid",
-                     __FILE__, __LINE__};
-    if (nc >= F.n) break;  // no reduction (the reference would not stop)
+    if (nc == 0 || nc >= F.n) break;  // no reduction (the reference would not stop)
     F.parent.ensure((size_t)F.n);
     C.n = nc;
     C.ni = nni;
@@ -523,21 +524,31 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
                        H.flag.p);
     WC_LAUNCH_CHECK();
     WC_HIP(hipMemsetAsync(H.flag.p + L.n, 0, sizeof(uint32_t), st));
-    L.nred = exclusive_scan(st, H.scan_tmp, H.flag.p, L.n + 1);
+    // the count lands in flag[n]; kept per level, read back with the error
+    // flags below (one host wait for the whole colour split)
+    exclusive_scan_async(st, H.scan_tmp, H.flag.p, L.n + 1);
+    WC_HIP(hipMemcpyAsync(H.nred.p + l, H.flag.p + L.n, sizeof(uint32_t),
+                          hipMemcpyDeviceToDevice, st));
     L.crow.ensure((size_t)L.n);
     L.clen.ensure((size_t)L.n);
     L.ccol.ensure((size_t)kSlots * L.n);
     L.cval.ensure((size_t)kSlots * L.n);
     hipLaunchKernelGGL(k_amg_compact, dim3(nblk(L.n, 256)), dim3(256), 0, st, L.A, L.red.p,
-                       H.flag.p, L.nred, L.crow.p, L.ccol.p, L.cval.p, L.clen.p, H.err.p);
+                       H.flag.p, L.crow.p, L.ccol.p, L.cval.p, L.clen.p, H.err.p);
     WC_LAUNCH_CHECK();
   }
   int herr = 0;
+  uint32_t nred[kAmgMaxLevels] = {};
   WC_HIP(hipMemcpyAsync(&herr, H.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  WC_HIP(hipMemcpyAsync(nred, H.nred.p, sizeof(uint32_t) * H.nl, hipMemcpyDeviceToHost, st));
   WC_HIP(hipStreamSynchronize(st));
+  if (herr & 2)
+    throw HipError{hipErrorInvalidValue, "AMG levels: a DOF cell lies outside the grid",
+                   __FILE__, __LINE__};
   if (herr & 1)
     throw HipError{hipErrorInvalidValue, "AMG levels: a coarse row has more than 7 entries",
                    __FILE__, __LINE__};
+  for (int l = 0; l < H.nl; ++l) H.L[l].nred = nred[l];
   H.compact = (herr & 4) == 0;
 }
 

@@ -201,6 +201,7 @@ struct AmgHier {
   int nl = 0;
   bool compact = false;         // every level has its colour-ordered rows
   DevBuf<uint32_t> cmin, flag, scan_tmp;
+  DevBuf<uint32_t> nred;        // red rows of each level, on the device
   DevBuf<int> err;
   void release();
 };
@@ -280,6 +281,9 @@ namespace wc {
 // In-place exclusive scan of n uint32 values; returns the total.
 uint32_t exclusive_scan(wc_ctx* ctx, uint32_t* data, int64_t n);
 uint32_t exclusive_scan(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* data, int64_t n);
+// the same without reading the total back (no host wait); the scratch must
+// not be reused before the scan has run
+void exclusive_scan_async(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* data, int64_t n);
 
 // ---- wc_particles.hip ----
 void particles_sort(wc_ctx* ctx);

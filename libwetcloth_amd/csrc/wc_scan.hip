@@ -77,14 +77,19 @@ void scan_rec(hipStream_t st, uint32_t* data, int64_t n, uint32_t* scratch,
 
 }  // namespace
 
-uint32_t exclusive_scan(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* data, int64_t n) {
-  if (n <= 0) return 0;
+void exclusive_scan_async(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* data, int64_t n) {
+  if (n <= 0) return;
   // scratch: block sums of every recursion level plus the total
   int64_t need = 2;
   for (int64_t m = n; m > 1; m = (m + kScanTile - 1) / kScanTile) need += m / kScanTile + 2;
   scratch.ensure((size_t)need + 8);
-  uint32_t* total_dev = scratch.p;
-  scan_rec(st, data, n, scratch.p + 2, total_dev);
+  scan_rec(st, data, n, scratch.p + 2, scratch.p);
+}
+
+uint32_t exclusive_scan(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* data, int64_t n) {
+  if (n <= 0) return 0;
+  exclusive_scan_async(st, scratch, data, n);
+  const uint32_t* total_dev = scratch.p;  // left there by the scan
   uint32_t total = 0;
   WC_HIP(hipMemcpyAsync(&total, total_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   WC_HIP(hipStreamSynchronize(st));
