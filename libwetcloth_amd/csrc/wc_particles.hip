@@ -21,8 +21,11 @@ namespace {
 constexpr int kP2GThreads = 256;  // the particle permutation in k_p2g assumes 256
 constexpr int kTile = 8;  // LDS nodes per axis: bucket (4) + halo 2 below, 2 above
 constexpr int kTileNodes = kTile * kTile * kTile;
-// P2G's LDS node layout: k planes 68 doubles apart, so the FP64 sums of the
-// 64 cells of a bucket fall on more distinct banks than at a stride of 64
+// P2G's LDS node layout: k planes 68 doubles apart.  A double spans two of
+// the 64 banks; with i, j strides of 1 and 8 doubles and this k stride, the
+// nodes of a bucket's 4x4x4 cells shift by 2 i + 16 j + 8 k banks, so a
+// wave's 64 lanes meet at most 2-way conflicts (a stride of 64 aliases every
+// k, 72 aliases k with j)
 constexpr int kPlane = 68;
 constexpr int kLdsNodes = kTile * kPlane;
 __device__ inline int p2g_lds(int li, int lj, int lk) { return lk * kPlane + lj * kTile + li; }
