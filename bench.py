@@ -87,6 +87,27 @@ def measured_traffic(kernel: str, workload: str):
     return e.get("bytes_per_launch")
 
 
+# particle-state words a transfer reads and writes per particle: P2G reads
+# x, v and C (3 + 3 + 9); G2P reads x and writes x, v and C (3 + 3 + 3 + 9).
+# The grid's bytes are not counted, so the rates are lower bounds.
+P2G_WORDS, G2P_WORDS = 15, 18
+
+
+def transfers(stats: dict, word: int) -> dict:
+    """Achieved HBM rate of the P2G and G2P kernels from the library's
+    HIP-event times (every launch of the timed substeps)."""
+    out = {}
+    for name, words in (("p2g", P2G_WORDS), ("g2p", G2P_WORDS)):
+        ms, nl, units = stats[name]
+        if nl == 0:
+            continue
+        b = units / nl * words * word
+        gbs = b / (ms / nl * 1e-3) / 1e9
+        out[name] = {"avg_ms": ms / nl, "launches": nl, "particles": units // nl,
+                     "particle_bytes_per_launch": b, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def roofline(stats: dict, word: int, workload: str) -> dict:
     """HBM roofline of the dominant kernel (the slower MIC(0) sweep) from the
     HIP-event times the library recorded on its own stream; word is the
@@ -235,6 +256,8 @@ def main():
     }
     out["roofline"] = roofline(m["kernel_stats"], 4, workload)
     out["fp64"]["roofline"] = roofline(f["kernel_stats"], 8, workload)
+    out["transfers"] = transfers(m["kernel_stats"], 4)
+    out["fp64"]["transfers"] = transfers(f["kernel_stats"], 8)
     if not a.no_amg:
         amg, g = run(W, W.WC_FP64, n, dt, a, solver=1)
         amg.close()

@@ -201,12 +201,16 @@ wc_status wc_set_option(wc_ctx* ctx, const char* name, int64_t value);
  * WC_FP64 context).  No reference counterpart. */
 wc_status wc_refinement_passes(const wc_ctx* ctx, int32_t* passes);
 
-/* Accumulated device time of one solver kernel class since "kernel_timing"
- * was last set: total ms, number of launches, and DOFs processed (the sum
- * over launches of the system size).  No reference counterpart. */
+/* Accumulated device time of one kernel class since "kernel_timing" was
+ * last set: total ms, number of launches, and units processed (the sum over
+ * launches of the system size in DOFs for the solver kernels, of the
+ * particle count for the transfers).  The transfers are timed at every
+ * launch whatever the "kernel_timing" stride.  No reference counterpart. */
 #define WC_KERNEL_SWEEP_FWD 0 /* MIC(0) forward sweep (solve_lower) */
 #define WC_KERNEL_SWEEP_BWD 1 /* MIC(0) backward sweep (solve_lower_transpose) */
 #define WC_KERNEL_SPMV 2      /* multiply(FixedSparseMatrix, s, z) */
+#define WC_KERNEL_P2G 3       /* the P2G kernel of wc_map_particle_nodes */
+#define WC_KERNEL_G2P 4       /* the G2P + advection kernel of wc_map_node_particles */
 wc_status wc_kernel_stats(const wc_ctx* ctx, int32_t kernel, double* ms,
                           int64_t* launches, int64_t* units);
 

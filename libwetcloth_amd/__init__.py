@@ -27,7 +27,7 @@ FIELD = {
 }
 _FIELD_DTYPE = {"act": np.uint8, "dof": np.int32, "valid": np.uint8}
 
-KERNELS = ("sweep_fwd", "sweep_bwd", "spmv")  # WC_KERNEL_* order
+KERNELS = ("sweep_fwd", "sweep_bwd", "spmv", "p2g", "g2p")  # WC_KERNEL_* order
 PHASES = ("rebucketize", "p2g", "step_velocity", "assemble", "solve", "apply_pressure", "g2p")
 
 _dp = C.POINTER(C.c_double)
@@ -322,8 +322,9 @@ class Scene:
         return int(p.value)
 
     def kernel_stats(self) -> dict:
-        """Per solver kernel class since set_option("kernel_timing", ...):
-        {name: (total ms, launches, DOFs processed)}."""
+        """Per kernel class since set_option("kernel_timing", ...): {name:
+        (total ms, launches, units processed)}; units are DOFs for the solver
+        kernels and particles for "p2g" and "g2p"."""
         out = {}
         for k, name in enumerate(KERNELS):
             ms = C.c_double(0)

@@ -70,7 +70,7 @@ struct DevCsr {
 // launching stream around each launch; pending pairs are read back only at
 // host synchronization points the solver already has.
 struct KTimer {
-  enum Kind { SWEEP_FWD = 0, SWEEP_BWD = 1, SPMV = 2, COUNT = 3 };
+  enum Kind { SWEEP_FWD = 0, SWEEP_BWD = 1, SPMV = 2, P2G = 3, G2P = 4, COUNT = 5 };
   bool on = false;
   int stride = 1;    // time the launches of every stride-th PCG iteration
   int64_t tick = 0;

@@ -743,9 +743,12 @@ void p2g_launch(wc_ctx* ctx, Planes<R> P) {
   A.fvol = (R)ctx->desc.particle_volume;
   A.r0 = (R)(g.dx * 0.883644 > ctx->desc.particle_radius ? g.dx * 0.883644
                                                           : ctx->desc.particle_radius);
+  KTimer& tm = ctx->pcg.timer;
+  hipEvent_t e0 = tm.on ? tm.mark(ctx->stream) : nullptr;
   hipLaunchKernelGGL(k_p2g<R>, dim3((unsigned)g.nbucket), dim3(kP2GThreads), 0,
                      ctx->stream, A);
   WC_LAUNCH_CHECK();
+  if (e0) tm.add(KTimer::P2G, ctx->np, e0, tm.mark(ctx->stream));
 }
 
 template <class R>
@@ -764,6 +767,8 @@ void g2p_launch(wc_ctx* ctx, Planes<R> P, double dt) {
   A.invD = (R)(4.0 / (ctx->g.dx * ctx->g.dx));
   A.solid_projection = ctx->desc.solid_projection;
   A.bstart = ctx->bstart.p;
+  KTimer& tm = ctx->pcg.timer;
+  hipEvent_t e0 = tm.on ? tm.mark(ctx->stream) : nullptr;
   // with bstart describing the particle arrays, one workgroup per bucket
   if (ctx->sorted)
     hipLaunchKernelGGL(k_g2p_bucket<R>, dim3((unsigned)ctx->g.nbucket), dim3(256), 0,
@@ -772,6 +777,7 @@ void g2p_launch(wc_ctx* ctx, Planes<R> P, double dt) {
     hipLaunchKernelGGL(k_g2p_advect<R>, dim3(blocks(ctx->np, 256)), dim3(256), 0,
                        ctx->stream, A);
   WC_LAUNCH_CHECK();
+  if (e0) tm.add(KTimer::G2P, ctx->np, e0, tm.mark(ctx->stream));
 }
 
 }  // namespace

@@ -109,3 +109,11 @@ def test_kernel_timing_counts_launches(oracle, wc):
         assert nl == 1 + it // 4 and units == nl * ndof and ms > 0
     ms, nl, units = st["spmv"]
     assert nl == it // 4 and units == nl * ndof
+    # the transfers: every launch, units = particles
+    np_ = s.num_particles()
+    s.set_option("kernel_timing", 4)
+    s.step(1.0 / 480, 1e-8, 2000)
+    st = s.kernel_stats()
+    for k in ("p2g", "g2p"):
+        ms, nl, units = st[k]
+        assert nl == 1 and units == np_ and ms > 0
