@@ -315,11 +315,14 @@ struct G2PArgs {
 // region(d, base) may return an LDS array holding face d's values at
 // [(c2 * kTile + c1) * kTile + c0] for the 27 stencil nodes base + (c0, c1,
 // c2) (then no node is fetched one by one), or null.
+// g2p_particle_at takes the particle's position as already loaded.
 template <class R, class Fetch, class Region>
-__device__ __attribute__((always_inline)) inline void g2p_particle(const G2PArgs<R>& A, int64_t p,
-                                                                   Fetch fetch, Region region) {
+__device__ __attribute__((always_inline)) inline void g2p_particle_at(const G2PArgs<R>& A,
+                                                                      int64_t p, R x0, R x1,
+                                                                      R x2, Fetch fetch,
+                                                                      Region region) {
   const Grid& g = A.g;
-  const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
+  const R px[3] = {x0, x1, x2};
   R fv[3] = {0, 0, 0};
   R B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -422,6 +425,12 @@ __device__ __attribute__((always_inline)) inline void g2p_particle(const G2PArgs
 }
 
 // One thread per particle, gathers from global memory.
+template <class R, class Fetch, class Region>
+__device__ __attribute__((always_inline)) inline void g2p_particle(const G2PArgs<R>& A, int64_t p,
+                                                                   Fetch fetch, Region region) {
+  g2p_particle_at<R>(A, p, A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p], fetch, region);
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -452,6 +461,15 @@ __global__ __launch_bounds__(256) void k_g2p_bucket(G2PArgs<R> A) {
   const int bk = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
   const int o0 = bi * 4 - 2, o1 = bj * 4 - 2, o2 = bk * 4 - 2;
   __shared__ double su[3][kTileNodes];
+  // the first particle's position is loaded before the region is staged,
+  // and each next one while the current one is processed
+  R nx[3] = {0, 0, 0};
+  if (pbeg + threadIdx.x < pend) {
+    const uint32_t q = pbeg + threadIdx.x;
+    nx[0] = A.x[q];
+    nx[1] = A.x[A.cap + q];
+    nx[2] = A.x[2 * A.cap + q];
+  }
   for (int l = threadIdx.x; l < kTileNodes; l += 256) {
     const int i = o0 + (l % kTile), j = o1 + ((l / kTile) % kTile), k = o2 + l / (kTile * kTile);
     const bool on = node_active(g, A.act, i, j, k);
@@ -462,8 +480,15 @@ __global__ __launch_bounds__(256) void k_g2p_bucket(G2PArgs<R> A) {
   for (uint32_t c0 = pbeg; c0 < pend; c0 += 256) {
     const uint32_t p = c0 + threadIdx.x;
     if (p >= pend) continue;
-    g2p_particle<R>(
-        A, p,
+    const R x0 = nx[0], x1 = nx[1], x2 = nx[2];
+    // this thread's next particle (no other thread writes it before then)
+    if (p + 256 < pend) {
+      nx[0] = A.x[p + 256];
+      nx[1] = A.x[A.cap + p + 256];
+      nx[2] = A.x[2 * A.cap + p + 256];
+    }
+    g2p_particle_at<R>(
+        A, p, x0, x1, x2,
         [&](int d, int i, int j, int k, R& u) {
           const int li = i - o0, lj = j - o1, lk = k - o2;
           if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile && lk < kTile) {
