@@ -16,7 +16,7 @@ from tests.test_oracle_amg import dam_system
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [24, 40])
+@pytest.mark.parametrize("n", [24, 40, 64])
 def test_vcycle_is_bitwise_the_oracles(oracle, wc, n):
     rs, col, val, b, ijk, nx = dam_system(oracle, n)
     assert len(oracle.amg_levels(rs, col, val, ijk, nx)) >= 2
@@ -168,3 +168,15 @@ def test_rows_longer_than_seven(oracle, wc):
     ok, x, it, res = wc.amgpcg_solve_csr(rs, col, val, ijk, nx, b, tol=1e-10, maxit=300)
     assert ok_r and ok and abs(it - it_r) <= 1
     assert np.abs(x - x_r).max() <= 1e-6 * np.abs(x_r).max()
+
+
+def test_amgpcg_not_converged_after_maxit(oracle, wc):
+    """maxit = 1: the solve returns false (WC_NOT_CONVERGED, not an error)
+    after one iteration, like the reference (AlgebraicMultigrid.h:352-360),
+    with the oracle's iterate."""
+    rs, col, val, b, ijk, nx = dam_system(oracle, 24)
+    ok_r, x_r, it_r, res_r = oracle.amgpcg(rs, col, val, ijk, nx, b, tol=1e-10, maxit=1)
+    ok, x, it, res = wc.amgpcg_solve_csr(rs, col, val, ijk, nx, b, tol=1e-10, maxit=1)
+    assert not ok_r and not ok and it == it_r == 1
+    assert np.abs(x - x_r).max() <= 1e-10 * np.abs(x_r).max()
+    assert abs(res - res_r) <= 1e-8 * res_r
