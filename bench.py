@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-amg", action="store_true",
                     help="skip the FP64 AMG-PCG run of the workload (the reference's "
                          "as-shipped pressure solver, reported under fp64_amg)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -140,17 +140,19 @@ def roofline(stats: dict, word: int, workload: str) -> dict:
 
 def cpu_baseline(n_particles_gpu: int, steps: int) -> dict:
     """The FP64 oracle (a plain-C port of the reference path) on one host core,
-    on a bounded sample: substeps of the 64^3 dam break (config 0)."""
+    on a bounded sample of the workload: the same block, velocity field and
+    tolerance at 128^3 (dt 1/960), whose substep takes ~18 s on one core."""
     from oracle import oracle as O
     O.build()
-    n = 64
-    g, x, v = O.dam_break(n)
+    n = 128
+    g, x, v = O.dam_break(n, block_hi=(0.5, 0.6, 1.0), seed=4, sigma=1.0)
     s = O.OracleScene((n, n, n), 1.0 / n, x, v)
-    dt = 1.0 / 480
-    s.substep(dt, 1e-6, 2000)  # untimed: first-touch of the arrays
+    dt = 1.0 / 960
     t0 = time.perf_counter()
+    iters = []
     for _ in range(steps):
         s.substep(dt, 1e-6, 2000)
+        iters.append(s.last_solve[1])
     t = (time.perf_counter() - t0) / steps
     np_sample = x.shape[1]
     scaled = (1.0 / t) * np_sample / max(n_particles_gpu, 1)
@@ -159,10 +161,11 @@ def cpu_baseline(n_particles_gpu: int, steps: int) -> dict:
         "unit": "substeps/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"{steps} substeps of the 64^3 dam break ({np_sample} particles, "
-                   f"{t:.2f} s each, {np_sample / t:.3e} particle-updates/s) on 1 core; "
-                   f"scaled to the {n_particles_gpu}-particle workload by particle count "
-                   "(the CPU solve grows faster than that, so this overstates the CPU rate)"),
+        "sample": (f"{steps} substep(s) of the workload's dam break at 128^3 ({np_sample} "
+                   f"particles, dt 1/960, {t:.1f} s each, PCG iterations {iters}, "
+                   f"{np_sample / t:.3e} particle-updates/s) on 1 core; scaled to the "
+                   f"{n_particles_gpu}-particle workload by particle count (the 256^3 solve "
+                   "takes more iterations, so this overstates the CPU rate)"),
     }
 
 
