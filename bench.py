@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-amg", action="store_true",
                     help="skip the FP64 AMG-PCG run of the workload (the reference's "
                          "as-shipped pressure solver, reported under fp64_amg)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
