@@ -171,18 +171,16 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   if (threadIdx.x < 27) stouch[threadIdx.x] = 0;
   __syncthreads();
 
-  // The bucket's particles in chunks of 512, two per thread: in pass q
-  // thread t takes particle 8 (t % 64) + t / 64 + 4 q (a bijection of
-  // 0..511), so the 64 lanes of a wave hold particles 8 apart: the bucket
-  // keeps its particles in their previous order, eight per cell when seeded,
-  // so the lanes sit in 64 different cells and collide less on the same LDS
-  // nodes.
-  const uint32_t tperm = (threadIdx.x & 63) * 8 + (threadIdx.x >> 6);
-#pragma unroll 1
-  for (uint32_t c0 = pbeg; c0 < pend; c0 += 2 * kP2GThreads)
-#pragma unroll 1
-  for (uint32_t q = 0; q < 2; ++q) {
-    const uint32_t p = c0 + tperm + 4 * q;
+  // The bucket's particles in chunks of 256; within a chunk thread t takes
+  // particle (t % 32) * 8 + t / 32 (a bijection of 0..255), so the lanes of a
+  // wave hold particles 8 apart: the bucket keeps its particles in their
+  // previous order, eight per cell when seeded, and lanes on different cells
+  // collide less on the same LDS nodes.  (Chunks of 512 with all 64 lanes on
+  // different cells were slower: the waves then fetch each particle line
+  // more often.)
+  const uint32_t tperm = (threadIdx.x & 31) * 8 + (threadIdx.x >> 5);
+  for (uint32_t c0 = pbeg; c0 < pend; c0 += kP2GThreads) {
+    const uint32_t p = c0 + tperm;
     if (p >= pend) continue;
     const R px[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
     // u, v, w faces: TwoDScene.cpp:6700-6713
