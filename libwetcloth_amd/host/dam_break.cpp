@@ -1,7 +1,8 @@
 // Drives the C++ host mirror the way a reference user would: a
 // robertbridson::PCGSolver<double> solve on a matrix assembled with
 // SparseMatrix::add_to_element, AMGPCGSolveSparse on a 3-D one with the
-// DOFs' cells, then liquid substeps through WetClothCore.
+// DOFs' cells, then liquid substeps through WetClothCore (MIC(0), then two
+// with the AMG solver).
 // Prints one JSON line.  Usage: dam_break <n> <steps> [outdir]
 #include <chrono>
 #include <cmath>
@@ -103,6 +104,7 @@ int main(int argc, char** argv) {
     for (int s = 0; s < steps; ++s) all_ok &= core.stepSystem(1.0 / 480);
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const int last_iterations = stepper.lastIterations();
     if (outdir) {
       // the reference's output files of the frame (TwoDSceneSerializer)
       wcamd::TwoDSceneSerializer ser;
@@ -113,14 +115,21 @@ int main(int argc, char** argv) {
     scene.getParticles(x4, v4, fB);
     double vy = 0.0;
     for (int64_t p = 0; p < np; ++p) vy += v4[4 * p + 1];
+    // 3. two more substeps with the reference's own solver, AMG-PCG
+    scene.setOption("solver", 1);
+    bool amg_steps_ok = true;
+    for (int s = 0; s < 2; ++s) amg_steps_ok &= core.stepSystem(1.0 / 480);
+    const int amg_step_iterations = stepper.lastIterations();
     std::printf(
         "{\"pcg_ok\": %s, \"pcg_iterations\": %d, \"pcg_residual\": %.17g, "
         "\"pcg_max_error\": %.17g, \"amg_ok\": %s, \"amg_iterations\": %d, "
         "\"amg_max_error\": %.17g, \"particles\": %lld, \"steps_ok\": %s, "
-        "\"last_iterations\": %d, \"mean_vy\": %.17g, \"seconds\": %.6f}\n",
+        "\"last_iterations\": %d, \"mean_vy\": %.17g, \"seconds\": %.6f, "
+        "\"amg_steps_ok\": %s, \"amg_step_iterations\": %d}\n",
         ok ? "true" : "false", iterations, residual, err / nrm, amg_ok ? "true" : "false",
         amg_it, amg_err / amg_nrm, (long long)np,
-        all_ok ? "true" : "false", stepper.lastIterations(), vy / (double)np, secs);
+        all_ok ? "true" : "false", last_iterations, vy / (double)np, secs,
+        amg_steps_ok ? "true" : "false", amg_step_iterations);
   } catch (const wcamd::Error& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return 1;

@@ -86,6 +86,10 @@ class TwoDScene {
   // and solidProjection, which take the substep size
   void mapNodeParticlesAPIC(double dt) { check(wc_map_node_particles(ctx_, dt), ctx_); }
 
+  // wc_set_option: "solver" 1 makes projectFine solve with AMGPCGSolveSparse
+  // (the reference's call at Pressure.cpp:797), 0 with the MIC(0) PCG
+  void setOption(const char* name, int64_t value) { check(wc_set_option(ctx_, name, value), ctx_); }
+
   wc_ctx* handle() const { return ctx_; }
   const wc_desc& desc() const { return desc_; }
 
@@ -103,8 +107,9 @@ class LinearizedImplicitEuler {
     check(wc_step_velocity(s.handle(), dt), s.handle());
     return true;
   }
-  // projectFine -> pressure::solveNodePressure with the MIC(0) PCG solve;
-  // like the reference, a failed solve is reported and the substep goes on
+  // projectFine -> pressure::solveNodePressure with the MIC(0) PCG solve, or
+  // AMG-PCG after setOption("solver", 1); like the reference, a failed solve
+  // is reported and the substep goes on
   bool projectFine(TwoDScene& s, double dt) {
     const wc_status st = wc_project(s.handle(), dt, criterion_, maxiters_, &residual_, &iterations_);
     check(st, s.handle());

@@ -25,6 +25,7 @@ def test_cpp_host_mirror(oracle, tmp_path):
     assert r["pcg_ok"] and r["pcg_max_error"] < 1e-9
     assert r["amg_ok"] and r["amg_max_error"] < 1e-9 and 0 < r["amg_iterations"] < 30
     assert r["steps_ok"]
+    assert r["amg_steps_ok"] and 0 < r["amg_step_iterations"] < 40
     g, x, v = oracle.dam_break(n)
     assert r["particles"] == x.shape[1]
     s = oracle.OracleScene((n, n, n), 1.0 / n, x, v)
