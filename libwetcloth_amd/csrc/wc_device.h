@@ -12,7 +12,15 @@
 
 namespace wc {
 
-constexpr int kGhost = 4;   // one bucket of ghost cells below the container
+// Two buckets of ghost cells below the container: the B2 stencils of
+// particles in the container reach one cell below it, into the bucket just
+// below, and the one-bucket dilation of the activated set
+// (expandFluidNodesMarked, TwoDScene.cpp:5021-5064) reaches one bucket
+// further.  The reference's grid extends three buckets past the particles'
+// bounding box (TwoDScene.cpp:1657-1690), so that dilation is never clipped
+// there; with two ghost buckets it is not clipped here either, and the
+// activated buckets, and with them the pressure DOFs, are the reference's.
+constexpr int kGhost = 8;
 constexpr int kBucket = 4;  // cells per bucket edge (every reference scene)
 
 struct Grid {
@@ -31,7 +39,8 @@ inline Grid make_grid(int ni, int nj, int nk, double dx) {
   g.n[2] = nk;
   g.dx = dx;
   for (int d = 0; d < 3; ++d) {
-    g.nb[d] = (g.n[d] + kBucket - 1) / kBucket + 3;
+    // two buckets below (kGhost) and two above the container
+    g.nb[d] = (g.n[d] + kBucket - 1) / kBucket + 4;
     g.nx[d] = g.nb[d] * kBucket;
   }
   g.nbucket = (int64_t)g.nb[0] * g.nb[1] * g.nb[2];

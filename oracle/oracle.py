@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_math.so")
-GHOST = 4
+GHOST = 8
 
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
@@ -65,73 +65,98 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        L.wco_quad_kernel.restype = C.c_double
-        L.wco_quad_kernel.argtypes = [C.c_double]
-        L.wco_fraction_inside2.restype = C.c_double
-        L.wco_fraction_inside2.argtypes = [C.c_double] * 2
-        L.wco_fraction_inside4.restype = C.c_double
-        L.wco_fraction_inside4.argtypes = [C.c_double] * 4
-        L.wco_box_container_phi.restype = C.c_double
-        L.wco_box_container_phi.argtypes = [_dp, _dp, _dp]
-        L.wco_hash3.restype = C.c_uint64
-        L.wco_hash3.argtypes = [C.c_uint64] * 3
-        L.wco_uniform.restype = C.c_double
-        L.wco_uniform.argtypes = [C.c_uint64]
-        L.wco_grid_init.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int, C.c_double]
-        L.wco_node_index.restype = C.c_int64
-        L.wco_node_index.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int]
-        L.wco_seed_block.restype = C.c_int64
-        L.wco_seed_block.argtypes = [C.POINTER(Grid), _dp, _dp, C.c_uint64, _dp, C.c_int64]
-        L.wco_seed_velocity.argtypes = [C.POINTER(Grid), C.c_uint64, C.c_double,
-                                        C.c_int64, _dp, _dp]
-        L.wco_scene_create.restype = C.POINTER(SceneStruct)
-        L.wco_scene_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
-                                       C.c_double, C.c_double, C.c_double, C.c_double,
-                                       C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
-                                       C.c_double]
-        L.wco_scene_destroy.argtypes = [C.POINTER(SceneStruct)]
-        for nm in ("wco_activate", "wco_update_liquid_phi", "wco_update_solid_phi",
-                   "wco_update_solid_weights", "wco_p2g", "wco_copy_vel_to_unew"):
-            getattr(L, nm).argtypes = [C.POINTER(SceneStruct)]
-        L.wco_step_velocity.argtypes = [C.POINTER(SceneStruct), C.c_double]
-        L.wco_assemble.restype = C.c_int64
-        L.wco_assemble.argtypes = [C.POINTER(SceneStruct), C.c_double]
-        L.wco_solve_pressure.restype = C.c_int
-        L.wco_solve_pressure.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_int]
-        L.wco_apply_pressure.argtypes = [C.POINTER(SceneStruct), C.c_double]
-        L.wco_g2p_advect.argtypes = [C.POINTER(SceneStruct), C.c_double]
-        L.wco_substep.restype = C.c_int
-        L.wco_substep.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_double, C.c_int]
-        L.wco_spmv.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
-        L.wco_mic0_factor.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double,
-                                      C.c_double, _dp, _dp]
-        L.wco_solve_lower.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp]
-        L.wco_solve_lower_transpose.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
-        L.wco_dot.restype = C.c_double
-        L.wco_dot.argtypes = [C.c_int64, _dp, _dp]
-        L.wco_abs_max.restype = C.c_double
-        L.wco_abs_max.argtypes = [C.c_int64, _dp]
-        L.wco_pcg_solve.restype = C.c_int
-        L.wco_pcg_solve.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
-                                    C.c_int, C.c_double, C.c_double, _dp,
-                                    C.POINTER(C.c_int)]
-        _i32p_ = C.POINTER(C.c_int32)
-        L.wco_amg_levels.restype = C.c_int
-        L.wco_amg_levels.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
-                                     C.c_int, C.POINTER(C.c_int64), C.c_int]
-        L.wco_amg_coarse.restype = C.c_int64
-        L.wco_amg_coarse.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
-                                     C.c_int, _u32p, _u32p, _u32p, _dp, C.c_int64]
-        L.wco_amg_rbgs.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, _dp, _dp, C.c_int]
-        L.wco_amg_vcycle.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
-                                     C.c_int, _dp, _dp]
-        L.wco_amgpcg_solve.restype = C.c_int
-        L.wco_amgpcg_solve.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
-                                       C.c_int, _dp, _dp, C.c_double, C.c_int, _dp,
-                                       C.POINTER(C.c_int)]
-        _lib = L
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+def build_variant(out_dir: str, plant: int, ghost: int | None = None) -> str:
+    """Compile the oracle with planted mistake ``plant`` (see WCO_PLANT in
+    wc_oracle.c), or with ``ghost`` ghost cells below the container instead
+    of WCO_GHOST, into ``out_dir``; returns the library's path.  For the
+    tests that show each pin fails on a named mistake."""
+    tag = f"plant{plant}" + (f"_ghost{ghost}" if ghost is not None else "")
+    out = os.path.join(out_dir, f"liboracle_{tag}.so")
+    defs = [f"-DWCO_PLANT={plant}"] + ([f"-DWCO_GHOST={ghost}"] if ghost is not None else [])
+    subprocess.run(["gcc", "-O2", "-std=c99", "-ffp-contract=off", "-fPIC", "-shared", *defs,
+                    "-o", out, os.path.join(HERE, "wc_oracle.c"),
+                    os.path.join(HERE, "wc_oracle_amg.c"), "-lm"], check=True)
+    return out
+
+
+def load_variant(path: str):
+    """A separately built oracle library (same interface as lib())."""
+    return _configure(C.CDLL(path))
+
+
+def _configure(L):
+    """Set the argument and result types of an oracle library."""
+    L.wco_quad_kernel.restype = C.c_double
+    L.wco_quad_kernel.argtypes = [C.c_double]
+    L.wco_fraction_inside2.restype = C.c_double
+    L.wco_fraction_inside2.argtypes = [C.c_double] * 2
+    L.wco_fraction_inside4.restype = C.c_double
+    L.wco_fraction_inside4.argtypes = [C.c_double] * 4
+    L.wco_box_container_phi.restype = C.c_double
+    L.wco_box_container_phi.argtypes = [_dp, _dp, _dp]
+    L.wco_hash3.restype = C.c_uint64
+    L.wco_hash3.argtypes = [C.c_uint64] * 3
+    L.wco_uniform.restype = C.c_double
+    L.wco_uniform.argtypes = [C.c_uint64]
+    L.wco_grid_init.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int, C.c_double]
+    L.wco_node_index.restype = C.c_int64
+    L.wco_node_index.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int]
+    L.wco_seed_block.restype = C.c_int64
+    L.wco_seed_block.argtypes = [C.POINTER(Grid), _dp, _dp, C.c_uint64, _dp, C.c_int64]
+    L.wco_seed_velocity.argtypes = [C.POINTER(Grid), C.c_uint64, C.c_double,
+                                    C.c_int64, _dp, _dp]
+    L.wco_scene_create.restype = C.POINTER(SceneStruct)
+    L.wco_scene_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
+                                   C.c_double]
+    L.wco_scene_destroy.argtypes = [C.POINTER(SceneStruct)]
+    for nm in ("wco_activate", "wco_update_liquid_phi", "wco_update_solid_phi",
+               "wco_update_solid_weights", "wco_p2g", "wco_copy_vel_to_unew"):
+        getattr(L, nm).argtypes = [C.POINTER(SceneStruct)]
+    L.wco_step_velocity.argtypes = [C.POINTER(SceneStruct), C.c_double]
+    L.wco_assemble.restype = C.c_int64
+    L.wco_assemble.argtypes = [C.POINTER(SceneStruct), C.c_double]
+    L.wco_solve_pressure.restype = C.c_int
+    L.wco_solve_pressure.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_int]
+    L.wco_apply_pressure.argtypes = [C.POINTER(SceneStruct), C.c_double]
+    L.wco_g2p_advect.argtypes = [C.POINTER(SceneStruct), C.c_double]
+    L.wco_substep.restype = C.c_int
+    L.wco_substep.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_double, C.c_int]
+    L.wco_spmv.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
+    L.wco_mic0_factor.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double,
+                                  C.c_double, _dp, _dp]
+    L.wco_solve_lower.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp]
+    L.wco_solve_lower_transpose.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp]
+    L.wco_dot.restype = C.c_double
+    L.wco_dot.argtypes = [C.c_int64, _dp, _dp]
+    L.wco_abs_max.restype = C.c_double
+    L.wco_abs_max.argtypes = [C.c_int64, _dp]
+    L.wco_pcg_solve.restype = C.c_int
+    L.wco_pcg_solve.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
+                                C.c_int, C.c_double, C.c_double, _dp,
+                                C.POINTER(C.c_int)]
+    _i32p_ = C.POINTER(C.c_int32)
+    L.wco_amg_levels.restype = C.c_int
+    L.wco_amg_levels.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                 C.c_int, C.POINTER(C.c_int64), C.c_int]
+    L.wco_amg_coarse.restype = C.c_int64
+    L.wco_amg_coarse.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                 C.c_int, _u32p, _u32p, _u32p, _dp, C.c_int64]
+    L.wco_amg_rbgs.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, _dp, _dp, C.c_int]
+    L.wco_amg_vcycle.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                 C.c_int, _dp, _dp]
+    L.wco_amgpcg_solve.restype = C.c_int
+    L.wco_amgpcg_solve.argtypes = [C.c_int64, _u32p, _u32p, _dp, _i32p_, C.c_int, C.c_int,
+                                   C.c_int, _dp, _dp, C.c_double, C.c_int, _dp,
+                                   C.POINTER(C.c_int)]
+    for nm in ("wco_extrapolate", "wco_constrain_liquid_velocity"):
+        getattr(L, nm).argtypes = [C.POINTER(SceneStruct)]
+    return L
 
 
 def ref_lib():
@@ -197,7 +222,7 @@ class OracleScene:
     """FP64 CPU scene: one reference substep per :meth:`substep` call."""
 
     def __init__(self, n, dx, x, v=None, Cmat=None, rho=1000.0, flip=0.99,
-                 gravity=(0.0, -9.81, 0.0), pvol=None, radius=None):
+                 gravity=(0.0, -9.81, 0.0), pvol=None, radius=None, L=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
         npart = x.shape[1]
         v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float64)
@@ -209,13 +234,14 @@ class OracleScene:
         self.n = tuple(int(a) for a in n)
         self.dx = float(dx)
         self.np = npart
-        self.s = lib().wco_scene_create(self.n[0], self.n[1], self.n[2], self.dx, rho, flip,
+        self.L = L if L is not None else lib()
+        self.s = self.L.wco_scene_create(self.n[0], self.n[1], self.n[2], self.dx, rho, flip,
                                         gravity[0], gravity[1], gravity[2], npart,
                                         ptr(x), ptr(v), ptr(Cm), pmass, pvol, radius)
 
     def __del__(self):
-        if getattr(self, "s", None) is not None and _lib is not None:
-            _lib.wco_scene_destroy(self.s)
+        if getattr(self, "s", None) is not None and getattr(self, "L", None) is not None:
+            self.L.wco_scene_destroy(self.s)
             self.s = None
 
     @property
@@ -269,20 +295,35 @@ class OracleScene:
         self.st.solid_projection = 1 if on else 0
 
     # reference phases
-    def activate(self): lib().wco_activate(self.s)
-    def update_liquid_phi(self): lib().wco_update_liquid_phi(self.s)
-    def update_solid_phi(self): lib().wco_update_solid_phi(self.s)
-    def update_solid_weights(self): lib().wco_update_solid_weights(self.s)
-    def p2g(self): lib().wco_p2g(self.s)
-    def copy_vel_to_unew(self): lib().wco_copy_vel_to_unew(self.s)
-    def step_velocity(self, dt): lib().wco_step_velocity(self.s, dt)
-    def assemble(self, dt): return lib().wco_assemble(self.s, dt)
-    def solve_pressure(self, tol, maxit): return lib().wco_solve_pressure(self.s, tol, maxit)
-    def apply_pressure(self, dt): lib().wco_apply_pressure(self.s, dt)
-    def g2p_advect(self, dt): lib().wco_g2p_advect(self.s, dt)
+    def activate(self): self.L.wco_activate(self.s)
+    def update_liquid_phi(self): self.L.wco_update_liquid_phi(self.s)
+    def update_solid_phi(self): self.L.wco_update_solid_phi(self.s)
+    def update_solid_weights(self): self.L.wco_update_solid_weights(self.s)
+    def p2g(self): self.L.wco_p2g(self.s)
+    def copy_vel_to_unew(self): self.L.wco_copy_vel_to_unew(self.s)
+    def step_velocity(self, dt): self.L.wco_step_velocity(self.s, dt)
+    def assemble(self, dt): return self.L.wco_assemble(self.s, dt)
+    def solve_pressure(self, tol, maxit): return self.L.wco_solve_pressure(self.s, tol, maxit)
+    def apply_pressure(self, dt): self.L.wco_apply_pressure(self.s, dt)
+    def extrapolate(self): self.L.wco_extrapolate(self.s)
+    def constrain_liquid_velocity(self): self.L.wco_constrain_liquid_velocity(self.s)
+    def g2p_advect(self, dt): self.L.wco_g2p_advect(self.s, dt)
+
+    def set_field(self, name: str, axis: int | None, values: np.ndarray):
+        """Overwrite a grid field of the scene (test hook)."""
+        st = self.st
+        nn = st.g.nnode
+        if name in ("unew", "vel", "ustar", "w", "mass", "vol", "rhs", "invC"):
+            dst = np.ctypeslib.as_array(getattr(st, name)[axis], shape=(nn,))
+            dst[:] = np.asarray(values, dtype=np.float64)
+        elif name == "valid":
+            dst = np.ctypeslib.as_array(st.valid[axis], shape=(nn,))
+            dst[:] = np.asarray(values, dtype=np.uint8)
+        else:
+            raise KeyError(name)
 
     def substep(self, dt, tol=1e-8, maxit=1000):
-        return lib().wco_substep(self.s, dt, tol, maxit)
+        return self.L.wco_substep(self.s, dt, tol, maxit)
 
     def pre_solve(self, dt):
         """Phases up to and including assembly."""

@@ -15,6 +15,20 @@
 
 #define THETA_CRITERION 0.01 /* Pressure.cpp:20 */
 
+/* Planted mistakes, for the tests only: tests/test_oracle_pins.py compiles
+ * this file again with -DWCO_PLANT=k to show that each pin of the oracle
+ * fails on mistake k.  The default build (WCO_PLANT 0) contains none of them.
+ *   1 activation without the one-bucket ring     2 activation with two rings
+ *   3 stencil buckets from the base node only     4 no solid override of the liquid phi
+ *   5 DOF test w >= 0 instead of w > 0            6 DOFs numbered i-fastest over the grid
+ *   7 extrapolation in place (no copies)          8 one extrapolation layer
+ *   9 flip_coeff applied to v instead of C        10 gravity with the wrong sign
+ *   11 u* without the dt factor on gravity        12 liquid phi radius without the
+ *                                                    0.883644 dx floor */
+#ifndef WCO_PLANT
+#define WCO_PLANT 0
+#endif
+
 /* ------------------------------------------------------------------ grid */
 
 void wco_grid_init(wco_grid* g, int ni, int nj, int nk, double dx) {
@@ -23,9 +37,14 @@ void wco_grid_init(wco_grid* g, int ni, int nj, int nk, double dx) {
   g->n[2] = nk;
   g->dx = dx;
   for (int d = 0; d < 3; ++d) {
-    /* one bucket of ghost cells below, the interior, then two more buckets
-     * (ghost cells above plus the far faces of the last cells) */
-    g->nb[d] = (g->n[d] + WCO_BUCKET - 1) / WCO_BUCKET + 3;
+    /* two buckets of ghost cells below, the interior, then two more buckets
+     * above: the stencils of particles in the container reach one cell past
+     * it and the activation dilates by one more bucket
+     * (expandFluidNodesMarked(1), TwoDScene.cpp:5021-5064); the reference's
+     * grid extends three buckets past the particles' bounding box
+     * (TwoDScene.cpp:1657-1690), so it never clips that dilation, and
+     * neither does this one */
+    g->nb[d] = (g->n[d] + WCO_BUCKET - 1) / WCO_BUCKET + 4;
     g->nx[d] = g->nb[d] * WCO_BUCKET;
   }
   g->nbucket = (int64_t)g->nb[0] * g->nb[1] * g->nb[2];
@@ -410,7 +429,7 @@ void wco_activate(wco_scene* s) {
       for (int d = 0; d < 3; ++d) {
         const int b = stencil_base(g, d, px[d], offs[gi][d]);
         blo[d] = b >> 2;
-        bhi[d] = (b + 2) >> 2;
+        bhi[d] = (WCO_PLANT == 3 ? b : b + 2) >> 2;
       }
       for (int bk = blo[2]; bk <= bhi[2]; ++bk)
         for (int bj = blo[1]; bj <= bhi[1]; ++bj)
@@ -422,9 +441,10 @@ void wco_activate(wco_scene* s) {
     for (int bj = 0; bj < g->nb[1]; ++bj)
       for (int bi = 0; bi < g->nb[0]; ++bi) {
         int a = 0;
-        for (int t = -1; t <= 1 && !a; ++t)
-          for (int u = -1; u <= 1 && !a; ++u)
-            for (int r = -1; r <= 1 && !a; ++r) {
+        const int ring = WCO_PLANT == 1 ? 0 : (WCO_PLANT == 2 ? 2 : 1);
+        for (int t = -ring; t <= ring && !a; ++t)
+          for (int u = -ring; u <= ring && !a; ++u)
+            for (int r = -ring; r <= ring && !a; ++r) {
               const int x = bi + r, y = bj + u, z = bk + t;
               if (x < 0 || y < 0 || z < 0 || x >= g->nb[0] || y >= g->nb[1] ||
                   z >= g->nb[2])
@@ -448,7 +468,8 @@ void wco_update_liquid_phi(wco_scene* s) {
   const double dx = g->dx;
   const double bucket_size = WCO_BUCKET * dx;
   for (int64_t c = 0; c < g->nnode; ++c) s->lphi[c] = 3.0 * bucket_size;
-  const double r0 = dx * 0.883644 > s->radius ? dx * 0.883644 : s->radius;
+  const double r0 = WCO_PLANT == 12 ? s->radius
+                                     : (dx * 0.883644 > s->radius ? dx * 0.883644 : s->radius);
   for (int64_t p = 0; p < s->np; ++p) {
     const double px[3] = {s->x[p], s->x[s->np + p], s->x[2 * s->np + p]};
     int base[3];
@@ -469,7 +490,7 @@ void wco_update_liquid_phi(wco_scene* s) {
   for (int k = 0; k < g->nx[2]; ++k)
     for (int j = 0; j < g->nx[1]; ++j)
       for (int i = 0; i < g->nx[0]; ++i) {
-        if (!node_active(s, i, j, k)) continue;
+        if (!node_active(s, i, j, k) || WCO_PLANT == 4) continue;
         if (cell_solid_phi(s, i, j, k) < 0.0)
           s->lphi[wco_node_index(g, i, j, k)] = -0.5 * dx;
       }
@@ -604,7 +625,12 @@ void wco_step_velocity(wco_scene* s, double dt) {
       }
       const double M = s->mass[d][n];
       double rhs = 0.0;
-      rhs -= M * gd * (-dt);
+      if (WCO_PLANT == 10)
+        rhs += M * gd * (-dt);
+      else if (WCO_PLANT == 11)
+        rhs -= M * gd * (-1.0);
+      else
+        rhs -= M * gd * (-dt);
       rhs += M * s->vel[d][n];
       s->rhs[d][n] = rhs;
       const double Q = 1.0, Ms = 0.0;
@@ -678,10 +704,20 @@ int64_t wco_assemble(wco_scene* s, double dt) {
       for (int f = 0; f < 6 && !open; ++f) {
         const int* q = fi[f];
         if (!node_active(s, q[0], q[1], q[2])) continue;
-        if (s->w[f >> 1][wco_node_index(g, q[0], q[1], q[2])] > 0) open = 1;
+        const double wf = s->w[f >> 1][wco_node_index(g, q[0], q[1], q[2])];
+        if (WCO_PLANT == 5 ? wf >= 0 : wf > 0) open = 1;
       }
       if (open) s->dof[n] = (int32_t)ndof++;
     }
+  }
+  if (WCO_PLANT == 6) { /* planted: numbered i-fastest over the whole grid */
+    int64_t c = 0;
+    for (int k = 0; k < g->nx[2]; ++k)
+      for (int j = 0; j < g->nx[1]; ++j)
+        for (int i = 0; i < g->nx[0]; ++i) {
+          const int64_t n = wco_node_index(g, i, j, k);
+          if (s->dof[n] >= 0) s->dof[n] = (int32_t)c++;
+        }
   }
   s->ndof = ndof;
   s->rowstart = (uint32_t*)xcalloc(ndof + 1, sizeof(uint32_t));
@@ -1022,27 +1058,45 @@ void wco_apply_pressure(wco_scene* s, double dt) {
           s->unew[d][n] = rhs * s->invC[d][n];
           s->valid[d][n] = valid;
         }
-    /* two extrapolation layers, each reading copies */
-    double* ucopy = (double*)xcalloc(g->nnode, sizeof(double));
-    uint8_t* vcopy = (uint8_t*)xcalloc(g->nnode, 1);
-    static const int off[6][3] = {{-1, 0, 0}, {1, 0, 0},  {0, -1, 0},
-                                  {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
-    for (int layer = 0; layer < 2; ++layer) {
+  }
+  wco_extrapolate(s);
+  wco_constrain_liquid_velocity(s);
+}
+
+/* Pressure.cpp:1826-1903 extrapolate, called twice per axis from
+ * applyPressureDragFluid: each layer gives every invalid face of an
+ * activated bucket the mean of its valid 6-neighbours, read from copies of
+ * the velocity and the valid flags made before the layer. */
+void wco_extrapolate(wco_scene* s) {
+  const wco_grid* g = &s->g;
+  double* ucopy = (double*)xcalloc(g->nnode, sizeof(double));
+  uint8_t* vcopy = (uint8_t*)xcalloc(g->nnode, 1);
+  static const int off[6][3] = {{-1, 0, 0}, {1, 0, 0},  {0, -1, 0},
+                                {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
+  const int layers = WCO_PLANT == 8 ? 1 : 2;
+  for (int d = 0; d < 3; ++d) {
+    for (int layer = 0; layer < layers; ++layer) {
+      const double* uread = ucopy;
+      const uint8_t* vread = vcopy;
+      if (WCO_PLANT == 7) { /* planted: reads the faces updated in this layer */
+        uread = s->unew[d];
+        vread = s->valid[d];
+      }
       memcpy(ucopy, s->unew[d], sizeof(double) * g->nnode);
       memcpy(vcopy, s->valid[d], g->nnode);
       for (int k = 0; k < g->nx[2]; ++k)
         for (int j = 0; j < g->nx[1]; ++j)
           for (int i = 0; i < g->nx[0]; ++i) {
             const int64_t n = wco_node_index(g, i, j, k);
-            if (!s->act[n >> 6] || vcopy[n]) continue;
+            if (!s->act[n >> 6] || vread[n]) continue;
             double sum = 0;
             int count = 0;
             for (int r = 0; r < 6; ++r) {
               const int a = i + off[r][0], b = j + off[r][1], c = k + off[r][2];
               if (!node_active(s, a, b, c)) continue;
               const int64_t m = wco_node_index(g, a, b, c);
-              if (!vcopy[m]) continue;
-              sum += ucopy[m];
+              if (!vread[m]) continue;
+              sum += uread[m];
               ++count;
             }
             if (count > 0) {
@@ -1051,9 +1105,17 @@ void wco_apply_pressure(wco_scene* s, double dt) {
             }
           }
     }
-    free(ucopy);
-    free(vcopy);
-    /* constrainLiquidVelocity: lerp toward the (static) solid velocity */
+  }
+  free(ucopy);
+  free(vcopy);
+}
+
+/* TwoDScene.cpp:1827-1866 constrainLiquidVelocity: faces with zero solid
+ * weight are lerped toward the (static) solid velocity by the boundary
+ * friction. */
+void wco_constrain_liquid_velocity(wco_scene* s) {
+  const wco_grid* g = &s->g;
+  for (int d = 0; d < 3; ++d)
     for (int64_t n = 0; n < g->nnode; ++n) {
       if (!s->act[n >> 6]) continue;
       if (s->w[d][n] == 0.0) {
@@ -1061,7 +1123,6 @@ void wco_apply_pressure(wco_scene* s, double dt) {
         s->unew[d][n] = (1 - fr) * s->unew[d][n] + fr * usolid;
       }
     }
-  }
 }
 
 /* TwoDScene.cpp:6936-7090 mapNodeParticlesAPIC (fluid branch), then
@@ -1100,9 +1161,10 @@ void wco_g2p_advect(wco_scene* s, double dt) {
             B[d * 3 + 2] += fnv * ww * e2 * invD;
           }
     }
-    for (int t = 0; t < 9; ++t) s->C[(int64_t)t * np + p] = B[t] * s->flip;
+    for (int t = 0; t < 9; ++t) s->C[(int64_t)t * np + p] = WCO_PLANT == 9 ? B[t] : B[t] * s->flip;
     double nx[3];
     for (int d = 0; d < 3; ++d) {
+      if (WCO_PLANT == 9) fv[d] *= s->flip;
       s->v[(int64_t)d * np + p] = fv[d];
       nx[d] = px[d] + fv[d] * dt;
     }

@@ -25,8 +25,18 @@
  *       sums of interior liquid rows and the textbook 7-point Laplacian.
  *   wco_apply_pressure  pinned by the reference's own divergence diagnostic
  *       (discrete divergence <= tol after projection).
- *   wco_update_liquid_phi / activation / extrapolation: pinned by hand cases
- *       on tiny grids (tests/test_oracle_scene.py).
+ *   wco_activate, wco_update_liquid_phi, wco_update_solid_phi/weights and the
+ *       DOF set and numbering of wco_assemble: pinned against a model of the
+ *       reference's sparse bucket grid in its own coordinates (origin and
+ *       extent from the particles' bounding box), and by the closed-form DOF
+ *       count m^3 + 6 m^2 of a lattice cube (tests/test_oracle_pins.py).
+ *   wco_extrapolate: pinned by a lone valid face reaching exactly its
+ *       two-step 6-neighbourhood with its own value, and a uniform field
+ *       staying uniform.
+ *   wco_step_velocity: u* = u + g dt at faces with mass.
+ *   G2P flip_coeff: C = flip B and v unscaled on an affine field.
+ *   Each of these pins fails on the oracle built with a planted mistake
+ *   (WCO_PLANT in wc_oracle.c).
  *   BLAS dot: a sequential sum; the reference uses Eigen's SIMD-blocked dot,
  *       so dot products match to rounding only (parity by tolerance).
  */
@@ -44,7 +54,9 @@ extern "C" {
  * dense index i = w + WCO_GHOST.  Nodes are stored in 4x4x4 buckets,
  * bucket-major (x fastest), then local k*16 + j*4 + i -- the reference's
  * bucket/local numbering (Core/Sorter.h:66-68, Core/TwoDScene.cpp:3754-3761). */
-#define WCO_GHOST 4
+#ifndef WCO_GHOST
+#define WCO_GHOST 8
+#endif
 #define WCO_BUCKET 4
 
 typedef struct {
@@ -138,6 +150,12 @@ void wco_step_velocity(wco_scene* s, double dt);
 int64_t wco_assemble(wco_scene* s, double dt);
 int wco_solve_pressure(wco_scene* s, double tol, int maxit);
 void wco_apply_pressure(wco_scene* s, double dt);
+/* the two parts of wco_apply_pressure after the direct faces, exposed so the
+ * tests can run them on chosen fields: two extrapolation layers of unew and
+ * valid on every axis (Pressure.cpp:1826-1903), then constrainLiquidVelocity
+ * (TwoDScene.cpp:1827-1866) */
+void wco_extrapolate(wco_scene* s);
+void wco_constrain_liquid_velocity(wco_scene* s);
 void wco_g2p_advect(wco_scene* s, double dt);
 /* test hook: G2P reads the P2G velocity instead of the projected one */
 void wco_copy_vel_to_unew(wco_scene* s);

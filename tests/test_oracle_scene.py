@@ -13,7 +13,7 @@ import pytest
 import scipy.ndimage as ndi
 import scipy.sparse as sp
 
-GHOST = 4
+GHOST = 8
 DT = 1.0 / 480
 
 
