@@ -131,13 +131,42 @@ wc_status wc_project(wc_ctx* ctx, double dt, double tol, int32_t maxit,
 /* applyPressureDragFluid (LinearizedImplicitEuler.cpp:2435), acceptVelocity
  * (:2176) and TwoDScene::constrainLiquidVelocity (TwoDScene.cpp:1827). */
 wc_status wc_apply_pressure(wc_ctx* ctx, double dt);
-/* TwoDScene::mapNodeParticlesAPIC (TwoDScene.h:493, TwoDScene.cpp:6936) fused
- * with SceneStepper::advectScene (SceneStepper.cpp:17) and
- * TwoDScene::solidProjection (TwoDScene.cpp:1776). */
-wc_status wc_map_node_particles(wc_ctx* ctx, double dt);
-/* All of the above for one substep (WetClothCore::stepSystem's loop body). */
+/* TwoDScene::mapNodeParticlesAPIC() (TwoDScene.h:493, TwoDScene.cpp:
+ * 6936-7090): the fluid particles' velocity and affine matrix from the
+ * projected grid velocity (no argument, like the reference call). */
+wc_status wc_map_node_particles(wc_ctx* ctx);
+/* SceneStepper::advectScene(scene, dt) (SceneStepper.cpp:17-41), fluid
+ * branch: x += v dt. */
+wc_status wc_advect_scene(wc_ctx* ctx, double dt);
+/* TwoDScene::solidProjection(dt) (TwoDScene.cpp:1776-1822): a particle whose
+ * solid distance would fall below 0 is moved back along the distance
+ * gradient.  As in the reference the corner-grid weights belong to the
+ * position the transfer weights were computed at (before wc_advect_scene)
+ * and the gradient points from the current one. */
+wc_status wc_solid_projection(wc_ctx* ctx, double dt);
+/* The three calls above in one pass over the particles (the same results,
+ * bit for bit): mapNodeParticlesAPIC, advectScene and, when the context's
+ * solid_projection is set, solidProjection (WetClothCore.cpp:248-262). */
+wc_status wc_map_node_particles_advect(wc_ctx* ctx, double dt);
+/* TwoDScene::getMaxFluidVelocity (TwoDScene.cpp:4180-4188): the largest
+ * fluid particle speed. */
+wc_status wc_max_fluid_velocity(wc_ctx* ctx, double* vmax);
+/* One substep: the phases above in WetClothCore.cpp:109-262 order with the
+ * fused G2P (one iteration of stepSystem's substep loop). */
+wc_status wc_substep(wc_ctx* ctx, double dt, double tol, int32_t maxit,
+                     double* residual_out, int32_t* iters_out);
+/* WetClothCore::stepSystem(dt) (WetClothCore.cpp:45-300): splits dt into
+ * max(1, ceil(dt / max_dt)) equal substeps, max_dt = min(3 dx / v_max,
+ * 1/30) with v_max from getMaxFluidVelocity (:52-62), and runs wc_substep
+ * on each.  Every substep runs even when a solve does not converge (the
+ * reference prints a warning and goes on); the status is then
+ * WC_NOT_CONVERGED.  residual_out is the last substep's, iters_out the sum
+ * over the substeps. */
 wc_status wc_step(wc_ctx* ctx, double dt, double tol, int32_t maxit,
                   double* residual_out, int32_t* iters_out);
+/* Substeps of the last wc_step.  No reference counterpart (stepSystem
+ * prints them). */
+wc_status wc_step_substeps(const wc_ctx* ctx, int32_t* n);
 
 /* ---- diagnostics (downloads for parity checks) ---- */
 #define WC_FIELD_ACT 0       /* uint8 per bucket: activated */

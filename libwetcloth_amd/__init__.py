@@ -89,8 +89,14 @@ def lib():
         L.wc_step_velocity.argtypes = [vp, C.c_double]
         L.wc_project.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _i32p]
         L.wc_apply_pressure.argtypes = [vp, C.c_double]
-        L.wc_map_node_particles.argtypes = [vp, C.c_double]
+        L.wc_map_node_particles.argtypes = [vp]
+        L.wc_advect_scene.argtypes = [vp, C.c_double]
+        L.wc_solid_projection.argtypes = [vp, C.c_double]
+        L.wc_map_node_particles_advect.argtypes = [vp, C.c_double]
+        L.wc_max_fluid_velocity.argtypes = [vp, _dp]
+        L.wc_substep.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _i32p]
         L.wc_step.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _i32p]
+        L.wc_step_substeps.argtypes = [vp, _i32p]
         L.wc_grid_info.argtypes = [vp, _i32p, _i64p, _i64p]
         L.wc_download_field.argtypes = [vp, C.c_int32, C.c_int32, vp]
         L.wc_download_system.argtypes = [vp, _i64p, _i64p, _u32p, _u32p, _dp, _dp, _dp]
@@ -255,16 +261,51 @@ class Scene:
     def applyPressureDragFluid(self, dt):
         _check(lib().wc_apply_pressure(self.h, dt), self.h)
 
-    def mapNodeParticlesAPIC(self, dt):
-        """G2P fused with advectScene and solidProjection."""
-        _check(lib().wc_map_node_particles(self.h, dt), self.h)
+    def mapNodeParticlesAPIC(self):
+        """TwoDScene::mapNodeParticlesAPIC(): particle v and C from the grid."""
+        _check(lib().wc_map_node_particles(self.h), self.h)
+
+    def advectScene(self, dt):
+        """SceneStepper::advectScene: x += v dt."""
+        _check(lib().wc_advect_scene(self.h, dt), self.h)
+
+    def solidProjection(self, dt):
+        """TwoDScene::solidProjection(dt)."""
+        _check(lib().wc_solid_projection(self.h, dt), self.h)
+
+    def map_node_particles_advect(self, dt):
+        """mapNodeParticlesAPIC, advectScene and solidProjection in one pass."""
+        _check(lib().wc_map_node_particles_advect(self.h, dt), self.h)
+
+    def getMaxFluidVelocity(self) -> float:
+        v = C.c_double()
+        _check(lib().wc_max_fluid_velocity(self.h, C.byref(v)), self.h)
+        return v.value
+
+    def substep(self, dt, tol=1e-8, maxit=1000):
+        """One substep of dt: (converged, PCG iterations, residual)."""
+        res = C.c_double()
+        it = C.c_int32()
+        st = lib().wc_substep(self.h, dt, tol, maxit, C.byref(res), C.byref(it))
+        _check(st, self.h, allow_nc=True)
+        return st == WC_OK, it.value, res.value
 
     def step(self, dt, tol=1e-8, maxit=1000):
+        """WetClothCore::stepSystem(dt): the CFL-split substeps of dt;
+        (all converged, PCG iterations summed, last residual)."""
         res = C.c_double()
         it = C.c_int32()
         st = lib().wc_step(self.h, dt, tol, maxit, C.byref(res), C.byref(it))
         _check(st, self.h, allow_nc=True)
         return st == WC_OK, it.value, res.value
+
+    stepSystem = step
+
+    def step_substeps(self) -> int:
+        """Substeps the last step() split its dt into."""
+        n = C.c_int32()
+        _check(lib().wc_step_substeps(self.h, C.byref(n)), self.h)
+        return n.value
 
     # ---- diagnostics ----
     def grid_info(self):

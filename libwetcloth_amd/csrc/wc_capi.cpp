@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <math.h>
 #include <stdio.h>
 
 #include <algorithm>
@@ -100,6 +101,9 @@ void release_all(wc_ctx* c) {
   c->pid.release();
   c->pid2.release();
   c->pkey.release();
+  c->xw.release();
+  c->fxw.release();
+  c->vmax_bits.release();
   c->bcount.release();
   c->bstart.release();
   c->bcursor.release();
@@ -333,7 +337,34 @@ wc_status wc_apply_pressure(wc_ctx* c, double dt) {
   return WC_OK;
 }
 
-wc_status wc_map_node_particles(wc_ctx* c, double dt) {
+wc_status wc_map_node_particles(wc_ctx* c) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::particles_g2p(c);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_advect_scene(wc_ctx* c, double dt) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::particles_advect(c, dt);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_solid_projection(wc_ctx* c, double dt) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  WC_TRY(c, {
+    wc::particles_solid_projection(c, dt);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_map_node_particles_advect(wc_ctx* c, double dt) {
   if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
   WC_TRY(c, {
     wc::particles_g2p_advect(c, dt);
@@ -342,40 +373,101 @@ wc_status wc_map_node_particles(wc_ctx* c, double dt) {
   return WC_OK;
 }
 
-wc_status wc_step(wc_ctx* c, double dt, double tol, int32_t maxit,
-                  double* residual_out, int32_t* iters_out) {
+wc_status wc_max_fluid_velocity(wc_ctx* c, double* vmax) {
+  if (!ready(c) || !vmax) return fail(c, WC_ERR_ARG, "bad argument");
+  WC_TRY(c, { *vmax = wc::particles_max_velocity(c); })
+  return WC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// WetClothCore::stepSystem's substep count (WetClothCore.cpp:52-62): liquid
+// particles may move 3 cells per substep, solids 1/3 (none here: their
+// bound is then 1/30 s), and a substep is at most 1/30 s.
+int cfl_substeps(double dt, double dx, double max_fluid_vel) {
+  const double max_elasto_vel = 0.0;
+  const double max_elasto_dt = std::min(dx / std::max(1e-63, max_elasto_vel) / 3.0, 1.0 / 30.0);
+  const double max_fluid_dt = std::min(dx / std::max(1e-63, max_fluid_vel) * 3.0, 1.0 / 30.0);
+  const double max_dt = std::min(max_elasto_dt, max_fluid_dt);
+  return std::max(1, (int)ceil(dt / max_dt));
+}
+
+// one substep in the order of WetClothCore.cpp:109-262, fused where the
+// reference's calls touch the same data (G2P, advection and projection)
+bool substep(wc_ctx* c, double dt, double tol, int maxit, double* res, int* it) {
+  record(c, 0);
+  wc::particles_sort(c);
+  record(c, 1);
+  wc::particles_p2g(c);
+  wc::grid_activate(c);
+  wc::grid_nodes(c);
+  record(c, 2);
+  wc::grid_step_velocity(c, dt);
+  record(c, 3);
+  wc::grid_assemble(c, dt);
+  record(c, 4);
+  const bool ok = solve_system(c, tol, maxit, res, it);
+  record(c, 5);
+  wc::grid_apply_pressure(c, dt);
+  record(c, 6);
+  wc::particles_g2p_advect(c, dt);
+  record(c, 7);
+  WC_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 7; ++i) {
+    float ms = 0.f;
+    WC_HIP(hipEventElapsedTime(&ms, c->events.ev[i], c->events.ev[i + 1]));
+    c->phase_ms[i] = ms;
+  }
+  return ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+wc_status wc_substep(wc_ctx* c, double dt, double tol, int32_t maxit, double* residual_out,
+                     int32_t* iters_out) {
   if (!ready(c) || maxit < 0) return fail(c, WC_ERR_ARG, "bad argument");
   bool ok = true;
   double res = 0;
   int it = 0;
+  WC_TRY(c, { ok = substep(c, dt, tol, maxit, &res, &it); })
+  if (residual_out) *residual_out = res;
+  if (iters_out) *iters_out = it;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_step(wc_ctx* c, double dt, double tol, int32_t maxit,
+                  double* residual_out, int32_t* iters_out) {
+  if (!ready(c) || maxit < 0 || !(dt >= 0)) return fail(c, WC_ERR_ARG, "bad argument");
+  bool ok = true;
+  double res = 0;
+  int it = 0;
   WC_TRY(c, {
-    record(c, 0);
-    wc::particles_sort(c);
-    record(c, 1);
-    wc::particles_p2g(c);
-    wc::grid_activate(c);
-    wc::grid_nodes(c);
-    record(c, 2);
-    wc::grid_step_velocity(c, dt);
-    record(c, 3);
-    wc::grid_assemble(c, dt);
-    record(c, 4);
-    ok = solve_system(c, tol, maxit, &res, &it);
-    record(c, 5);
-    wc::grid_apply_pressure(c, dt);
-    record(c, 6);
-    wc::particles_g2p_advect(c, dt);
-    record(c, 7);
-    WC_HIP(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < 7; ++i) {
-      float ms = 0.f;
-      WC_HIP(hipEventElapsedTime(&ms, c->events.ev[i], c->events.ev[i + 1]));
-      c->phase_ms[i] = ms;
+    const int n = cfl_substeps(dt, c->g.dx, wc::particles_max_velocity(c));
+    const double sub_dt = dt / (double)n;
+    c->last_substeps = n;
+    // like the reference, every substep runs even when a solve does not
+    // converge; the step reports it
+    for (int k = 0; k < n; ++k) {
+      double r = 0;
+      int i = 0;
+      if (!substep(c, sub_dt, tol, maxit, &r, &i)) ok = false;
+      res = r;
+      it += i;
     }
   })
   if (residual_out) *residual_out = res;
   if (iters_out) *iters_out = it;
   return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_step_substeps(const wc_ctx* c, int32_t* n) {
+  if (!c || !n) return WC_ERR_ARG;
+  *n = c->last_substeps;
+  return WC_OK;
 }
 
 wc_status wc_grid_info(const wc_ctx* c, int32_t nx[3], int64_t* nnode,

@@ -242,6 +242,12 @@ struct wc_ctx {
   wc::DevBuf<uint32_t> pid, pid2, pkey;
   wc::DevBuf<uint32_t> bcount, bstart, bcursor;  // per bucket (+1)
   bool sorted = false;  // bstart describes the current particle arrays
+  // positions before the last separate advection pass (wc_advect_scene), the
+  // ones the transfer weights belong to; solidProjection weighs at them
+  wc::DevBuf<double> xw;
+  wc::DevBuf<float> fxw;
+  bool xw_valid = false;
+  wc::DevBuf<unsigned long long> vmax_bits;
 
   // grid (nnode entries unless noted)
   wc::DevBuf<uint8_t> touched, act;  // per bucket
@@ -265,6 +271,7 @@ struct wc_ctx {
 
   double last_residual = 0.0;
   int last_iters = 0;
+  int last_substeps = 0;  // substeps of the last wc_step (the CFL split)
   // pressure solver: 0 = MIC(0) PCGSolver, 1 = AMGPCGSolveSparse (the
   // reference's call at Pressure.cpp:797), with its hierarchy and the DOFs'
   // cell coordinates
@@ -289,6 +296,13 @@ void exclusive_scan_async(hipStream_t st, DevBuf<uint32_t>& scratch, uint32_t* d
 void particles_sort(wc_ctx* ctx);
 void particles_p2g(wc_ctx* ctx);
 void particles_g2p_advect(wc_ctx* ctx, double dt);
+// the three parts of particles_g2p_advect as separate passes:
+// mapNodeParticlesAPIC, advectScene, solidProjection
+void particles_g2p(wc_ctx* ctx);
+void particles_advect(wc_ctx* ctx, double dt);
+void particles_solid_projection(wc_ctx* ctx, double dt);
+// getMaxFluidVelocity (waits for the stream)
+double particles_max_velocity(wc_ctx* ctx);
 int64_t particles_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3],
                              uint64_t seed, double sigma);
 void particles_set_soa(wc_ctx* ctx, int64_t n, const double* x, const double* v,

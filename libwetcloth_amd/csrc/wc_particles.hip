@@ -10,6 +10,9 @@
 // (updateLiquidPhi) and marks the buckets the stencils touch (resampleNodes'
 // activation).
 #include <math.h>
+#include <string.h>
+
+#include <algorithm>
 
 #include <vector>
 
@@ -314,7 +317,65 @@ struct G2PArgs {
 // [(c2 * kTile + c1) * kTile + c0] for the 27 stencil nodes base + (c0, c1,
 // c2) (then no node is fetched one by one), or null.
 // g2p_particle_at takes the particle's position as already loaded.
-template <class R, class Fetch, class Region>
+// solidProjection (TwoDScene.cpp:1782-1821) of one particle: the corner-grid
+// weights of the position px the transfer weights were computed at, the
+// gradient toward the current position nx (advected), the displacement
+// fv dt; nx is moved out of the solid when phi_now < 0.
+template <class R>
+__device__ __attribute__((always_inline)) inline void solid_project(const G2PArgs<R>& A,
+                                                                    const R px[3], R nx[3],
+                                                                    const R fv[3]) {
+  const Grid& g = A.g;
+  // solidProjection moves the particle only if phi_now < 0.  The box distance
+  // is 1-Lipschitz, so every stencil node's phi (or the inactive value 3 dx)
+  // is at least min(3 dx, phi(px) - 2.6 dx), phi_ori, their convex
+  // combination, too, and phi_now >= phi_ori - |dp| with |gr| <= 1: a
+  // particle with phi(px) > 4 dx + |v|_1 dt and |v|_1 dt < 2 dx is never
+  // moved, and skips the evaluation.
+  {
+    const R step = (fabs(fv[0]) + fabs(fv[1]) + fabs(fv[2])) * A.dt;
+    const R cdx = (R)g.dx;
+    if (container_phi_r<R>(g, px[0], px[1], px[2]) > (R)4 * cdx + step && step < (R)2 * cdx)
+      return;
+  }
+  int base[3];
+  R w[3][3];
+  for (int a = 0; a < 3; ++a) {
+    base[a] = stencil_base_r<R>(g, a, px[a], (R)0);
+    stencil_weights_r<R>(g, px[a], base[a], (R)0, w[a]);
+  }
+  R phi_ori = 0, gr[3] = {0, 0, 0};
+  for (int c2 = 0; c2 < 3; ++c2)
+    for (int c1 = 0; c1 < 3; ++c1)
+      for (int c0 = 0; c0 < 3; ++c0) {
+        const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+        const R q0 = node_pos_r<R>(g, i, (R)0), q1 = node_pos_r<R>(g, j, (R)0),
+                q2 = node_pos_r<R>(g, k, (R)0);
+        const R phi = node_active(g, A.act, i, j, k) ? container_phi_r<R>(g, q0, q1, q2)
+                                                       : (R)3 * (R)g.dx;
+        const R ww = w[0][c0] * w[1][c1] * w[2][c2];
+        phi_ori += phi * ww;
+        gr[0] += phi * A.invD * ww * (q0 - nx[0]);
+        gr[1] += phi * A.invD * ww * (q1 - nx[1]);
+        gr[2] += phi * A.invD * ww * (q2 - nx[2]);
+      }
+  const R gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+  if (gn > (R)1e-20) {
+    gr[0] /= gn;
+    gr[1] /= gn;
+    gr[2] /= gn;
+  }
+  const R dp0 = fv[0] * A.dt, dp1 = fv[1] * A.dt, dp2 = fv[2] * A.dt;
+  const R phi_now = phi_ori + (gr[0] * dp0 + gr[1] * dp1 + gr[2] * dp2);
+  if (phi_now < (R)0) {
+    nx[0] -= phi_now * gr[0];
+    nx[1] -= phi_now * gr[1];
+    nx[2] -= phi_now * gr[2];
+  }
+}
+
+// With ADVECT false only v and C are written (mapNodeParticlesAPIC alone).
+template <class R, bool ADVECT = true, class Fetch, class Region>
 __device__ __attribute__((always_inline)) inline void g2p_particle_at(const G2PArgs<R>& A,
                                                                       int64_t p, R x0, R x1,
                                                                       R x2, Fetch fetch,
@@ -365,75 +426,27 @@ __device__ __attribute__((always_inline)) inline void g2p_particle_at(const G2PA
     }
   }
   for (int t = 0; t < 9; ++t) A.C[t * A.cap + p] = B[t] * A.flip;
-  R nx[3];
-  for (int d = 0; d < 3; ++d) {
-    A.v[d * A.cap + p] = fv[d];
-    nx[d] = px[d] + fv[d] * A.dt;
+  for (int d = 0; d < 3; ++d) A.v[d * A.cap + p] = fv[d];
+  if constexpr (ADVECT) {
+    R nx[3];
+    for (int d = 0; d < 3; ++d) nx[d] = px[d] + fv[d] * A.dt;
+    if (A.solid_projection) solid_project<R>(A, px, nx, fv);
+    for (int d = 0; d < 3; ++d) A.x[d * A.cap + p] = nx[d];
   }
-  // solidProjection moves the particle only if phi_now < 0.  The box distance
-  // is 1-Lipschitz, so every stencil node's phi (or the inactive value 3 dx)
-  // is at least min(3 dx, phi(px) - 2.6 dx), phi_ori, their convex
-  // combination, too, and phi_now >= phi_ori - |dp| with |gr| <= 1: a
-  // particle with phi(px) > 4 dx + |v|_1 dt and |v|_1 dt < 2 dx is never
-  // moved, and skips the evaluation.
-  bool near_wall = true;
-  if (A.solid_projection) {
-    const R step = (fabs(fv[0]) + fabs(fv[1]) + fabs(fv[2])) * A.dt;
-    const R cdx = (R)g.dx;
-    near_wall = !(container_phi_r<R>(g, px[0], px[1], px[2]) > (R)4 * cdx + step &&
-                  step < (R)2 * cdx);
-  }
-  if (A.solid_projection && near_wall) {
-    int base[3];
-    R w[3][3];
-    for (int a = 0; a < 3; ++a) {
-      base[a] = stencil_base_r<R>(g, a, px[a], (R)0);
-      stencil_weights_r<R>(g, px[a], base[a], (R)0, w[a]);
-    }
-    R phi_ori = 0, gr[3] = {0, 0, 0};
-    for (int c2 = 0; c2 < 3; ++c2)
-      for (int c1 = 0; c1 < 3; ++c1)
-        for (int c0 = 0; c0 < 3; ++c0) {
-          const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
-          const R q0 = node_pos_r<R>(g, i, (R)0), q1 = node_pos_r<R>(g, j, (R)0),
-                  q2 = node_pos_r<R>(g, k, (R)0);
-          const R phi = node_active(g, A.act, i, j, k) ? container_phi_r<R>(g, q0, q1, q2)
-                                                         : (R)3 * (R)g.dx;
-          const R ww = w[0][c0] * w[1][c1] * w[2][c2];
-          phi_ori += phi * ww;
-          gr[0] += phi * A.invD * ww * (q0 - nx[0]);
-          gr[1] += phi * A.invD * ww * (q1 - nx[1]);
-          gr[2] += phi * A.invD * ww * (q2 - nx[2]);
-        }
-    const R gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
-    if (gn > (R)1e-20) {
-      gr[0] /= gn;
-      gr[1] /= gn;
-      gr[2] /= gn;
-    }
-    const R dp0 = fv[0] * A.dt, dp1 = fv[1] * A.dt, dp2 = fv[2] * A.dt;
-    const R phi_now = phi_ori + (gr[0] * dp0 + gr[1] * dp1 + gr[2] * dp2);
-    if (phi_now < (R)0) {
-      nx[0] -= phi_now * gr[0];
-      nx[1] -= phi_now * gr[1];
-      nx[2] -= phi_now * gr[2];
-    }
-  }
-  for (int d = 0; d < 3; ++d) A.x[d * A.cap + p] = nx[d];
 }
 
 // One thread per particle, gathers from global memory.
-template <class R, class Fetch, class Region>
+template <class R, bool ADVECT, class Fetch, class Region>
 __device__ __attribute__((always_inline)) inline void g2p_particle(const G2PArgs<R>& A, int64_t p,
                                                                    Fetch fetch, Region region) {
-  g2p_particle_at<R>(A, p, A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p], fetch, region);
+  g2p_particle_at<R, ADVECT>(A, p, A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p], fetch, region);
 }
 
-template <class R>
+template <class R, bool ADVECT>
 __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= A.n) return;
-  g2p_particle<R>(
+  g2p_particle<R, ADVECT>(
       A, p,
       [&](int d, int i, int j, int k, R& u) {
         if (!node_active(A.g, A.act, i, j, k)) return false;
@@ -448,7 +461,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs<R> A) {
 // of k_p2g; inactive nodes hold 0) in LDS once, and each particle gathers
 // from there (a node outside the region, which only a particle outside its
 // bucket reaches, is read from global memory as above).
-template <class R>
+template <class R, bool ADVECT>
 __global__ __launch_bounds__(256) void k_g2p_bucket(G2PArgs<R> A) {
   const int64_t b = blockIdx.x;
   const uint32_t pbeg = A.bstart[b], pend = A.bstart[b + 1];
@@ -485,7 +498,7 @@ __global__ __launch_bounds__(256) void k_g2p_bucket(G2PArgs<R> A) {
       nx[1] = A.x[A.cap + p + 256];
       nx[2] = A.x[2 * A.cap + p + 256];
     }
-    g2p_particle_at<R>(
+    g2p_particle_at<R, ADVECT>(
         A, p, x0, x1, x2,
         [&](int d, int i, int j, int k, R& u) {
           const int li = i - o0, lj = j - o1, lk = k - o2;
@@ -506,6 +519,61 @@ __global__ __launch_bounds__(256) void k_g2p_bucket(G2PArgs<R> A) {
                           (unsigned)l2 <= (unsigned)(kTile - 3);
           return in ? &su[d][(l2 * kTile + l1) * kTile + l0] : nullptr;
         });
+  }
+}
+
+// SceneStepper::advectScene, fluid branch (SceneStepper.cpp:30-31): x += v dt.
+// The position before the move is kept in xw: solidProjection weighs the
+// corner grid at the position the transfer weights were computed at.
+template <class R>
+__global__ __launch_bounds__(256) void k_advect(R* x, const R* v, R* xw, int64_t n, int64_t cap,
+                                               R dt) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int d = 0; d < 3; ++d) {
+    const R x0 = x[d * cap + p];
+    xw[d * cap + p] = x0;
+    x[d * cap + p] = x0 + v[d * cap + p] * dt;
+  }
+}
+
+// TwoDScene::solidProjection (TwoDScene.cpp:1776-1822) as its own pass: the
+// weights at xw (or at x when no advection has run since the transfer), the
+// gradient toward the current x.
+template <class R>
+__global__ __launch_bounds__(256) void k_solid_projection(G2PArgs<R> A, const R* xw) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= A.n) return;
+  const R* xs = xw ? xw : A.x;
+  const R px[3] = {xs[p], xs[A.cap + p], xs[2 * A.cap + p]};
+  R nx[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
+  const R fv[3] = {A.v[p], A.v[A.cap + p], A.v[2 * A.cap + p]};
+  solid_project<R>(A, px, nx, fv);
+  for (int d = 0; d < 3; ++d) A.x[d * A.cap + p] = nx[d];
+}
+
+// TwoDScene::getMaxFluidVelocity (TwoDScene.cpp:4180-4188): the largest
+// squared norm of a fluid velocity, in FP64, as the bits of a non-negative
+// double (their order is the values' order) by atomic max.
+template <class R>
+__global__ __launch_bounds__(256) void k_max_vel2(const R* v, int64_t n, int64_t cap,
+                                                 unsigned long long* out) {
+  double m = 0.0;
+  for (int64_t p = blockIdx.x * 256ll + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
+    const double a = (double)v[p], b = (double)v[cap + p], c = (double)v[2 * cap + p];
+    const double q = a * a + b * b + c * c;
+    m = q > m ? q : m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(m, o, 64);
+    m = t > m ? t : m;
+  }
+  __shared__ double sm[4];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) m = sm[w] > m ? sm[w] : m;
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
   }
 }
 
@@ -681,6 +749,7 @@ void ensure_particles(wc_ctx* ctx, int64_t n) {
   }
   ctx->np = n;
   ctx->sorted = false;
+  ctx->xw_valid = false;
 }
 
 __global__ void k_narrow(const double* a, float* b, int64_t n) {
@@ -743,6 +812,7 @@ void sort_impl(wc_ctx* ctx, Planes<R> P) {
   std::swap(*P.C, *P.C2);
   std::swap(ctx->pid, ctx->pid2);
   ctx->sorted = true;
+  ctx->xw_valid = false;
 }
 
 template <class R>
@@ -775,7 +845,7 @@ void p2g_launch(wc_ctx* ctx, Planes<R> P) {
 }
 
 template <class R>
-void g2p_launch(wc_ctx* ctx, Planes<R> P, double dt) {
+G2PArgs<R> g2p_args(wc_ctx* ctx, Planes<R> P, double dt) {
   G2PArgs<R> A;
   A.g = ctx->g;
   A.x = P.x->p;
@@ -790,17 +860,63 @@ void g2p_launch(wc_ctx* ctx, Planes<R> P, double dt) {
   A.invD = (R)(4.0 / (ctx->g.dx * ctx->g.dx));
   A.solid_projection = ctx->desc.solid_projection;
   A.bstart = ctx->bstart.p;
+  return A;
+}
+
+template <class R, bool ADVECT>
+void g2p_launch(wc_ctx* ctx, Planes<R> P, double dt) {
+  const G2PArgs<R> A = g2p_args(ctx, P, dt);
   KTimer& tm = ctx->pcg.timer;
   hipEvent_t e0 = tm.on ? tm.mark(ctx->stream) : nullptr;
   // with bstart describing the particle arrays, one workgroup per bucket
   if (ctx->sorted)
-    hipLaunchKernelGGL(k_g2p_bucket<R>, dim3((unsigned)ctx->g.nbucket), dim3(256), 0,
+    hipLaunchKernelGGL((k_g2p_bucket<R, ADVECT>), dim3((unsigned)ctx->g.nbucket), dim3(256), 0,
                        ctx->stream, A);
   else
-    hipLaunchKernelGGL(k_g2p_advect<R>, dim3(blocks(ctx->np, 256)), dim3(256), 0,
+    hipLaunchKernelGGL((k_g2p_advect<R, ADVECT>), dim3(blocks(ctx->np, 256)), dim3(256), 0,
                        ctx->stream, A);
   WC_LAUNCH_CHECK();
   if (e0) tm.add(KTimer::G2P, ctx->np, e0, tm.mark(ctx->stream));
+}
+
+// the weights-position planes of the separate advection / projection passes
+inline DevBuf<double>& xw_planes(wc_ctx* c, double) { return c->xw; }
+inline DevBuf<float>& xw_planes(wc_ctx* c, float) { return c->fxw; }
+
+template <class R>
+void advect_launch(wc_ctx* ctx, Planes<R> P, double dt) {
+  DevBuf<R>& xw = xw_planes(ctx, R());
+  xw.ensure(3 * (size_t)ctx->pcap);
+  hipLaunchKernelGGL(k_advect<R>, dim3(blocks(ctx->np, 256)), dim3(256), 0, ctx->stream,
+                     P.x->p, P.v->p, xw.p, ctx->np, ctx->pcap, (R)dt);
+  WC_LAUNCH_CHECK();
+  ctx->xw_valid = true;
+}
+
+template <class R>
+void projection_launch(wc_ctx* ctx, Planes<R> P, double dt) {
+  G2PArgs<R> A = g2p_args(ctx, P, dt);
+  const R* xw = ctx->xw_valid ? xw_planes(ctx, R()).p : nullptr;
+  hipLaunchKernelGGL(k_solid_projection<R>, dim3(blocks(ctx->np, 256)), dim3(256), 0,
+                     ctx->stream, A, xw);
+  WC_LAUNCH_CHECK();
+}
+
+template <class R>
+double max_vel_launch(wc_ctx* ctx, Planes<R> P) {
+  ctx->vmax_bits.ensure(1);
+  WC_HIP(hipMemsetAsync(ctx->vmax_bits.p, 0, sizeof(unsigned long long), ctx->stream));
+  const unsigned nb = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, blocks(ctx->np, 256)));
+  hipLaunchKernelGGL(k_max_vel2<R>, dim3(nb), dim3(256), 0, ctx->stream, P.v->p, ctx->np,
+                     ctx->pcap, ctx->vmax_bits.p);
+  WC_LAUNCH_CHECK();
+  unsigned long long bits = 0;
+  WC_HIP(hipMemcpyAsync(&bits, ctx->vmax_bits.p, sizeof(bits), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WC_HIP(hipStreamSynchronize(ctx->stream));
+  double m2;
+  memcpy(&m2, &bits, sizeof(m2));
+  return sqrt(m2);
 }
 
 }  // namespace
@@ -996,9 +1112,40 @@ void particles_p2g(wc_ctx* ctx) {
 void particles_g2p_advect(wc_ctx* ctx, double dt) {
   if (ctx->np == 0) return;
   if (ctx->mixed)
-    g2p_launch(ctx, planes(ctx, 0.0f), dt);
+    g2p_launch<float, true>(ctx, planes(ctx, 0.0f), dt);
   else
-    g2p_launch(ctx, planes(ctx, 0.0), dt);
+    g2p_launch<double, true>(ctx, planes(ctx, 0.0), dt);
+  ctx->xw_valid = false;
+}
+
+void particles_g2p(wc_ctx* ctx) {
+  if (ctx->np == 0) return;
+  if (ctx->mixed)
+    g2p_launch<float, false>(ctx, planes(ctx, 0.0f), 0.0);
+  else
+    g2p_launch<double, false>(ctx, planes(ctx, 0.0), 0.0);
+  ctx->xw_valid = false;
+}
+
+void particles_advect(wc_ctx* ctx, double dt) {
+  if (ctx->np == 0) return;
+  if (ctx->mixed)
+    advect_launch(ctx, planes(ctx, 0.0f), dt);
+  else
+    advect_launch(ctx, planes(ctx, 0.0), dt);
+}
+
+void particles_solid_projection(wc_ctx* ctx, double dt) {
+  if (ctx->np == 0) return;
+  if (ctx->mixed)
+    projection_launch(ctx, planes(ctx, 0.0f), dt);
+  else
+    projection_launch(ctx, planes(ctx, 0.0), dt);
+}
+
+double particles_max_velocity(wc_ctx* ctx) {
+  if (ctx->np == 0) return 0.0;
+  return ctx->mixed ? max_vel_launch(ctx, planes(ctx, 0.0f)) : max_vel_launch(ctx, planes(ctx, 0.0));
 }
 
 }  // namespace wc

@@ -115,6 +115,32 @@ int main(int argc, char** argv) {
     scene.getParticles(x4, v4, fB);
     double vy = 0.0;
     for (int64_t p = 0; p < np; ++p) vy += v4[4 * p + 1];
+    // 2b. the reference's three calls after the projection against the fused
+    // pass, on the same grid: positions, velocities and affine matrices are
+    // bit for bit the same
+    bool split_fused_bitwise = false;
+    {
+      const double dt = 1.0 / 480;
+      scene.rebucketizeParticles();
+      scene.mapParticleNodesAPIC();
+      stepper.stepVelocity(scene, dt);
+      stepper.projectFine(scene, dt);
+      stepper.applyPressureDragFluid(scene, dt);
+      std::vector<double> x0, v0, B0, xs, vs, Bs, xf, vf, Bf;
+      scene.getParticles(x0, v0, B0);
+      scene.mapNodeParticlesAPIC();
+      stepper.advectScene(scene, dt);
+      scene.solidProjection(dt);
+      scene.getParticles(xs, vs, Bs);
+      scene.setParticles(x0, v0, B0);
+      scene.mapNodeParticlesAdvect(dt);
+      scene.getParticles(xf, vf, Bf);
+      split_fused_bitwise = xs == xf && vs == vf && Bs == Bf;
+    }
+    // 2c. a long step splits into CFL substeps (WetClothCore.cpp:52-62)
+    const double vmax = scene.getMaxFluidVelocity();
+    const bool big_ok = core.stepSystem(0.1);
+    const int big_substeps = core.lastSubsteps();
     // 3. two more substeps with the reference's own solver, AMG-PCG
     scene.setOption("solver", 1);
     bool amg_steps_ok = true;
@@ -125,11 +151,14 @@ int main(int argc, char** argv) {
         "\"pcg_max_error\": %.17g, \"amg_ok\": %s, \"amg_iterations\": %d, "
         "\"amg_max_error\": %.17g, \"particles\": %lld, \"steps_ok\": %s, "
         "\"last_iterations\": %d, \"mean_vy\": %.17g, \"seconds\": %.6f, "
-        "\"amg_steps_ok\": %s, \"amg_step_iterations\": %d}\n",
+        "\"amg_steps_ok\": %s, \"amg_step_iterations\": %d, "
+        "\"split_fused_bitwise\": %s, \"vmax\": %.17g, \"big_step_ok\": %s, "
+        "\"big_step_substeps\": %d}\n",
         ok ? "true" : "false", iterations, residual, err / nrm, amg_ok ? "true" : "false",
         amg_it, amg_err / amg_nrm, (long long)np,
         all_ok ? "true" : "false", last_iterations, vy / (double)np, secs,
-        amg_steps_ok ? "true" : "false", amg_step_iterations);
+        amg_steps_ok ? "true" : "false", amg_step_iterations,
+        split_fused_bitwise ? "true" : "false", vmax, big_ok ? "true" : "false", big_substeps);
   } catch (const wcamd::Error& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return 1;

@@ -17,6 +17,8 @@
 // reference has no exceptions because it has no failure modes of this kind.
 #pragma once
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -82,9 +84,20 @@ class TwoDScene {
   // resampleNodes activation, updateLiquidPhi, updateSolidPhi, updateSolidWeights
   // and TwoDScene::mapParticleNodesAPIC (TwoDScene.h:487)
   void mapParticleNodesAPIC() { check(wc_map_particle_nodes(ctx_), ctx_); }
-  // TwoDScene::mapNodeParticlesAPIC (TwoDScene.h:493) fused with advectScene
-  // and solidProjection, which take the substep size
-  void mapNodeParticlesAPIC(double dt) { check(wc_map_node_particles(ctx_, dt), ctx_); }
+  // TwoDScene::mapNodeParticlesAPIC() (TwoDScene.h:493): v and C from the grid
+  void mapNodeParticlesAPIC() { check(wc_map_node_particles(ctx_), ctx_); }
+  // TwoDScene::solidProjection(dt) (TwoDScene.cpp:1776-1822)
+  void solidProjection(double dt) { check(wc_solid_projection(ctx_, dt), ctx_); }
+  // mapNodeParticlesAPIC, SceneStepper::advectScene and solidProjection in one
+  // pass over the particles (bitwise the same as the three calls)
+  void mapNodeParticlesAdvect(double dt) { check(wc_map_node_particles_advect(ctx_, dt), ctx_); }
+  // TwoDScene::getMaxFluidVelocity (TwoDScene.cpp:4180-4188)
+  double getMaxFluidVelocity() const {
+    double v = 0.0;
+    check(wc_max_fluid_velocity(ctx_, &v), ctx_);
+    return v;
+  }
+  double getCellSize() const { return desc_.dx; }
 
   // wc_set_option: "solver" 1 makes projectFine solve with AMGPCGSolveSparse
   // (the reference's call at Pressure.cpp:797), 0 with the MIC(0) PCG
@@ -119,6 +132,11 @@ class LinearizedImplicitEuler {
     check(wc_apply_pressure(s.handle(), dt), s.handle());
     return true;
   }
+  // SceneStepper::advectScene (SceneStepper.cpp:17-41)
+  bool advectScene(TwoDScene& s, double dt) {
+    check(wc_advect_scene(s.handle(), dt), s.handle());
+    return true;
+  }
   double lastResidual() const { return residual_; }
   int lastIterations() const { return iterations_; }
 
@@ -148,24 +166,48 @@ class TwoDSceneSerializer {
   }
 };
 
-// WetClothCore::stepSystem for liquid scenes: one substep per call
+// WetClothCore::stepSystem (WetClothCore.cpp:45-330) for liquid scenes:
+// the CFL split of dt (:52-62) and, per substep, the liquid path's calls in
+// the reference's order (:115-262).  With fused set, the transfer back,
+// advection and projection run as one pass (the same result, bit for bit).
 class WetClothCore {
  public:
-  WetClothCore(TwoDScene& scene, LinearizedImplicitEuler& stepper)
-      : scene_(scene), stepper_(stepper) {}
+  WetClothCore(TwoDScene& scene, LinearizedImplicitEuler& stepper, bool fused = false)
+      : scene_(scene), stepper_(stepper), fused_(fused) {}
   bool stepSystem(double dt) {
-    scene_.rebucketizeParticles();
-    scene_.mapParticleNodesAPIC();
-    stepper_.stepVelocity(scene_, dt);
-    const bool ok = stepper_.projectFine(scene_, dt);
-    stepper_.applyPressureDragFluid(scene_, dt);
-    scene_.mapNodeParticlesAPIC(dt);
+    const double max_elasto_vel = 0.0;  // no elastic particles
+    const double max_fluid_vel = scene_.getMaxFluidVelocity();
+    const double dx = scene_.getCellSize();
+    const double max_elasto_dt = std::min(dx / std::max(1e-63, max_elasto_vel) / 3.0, 1.0 / 30.0);
+    const double max_fluid_dt = std::min(dx / std::max(1e-63, max_fluid_vel) * 3.0, 1.0 / 30.0);
+    const double max_dt = std::min(max_elasto_dt, max_fluid_dt);
+    const int num_substeps = std::max(1, (int)std::ceil(dt / max_dt));
+    const double sub_dt = dt / (double)num_substeps;
+    last_substeps_ = num_substeps;
+    bool ok = true;
+    for (int k = 0; k < num_substeps; ++k) {
+      scene_.rebucketizeParticles();
+      scene_.mapParticleNodesAPIC();
+      stepper_.stepVelocity(scene_, sub_dt);
+      ok &= stepper_.projectFine(scene_, sub_dt);
+      stepper_.applyPressureDragFluid(scene_, sub_dt);
+      if (fused_) {
+        scene_.mapNodeParticlesAdvect(sub_dt);
+      } else {
+        scene_.mapNodeParticlesAPIC();
+        stepper_.advectScene(scene_, sub_dt);
+        if (scene_.desc().solid_projection) scene_.solidProjection(sub_dt);
+      }
+    }
     return ok;
   }
+  int lastSubsteps() const { return last_substeps_; }
 
  private:
   TwoDScene& scene_;
   LinearizedImplicitEuler& stepper_;
+  bool fused_;
+  int last_substeps_ = 0;
 };
 
 }  // namespace wcamd

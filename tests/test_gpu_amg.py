@@ -94,7 +94,7 @@ def test_context_solver_option(oracle, wc):
     assert np.array_equal(wc.amg_apply_csr(rs, col, val, ijk, dims, r),
                           oracle.amg_vcycle(rs, col, val, ijk, dims, r))
     s.applyPressureDragFluid(dt)
-    s.mapNodeParticlesAPIC(dt)
+    s.map_node_particles_advect(dt)
     for _ in range(3):
         ok, it, res = s.step(dt, tol=1e-8, maxit=300)
         assert ok

@@ -26,6 +26,12 @@ def test_cpp_host_mirror(oracle, tmp_path):
     assert r["amg_ok"] and r["amg_max_error"] < 1e-9 and 0 < r["amg_iterations"] < 30
     assert r["steps_ok"]
     assert r["amg_steps_ok"] and 0 < r["amg_step_iterations"] < 40
+    # mapNodeParticlesAPIC(), advectScene(dt), solidProjection(dt) as the
+    # reference calls them give the fused pass's particles bit for bit
+    assert r["split_fused_bitwise"]
+    # stepSystem(0.1) splits into ceil(0.1 / min(3 dx / v_max, 1/30)) substeps
+    exp = max(1, int(np.ceil(0.1 / min((1.0 / n) / max(1e-63, r["vmax"]) * 3.0, 1.0 / 30.0))))
+    assert r["big_step_ok"] and r["big_step_substeps"] == exp >= 3
     g, x, v = oracle.dam_break(n)
     assert r["particles"] == x.shape[1]
     s = oracle.OracleScene((n, n, n), 1.0 / n, x, v)

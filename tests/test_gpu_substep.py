@@ -75,9 +75,11 @@ def test_phases_match_oracle(oracle, wc, n):
     for d in range(3):
         assert np.array_equal(gpu.field("valid", d), cpu.field("valid", d))
         assert rel(gpu.field("unew", d), cpu.field("unew", d)) < 1e-7
-    # --- G2P, advection, solid projection ---
+    # --- G2P, advection, solid projection: the reference's three calls ---
     cpu.g2p_advect(DT)
-    gpu.mapNodeParticlesAPIC(DT)
+    gpu.mapNodeParticlesAPIC()
+    gpu.advectScene(DT)
+    gpu.solidProjection(DT)
     xg, vg, Cg = gpu.particles()
     xc, vc, Cc = cpu.particles()
     dx = 1.0 / n
@@ -155,10 +157,43 @@ def test_matches_committed_golden(wc):
     assert np.abs(b - ref["rhs"]).max() <= 1e-11 * np.abs(ref["rhs"]).max()
     assert rel(p, ref["pressure"]) < 1e-8
     s.applyPressureDragFluid(dt)
-    s.mapNodeParticlesAPIC(dt)
+    s.map_node_particles_advect(dt)
     for _ in range(2):
         ok, it, res = s.step(dt, tol, 2000)
         assert ok
     x3, v3, _ = s.particles()
     assert np.abs(x3 - ref["x3"]).max() < 1e-8 / n
     assert rel(v3, ref["v3"]) < 1e-6
+
+
+def test_cfl_substep_split(oracle, wc):
+    """WetClothCore::stepSystem's split (WetClothCore.cpp:52-62): with v_max =
+    getMaxFluidVelocity, dt runs as max(1, ceil(dt / min(3 dx / v_max,
+    1/30))) equal substeps; the result is the oracle's over that many
+    substeps of dt / n."""
+    n = 16
+    dx = 1.0 / n
+    g, x, v = oracle.dam_break(n, block_hi=(0.4, 0.6, 1.0), seed=3, sigma=0.3)
+    v[0] += 2.5  # fast enough that 3 dx / v_max < 1/30
+    gpu = wc.Scene((n, n, n), dx)
+    gpu.set_particles(x, v)
+    vmax = gpu.getMaxFluidVelocity()
+    assert vmax == np.sqrt((v * v).sum(axis=0)).max()
+    dt = 0.05
+    expect = max(1, int(np.ceil(dt / min(min(dx / 1e-63 / 3.0, 1.0 / 30.0),
+                                         min(dx / max(1e-63, vmax) * 3.0, 1.0 / 30.0)))))
+    assert expect >= 2
+    ok, it, res = gpu.step(dt, 1e-8, 2000)
+    assert ok and gpu.step_substeps() == expect
+    cpu = oracle.OracleScene((n, n, n), dx, x, v)
+    for _ in range(expect):
+        assert cpu.substep(dt / expect, 1e-8, 2000)
+    xg, vg, _ = gpu.particles()
+    xc, vc, _ = cpu.particles()
+    assert np.abs(xg - xc).max() < 1e-8 * dx
+    assert rel(vg, vc) < 1e-6
+    # a scene at rest takes the 1/30 s bound: 0.05 s is two substeps
+    gpu2 = wc.Scene((n, n, n), dx)
+    gpu2.set_particles(x)
+    assert gpu2.getMaxFluidVelocity() == 0.0
+    assert gpu2.step(0.05, 1e-8, 2000)[0] and gpu2.step_substeps() == 2
