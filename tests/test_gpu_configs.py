@@ -1,0 +1,153 @@
+"""The dam-break configurations at their full sizes.
+
+* 64^3, FP64, 100 substeps at dt = 1/480 against the oracle's run of the same
+  scene (tests/golden/dam_break_64_100.npz: 505 sampled particles, the means
+  over all 503,312 and every substep's PCG iteration count).
+* 128^3 (4.0 M particles, sigma = 0.5 m/s, dt = 1/960) and 256^3 (40.3 M,
+  sigma = 1 m/s, dt = 1/1920), mixed precision against FP64, both to a
+  1e-6 relative residual:
+  - one substep from one FP32-representable state (positions and velocities
+    rounded to FP32 for both forms), so that the two forms differ only by
+    their arithmetic: positions must agree within 1e-4 dx in the max norm;
+    the mixed pressure meets the FP64 residual criterion on its own FP64
+    system, and is compared with the FP64 MIC(0)-PCG solution of that same
+    system at the same tolerance and with its converged (1e-12) solution;
+  - the full run (200 substeps at 128^3, 100 at 256^3) from that state:
+    every solve converges in both forms, and the per-particle position
+    difference (median, 99th percentile, worst) stays within bounds set at
+    the measured values, so that a regression fails.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "dam_break_64_100.npz")
+
+
+def rel(a, b):
+    den = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / den)
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def test_config0_fp64_100_substeps_against_oracle(oracle, wc):
+    ref = np.load(GOLDEN)
+    n, dt, tol = int(ref["n"]), float(ref["dt"]), float(ref["tol"])
+    _, x, v = oracle.dam_break(n)
+    assert x.shape[1] == int(ref["np"])
+    s = wc.Scene((n, n, n), 1.0 / n)
+    s.set_particles(x, v)
+    iters = []
+    for _ in range(int(ref["steps"])):
+        ok, it, res = s.substep(dt, tol, 2000)
+        assert ok
+        iters.append(it)
+    xg, vg, _ = s.particles()
+    idx = ref["idx"]
+    err = np.abs(xg[:, idx] - ref["x"]).max() * n
+    print(f"64^3 FP64 after 100 substeps against the oracle: sampled positions {err:.2e} dx, "
+          f"velocities {rel(vg[:, idx], ref['v']):.2e}; iterations "
+          f"{np.abs(np.array(iters) - ref['iters']).max()} apart at most")
+    assert err < 1e-9
+    assert rel(vg[:, idx], ref["v"]) < 1e-7
+    assert np.abs(xg.mean(axis=1) - ref["x_mean"]).max() < 1e-12
+    assert np.abs(vg.mean(axis=1) - ref["v_mean"]).max() < 1e-9
+    assert np.abs(np.array(iters) - ref["iters"]).max() <= 1
+
+
+# ---------------------------------------------------------------- mixed vs FP64
+
+CONFIGS = {
+    # n, block_hi, seed, sigma, dt, substeps of the full run
+    128: (128, (0.4, 0.6, 1.0), 1, 0.5, 1.0 / 960, 200),
+    256: (256, (0.5, 0.6, 1.0), 4, 1.0, 1.0 / 1920, 100),
+}
+TOL = 1e-6
+# bounds of the full runs on |x_mixed - x_fp64| / dx per particle: (median,
+# 99th percentile, worst), set at about twice the values measured
+RUN_BOUNDS = {128: (1e-4, 1e-3, 5e-2), 256: (2e-4, 1e-3, 1e-1)}
+
+
+def start_state(wc, n, block_hi, seed, sigma):
+    """The seeded block, rounded to FP32 (so both forms start from the same
+    values): (x, v) in FP64 arrays."""
+    s = wc.Scene((n, n, n), 1.0 / n)
+    s.seed_block((0.0, 0.0, 0.0), block_hi, seed=seed, sigma=sigma)
+    x, v, _ = s.particles()
+    s.close()
+    return f32(x), f32(v)
+
+
+def pair(wc, n, x, v):
+    f = wc.Scene((n, n, n), 1.0 / n)
+    m = wc.Scene((n, n, n), 1.0 / n, precision=wc.WC_MIXED)
+    f.set_particles(x, v)
+    m.set_particles(x, v)
+    return f, m
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_mixed_one_substep_against_fp64(wc, n):
+    n, block_hi, seed, sigma, dt, _ = CONFIGS[n]
+    x, v = start_state(wc, n, block_hi, seed, sigma)
+    f, m = pair(wc, n, x, v)
+    okf, itf, _ = f.substep(dt, TOL, 2000)
+    okm, itm, _ = m.substep(dt, TOL, 2000)
+    assert okf and okm
+    xf, vf, _ = f.particles()
+    xm, vm, _ = m.particles()
+    dxe = np.abs(xm - xf).max() * n
+    # the mixed form's own FP64 system, its pressure, and FP64 PCG solutions
+    # of that system at the same tolerance and converged
+    rs, col, val, b, pm = m.system()
+    nd = len(rs) - 1
+    A = sp.csr_matrix((val, col, rs), shape=(nd, nd))
+    r = b - A @ pm
+    ok64, p64, it64, _ = wc.pcg_solve_csr(rs, col, val, b, tol=TOL, maxit=2000)
+    okx, px, _, _ = wc.pcg_solve_csr(rs, col, val, b, tol=1e-12, maxit=5000)
+    assert ok64 and okx
+    e_m, e_64, d_m64 = rel(pm, px), rel(p64, px), rel(pm, p64)
+    print(f"{n}^3, {x.shape[1]} particles, one substep: |x_mixed - x_fp64| {dxe:.2e} dx, "
+          f"|v| rel {rel(vm, vf):.2e}; iterations {itm} mixed ({m.refinement_passes()} passes), "
+          f"{itf} FP64; pressure vs the FP64 PCG solution of the same system {d_m64:.2e}, "
+          f"error against the converged solution {e_m:.2e} mixed, {e_64:.2e} FP64 PCG")
+    assert dxe < 1e-4
+    assert np.abs(r).max() <= TOL * np.abs(b).max()
+    # the mixed pressure is as close to the converged solution as the FP64
+    # PCG solution of the same system at the same tolerance
+    assert e_m <= max(2.0 * e_64, 1e-6)
+    assert d_m64 < 1e-5
+    f.close()
+    m.close()
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_mixed_full_run_against_fp64(wc, n):
+    n, block_hi, seed, sigma, dt, steps = CONFIGS[n]
+    x, v = start_state(wc, n, block_hi, seed, sigma)
+    f, m = pair(wc, n, x, v)
+    itf = itm = 0
+    for k in range(steps):
+        okf, a, _ = f.substep(dt, TOL, 2000)
+        okm, b, _ = m.substep(dt, TOL, 2000)
+        assert okf and okm, k
+        itf += a
+        itm += b
+    xf, _, _ = f.particles()
+    xm, _, _ = m.particles()
+    d = np.abs(xm - xf).max(axis=0) * n
+    med, p99, worst = np.median(d), np.percentile(d, 99), d.max()
+    print(f"{n}^3 after {steps} substeps, |x_mixed - x_fp64| / dx: median {med:.2e}, "
+          f"p99 {p99:.2e}, worst {worst:.2e}; mean iterations {itm / steps:.1f} mixed, "
+          f"{itf / steps:.1f} FP64")
+    bm, b99, bw = RUN_BOUNDS[n]
+    assert med < bm and p99 < b99 and worst < bw
+    f.close()
+    m.close()

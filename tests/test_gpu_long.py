@@ -5,14 +5,18 @@
   after 100 substeps; bound 1e-10 both).
 * 32^3, mixed against the oracle: positions within 1e-4 dx (measured
   4.5e-5), node pressure within 1e-4 relative (measured 1.2e-5; the systems
-  differ by then, since the particles do).
+  differ by then, since the particles do).  The 1e-6 pressure target of
+  the mixed form is not met after 100 substeps: that check is kept, marked
+  as an expected failure.
 * 64^3 (the 0.5 M-particle dam break at dt = 1/480, PCG to 1e-8), mixed
   against the FP64 GPU form: the scene amplifies any perturbation at a few
   particles.  FP64 started from the FP32-rounded positions (a start
   difference of at most 2.5e-6 dx) ends 6.4e-4 dx away at its worst
   particle but 1e-6 dx at the median.  So the bound is on the 99th
   percentile (1e-4 dx; measured 3.9e-5) and the median (1e-5 dx; measured
-  7.4e-6), not on the worst particle (measured 4.1e-3).
+  7.4e-6); the worst particle (measured 4.1e-3) is bounded at 1e-2 dx so
+  that a regression there still fails, not at the 1e-4 dx target, which no
+  form with FP32 positions meets on this scene.
 """
 import numpy as np
 import pytest
@@ -69,6 +73,14 @@ def test_mixed_100_substeps_against_oracle(runs_32):
     assert perr < 1e-4
 
 
+@pytest.mark.xfail(reason="mixed-precision node pressure after 100 substeps at 32^3 is "
+                          "1.2e-5 relative to the oracle's (measured), against a 1e-6 target: "
+                          "the particles, and so the systems, differ by then", strict=False)
+def test_mixed_100_substeps_pressure_target(runs_32):
+    n, cpu, _, gm = runs_32
+    assert rel(gm.field("pressure"), cpu.field("pressure")) < 1e-6
+
+
 def test_mixed_100_substeps_64(oracle, wc):
     n = 64
     _, x, v = oracle.dam_break(n)
@@ -92,3 +104,4 @@ def test_mixed_100_substeps_64(oracle, wc):
           f"median {np.median(c):.2e} p99 {np.percentile(c, 99):.2e} max {c.max():.2e}")
     assert np.median(d) < 1e-5
     assert np.percentile(d, 99) < 1e-4
+    assert d.max() < 1e-2
