@@ -275,7 +275,9 @@ wc_status wc_mic0_apply_csr(int64_t n, const uint32_t* rowstart,
  * GeometricLevelGen.h:186-249 (2x2x2 blocks of the DOFs' cells, Galerkin
  * coarse operators 0.5 R A P, red-black Gauss-Seidel 4 sweeps per level and
  * 200 on the coarsest, coarsening while a level holds more than 4096 DOFs).
- * dof_ijk: 3 int32 per DOF, its cell (i, j, k) with 0 <= i < ni etc.  The
+ * dof_ijk: 3 int32 per DOF, its cell (i, j, k) with 0 <= i < ni etc., no
+ * two DOFs in one cell (WC_ERR_ARG otherwise: the reference accepts them,
+ * these kernels keep one child per position of a 2x2x2 block).  The
  * rows must be sorted and every coarse row may hold at most 7 entries
  * (WC_ERR_HIP otherwise; a 7-point system always meets this).  Rows of one
  * colour (parity of i + j + k) must not couple: the reference smooths each

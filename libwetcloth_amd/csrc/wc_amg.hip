@@ -89,7 +89,7 @@ __global__ void k_amg_first(int64_t n, const int32_t* ijk, int nni, int nnj, int
 // DOF's children by their position in the block (GeometricLevelGen.h:138-156)
 __global__ void k_amg_parent(int64_t n, const int32_t* ijk, int nni, int nnj, int nnk,
                              const uint32_t* cmin, const uint32_t* first, uint32_t* parent,
-                             int32_t* cijk, uint32_t* kids) {
+                             int32_t* cijk, uint32_t* kids, int* err) {
   const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f >= n) return;
   const int64_t key = cell_key(ijk, f, nni, nnj, nnk);
@@ -106,7 +106,11 @@ __global__ void k_amg_parent(int64_t n, const int32_t* ijk, int nni, int nnj, in
     cijk[3 * (int64_t)I + 1] = j / 2;
     cijk[3 * (int64_t)I + 2] = k / 2;
   }
-  kids[8 * (int64_t)I + (i & 1) + 2 * (j & 1) + 4 * (k & 1)] = (uint32_t)f;
+  // a child's slot is its position in the 2x2x2 block; two DOFs of one cell
+  // would claim the same slot (the caller's cells must be distinct)
+  const uint32_t was =
+      atomicCAS(&kids[8 * (int64_t)I + (i & 1) + 2 * (j & 1) + 4 * (k & 1)], kNone, (uint32_t)f);
+  if (was != kNone) atomicOr(err, 8);
 }
 
 // A_c = 0.5 R (A P) row by row (SparseMatrix.h:327-348 twice); the children
@@ -492,7 +496,7 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
     C.kids.ensure(8 * (size_t)nc);
     WC_HIP(hipMemsetAsync(C.kids.p, 0xFF, sizeof(uint32_t) * 8 * (size_t)nc, st));
     hipLaunchKernelGGL(k_amg_parent, dim3(nblk(F.n, 256)), dim3(256), 0, st, F.n, F.ijk.p, nni,
-                       nnj, nnk, H.cmin.p, H.flag.p, F.parent.p, C.ijk.p, C.kids.p);
+                       nnj, nnk, H.cmin.p, H.flag.p, F.parent.p, C.ijk.p, C.kids.p, H.err.p);
     C.col.ensure(7 * (size_t)nc);
     C.val.ensure(7 * (size_t)nc);
     C.len.ensure((size_t)nc);
@@ -548,6 +552,8 @@ void amg_build(hipStream_t st, AmgHier& H, const DevCsr& A, const int32_t* ijk, 
   if (herr & 1)
     throw HipError{hipErrorInvalidValue, "AMG levels: a coarse row has more than 7 entries",
                    __FILE__, __LINE__};
+  if (herr & 8)
+    throw HipError{hipErrorInvalidValue, "AMG levels: two DOFs share a cell", __FILE__, __LINE__};
   for (int l = 0; l < H.nl; ++l) H.L[l].nred = nred[l];
   H.compact = (herr & 4) == 0;
 }

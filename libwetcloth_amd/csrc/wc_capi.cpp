@@ -154,6 +154,30 @@ void record(wc_ctx* c, int i) {
   if (c->events.have) WC_HIP(hipEventRecord(c->events.ev[i], c->stream));
 }
 
+// The reference's bucket grid around the particles: updateParticleBoundingBox
+// rounds the bounding box out to multiples of two buckets (TwoDScene.cpp:
+// 1606-1611) and rebucketizeParticles pads it by three buckets and counts
+// its buckets (:1660-1690).  origin: the dense node coordinates of the
+// grid's first cell; dims: its node counts (buckets * 4), as
+// solveNodePressure passes them to the AMG (Pressure.cpp:316-318).
+void reference_bucket_frame(wc_ctx* c, int origin[3], int dims[3]) {
+  double lo[3], hi[3];
+  wc::particles_bbox(c, lo, hi);
+  const double dx = c->g.dx;
+  const double bucket = wc::kBucket * dx;
+  const double two = bucket * 2.0;
+  for (int d = 0; d < 3; ++d) {
+    const double bmin = floor(lo[d] / two) * two, bmax = ceil(hi[d] / two) * two;
+    const double content = bmax - bmin + bucket * 3.0 * 2.0;
+    const int cells = std::max(1, (int)ceil(content / dx));
+    const double size = (double)cells * dx;
+    const int nbk = std::max(1, (int)ceil(size / bucket));
+    const double corner = bmin - bucket * 3.0;
+    origin[d] = (int)llround(corner / dx) + wc::kGhost;
+    dims[d] = nbk * wc::kBucket;
+  }
+}
+
 // solve the assembled system into c->p and scatter it onto the grid
 bool solve_system(wc_ctx* c, double tol, int maxit, double* res, int* it) {
   const wc::DevCsr A = system_csr(c);
@@ -164,22 +188,24 @@ bool solve_system(wc_ctx* c, double tol, int maxit, double* res, int* it) {
   gs.dofscan = c->dofscan.p;
   gs.rowlen = c->rowlen.p;
   gs.col = c->col.p;
-  wc::pcg_prepare(c->stream, c->pcg, A, c->dof_level.p,
-                  c->level_max - c->level_min + 1);
   bool ok;
   if (c->solver == 1 && A.n > 0) {
+    wc::pcg_prepare_vectors(c->stream, c->pcg, A.n);
     // AMGPCGSolveSparse (Pressure.cpp:797) on the DOFs' cells, bucket
     // handle * 4 + node in the bucket, and the bucket grid's node counts
-    // (Pressure.cpp:316-318, 433-436)
+    // (Pressure.cpp:316-318, 433-436), in the frame of the reference's
+    // bucket grid: the aggregation and colouring of its levels depend on it
     c->pcg.sweep.valid = false;
     c->pcg.factored = false;
     c->pcg.passes = 0;  // FP64 throughout: no refinement passes
+    int origin[3], dims[3];
+    reference_bucket_frame(c, origin, dims);
     c->dof_ijk.ensure(3 * (size_t)A.n);
-    wc::grid_dof_cells(c, c->dof_ijk.p);
-    const wc::Grid& g = c->g;
-    wc::amg_build(c->stream, c->amg, A, c->dof_ijk.p, g.nx[0], g.nx[1], g.nx[2]);
+    wc::grid_dof_cells(c, c->dof_ijk.p, origin);
+    wc::amg_build(c->stream, c->amg, A, c->dof_ijk.p, dims[0], dims[1], dims[2]);
     ok = wc::amgpcg_solve(c->stream, c->pcg, c->amg, A, c->b.p, c->p.p, tol, maxit, res, it);
   } else {
+    wc::pcg_prepare(c->stream, c->pcg, A, c->dof_level.p, c->level_max - c->level_min + 1);
     ok = wc::pcg_solve(c->stream, c->pcg, A, c->b.p, c->p.p, tol, maxit, 0.97, 0.25, res, it);
   }
   wc::grid_scatter_pressure(c);
@@ -948,6 +974,14 @@ wc_status check_ijk(int64_t n, const int32_t* ijk, int32_t ni, int32_t nj, int32
     if (ijk[3 * f] < 0 || ijk[3 * f] >= ni || ijk[3 * f + 1] < 0 || ijk[3 * f + 1] >= nj ||
         ijk[3 * f + 2] < 0 || ijk[3 * f + 2] >= nk)
       return WC_ERR_ARG;
+  // every DOF its own cell: a coarse DOF keeps one child per position in
+  // its 2x2x2 block
+  std::vector<int64_t> key((size_t)n);
+  for (int64_t f = 0; f < n; ++f)
+    key[f] = ((int64_t)ijk[3 * f + 2] * nj + ijk[3 * f + 1]) * ni + ijk[3 * f];
+  std::sort(key.begin(), key.end());
+  for (int64_t f = 1; f < n; ++f)
+    if (key[f] == key[f - 1]) return WC_ERR_ARG;
   return WC_OK;
 }
 
@@ -958,7 +992,8 @@ wc_status with_amg(int64_t n, const uint32_t* rowstart, const uint32_t* col, con
   if (check_csr(n, rowstart, col) != WC_OK || (n > 0 && !val))
     return fail(nullptr, WC_ERR_ARG, "invalid CSR matrix (rows must be sorted, columns < n)");
   if (check_ijk(n, ijk, ni, nj, nk) != WC_OK)
-    return fail(nullptr, WC_ERR_ARG, "invalid DOF cells (need 0 <= i < ni, 0 <= j < nj, 0 <= k < nk)");
+    return fail(nullptr, WC_ERR_ARG,
+                "invalid DOF cells (need 0 <= i < ni, 0 <= j < nj, 0 <= k < nk, and no two DOFs in one cell)");
   try {
     FreeSession S;
     wc::AmgHier H;
@@ -1000,7 +1035,7 @@ wc_status wc_amgpcg_solve_csr(int64_t n, const uint32_t* rowstart, const uint32_
     b.ensure(n);
     x.ensure(n);
     WC_HIP(hipMemcpyAsync(b.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, S.st));
-    wc::pcg_prepare(S.st, S.work, S.csr.A, nullptr, 0);
+    wc::pcg_prepare_vectors(S.st, S.work, n);
     ok = wc::amgpcg_solve(S.st, S.work, H, S.csr.A, b.p, x.p, tol, maxit, &res, &it);
     WC_HIP(hipMemcpyAsync(result, x.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
     WC_HIP(hipStreamSynchronize(S.st));

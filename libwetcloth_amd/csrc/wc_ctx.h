@@ -303,6 +303,8 @@ void particles_advect(wc_ctx* ctx, double dt);
 void particles_solid_projection(wc_ctx* ctx, double dt);
 // getMaxFluidVelocity (waits for the stream)
 double particles_max_velocity(wc_ctx* ctx);
+// the particles' bounding box (waits for the stream)
+void particles_bbox(wc_ctx* ctx, double lo[3], double hi[3]);
 int64_t particles_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3],
                              uint64_t seed, double sigma);
 void particles_set_soa(wc_ctx* ctx, int64_t n, const double* x, const double* v,
@@ -316,9 +318,9 @@ void grid_nodes(wc_ctx* ctx);
 void grid_step_velocity(wc_ctx* ctx, double dt);
 int64_t grid_assemble(wc_ctx* ctx, double dt);
 void grid_scatter_pressure(wc_ctx* ctx);
-// each DOF's node (i, j, k) on the dense bucket grid, bucket handle * 4 plus
-// the node in the bucket (Pressure.cpp:433-436), 3 per DOF, into ijk
-void grid_dof_cells(wc_ctx* ctx, int32_t* ijk);
+// each DOF's cell (i, j, k) minus origin[3] (dense node coordinates), 3 per
+// DOF, into ijk
+void grid_dof_cells(wc_ctx* ctx, int32_t* ijk, const int origin[3]);
 void grid_apply_pressure(wc_ctx* ctx, double dt);
 
 // ---- wc_sweep.hip ----
@@ -358,6 +360,9 @@ bool sweep_aborted(hipStream_t st, SweepPlan& P);
 // grid: i+j+k) or computed by relaxation when level == nullptr.
 void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
                  const int32_t* level, int32_t level_span);
+// only the vectors, partials and scalars of the PCG loop (AMG-PCG, which has
+// no MIC(0) factor or level schedule)
+void pcg_prepare_vectors(hipStream_t st, PcgWork& w, int64_t n);
 void pcg_factor(hipStream_t st, PcgWork& w, const DevCsr& A, double tau,
                 double mdr);
 void pcg_apply(hipStream_t st, PcgWork& w, const DevCsr& A, const double* r,

@@ -278,6 +278,14 @@ __device__ inline unsigned long long phi_key(double v) {
   unsigned long long b = (unsigned long long)__double_as_longlong(v);
   return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
 }
+// phi_from_key on the host
+inline double key_value(unsigned long long k) {
+  const unsigned long long b =
+      (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  double v;
+  __builtin_memcpy(&v, &b, sizeof(v));
+  return v;
+}
 __device__ inline double phi_from_key(unsigned long long k) {
   const unsigned long long b =
       (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;

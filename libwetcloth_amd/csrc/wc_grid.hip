@@ -506,22 +506,22 @@ int64_t grid_assemble(wc_ctx* ctx, double dt) {
 
 namespace {
 __global__ void k_dof_cells(Grid g, const uint32_t* isdof, const uint32_t* dofscan,
-                            int32_t* ijk) {
+                            int32_t* ijk, int o0, int o1, int o2) {
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (n >= g.nnode || !isdof[n]) return;
   int i, j, k;
   node_coords(g, n, i, j, k);
   const int64_t d = dofscan[n];
-  ijk[3 * d] = i;
-  ijk[3 * d + 1] = j;
-  ijk[3 * d + 2] = k;
+  ijk[3 * d] = i - o0;
+  ijk[3 * d + 1] = j - o1;
+  ijk[3 * d + 2] = k - o2;
 }
 }  // namespace
 
-void grid_dof_cells(wc_ctx* ctx, int32_t* ijk) {
+void grid_dof_cells(wc_ctx* ctx, int32_t* ijk, const int origin[3]) {
   const Grid& g = ctx->g;
   hipLaunchKernelGGL(k_dof_cells, dim3(blocks(g.nnode, 256)), dim3(256), 0, ctx->stream, g,
-                     ctx->isdof.p, ctx->dofscan.p, ijk);
+                     ctx->isdof.p, ctx->dofscan.p, ijk, origin[0], origin[1], origin[2]);
   WC_LAUNCH_CHECK();
 }
 

@@ -577,6 +577,32 @@ __global__ __launch_bounds__(256) void k_max_vel2(const R* v, int64_t n, int64_t
   }
 }
 
+// TwoDScene::updateParticleBoundingBox's reduction (TwoDScene.cpp:1588-1604):
+// the particles' componentwise min and max, as order-preserving 64-bit keys
+// (out[0..2] min, out[3..5] max; min slots start at ~0, max slots at 0).
+template <class R>
+__global__ __launch_bounds__(256) void k_bbox(const R* x, int64_t n, int64_t cap,
+                                             unsigned long long* out) {
+  unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0, 0, 0};
+  for (int64_t p = blockIdx.x * 256ll + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+    for (int d = 0; d < 3; ++d) {
+      const unsigned long long k = phi_key((double)x[d * cap + p]);
+      lo[d] = k < lo[d] ? k : lo[d];
+      hi[d] = k > hi[d] ? k : hi[d];
+    }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      const unsigned long long a = __shfl_xor(lo[d], o, 64), b = __shfl_xor(hi[d], o, 64);
+      lo[d] = a < lo[d] ? a : lo[d];
+      hi[d] = b > hi[d] ? b : hi[d];
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&out[d], lo[d]);
+      atomicMax(&out[3 + d], hi[d]);
+    }
+}
+
 // ---- seeding ----
 
 __global__ void k_seed_flags(Grid g, int c00, int c01, int c02, int cn0,
@@ -1141,6 +1167,33 @@ void particles_solid_projection(wc_ctx* ctx, double dt) {
     projection_launch(ctx, planes(ctx, 0.0f), dt);
   else
     projection_launch(ctx, planes(ctx, 0.0), dt);
+}
+
+void particles_bbox(wc_ctx* ctx, double lo[3], double hi[3]) {
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = 1e+20;  // the reduction's identities (TwoDScene.cpp:1589-1590)
+    hi[d] = -1e+20;
+  }
+  if (ctx->np == 0) return;
+  ctx->vmax_bits.ensure(6);
+  unsigned long long init[6] = {~0ull, ~0ull, ~0ull, 0, 0, 0};
+  WC_HIP(hipMemcpyAsync(ctx->vmax_bits.p, init, sizeof(init), hipMemcpyHostToDevice,
+                        ctx->stream));
+  const unsigned nb = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, blocks(ctx->np, 256)));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_bbox<float>, dim3(nb), dim3(256), 0, ctx->stream, ctx->fx.p, ctx->np,
+                       ctx->pcap, ctx->vmax_bits.p);
+  else
+    hipLaunchKernelGGL(k_bbox<double>, dim3(nb), dim3(256), 0, ctx->stream, ctx->x.p, ctx->np,
+                       ctx->pcap, ctx->vmax_bits.p);
+  WC_LAUNCH_CHECK();
+  unsigned long long k[6];
+  WC_HIP(hipMemcpyAsync(k, ctx->vmax_bits.p, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
+  WC_HIP(hipStreamSynchronize(ctx->stream));
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = key_value(k[d]);
+    hi[d] = key_value(k[3 + d]);
+  }
 }
 
 double particles_max_velocity(wc_ctx* ctx) {

@@ -533,6 +533,18 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
   w.nlevels = 0;
 }
 
+void pcg_prepare_vectors(hipStream_t st, PcgWork& w, int64_t n) {
+  w.r.ensure(n);
+  w.z.ensure(n);
+  w.s.ensure(n);
+  w.red_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
+  w.partial.ensure(w.red_blocks);
+  w.scal.ensure(S_NSCAL);
+  if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
+  w.levels_ready = false;
+  w.nlevels = 0;
+}
+
 // The level schedule of the level-scheduled kernels (built on first use, so
 // systems solved with the pipelined sweeps never pay for it).
 static void pcg_levels(hipStream_t st, PcgWork& w, const DevCsr& A) {

@@ -43,3 +43,19 @@ def dam_break_system(oracle, n=16):
     s.pre_solve(1.0 / 480)
     rs, col, val, b, _ = s.system()
     return rs, col, val, b
+
+
+def reference_bucket_frame(x, dx):
+    """The reference's bucket grid around particles x (3 x N): the world cell
+    of its first cell and its node counts per axis (buckets * 4).
+    updateParticleBoundingBox rounds the bounding box out to multiples of two
+    buckets (TwoDScene.cpp:1606-1611); rebucketizeParticles pads it by three
+    buckets and counts the buckets (:1660-1690)."""
+    B = 4 * dx
+    D = 2 * B
+    bmin = np.floor(x.min(axis=1) / D) * D
+    bmax = np.ceil(x.max(axis=1) / D) * D
+    cells = np.maximum(1, np.ceil((bmax - bmin + B * 3.0 * 2.0) / dx).astype(int))
+    nbk = np.maximum(1, np.ceil(cells * dx / B).astype(int))
+    origin = np.rint((bmin - B * 3.0) / dx).astype(int)
+    return origin, tuple(int(4 * b) for b in nbk)

@@ -11,6 +11,7 @@ tests/test_oracle_amg.py), on the oracle's dam-break pressure systems:
 import numpy as np
 import pytest
 
+from tests.helpers import reference_bucket_frame
 from tests.test_oracle_amg import dam_system
 
 pytestmark = pytest.mark.gpu
@@ -50,6 +51,12 @@ def test_amg_rejects_bad_input(wc, oracle):
         wc.amgpcg_solve_csr(rs, col, val, bad, nx, b)
     with pytest.raises(wc.WetclothError):
         wc.amg_apply_csr(rs, col, val, ijk, (nx[0], nx[1], 0), b)
+    # two DOFs in one cell are refused (a coarse DOF keeps one child per
+    # position of its 2x2x2 block)
+    dup = ijk.copy()
+    dup[5] = dup[4]
+    with pytest.raises(wc.WetclothError, match="no two DOFs"):
+        wc.amgpcg_solve_csr(rs, col, val, dup, nx, b)
     # a zero right-hand side returns at once
     ok, x, it, res = wc.amgpcg_solve_csr(rs, col, val, ijk, nx, np.zeros(len(b)))
     assert ok and it == 0 and not x.any()
@@ -83,9 +90,13 @@ def test_context_solver_option(oracle, wc):
                              indexing="ij")
     d = dof[oracle.node_index_array(cpu.grid, ii.ravel(), jj.ravel(), kk.ravel())]
     m = d >= 0
+    # the cells in the frame of the reference's bucket grid, whose origin and
+    # size follow the particles' bounding box (TwoDScene.cpp:1588-1690)
+    origin, dims = reference_bucket_frame(x, 1.0 / n)
     ijk = np.zeros((len(b), 3), dtype=np.int32)
-    ijk[d[m]] = np.stack([ii.ravel()[m], jj.ravel()[m], kk.ravel()[m]], axis=1)
-    dims = nx
+    ijk[d[m]] = np.stack([ii.ravel()[m], jj.ravel()[m], kk.ravel()[m]], axis=1) \
+        - (origin + oracle.GHOST)
+    assert ijk.min() >= 0 and np.all(ijk < np.array(dims))
     ok_r, x_r, it_r, _ = oracle.amgpcg(rs, col, val, ijk, dims, b, tol=1e-10, maxit=300)
     assert ok_r and abs(it - it_r) <= 1
     assert np.abs(p - x_r).max() <= 1e-6 * np.abs(x_r).max()
