@@ -46,6 +46,9 @@ class SceneStruct(C.Structure):
         ("ndof", C.c_int64), ("rowstart", _u32p), ("colidx", _u32p),
         ("val", _dp), ("b", _dp), ("p", _dp), ("nnz_cap", C.c_int64),
         ("last_residual", C.c_double), ("last_iters", C.c_int), ("last_ok", C.c_int),
+        ("frz_mode", C.c_int), ("lphi_arg", _i32p), ("frz_act", _u8p), ("frz_arg", _i32p),
+        ("frz_liquid", _u8p), ("frz_dof", _i32p), ("frz_ndof", C.c_int64),
+        ("frz_clamp", _u8p * 3), ("frz_mass", _u8p * 3), ("frz_proj", _u8p),
     ]
 
 
@@ -156,6 +159,10 @@ def _configure(L):
                                    C.POINTER(C.c_int)]
     for nm in ("wco_extrapolate", "wco_constrain_liquid_velocity"):
         getattr(L, nm).argtypes = [C.POINTER(SceneStruct)]
+    L.wco_frozen_set.argtypes = [C.POINTER(SceneStruct), C.c_int]
+    L.wco_substep_adjoint.restype = C.c_int
+    L.wco_substep_adjoint.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_double, C.c_int,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     return L
 
 
@@ -325,6 +332,53 @@ class OracleScene:
     def substep(self, dt, tol=1e-8, maxit=1000):
         return self.L.wco_substep(self.s, dt, tol, maxit)
 
+    # ---- the adjoint (decisions recorded, replayed; wco_substep_adjoint) ----
+    def set_particles(self, x, v, Cm):
+        st = self.st
+        n = self.np
+        for name, a, k in (("x", x, 3), ("v", v, 3), ("C", Cm, 9)):
+            dst = np.ctypeslib.as_array(getattr(st, name), shape=(k * n,))
+            dst[:] = np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+    def frozen(self):
+        """A copy of the recorded decisions of the last substep."""
+        st = self.st
+        nn, nb = st.g.nnode, st.g.nbucket
+        return dict(act=self._arr(st.frz_act, nb, np.uint8), arg=self._arr(st.frz_arg, nn, np.int32),
+                    liquid=self._arr(st.frz_liquid, nn, np.uint8),
+                    dof=self._arr(st.frz_dof, nn, np.int32), ndof=int(st.frz_ndof),
+                    clamp=[self._arr(st.frz_clamp[d], nn, np.uint8) for d in range(3)],
+                    mass=[self._arr(st.frz_mass[d], nn, np.uint8) for d in range(3)],
+                    proj=self._arr(st.frz_proj, self.np, np.uint8))
+
+    def set_frozen(self, f):
+        st = self.st
+        nn, nb = st.g.nnode, st.g.nbucket
+
+        def put(ptr, n, a):
+            np.ctypeslib.as_array(ptr, shape=(n,))[:] = a
+        put(st.frz_act, nb, f["act"])
+        put(st.frz_arg, nn, f["arg"])
+        put(st.frz_liquid, nn, f["liquid"])
+        put(st.frz_dof, nn, f["dof"])
+        st.frz_ndof = f["ndof"]
+        for d in range(3):
+            put(st.frz_clamp[d], nn, f["clamp"][d])
+            put(st.frz_mass[d], nn, f["mass"][d])
+        put(st.frz_proj, self.np, f["proj"])
+
+    def frozen_mode(self, mode):
+        self.L.wco_frozen_set(self.s, mode)
+
+    def substep_adjoint(self, dt, tol, maxit, x0, v0, C0, xb, vb, Cb):
+        """dL/d(step inputs) from dL/d(step outputs) and dL/d|g| of this step."""
+        x0, v0, C0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (x0, v0, C0))
+        xb, vb, Cb = (np.array(a, dtype=np.float64, order="C") for a in (xb, vb, Cb))
+        gb = C.c_double(0.0)
+        ok = self.L.wco_substep_adjoint(self.s, dt, tol, maxit, ptr(x0), ptr(v0), ptr(C0),
+                                        ptr(xb), ptr(vb), ptr(Cb), C.byref(gb))
+        return bool(ok), xb, vb, Cb, gb.value
+
     def pre_solve(self, dt):
         """Phases up to and including assembly."""
         self.activate()
@@ -479,3 +533,63 @@ def dam_break(n: int, block_hi=(0.4, 0.6, 1.0), seed: int = 0, sigma: float = 0.
     x = seed_block(g, (0.0, 0.0, 0.0), block_hi, seed)
     v = seed_velocity(g, seed, sigma, x) if sigma > 0 else np.zeros_like(x)
     return g, x, v
+
+
+# ------------------------------------------------------------ the adjoint
+
+class AdjointRun:
+    """A run of `steps` substeps whose decisions are recorded on the first
+    forward pass and replayed afterwards, with the loss sum |x - target|^2 of
+    the final positions and its gradient by the adjoint (wco_substep_adjoint)
+    with respect to the initial velocities and the gravity magnitude."""
+
+    def __init__(self, n, dx, x0, v0, C0, steps, dt, g=9.81, tol=1e-13, L=None):
+        self.n, self.dx, self.steps, self.dt, self.tol = n, dx, steps, dt, tol
+        self.x0, self.C0 = x0, C0
+        self.scene = OracleScene(n, dx, x0, v0, C0, gravity=(0.0, -g, 0.0), L=L)
+        self.scene.frozen_mode(1)
+        self.states, self.records = [], []
+        for _ in range(steps):
+            self.states.append(self.scene.particles())
+            assert self.scene.substep(dt, tol, 5000)
+            self.records.append(self.scene.frozen())
+        self.scene.frozen_mode(2)
+
+    def forward(self, v0, g=None):
+        """Final positions from initial velocities v0 (and gravity g), with
+        the recorded decisions."""
+        s = self.scene
+        if g is not None:
+            s.st.gravity[1] = -g
+        s.set_particles(self.x0, v0, self.C0)
+        for k in range(self.steps):
+            s.set_frozen(self.records[k])
+            assert s.substep(self.dt, self.tol, 5000)
+        return s.particles()[0]
+
+    def gradient(self, v0, target, g=None):
+        """(loss, dL/dv0, dL/d|g|)."""
+        s = self.scene
+        xf = self.forward(v0, g)
+        loss = float(((xf - target) ** 2).sum())
+        xb = 2.0 * (xf - target)
+        vb = np.zeros_like(xb)
+        Cb = np.zeros((9, xb.shape[1]))
+        gsum = 0.0
+        states = [None] * self.steps
+        # forward again, keeping each step's inputs (the stored record states
+        # belong to the recorded velocities, not to v0)
+        s.set_particles(self.x0, v0, self.C0)
+        for k in range(self.steps):
+            states[k] = s.particles()
+            s.set_frozen(self.records[k])
+            assert s.substep(self.dt, self.tol, 5000)
+        for k in range(self.steps - 1, -1, -1):
+            x, v, Cm = states[k]
+            s.set_particles(x, v, Cm)
+            s.set_frozen(self.records[k])
+            assert s.substep(self.dt, self.tol, 5000)
+            ok, xb, vb, Cb, gb = s.substep_adjoint(self.dt, self.tol, 5000, x, v, Cm, xb, vb, Cb)
+            assert ok
+            gsum += gb
+        return loss, vb, gsum

@@ -24,7 +24,9 @@
  *   7 extrapolation in place (no copies)          8 one extrapolation layer
  *   9 flip_coeff applied to v instead of C        10 gravity with the wrong sign
  *   11 u* without the dt factor on gravity        12 liquid phi radius without the
- *                                                    0.883644 dx floor */
+ *                                                    0.883644 dx floor
+ *   adjoint (wco_substep_adjoint) without:  13 the theta terms  14 the solid
+ *   projection  15 the liquid phi's particle term  16 P2G's weight derivative */
 #ifndef WCO_PLANT
 #define WCO_PLANT 0
 #endif
@@ -73,6 +75,41 @@ static int node_active(const wco_scene* s, int i, int j, int k) {
 /* world coordinate of dense node index i on an axis, shifted by off cells */
 static double node_pos(const wco_grid* g, int i, double off) {
   return ((double)(i - WCO_GHOST) + off) * g->dx;
+}
+
+/* The decisions a replay holds fixed (wco_frozen_set): whether a cell is
+ * liquid and whether a face has mass. */
+static int is_liquid(const wco_scene* s, int64_t n) {
+  return s->frz_mode == 2 ? s->frz_liquid[n] : s->lphi[n] < 0.0;
+}
+static int has_mass(const wco_scene* s, int d, int64_t n) {
+  return s->frz_mode == 2 ? s->frz_mass[d][n] : s->mass[d][n] > 1e-20;
+}
+
+/* theta of the face (axis d, node fn) between cells nl and nr (nl < 0: no
+ * cell) with liquid phis phil, phir: fraction_inside(phil, phir)
+ * (MathUtilities.h:286-293) by the cells' liquid flags, clamped at the
+ * theta criterion (Pressure.cpp:20); the clamp is a decision a replay keeps */
+static double face_theta(wco_scene* s, int d, int64_t fn, int64_t nl, int64_t nr, double phil,
+                         double phir) {
+  const int ll = nl >= 0 && is_liquid(s, nl), lr = is_liquid(s, nr);
+  double th;
+  if (ll && lr)
+    th = 1;
+  else if (ll)
+    th = phil / (phil - phir);
+  else if (lr)
+    th = phir / (phir - phil);
+  else
+    th = 0;
+  int clamp;
+  if (s->frz_mode == 2) {
+    clamp = s->frz_clamp[d][fn];
+  } else {
+    clamp = th < THETA_CRITERION;
+    if (s->frz_mode == 1) s->frz_clamp[d][fn] = (uint8_t)clamp;
+  }
+  return clamp ? THETA_CRITERION : th;
 }
 
 /* ----------------------------------------------------------- math kernels */
@@ -368,6 +405,7 @@ wco_scene* wco_scene_create(int ni, int nj, int nk, double dx, double rho,
   }
   s->dof = (int32_t*)xcalloc(nn, sizeof(int32_t));
   s->pressure = (double*)xcalloc(nn, sizeof(double));
+  s->lphi_arg = (int32_t*)xcalloc(nn, sizeof(int32_t));
   return s;
 }
 
@@ -406,6 +444,16 @@ void wco_scene_destroy(wco_scene* s) {
   }
   free(s->dof);
   free(s->pressure);
+  free(s->lphi_arg);
+  free(s->frz_act);
+  free(s->frz_arg);
+  free(s->frz_liquid);
+  free(s->frz_dof);
+  for (int d = 0; d < 3; ++d) {
+    free(s->frz_clamp[d]);
+    free(s->frz_mass[d]);
+  }
+  free(s->frz_proj);
   free_system(s);
   free(s);
 }
@@ -416,6 +464,10 @@ void wco_scene_destroy(wco_scene* s) {
  * (expandFluidNodesMarked(1), TwoDScene.cpp:5021-5064). */
 void wco_activate(wco_scene* s) {
   const wco_grid* g = &s->g;
+  if (s->frz_mode == 2) {
+    memcpy(s->act, s->frz_act, g->nbucket);
+    return;
+  }
   uint8_t* touched = (uint8_t*)xcalloc(g->nbucket, 1);
   static const double offs[5][3] = {{0.0, 0.5, 0.5},
                                     {0.5, 0.0, 0.5},
@@ -454,6 +506,7 @@ void wco_activate(wco_scene* s) {
         s->act[wco_bucket_index(g, bi, bj, bk)] = (uint8_t)a;
       }
   free(touched);
+  if (s->frz_mode == 1) memcpy(s->frz_act, s->act, g->nbucket);
 }
 
 static double cell_solid_phi(const wco_scene* s, int i, int j, int k) {
@@ -467,9 +520,31 @@ void wco_update_liquid_phi(wco_scene* s) {
   const wco_grid* g = &s->g;
   const double dx = g->dx;
   const double bucket_size = WCO_BUCKET * dx;
-  for (int64_t c = 0; c < g->nnode; ++c) s->lphi[c] = 3.0 * bucket_size;
+  for (int64_t c = 0; c < g->nnode; ++c) {
+    s->lphi[c] = 3.0 * bucket_size;
+    s->lphi_arg[c] = -1;
+  }
   const double r0 = WCO_PLANT == 12 ? s->radius
                                      : (dx * 0.883644 > s->radius ? dx * 0.883644 : s->radius);
+  if (s->frz_mode == 2) {
+    /* the recorded particle of each cell gives its distance */
+    for (int k = 0; k < g->nx[2]; ++k)
+      for (int j = 0; j < g->nx[1]; ++j)
+        for (int i = 0; i < g->nx[0]; ++i) {
+          const int64_t n = wco_node_index(g, i, j, k);
+          const int32_t a = s->frz_arg[n];
+          s->lphi_arg[n] = a;
+          if (a == -2) {
+            s->lphi[n] = -0.5 * dx;
+          } else if (a >= 0) {
+            const double d0 = s->x[a] - node_pos(g, i, 0.5);
+            const double d1 = s->x[s->np + a] - node_pos(g, j, 0.5);
+            const double d2 = s->x[2 * s->np + a] - node_pos(g, k, 0.5);
+            s->lphi[n] = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - r0;
+          }
+        }
+    return;
+  }
   for (int64_t p = 0; p < s->np; ++p) {
     const double px[3] = {s->x[p], s->x[s->np + p], s->x[2 * s->np + p]};
     int base[3];
@@ -484,16 +559,25 @@ void wco_update_liquid_phi(wco_scene* s) {
           const double d2 = px[2] - node_pos(g, k, 0.5);
           const double phi = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - r0;
           const int64_t n = wco_node_index(g, i, j, k);
-          if (phi < s->lphi[n]) s->lphi[n] = phi;
+          if (phi < s->lphi[n]) {
+            s->lphi[n] = phi;
+            s->lphi_arg[n] = (int32_t)p;
+          }
         }
   }
   for (int k = 0; k < g->nx[2]; ++k)
     for (int j = 0; j < g->nx[1]; ++j)
       for (int i = 0; i < g->nx[0]; ++i) {
         if (!node_active(s, i, j, k) || WCO_PLANT == 4) continue;
-        if (cell_solid_phi(s, i, j, k) < 0.0)
+        if (cell_solid_phi(s, i, j, k) < 0.0) {
           s->lphi[wco_node_index(g, i, j, k)] = -0.5 * dx;
+          s->lphi_arg[wco_node_index(g, i, j, k)] = -2;
+        }
       }
+  if (s->frz_mode == 1) {
+    memcpy(s->frz_arg, s->lphi_arg, sizeof(int32_t) * g->nnode);
+    for (int64_t n = 0; n < g->nnode; ++n) s->frz_liquid[n] = s->lphi[n] < 0.0;
+  }
 }
 
 /* TwoDScene.cpp:7940-7970 updateSolidPhi: corner phi of the container in
@@ -601,8 +685,10 @@ void wco_p2g(wco_scene* s) {
     }
   }
   for (int d = 0; d < 3; ++d) {
-    for (int64_t n = 0; n < g->nnode; ++n)
-      if (s->mass[d][n] > 1e-20) s->vel[d][n] = mom[d][n] / s->mass[d][n];
+    for (int64_t n = 0; n < g->nnode; ++n) {
+      if (s->frz_mode == 1) s->frz_mass[d][n] = s->mass[d][n] > 1e-20;
+      if (has_mass(s, d, n)) s->vel[d][n] = mom[d][n] / s->mass[d][n];
+    }
     free(mom[d]);
   }
 }
@@ -635,11 +721,12 @@ void wco_step_velocity(wco_scene* s, double dt) {
       s->rhs[d][n] = rhs;
       const double Q = 1.0, Ms = 0.0;
       const double Cc = M + Q * Ms;
-      const double invC = Cc > 1e-20 ? 1.0 / Cc : 1.0;
+      const int hm = has_mass(s, d, n); /* M > 1e-20 (and so Cc) */
+      const double invC = hm ? 1.0 / Cc : 1.0;
       s->invC[d][n] = invC;
-      s->ustar[d][n] = M > 1e-20 ? rhs / M : rhs;
+      s->ustar[d][n] = hm ? rhs / M : rhs;
       const double psi = 0.0, invCs = 0.0;
-      const double P = M > 1e-20 ? 0.0 / M : 1.0;
+      const double P = hm ? 0.0 / M : 1.0;
       s->psi_fs[d][n] = ((1.0 - psi) * invC + psi * invCs * P) * M;
     }
   }
@@ -687,7 +774,7 @@ int64_t wco_assemble(wco_scene* s, double dt) {
     s->pressure[n] = 0.0;
   }
   /* storage order is bucket-major, so a running count is the partial sum */
-  for (int64_t bucket = 0; bucket < g->nbucket; ++bucket) {
+  for (int64_t bucket = 0; bucket < g->nbucket && s->frz_mode != 2; ++bucket) {
     if (!s->act[bucket]) continue;
     const int bk = (int)(bucket / ((int64_t)g->nb[0] * g->nb[1]));
     const int bj = (int)((bucket / g->nb[0]) % g->nb[1]);
@@ -709,6 +796,13 @@ int64_t wco_assemble(wco_scene* s, double dt) {
       }
       if (open) s->dof[n] = (int32_t)ndof++;
     }
+  }
+  if (s->frz_mode == 2) {
+    memcpy(s->dof, s->frz_dof, sizeof(int32_t) * g->nnode);
+    ndof = s->frz_ndof;
+  } else if (s->frz_mode == 1) {
+    memcpy(s->frz_dof, s->dof, sizeof(int32_t) * g->nnode);
+    s->frz_ndof = ndof;
   }
   if (WCO_PLANT == 6) { /* planted: numbered i-fastest over the whole grid */
     int64_t c = 0;
@@ -769,15 +863,16 @@ int64_t wco_assemble(wco_scene* s, double dt) {
           if (!cell_exists(s, nb[0], nb[1], nb[2])) continue;
           const int64_t nn = wco_node_index(g, nb[0], nb[1], nb[2]);
           const double nphi = s->lphi[nn];
-          if (nphi < 0.0) {
+          if (is_liquid(s, nn)) {
             const int32_t other = s->dof[nn];
             if (other >= 0) {
               insert_sorted(row, &len, (uint32_t)me, term);
               insert_sorted(row, &len, (uint32_t)other, -term);
             }
           } else {
-            double theta = wco_fraction_inside2(center_phi, nphi);
-            if (theta < THETA_CRITERION) theta = THETA_CRITERION;
+            /* the face between this cell and the air cell nn */
+            const double theta = hi ? face_theta(s, axis, fn, n, nn, center_phi, nphi)
+                                    : face_theta(s, axis, fn, nn, n, nphi, center_phi);
             insert_sorted(row, &len, (uint32_t)me, term / theta);
           }
         }
@@ -1040,12 +1135,10 @@ void wco_apply_pressure(wco_scene* s, double dt) {
             const int64_t ln = lex ? wco_node_index(g, l[0], l[1], l[2]) : -1;
             const double left_phi = lex ? s->lphi[ln] : 0.0;
             const double right_phi = rex ? s->lphi[n] : 0.0;
-            if (left_phi < 0.0 || right_phi < 0.0) {
+            const int lliq = lex && is_liquid(s, ln), rliq = is_liquid(s, n);
+            if (lliq || rliq) {
               double theta = 1.;
-              if (right_phi >= 0.0 || left_phi >= 0.0) {
-                theta = wco_fraction_inside2(left_phi, right_phi);
-                if (theta < THETA_CRITERION) theta = THETA_CRITERION;
-              }
+              if (!rliq || !lliq) theta = face_theta(s, d, n, ln, n, left_phi, right_phi);
               double pg = 0;
               if (lex) pg -= s->pressure[ln];
               if (rex) pg += s->pressure[n];
@@ -1197,7 +1290,14 @@ void wco_g2p_advect(wco_scene* s, double dt) {
       }
       const double dp[3] = {fv[0] * dt, fv[1] * dt, fv[2] * dt};
       const double phi_now = phi_ori + (gr[0] * dp[0] + gr[1] * dp[1] + gr[2] * dp[2]);
-      if (phi_now < 0.0) {
+      int proj;
+      if (s->frz_mode == 2) {
+        proj = s->frz_proj[p];
+      } else {
+        proj = phi_now < 0.0;
+        if (s->frz_mode == 1) s->frz_proj[p] = (uint8_t)proj;
+      }
+      if (proj) {
         nx[0] -= phi_now * gr[0];
         nx[1] -= phi_now * gr[1];
         nx[2] -= phi_now * gr[2];
@@ -1225,5 +1325,437 @@ int wco_substep(wco_scene* s, double dt, double tol, int maxit) {
   const int ok = wco_solve_pressure(s, tol, maxit);
   wco_apply_pressure(s, dt);     /* :224, :230, :239 */
   wco_g2p_advect(s, dt);         /* :248, :258, :262 */
+  return ok;
+}
+
+/* ------------------------------------------------------------- adjoint */
+
+void wco_frozen_set(wco_scene* s, int mode) {
+  const int64_t nn = s->g.nnode;
+  if (!s->frz_act) {
+    s->frz_act = (uint8_t*)xcalloc(s->g.nbucket, 1);
+    s->frz_arg = (int32_t*)xcalloc(nn, sizeof(int32_t));
+    s->frz_liquid = (uint8_t*)xcalloc(nn, 1);
+    s->frz_dof = (int32_t*)xcalloc(nn, sizeof(int32_t));
+    for (int d = 0; d < 3; ++d) {
+      s->frz_clamp[d] = (uint8_t*)xcalloc(nn, 1);
+      s->frz_mass[d] = (uint8_t*)xcalloc(nn, 1);
+    }
+    s->frz_proj = (uint8_t*)xcalloc(s->np, 1);
+  }
+  s->frz_mode = mode;
+}
+
+/* derivative of MathUtilities.h:416-424 quad_kernel */
+static double quad_kernel_d(double x) {
+  const double ax = fabs(x);
+  if (ax < 0.5) return -2.0 * x;
+  if (ax < 1.5) return x > 0 ? -(1.5 - ax) : (1.5 - ax);
+  return 0.0;
+}
+
+/* the three per-axis weights of a stencil and their derivatives by x */
+static void stencil_weights_d(const wco_grid* g, double x, int base, double off, double w[3],
+                              double dw[3]) {
+  for (int a = 0; a < 3; ++a) {
+    const double r = (x - node_pos(g, base + a, off)) / g->dx;
+    w[a] = wco_quad_kernel(r);
+    dw[a] = quad_kernel_d(r) / g->dx;
+  }
+}
+
+/* whether the pressure application (Pressure.cpp:1184-1211) sets face (d, n)
+ * directly: a positive solid weight and a liquid cell on either side */
+static int direct_face(const wco_scene* s, int d, int i, int j, int k) {
+  const int64_t n = wco_node_index(&s->g, i, j, k);
+  if (!s->act[n >> 6] || !(s->w[d][n] > 0.)) return 0;
+  int l[3] = {i, j, k};
+  l[d] -= 1;
+  const int lex = node_active(s, l[0], l[1], l[2]);
+  const int64_t ln = lex ? wco_node_index(&s->g, l[0], l[1], l[2]) : -1;
+  return (lex && is_liquid(s, ln)) || is_liquid(s, n);
+}
+
+/* d theta / d phi of a face between cells nl (may be absent: < 0) and nr:
+ * adds tb * dtheta/dphi to phib of each cell */
+static void theta_adjoint(const wco_scene* s, int d, int64_t fn, int64_t nl, int64_t nr,
+                          double tb, double* phib) {
+  if (WCO_PLANT == 13) return;
+  const int ll = nl >= 0 && is_liquid(s, nl), lr = is_liquid(s, nr);
+  if (ll == lr) return;  /* theta = 1 (or no liquid): constant */
+  if (s->frz_clamp[d][fn]) return; /* recorded (mode 1) or replayed (mode 2) */
+  const double pl = nl >= 0 ? s->lphi[nl] : 0.0, pr = s->lphi[nr];
+  const double a = ll ? pl : pr, b = ll ? pr : pl; /* liquid and air phi */
+  const double den = (a - b) * (a - b);
+  const double da = -b / den, db = a / den; /* theta = a / (a - b) */
+  if (ll) {
+    if (nl >= 0) phib[nl] += tb * da;
+    phib[nr] += tb * db;
+  } else {
+    phib[nr] += tb * da;
+    if (nl >= 0) phib[nl] += tb * db;
+  }
+}
+
+/* Adjoint of wco_substep in the replayed (or recorded) decisions. */
+int wco_substep_adjoint(wco_scene* s, double dt, double tol, int maxit, const double* x0,
+                        const double* v0, const double* C0, double* xb, double* vb, double* Cb,
+                        double* gb) {
+  const wco_grid* g = &s->g;
+  const int64_t np = s->np, nn = g->nnode;
+  const double dx = g->dx, invD = 4.0 / (dx * dx), fm = s->pmass;
+  double* ub[3]; /* dL/d(final face velocity), then dL/d(P2G face velocity) */
+  double* ub0[3];
+  for (int d = 0; d < 3; ++d) {
+    ub[d] = (double*)xcalloc(nn, sizeof(double));
+    ub0[d] = (double*)xcalloc(nn, sizeof(double));
+  }
+  double* xin = (double*)xcalloc(3 * np, sizeof(double)); /* dL/dx0 */
+  double* vin = (double*)xcalloc(3 * np, sizeof(double));
+  double* Cin = (double*)xcalloc(9 * np, sizeof(double));
+
+  /* 1. G2P (TDS:6936-7090), advection and solid projection, per particle */
+  for (int64_t p = 0; p < np; ++p) {
+    const double px[3] = {x0[p], x0[np + p], x0[2 * np + p]};
+    const double fv[3] = {s->v[p], s->v[np + p], s->v[2 * np + p]};
+    double nx[3];
+    for (int d = 0; d < 3; ++d) nx[d] = px[d] + fv[d] * dt;
+    double xbar_out[3] = {xb[p], xb[np + p], xb[2 * np + p]};
+    double nxb[3] = {xbar_out[0], xbar_out[1], xbar_out[2]};
+    double fvb[3] = {vb[p], vb[np + p], vb[2 * np + p]};
+    double pxb[3] = {0, 0, 0};
+    if (s->solid_projection && s->frz_proj[p] && WCO_PLANT != 14) {
+      int base[3];
+      double w[3][3], dw[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], 0.0);
+        stencil_weights_d(g, px[a], base[a], 0.0, w[a], dw[a]);
+      }
+      double phi_ori = 0.0, gr[3] = {0, 0, 0};
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            const double phi = node_active(s, i, j, k) ? s->sphi[wco_node_index(g, i, j, k)]
+                                                       : 3.0 * dx;
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            phi_ori += phi * ww;
+            gr[0] += phi * invD * ww * (node_pos(g, i, 0.0) - nx[0]);
+            gr[1] += phi * invD * ww * (node_pos(g, j, 0.0) - nx[1]);
+            gr[2] += phi * invD * ww * (node_pos(g, k, 0.0) - nx[2]);
+          }
+      const double gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+      double gh[3] = {gr[0], gr[1], gr[2]};
+      if (gn > 1e-20)
+        for (int a = 0; a < 3; ++a) gh[a] /= gn;
+      const double dp[3] = {fv[0] * dt, fv[1] * dt, fv[2] * dt};
+      const double phi_now = phi_ori + (gh[0] * dp[0] + gh[1] * dp[1] + gh[2] * dp[2]);
+      /* x' = nx - phi_now gh */
+      const double pnb = -(gh[0] * xbar_out[0] + gh[1] * xbar_out[1] + gh[2] * xbar_out[2]);
+      double ghb[3], grb[3];
+      for (int a = 0; a < 3; ++a) {
+        ghb[a] = -phi_now * xbar_out[a] + pnb * dp[a];
+        fvb[a] += dt * pnb * gh[a]; /* dp = fv dt */
+      }
+      if (gn > 1e-20) {
+        const double dd = gh[0] * ghb[0] + gh[1] * ghb[1] + gh[2] * ghb[2];
+        for (int a = 0; a < 3; ++a) grb[a] = (ghb[a] - gh[a] * dd) / gn;
+      } else {
+        for (int a = 0; a < 3; ++a) grb[a] = ghb[a];
+      }
+      for (int a = 0; a < 3; ++a) nxb[a] += -invD * phi_ori * grb[a];
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            const double phi = node_active(s, i, j, k) ? s->sphi[wco_node_index(g, i, j, k)]
+                                                       : 3.0 * dx;
+            const double q[3] = {node_pos(g, i, 0.0), node_pos(g, j, 0.0), node_pos(g, k, 0.0)};
+            const double coef =
+                pnb + invD * ((q[0] - nx[0]) * grb[0] + (q[1] - nx[1]) * grb[1] +
+                              (q[2] - nx[2]) * grb[2]);
+            pxb[0] += phi * coef * dw[0][c0] * w[1][c1] * w[2][c2];
+            pxb[1] += phi * coef * w[0][c0] * dw[1][c1] * w[2][c2];
+            pxb[2] += phi * coef * w[0][c0] * w[1][c1] * dw[2][c2];
+          }
+    }
+    /* nx = px + fv dt */
+    for (int a = 0; a < 3; ++a) {
+      pxb[a] += nxb[a];
+      fvb[a] += dt * nxb[a];
+    }
+    /* fv_d = sum w u;  C_d = flip sum w u (x_n - x_p) invD */
+    for (int d = 0; d < 3; ++d) {
+      int base[3];
+      double w[3][3], dw[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], face_off[d][a]);
+        stencil_weights_d(g, px[a], base[a], face_off[d][a], w[a], dw[a]);
+      }
+      const double Bb[3] = {s->flip * Cb[(int64_t)(d * 3 + 0) * np + p],
+                            s->flip * Cb[(int64_t)(d * 3 + 1) * np + p],
+                            s->flip * Cb[(int64_t)(d * 3 + 2) * np + p]};
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            if (!node_active(s, i, j, k)) continue;
+            const int64_t n = wco_node_index(g, i, j, k);
+            const double u = s->unew[d][n];
+            const double e[3] = {node_pos(g, i, face_off[d][0]) - px[0],
+                                 node_pos(g, j, face_off[d][1]) - px[1],
+                                 node_pos(g, k, face_off[d][2]) - px[2]};
+            const double G = fvb[d] + invD * (Bb[0] * e[0] + Bb[1] * e[1] + Bb[2] * e[2]);
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            ub[d][n] += ww * G;
+            pxb[0] += u * G * dw[0][c0] * w[1][c1] * w[2][c2] - ww * u * invD * Bb[0];
+            pxb[1] += u * G * w[0][c0] * dw[1][c1] * w[2][c2] - ww * u * invD * Bb[1];
+            pxb[2] += u * G * w[0][c0] * w[1][c1] * dw[2][c2] - ww * u * invD * Bb[2];
+          }
+    }
+    for (int a = 0; a < 3; ++a) xin[(int64_t)a * np + p] = pxb[a];
+  }
+
+  /* 2. constrainLiquidVelocity (TDS:1827-1866): faces with zero weight */
+  for (int d = 0; d < 3; ++d)
+    for (int64_t n = 0; n < nn; ++n)
+      if (s->act[n >> 6] && s->w[d][n] == 0.0) ub[d][n] *= (1 - s->friction);
+
+  /* 3. the two extrapolation layers (Pressure.cpp:1826-1903) in reverse:
+   * layer L fills each invalid face with the mean of its neighbours valid
+   * before the layer (vset[L]); its adjoint hands each filled face's
+   * gradient to those neighbours in equal shares, and the face's value
+   * before the layer, overwritten, gets none */
+  {
+    static const int off[6][3] = {{-1, 0, 0}, {1, 0, 0},  {0, -1, 0},
+                                  {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
+    uint8_t* vset[3];
+    uint8_t* fill[2];
+    int32_t* cnt[2];
+    for (int L = 0; L < 3; ++L) vset[L] = (uint8_t*)xcalloc(nn, 1);
+    for (int L = 0; L < 2; ++L) {
+      fill[L] = (uint8_t*)xcalloc(nn, 1);
+      cnt[L] = (int32_t*)xcalloc(nn, sizeof(int32_t));
+    }
+    for (int d = 0; d < 3; ++d) {
+      for (int k = 0; k < g->nx[2]; ++k)
+        for (int j = 0; j < g->nx[1]; ++j)
+          for (int i = 0; i < g->nx[0]; ++i)
+            vset[0][wco_node_index(g, i, j, k)] = (uint8_t)direct_face(s, d, i, j, k);
+      for (int L = 0; L < 2; ++L) {
+        memcpy(vset[L + 1], vset[L], nn);
+        for (int k = 0; k < g->nx[2]; ++k)
+          for (int j = 0; j < g->nx[1]; ++j)
+            for (int i = 0; i < g->nx[0]; ++i) {
+              const int64_t n = wco_node_index(g, i, j, k);
+              fill[L][n] = 0;
+              if (!s->act[n >> 6] || vset[L][n]) continue;
+              int c = 0;
+              for (int r = 0; r < 6; ++r) {
+                const int a = i + off[r][0], b = j + off[r][1], q = k + off[r][2];
+                if (node_active(s, a, b, q) && vset[L][wco_node_index(g, a, b, q)]) ++c;
+              }
+              if (c > 0) {
+                fill[L][n] = 1;
+                cnt[L][n] = c;
+                vset[L + 1][n] = 1;
+              }
+            }
+      }
+      for (int L = 1; L >= 0; --L)
+        for (int k = 0; k < g->nx[2]; ++k)
+          for (int j = 0; j < g->nx[1]; ++j)
+            for (int i = 0; i < g->nx[0]; ++i) {
+              const int64_t n = wco_node_index(g, i, j, k);
+              if (!fill[L][n]) continue;
+              const double share = ub[d][n] / (double)cnt[L][n];
+              for (int r = 0; r < 6; ++r) {
+                const int a = i + off[r][0], b = j + off[r][1], q = k + off[r][2];
+                if (!node_active(s, a, b, q)) continue;
+                const int64_t m = wco_node_index(g, a, b, q);
+                if (vset[L][m]) ub[d][m] += share;
+              }
+              ub[d][n] = 0.0;
+            }
+    }
+    for (int L = 0; L < 3; ++L) free(vset[L]);
+    for (int L = 0; L < 2; ++L) {
+      free(fill[L]);
+      free(cnt[L]);
+    }
+  }
+
+  /* 4. the pressure application (Pressure.cpp:1137-1314, LIE:65-104):
+   * u0 = invC rhs - invC dt V (p_R - p_L) / (dx theta), rhs = M (u + g dt) */
+  const int64_t ndof = s->ndof;
+  double* pb = (double*)xcalloc(ndof ? ndof : 1, sizeof(double));
+  double* phib = (double*)xcalloc(nn, sizeof(double));
+  double* rb[3]; /* dL/d rhs of the grid update per face */
+  for (int d = 0; d < 3; ++d) rb[d] = (double*)xcalloc(nn, sizeof(double));
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < g->nx[2]; ++k)
+      for (int j = 0; j < g->nx[1]; ++j)
+        for (int i = 0; i < g->nx[0]; ++i) {
+          const int64_t n = wco_node_index(g, i, j, k);
+          if (!s->act[n >> 6]) continue;
+          const double ubn = ub[d][n];
+          const double invC = s->invC[d][n];
+          rb[d][n] += invC * ubn;
+          if (!direct_face(s, d, i, j, k)) continue;
+          int l[3] = {i, j, k};
+          l[d] -= 1;
+          const int lex = node_active(s, l[0], l[1], l[2]);
+          const int64_t ln = lex ? wco_node_index(g, l[0], l[1], l[2]) : -1;
+          const int lliq = lex && is_liquid(s, ln), rliq = is_liquid(s, n);
+          double theta = 1.0;
+          if (!lliq || !rliq) theta = face_theta(s, d, n, ln, n, lex ? s->lphi[ln] : 0.0, s->lphi[n]);
+          const double c = invC * dt * s->vol[d][n] / (dx * theta);
+          /* u0 -= c (p_n - p_ln) */
+          if (s->dof[n] >= 0) pb[s->dof[n]] -= c * ubn;
+          if (lex && s->dof[ln] >= 0) pb[s->dof[ln]] += c * ubn;
+          if (!lliq || !rliq) {
+            const double pn = s->pressure[n], pl = lex ? s->pressure[ln] : 0.0;
+            const double tb = ubn * invC * dt * s->vol[d][n] * (pn - pl) / (dx * theta * theta);
+            theta_adjoint(s, d, n, ln, n, tb, phib);
+          }
+        }
+
+  /* 5. the pressure solve: bbar = A^-1 pbar (A symmetric); A_ii holds
+   * term / theta for each air face of a DOF */
+  double* bb = (double*)xcalloc(ndof ? ndof : 1, sizeof(double));
+  double res = 0;
+  int it = 0;
+  int ok = 1;
+  if (ndof > 0)
+    ok = wco_pcg_solve(ndof, s->rowstart, s->colidx, s->val, pb, bb, tol, maxit, 0.97, 0.25,
+                       &res, &it);
+  const double coeff = dt / (dx * dx);
+  for (int64_t bucket = 0; bucket < g->nbucket; ++bucket) {
+    if (!s->act[bucket]) continue;
+    const int bk = (int)(bucket / ((int64_t)g->nb[0] * g->nb[1]));
+    const int bj = (int)((bucket / g->nb[0]) % g->nb[1]);
+    const int bi = (int)(bucket % g->nb[0]);
+    for (int l = 0; l < 64; ++l) {
+      const int64_t n = (bucket << 6) | l;
+      const int32_t me = s->dof[n];
+      if (me < 0) continue;
+      const int i = bi * 4 + (l & 3), j = bj * 4 + ((l >> 2) & 3), k = bk * 4 + (l >> 4);
+      for (int f = 0; f < 6; ++f) {
+        const int axis = f >> 1, hi = f & 1;
+        int q[3] = {i, j, k};
+        q[axis] += hi;
+        if (!node_active(s, q[0], q[1], q[2])) continue;
+        const int64_t fn = wco_node_index(g, q[0], q[1], q[2]);
+        const double w = s->w[axis][fn];
+        /* b: rhs += (lo face) or -= (hi face) u* psi w / dx, u* psi = invC rhs */
+        const double sgn = hi ? -1.0 : 1.0;
+        rb[axis][fn] += sgn * bb[me] * s->invC[axis][fn] * w / dx;
+        if (!(w > 0.0)) continue;
+        int nb[3] = {i, j, k};
+        nb[axis] += hi ? 1 : -1;
+        if (!node_active(s, nb[0], nb[1], nb[2])) continue;
+        const int64_t nnb = wco_node_index(g, nb[0], nb[1], nb[2]);
+        if (is_liquid(s, nnb)) continue;
+        const double term = w * coeff * (s->invC[axis][fn] * s->vol[axis][fn]);
+        const int64_t nl = hi ? n : nnb, nr = hi ? nnb : n;
+        const double theta = face_theta(s, axis, fn, nl, nr, s->lphi[nl], s->lphi[nr]);
+        /* A_ii += term / theta: dL = -bbar_i dA_ii p_i */
+        const double tb = bb[me] * s->p[me] * term / (theta * theta);
+        theta_adjoint(s, axis, fn, nl, nr, tb, phib);
+      }
+    }
+  }
+
+  /* 6. the grid update (LIE:224-337): rhs = M (u + g dt) */
+  double gbar_y = 0.0;
+  for (int d = 0; d < 3; ++d)
+    for (int64_t n = 0; n < nn; ++n) {
+      if (!s->act[n >> 6] || !has_mass(s, d, n)) {
+        ub0[d][n] = 0.0;
+        continue;
+      }
+      const double M = s->mass[d][n];
+      ub0[d][n] = M * rb[d][n];
+      if (d == 1) gbar_y += M * dt * rb[d][n];
+    }
+  *gb += -gbar_y; /* gravity (0, -|g|, 0) */
+
+  /* 7. P2G (TDS:6700-6713): u = P / M, P = fm sum w (v + C e), M = fm sum w */
+  for (int64_t p = 0; p < np; ++p) {
+    const double px[3] = {x0[p], x0[np + p], x0[2 * np + p]};
+    double pxb[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+      int base[3];
+      double w[3][3], dw[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], face_off[d][a]);
+        stencil_weights_d(g, px[a], base[a], face_off[d][a], w[a], dw[a]);
+      }
+      const double vd = v0[(int64_t)d * np + p];
+      const double b0 = C0[(int64_t)(d * 3 + 0) * np + p];
+      const double b1 = C0[(int64_t)(d * 3 + 1) * np + p];
+      const double b2 = C0[(int64_t)(d * 3 + 2) * np + p];
+      double vdb = 0, cb[3] = {0, 0, 0};
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            if (!node_active(s, i, j, k)) continue;
+            const int64_t n = wco_node_index(g, i, j, k);
+            if (!has_mass(s, d, n)) continue;
+            const double M = s->mass[d][n];
+            const double Pb = ub0[d][n] / M;
+            const double Mb = -ub0[d][n] * s->vel[d][n] / M;
+            const double e[3] = {node_pos(g, i, face_off[d][0]) - px[0],
+                                 node_pos(g, j, face_off[d][1]) - px[1],
+                                 node_pos(g, k, face_off[d][2]) - px[2]};
+            const double vel = vd + (b0 * e[0] + b1 * e[1] + b2 * e[2]);
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            vdb += fm * ww * Pb;
+            for (int a = 0; a < 3; ++a) cb[a] += fm * ww * Pb * e[a];
+            const double G = WCO_PLANT == 16 ? 0.0 : fm * (Pb * vel + Mb);
+            pxb[0] += G * dw[0][c0] * w[1][c1] * w[2][c2] - fm * ww * Pb * b0;
+            pxb[1] += G * w[0][c0] * dw[1][c1] * w[2][c2] - fm * ww * Pb * b1;
+            pxb[2] += G * w[0][c0] * w[1][c1] * dw[2][c2] - fm * ww * Pb * b2;
+          }
+      vin[(int64_t)d * np + p] += vdb;
+      for (int a = 0; a < 3; ++a) Cin[(int64_t)(d * 3 + a) * np + p] += cb[a];
+    }
+    for (int a = 0; a < 3; ++a) xin[(int64_t)a * np + p] += pxb[a];
+  }
+
+  /* 8. the liquid phi (TDS:3307-3348): phi_c = |x_p - x_c| - r0 for the
+   * cell's recorded particle */
+  for (int k = 0; k < g->nx[2]; ++k)
+    for (int j = 0; j < g->nx[1]; ++j)
+      for (int i = 0; i < g->nx[0]; ++i) {
+        const int64_t n = wco_node_index(g, i, j, k);
+        const int32_t a = s->lphi_arg[n];
+        if (a < 0 || phib[n] == 0.0 || WCO_PLANT == 15) continue;
+        const double d0 = x0[a] - node_pos(g, i, 0.5);
+        const double d1 = x0[np + a] - node_pos(g, j, 0.5);
+        const double d2 = x0[2 * np + a] - node_pos(g, k, 0.5);
+        const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        if (r > 0) {
+          xin[a] += phib[n] * d0 / r;
+          xin[np + a] += phib[n] * d1 / r;
+          xin[2 * np + a] += phib[n] * d2 / r;
+        }
+      }
+
+  memcpy(xb, xin, sizeof(double) * 3 * np);
+  memcpy(vb, vin, sizeof(double) * 3 * np);
+  memcpy(Cb, Cin, sizeof(double) * 9 * np);
+  for (int d = 0; d < 3; ++d) {
+    free(ub[d]);
+    free(ub0[d]);
+    free(rb[d]);
+  }
+  free(xin);
+  free(vin);
+  free(Cin);
+  free(pb);
+  free(bb);
+  free(phib);
   return ok;
 }

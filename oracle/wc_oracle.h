@@ -132,6 +132,24 @@ typedef struct {
   double last_residual;
   int last_iters;
   int last_ok;
+
+  /* Decisions of the substep held fixed for the adjoint and its central
+   * differences (wc_oracle_adj.c): frz_mode 0 computes them, 1 computes and
+   * records them, 2 replays the recorded ones.  Recorded: the activated
+   * buckets, per cell the particle whose distance gives the liquid phi (-1:
+   * the default 3 buckets, -2: the solid override), whether it is liquid and
+   * its DOF number, per face whether theta is clamped at 0.01 and whether
+   * it has mass, per particle whether the solid projection moves it. */
+  int frz_mode;
+  int32_t* lphi_arg;     /* per cell, always kept: the liquid phi's particle */
+  uint8_t* frz_act;
+  int32_t* frz_arg;
+  uint8_t* frz_liquid;
+  int32_t* frz_dof;
+  int64_t frz_ndof;
+  uint8_t* frz_clamp[3];
+  uint8_t* frz_mass[3];
+  uint8_t* frz_proj;
 } wco_scene;
 
 wco_scene* wco_scene_create(int ni, int nj, int nk, double dx, double rho,
@@ -161,6 +179,20 @@ void wco_g2p_advect(wco_scene* s, double dt);
 void wco_copy_vel_to_unew(wco_scene* s);
 /* one substep in the reference order (Core/WetClothCore.cpp:77-300) */
 int wco_substep(wco_scene* s, double dt, double tol, int maxit);
+
+/* ---- the adjoint of a substep (wc_oracle_adj.c) ----
+ * wco_frozen_set(s, mode): 0 compute every decision, 1 record them, 2 replay
+ * the recorded ones (the arrays are allocated on first use).
+ * wco_substep_adjoint: the scene holds the forward run (mode 1 or 2) of one
+ * substep from the particle state x0, v0, C0 (SoA, as the scene's).  On
+ * entry xb, vb, Cb hold dL/d(the substep's x, v, C); on return dL/d(x0, v0,
+ * C0).  *gb accumulates dL/d|g| for gravity (0, -|g|, 0).  The adjoint
+ * pressure solve runs MIC(0)-PCG to tol (the matrix is symmetric).  Returns
+ * 1 when that solve converged. */
+void wco_frozen_set(wco_scene* s, int mode);
+int wco_substep_adjoint(wco_scene* s, double dt, double tol, int maxit, const double* x0,
+                        const double* v0, const double* C0, double* xb, double* vb, double* Cb,
+                        double* gb);
 
 /* ---- PCGSolver restatement on a sorted CSR matrix ---- */
 void wco_spmv(int64_t n, const uint32_t* rowstart, const uint32_t* col,
