@@ -257,6 +257,27 @@ wc_status wc_pcg_solve_csr(int64_t n, const uint32_t* rowstart,
                            const double* rhs, double* result, double tol,
                            int32_t maxit, double* residual_out,
                            int32_t* iters_out);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): wc_pcg_solve_csr, with the same
+ * arguments, results and kernels, taking all its device memory -- the
+ * uploaded matrix and vectors, the MIC(0) factor, the level schedule, the
+ * PCG vectors and reduction partials -- from one fixed array of
+ * WC_NOALLOC_SCRATCH_BYTES per GPU that the library holds from the first
+ * call on that GPU until the process ends.  No device or pinned host memory
+ * is allocated (the first call on a GPU also creates the stream it uses).
+ * The array holds n + 1 + 2 nnz 32-bit words, 2 nnz + 7 n doubles and 4 n
+ * 32-bit words, each part rounded up to 256 bytes, plus 9 KB: about 520,000
+ * rows of a 7-point system.  Larger systems are refused with WC_ERR_ARG.
+ * Every call on one GPU uses that GPU's array and stream: calls that
+ * overlap on one GPU (from several threads) race and must be serialised by
+ * the caller. */
+#define WC_NOALLOC_SCRATCH_BYTES (128u << 20)
+wc_status wc_pcg_solve_csr_noalloc(int64_t n, const uint32_t* rowstart,
+                                   const uint32_t* col, const double* val,
+                                   const double* rhs, double* result, double tol,
+                                   int32_t maxit, double* residual_out,
+                                   int32_t* iters_out);
 /* factor_modified_incomplete_cholesky0 (PCGSolver.h:97-205).  lv is
  * indexed like val: lv[e] = L(col[e], row) for entries above the diagonal. */
 wc_status wc_mic0_factor_csr(int64_t n, const uint32_t* rowstart,

@@ -110,6 +110,7 @@ def lib():
         L.wc_system_multiply.argtypes = [vp, _dp, _dp]
         L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
                                        C.c_int32, _dp, _i32p]
+        L.wc_pcg_solve_csr_noalloc.argtypes = L.wc_pcg_solve_csr.argtypes
         L.wc_mic0_factor_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double, C.c_double,
                                          _dp, _dp]
         L.wc_mic0_apply_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp]
@@ -388,8 +389,10 @@ def _csr(rs, col):
     return (np.ascontiguousarray(rs, dtype=np.uint32), np.ascontiguousarray(col, dtype=np.uint32))
 
 
-def pcg_solve_csr(rs, col, val, rhs, tol=1e-8, maxit=500):
-    """PCGSolver<double>::solve: returns (converged, x, iterations, residual)."""
+def pcg_solve_csr(rs, col, val, rhs, tol=1e-8, maxit=500, no_alloc=False):
+    """PCGSolver<double>::solve: returns (converged, x, iterations, residual).
+    no_alloc=True calls wc_pcg_solve_csr_noalloc (the library's fixed device
+    array instead of allocations; systems beyond it raise)."""
     rs, col = _csr(rs, col)
     val = np.ascontiguousarray(val, dtype=np.float64)
     rhs = np.ascontiguousarray(rhs, dtype=np.float64)
@@ -397,8 +400,9 @@ def pcg_solve_csr(rs, col, val, rhs, tol=1e-8, maxit=500):
     x = np.zeros(n)
     res = C.c_double()
     it = C.c_int32()
-    st = lib().wc_pcg_solve_csr(n, _ptr(rs, _u32p), _ptr(col, _u32p), _ptr(val), _ptr(rhs),
-                                _ptr(x), tol, maxit, C.byref(res), C.byref(it))
+    fn = lib().wc_pcg_solve_csr_noalloc if no_alloc else lib().wc_pcg_solve_csr
+    st = fn(n, _ptr(rs, _u32p), _ptr(col, _u32p), _ptr(val), _ptr(rhs), _ptr(x), tol, maxit,
+            C.byref(res), C.byref(it))
     _check(st, None, allow_nc=True)
     return st == WC_OK, x, it.value, res.value
 

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <charconv>
 #include <climits>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -956,6 +957,161 @@ wc_status wc_pcg_solve_csr(int64_t n, const uint32_t* rowstart,
       x.release();
     }
   } catch (const HipError& e) {
+    return hip_fail(nullptr, e);
+  }
+  if (residual_out) *residual_out = res;
+  if (iters_out) *iters_out = it;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+}  // extern "C"
+
+// ---- wc_pcg_solve_csr without allocation ----
+
+namespace {
+
+// The device memory of wc_pcg_solve_csr_noalloc: one fixed array per GPU,
+// held until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: the array's device address on each device is
+// looked up once (hipGetSymbolAddress, with that device current) and only
+// that address is handed to copies and kernels.
+static __device__ __align__(256) unsigned char scratch[WC_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kNoAllocDevices = 64;
+struct NoAllocDevice {
+  std::once_flag once;
+  hipError_t err = hipSuccess;
+  unsigned char* ptr = nullptr;  // this device's copy of `scratch`
+  hipStream_t st = nullptr;      // created with it, kept with it
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+// Set-up on the current device, once: the array's address and a stream.
+NoAllocDevice& noalloc_device() {
+  int dev = 0;
+  WC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices)
+    throw HipError{hipErrorInvalidDevice, "device ordinal beyond the no-allocation table",
+                   __FILE__, __LINE__};
+  NoAllocDevice& d = g_noalloc[dev];
+  std::call_once(d.once, [&d] {
+    d.err = hipGetSymbolAddress((void**)&d.ptr, HIP_SYMBOL(scratch));
+    if (d.err == hipSuccess && !d.ptr) d.err = hipErrorInvalidSymbol;
+    if (d.err == hipSuccess) d.err = hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking);
+  });
+  WC_HIP(d.err);
+  return d;
+}
+
+// Every buffer of the level-scheduled PCG solve of an n-row system with nnz
+// stored entries, carved from one block at 256-byte boundaries.  With
+// base == nullptr only the size is computed.
+struct NoAllocParts {
+  uint32_t *rowptr, *col, *tpos, *lvl_count, *lvl_cursor, *lvl_list;
+  double *val, *b, *x, *lv, *invdiag, *adiag, *z, *r, *s, *partial, *scal;
+  int* flags;
+  int32_t* level;
+  size_t bytes;
+};
+
+NoAllocParts noalloc_layout(unsigned char* base, int64_t n, int64_t nnz) {
+  size_t off = 0;
+  auto take = [&](size_t elems, size_t elem_bytes) -> void* {
+    const size_t o = (off + 255) & ~size_t(255);
+    off = o + elems * elem_bytes;
+    return base ? base + o : nullptr;
+  };
+  NoAllocParts q;
+  q.rowptr = (uint32_t*)take(n + 1, 4);
+  q.col = (uint32_t*)take(nnz, 4);
+  q.val = (double*)take(nnz, 8);
+  q.b = (double*)take(n, 8);
+  q.x = (double*)take(n, 8);
+  q.tpos = (uint32_t*)take(nnz, 4);
+  q.lv = (double*)take(nnz, 8);
+  q.invdiag = (double*)take(n, 8);
+  q.adiag = (double*)take(n, 8);
+  q.z = (double*)take(n, 8);
+  q.r = (double*)take(n, 8);
+  q.s = (double*)take(n, 8);
+  q.flags = (int*)take(4, 4);
+  q.level = (int32_t*)take(n, 4);
+  q.lvl_count = (uint32_t*)take(n, 4);  // a row's level is below its index: n levels at most
+  q.lvl_cursor = (uint32_t*)take(n, 4);
+  q.lvl_list = (uint32_t*)take(n, 4);
+  q.partial = (double*)take(1024, 8);
+  q.scal = (double*)take(8, 8);
+  q.bytes = off;
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+wc_status wc_pcg_solve_csr_noalloc(int64_t n, const uint32_t* rowstart, const uint32_t* col,
+                                   const double* val, const double* rhs, double* result,
+                                   double tol, int32_t maxit, double* residual_out,
+                                   int32_t* iters_out) {
+  if (check_csr(n, rowstart, col) != WC_OK || (n > 0 && (!val || !rhs || !result)))
+    return fail(nullptr, WC_ERR_ARG, "invalid CSR matrix (rows must be sorted, columns < n)");
+  if (maxit < 0) return fail(nullptr, WC_ERR_ARG, "maxit < 0");
+  const int64_t nnz = n > 0 ? (int64_t)rowstart[n] : 0;
+  const size_t need = noalloc_layout(nullptr, n, nnz).bytes;
+  if (need > (size_t)WC_NOALLOC_SCRATCH_BYTES)
+    return fail(nullptr, WC_ERR_ARG,
+                "wc_pcg_solve_csr_noalloc: the system (" + std::to_string(n) + " rows, " +
+                    std::to_string(nnz) + " entries) needs " + std::to_string(need) +
+                    " bytes of device memory, more than the fixed " +
+                    std::to_string((size_t)WC_NOALLOC_SCRATCH_BYTES) +
+                    " (wc_pcg_solve_csr has no such limit)");
+  double res = 0;
+  int it = 0;
+  bool ok = true;
+  NoAllocDevice* d = nullptr;
+  try {
+    d = &noalloc_device();
+    if (n > 0) {
+      const NoAllocParts q = noalloc_layout(d->ptr, n, nnz);
+      const hipStream_t st = d->st;
+      WC_HIP(hipMemcpyAsync(q.rowptr, rowstart, sizeof(uint32_t) * (n + 1),
+                            hipMemcpyHostToDevice, st));
+      if (nnz) {
+        WC_HIP(hipMemcpyAsync(q.col, col, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
+        WC_HIP(hipMemcpyAsync(q.val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+      }
+      WC_HIP(hipMemcpyAsync(q.b, rhs, sizeof(double) * n, hipMemcpyHostToDevice, st));
+      wc::DevCsr A;
+      A.n = n;
+      A.rowptr = q.rowptr;
+      A.col = q.col;
+      A.val = q.val;
+      A.nnz_slots = nnz;
+      wc::PcgWork w;
+      w.no_alloc = true;
+      w.tpos.lend(q.tpos, nnz);
+      w.lv.lend(q.lv, nnz);
+      w.invdiag.lend(q.invdiag, n);
+      w.adiag.lend(q.adiag, n);
+      w.z.lend(q.z, n);
+      w.r.lend(q.r, n);
+      w.s.lend(q.s, n);
+      w.x.lend(q.x, n);
+      w.flags.lend(q.flags, 4);
+      w.level.lend(q.level, n);
+      w.lvl_count.lend(q.lvl_count, n);
+      w.lvl_cursor.lend(q.lvl_cursor, n);
+      w.lvl_list.lend(q.lvl_list, n);
+      w.partial.lend(q.partial, 1024);
+      w.scal.lend(q.scal, 8);
+      wc::pcg_prepare(st, w, A, nullptr, 0);
+      ok = wc::pcg_solve(st, w, A, q.b, q.x, tol, maxit, 0.97, 0.25, &res, &it);
+      WC_HIP(hipMemcpyAsync(result, q.x, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+      WC_HIP(hipStreamSynchronize(st));
+      w.release();
+    }
+  } catch (const HipError& e) {
+    if (d && d->st) (void)hipStreamSynchronize(d->st);
     return hip_fail(nullptr, e);
   }
   if (residual_out) *residual_out = res;

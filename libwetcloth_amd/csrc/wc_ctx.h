@@ -35,8 +35,18 @@ struct DevBuf {
   // A buffer that has to grow again (the DOF count and the sweep plan vary
   // from substep to substep) gets 25 % headroom, so that the free and
   // allocation, which stall the device, are rare.
+  bool lent = false;  // memory lent by the caller: never freed or regrown here
+  void lend(T* q, size_t n) {
+    release();
+    p = q;
+    cap = n;
+    lent = true;
+  }
   void ensure(size_t n) {
     if (n <= cap && p) return;
+    if (lent)
+      throw HipError{hipErrorOutOfMemory, "a lent buffer is smaller than the request", __FILE__,
+                     __LINE__};
     const bool regrow = p != nullptr;
     if (p) WC_HIP(hipFree(p));
     p = nullptr;
@@ -47,9 +57,10 @@ struct DevBuf {
     cap = c;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !lent) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    lent = false;
   }
 };
 
@@ -163,7 +174,9 @@ struct PcgWork {
   int32_t level_span = 0;
   bool levels_ready = false;
   int red_blocks = 0;
-  double* host_scal = nullptr;  // pinned
+  double* host_scal = nullptr;  // pinned, or host_scal_own when no_alloc
+  bool no_alloc = false;        // every buffer lent (DevBuf::lend); no pinned memory either
+  double host_scal_own[8] = {};
   int* host_flag = nullptr;     // mapped host memory: the sweeps' abort flag
   double* host_part = nullptr;  // mapped host memory, red_blocks |r| partials
   double* host_part_dev = nullptr;  // its device address

@@ -485,7 +485,7 @@ void PcgWork::release() {
   lvl_count.release();
   lvl_cursor.release();
   lvl_list.release();
-  if (host_scal) (void)hipHostFree(host_scal);
+  if (host_scal && host_scal != host_scal_own) (void)hipHostFree(host_scal);
   host_scal = nullptr;
   if (host_flag) (void)hipHostFree(host_flag);
   host_flag = nullptr;
@@ -513,6 +513,19 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
   w.x.ensure(n);
   w.red_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
   w.partial.ensure(w.red_blocks);
+  w.scal.ensure(S_NSCAL);
+  w.level_in = level;
+  w.level_span = level_span;
+  w.levels_ready = false;
+  w.nlevels = 0;
+  if (w.no_alloc) {
+    // lent buffers only: scalars come back through pageable host memory
+    // (the copy is followed by a stream synchronisation in read_scal); the
+    // box loops, which need mapped memory, are not used
+    static_assert(S_NSCAL <= sizeof(w.host_scal_own) / sizeof(double), "scalar slots");
+    w.host_scal = w.host_scal_own;
+    return;
+  }
   // written by the update kernels of the box loops, read by the host
   if (!w.host_part) {
     WC_HIP(hipHostMalloc((void**)&w.host_part, sizeof(double) * 1024,
@@ -520,17 +533,12 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
     WC_HIP(hipHostGetDevicePointer((void**)&w.host_part_dev, w.host_part, 0));
   }
   if (!w.res_event) WC_HIP(hipEventCreateWithFlags(&w.res_event, hipEventDisableTiming));
-  w.scal.ensure(S_NSCAL);
   if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
   if (!w.host_flag) {
     WC_HIP(hipHostMalloc((void**)&w.host_flag, sizeof(int),
                          hipHostMallocMapped | hipHostMallocCoherent));
     WC_HIP(hipHostGetDevicePointer((void**)&w.host_flag_dev, w.host_flag, 0));
   }
-  w.level_in = level;
-  w.level_span = level_span;
-  w.levels_ready = false;
-  w.nlevels = 0;
 }
 
 void pcg_prepare_vectors(hipStream_t st, PcgWork& w, int64_t n) {
