@@ -168,6 +168,64 @@ wc_status wc_step(wc_ctx* ctx, double dt, double tol, int32_t maxit,
  * prints them). */
 wc_status wc_step_substeps(const wc_ctx* ctx, int32_t* n);
 
+/* ---- the adjoint of the substep (no reference counterpart: the reference
+ * has no differentiation) ----
+ * The substep is differentiated with its discrete decisions held fixed: the
+ * activated buckets, each cell's liquid-phi particle (the particle whose
+ * distance sets the cell's phi, or the default or solid value), the liquid
+ * cells and with them the pressure DOFs, the theta clamps at 0.01, the
+ * faces with mass (M > 1e-20) and the particles the solid projection
+ * moves.  A substep run in mode 1 records them into a slot of the context,
+ * one in mode 2 replays a slot (so that a perturbed run is the same smooth
+ * map, as central differences need).  WC_FP64 contexts only. */
+/* The gravity vector of the following substeps (the scene's
+ * SimpleGravityForce, SimpleGravityForce.cpp:26-54, fixed at wc_create
+ * otherwise), e.g. to take differences in |g| on one context. */
+wc_status wc_set_gravity(wc_ctx* ctx, const double g[3]);
+#define WC_FROZEN_OFF 0
+#define WC_FROZEN_RECORD 1
+#define WC_FROZEN_REPLAY 2
+/* Mode and slot (0 <= slot < 2^20) for the following substeps.  Replaying a
+ * slot that holds no record of the same particle count is WC_ERR_STATE. */
+wc_status wc_set_frozen(wc_ctx* ctx, int32_t mode, int32_t slot);
+/* Decisions of replayed substeps that the run would have taken differently
+ * from the record (a liquid cell's sign, a face's mass above 1e-20, a theta
+ * clamp, a particle's solid projection), counted since the last
+ * wc_set_frozen with mode 0 or 1 or the last wc_rollout_forward.  A
+ * replayed face that has lost all its mass is taken as massless.  Zero
+ * means the replay is the run's own smooth neighbourhood. */
+wc_status wc_frozen_departures(wc_ctx* ctx, int64_t* count);
+/* The particle state (x, v, C) copied to (restore = 0) or from (1) a
+ * device-resident slot, in the caller's particle order. */
+wc_status wc_checkpoint(wc_ctx* ctx, int32_t slot, int32_t restore);
+/* The adjoint of the substep wc_substep has just run in mode 1 or 2
+ * (WC_ERR_STATE otherwise).  xb, vb, Cb (SoA in the caller's order, as
+ * wc_get_particles_soa: 3n, 3n, 9n; NULL = zero) hold dL/d(x, v, C after
+ * the substep) on entry and dL/d(x, v, C before it) on return; *gb is
+ * increased by dL/d|g| for gravity (0, -|g|, 0).  The adjoint pressure
+ * solve is the forward's MIC(0)-PCG on the same symmetric matrix, to tol;
+ * WC_NOT_CONVERGED if it stops at maxit. */
+wc_status wc_substep_adjoint(wc_ctx* ctx, double dt, double tol, int32_t maxit, double* xb,
+                             double* vb, double* Cb, double* gb, double* residual_out,
+                             int32_t* iters_out);
+/* A rollout of `steps` substeps of dt from the current particles, the
+ * forward half of a forward/backward pair: with replay = 0 it records each
+ * substep's decisions (slot k for substep k) and checkpoints the particle
+ * state before every `every`-th substep; with replay = 1 it replays the
+ * recorded decisions (the same `steps`), e.g. from perturbed particles.
+ * iters_out: PCG iterations over all substeps. */
+wc_status wc_rollout_forward(wc_ctx* ctx, int32_t steps, double dt, double tol,
+                             int32_t maxit, int32_t every, int32_t replay,
+                             int32_t* iters_out);
+/* The backward half: xb, vb, Cb hold dL/d(final x, v, C) on entry and
+ * dL/d(initial x, v, C) on return (layout as wc_substep_adjoint), *gb is
+ * increased by dL/d|g|.  Each segment of `every` substeps is recomputed from
+ * its checkpoint (every state of the segment kept on the device), then
+ * differentiated substep by substep in reverse.  On return the particles
+ * hold the rollout's initial state. */
+wc_status wc_rollout_backward(wc_ctx* ctx, double dt, double tol, int32_t maxit, double* xb,
+                              double* vb, double* Cb, double* gb, int32_t* iters_out);
+
 /* ---- diagnostics (downloads for parity checks) ---- */
 #define WC_FIELD_ACT 0       /* uint8 per bucket: activated */
 #define WC_FIELD_LPHI 1      /* double per node: liquid phi at cell centres */

@@ -111,6 +111,16 @@ def lib():
         L.wc_pcg_solve_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, C.c_double,
                                        C.c_int32, _dp, _i32p]
         L.wc_pcg_solve_csr_noalloc.argtypes = L.wc_pcg_solve_csr.argtypes
+        L.wc_set_gravity.argtypes = [vp, _dp]
+        L.wc_set_frozen.argtypes = [vp, C.c_int32, C.c_int32]
+        L.wc_checkpoint.argtypes = [vp, C.c_int32, C.c_int32]
+        L.wc_frozen_departures.argtypes = [vp, _i64p]
+        L.wc_substep_adjoint.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _dp, _dp,
+                                         _dp, _dp, _i32p]
+        L.wc_rollout_forward.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                         C.c_int32, C.c_int32, _i32p]
+        L.wc_rollout_backward.argtypes = [vp, C.c_double, C.c_double, C.c_int32, _dp, _dp, _dp,
+                                          _dp, _i32p]
         L.wc_mic0_factor_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, C.c_double, C.c_double,
                                          _dp, _dp]
         L.wc_mic0_apply_csr.argtypes = [C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp]
@@ -307,6 +317,66 @@ class Scene:
         n = C.c_int32()
         _check(lib().wc_step_substeps(self.h, C.byref(n)), self.h)
         return n.value
+
+    def set_gravity(self, g):
+        """The gravity vector of the following substeps."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        _check(lib().wc_set_gravity(self.h, _ptr(g)), self.h)
+
+    # ---- the adjoint (no reference counterpart) ----
+    def set_frozen(self, mode, slot=0):
+        """Record (1) the next substeps' decisions into slot, replay (2) a
+        recorded slot, or neither (0)."""
+        _check(lib().wc_set_frozen(self.h, int(mode), int(slot)), self.h)
+
+    def frozen_departures(self) -> int:
+        """Replayed decisions the run would have taken differently."""
+        n = C.c_int64()
+        _check(lib().wc_frozen_departures(self.h, C.byref(n)), self.h)
+        return n.value
+
+    def checkpoint(self, slot, restore=False):
+        """Save the particle state to a device slot, or restore it."""
+        _check(lib().wc_checkpoint(self.h, int(slot), 1 if restore else 0), self.h)
+
+    def _grads(self, xb, vb, Cb):
+        n = self.num_particles()
+        out = []
+        for a, rows in ((xb, 3), (vb, 3), (Cb, 9)):
+            out.append(np.zeros((rows, n)) if a is None
+                       else np.array(a, dtype=np.float64, order="C").reshape(rows, n))
+        return out
+
+    def substep_adjoint(self, dt, xb, vb=None, Cb=None, tol=1e-12, maxit=5000):
+        """Adjoint of the substep just run in mode 1 or 2: from dL/d(x, v, C)
+        after it, (converged, dL/dx, dL/dv, dL/dC before it, dL/d|g|)."""
+        xb, vb, Cb = self._grads(xb, vb, Cb)
+        gb = C.c_double(0.0)
+        res = C.c_double()
+        it = C.c_int32()
+        st = lib().wc_substep_adjoint(self.h, dt, tol, maxit, _ptr(xb), _ptr(vb), _ptr(Cb),
+                                      C.byref(gb), C.byref(res), C.byref(it))
+        _check(st, self.h, allow_nc=True)
+        return st == WC_OK, xb, vb, Cb, gb.value
+
+    def rollout_forward(self, steps, dt, tol=1e-12, maxit=5000, every=10, replay=False):
+        """`steps` substeps recording their decisions (checkpoints every
+        `every`), or replaying the recorded ones: (converged, iterations)."""
+        it = C.c_int32()
+        st = lib().wc_rollout_forward(self.h, int(steps), dt, tol, maxit, int(every),
+                                      1 if replay else 0, C.byref(it))
+        _check(st, self.h, allow_nc=True)
+        return st == WC_OK, it.value
+
+    def rollout_backward(self, dt, xb, vb=None, Cb=None, tol=1e-12, maxit=5000):
+        """From dL/d(final x, v, C): (converged, dL/dx0, dL/dv0, dL/dC0, dL/d|g|)."""
+        xb, vb, Cb = self._grads(xb, vb, Cb)
+        gb = C.c_double(0.0)
+        it = C.c_int32()
+        st = lib().wc_rollout_backward(self.h, dt, tol, maxit, _ptr(xb), _ptr(vb), _ptr(Cb),
+                                       C.byref(gb), C.byref(it))
+        _check(st, self.h, allow_nc=True)
+        return st == WC_OK, xb, vb, Cb, gb.value
 
     # ---- diagnostics ----
     def grid_info(self):

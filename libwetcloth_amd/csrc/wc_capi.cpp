@@ -138,6 +138,7 @@ void release_all(wc_ctx* c) {
   c->amg.release();
   c->dof_ijk.release();
   c->scan_tmp.release();
+  wc::adjoint_release(c);
 }
 
 wc::DevCsr system_csr(const wc_ctx* c) {
@@ -442,6 +443,9 @@ bool substep(wc_ctx* c, double dt, double tol, int maxit, double* res, int* it) 
   wc::particles_g2p_advect(c, dt);
   record(c, 7);
   WC_HIP(hipStreamSynchronize(c->stream));
+  // the adjoint of this substep may follow (its decisions are recorded or
+  // replayed and its start state kept)
+  c->adj.ready = c->frz_mode != 0 && c->adj.have_start;
   for (int i = 0; i < 7; ++i) {
     float ms = 0.f;
     WC_HIP(hipEventElapsedTime(&ms, c->events.ev[i], c->events.ev[i + 1]));
@@ -495,6 +499,220 @@ wc_status wc_step_substeps(const wc_ctx* c, int32_t* n) {
   if (!c || !n) return WC_ERR_ARG;
   *n = c->last_substeps;
   return WC_OK;
+}
+
+}  // extern "C"
+
+// ---- the adjoint ----
+
+namespace {
+
+constexpr int kMaxSlots = 1 << 20;
+
+// record (mode 1) into or replay (mode 2) from slot; mode 0 turns both off
+wc_status set_frozen(wc_ctx* c, int mode, int slot) {
+  if (mode < 0 || mode > 2 || slot < 0 || slot >= kMaxSlots)
+    return fail(c, WC_ERR_ARG, "mode must be 0, 1 or 2 and slot in [0, 2^20)");
+  if (mode != 0 && c->mixed)
+    return fail(c, WC_ERR_STATE, "recorded decisions and the adjoint need a WC_FP64 context");
+  if (mode == 2 && ((size_t)slot >= c->frz.size() || !c->frz[(size_t)slot].recorded ||
+                    c->frz[(size_t)slot].np != c->np))
+    return fail(c, WC_ERR_STATE,
+                "slot " + std::to_string(slot) + " holds no recorded substep of these particles");
+  if (mode != 2 && c->frz_dep.p)
+    WC_HIP(hipMemsetAsync(c->frz_dep.p, 0, sizeof(unsigned long long), c->stream));
+  c->frz_mode = mode;
+  c->frz_slot = slot;
+  if (mode == 1) {
+    wc::FrozenRecord& r = wc::frozen_record(c);
+    WC_HIP(hipMemsetAsync(r.proj.p, 0, std::max<int64_t>(1, c->np), c->stream));
+    r.recorded = true;
+    r.np = c->np;
+  }
+  return WC_OK;
+}
+
+void upload_grad(wc_ctx* c, const double* xb, const double* vb, const double* Cb) {
+  wc::AdjointWork& a = c->adj;
+  const int64_t n = c->np;
+  a.gx.ensure(std::max<int64_t>(1, 3 * n));
+  a.gv.ensure(std::max<int64_t>(1, 3 * n));
+  a.gC.ensure(std::max<int64_t>(1, 9 * n));
+  a.n = n;
+  hipStream_t st = c->stream;
+  auto up = [&](double* dst, const double* src, int64_t cnt) {
+    if (src)
+      WC_HIP(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, st));
+    else
+      WC_HIP(hipMemsetAsync(dst, 0, sizeof(double) * cnt, st));
+  };
+  if (n > 0) {
+    up(a.gx.p, xb, 3 * n);
+    up(a.gv.p, vb, 3 * n);
+    up(a.gC.p, Cb, 9 * n);
+  }
+}
+
+void download_grad(wc_ctx* c, double* xb, double* vb, double* Cb) {
+  const wc::AdjointWork& a = c->adj;
+  const int64_t n = a.n;
+  hipStream_t st = c->stream;
+  if (n > 0) {
+    if (xb) WC_HIP(hipMemcpyAsync(xb, a.gx.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    if (vb) WC_HIP(hipMemcpyAsync(vb, a.gv.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    if (Cb) WC_HIP(hipMemcpyAsync(Cb, a.gC.p, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, st));
+  }
+  WC_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+extern "C" {
+
+wc_status wc_set_gravity(wc_ctx* c, const double g[3]) {
+  if (!ready(c) || !g) return fail(c, WC_ERR_ARG, "bad argument");
+  for (int d = 0; d < 3; ++d) c->desc.gravity[d] = g[d];
+  return WC_OK;
+}
+
+wc_status wc_set_frozen(wc_ctx* c, int32_t mode, int32_t slot) {
+  if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
+  wc_status st = WC_OK;
+  WC_TRY(c, { st = set_frozen(c, mode, slot); })
+  return st;
+}
+
+wc_status wc_frozen_departures(wc_ctx* c, int64_t* count) {
+  if (!ready(c) || !count) return fail(c, WC_ERR_ARG, "bad argument");
+  unsigned long long n = 0;
+  WC_TRY(c, {
+    if (c->frz_dep.p)
+      WC_HIP(hipMemcpyAsync(&n, c->frz_dep.p, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  *count = (int64_t)n;
+  return WC_OK;
+}
+
+wc_status wc_checkpoint(wc_ctx* c, int32_t slot, int32_t restore) {
+  if (!ready(c) || slot < 0 || slot >= kMaxSlots) return fail(c, WC_ERR_ARG, "bad argument");
+  if (c->mixed) return fail(c, WC_ERR_STATE, "checkpoints need a WC_FP64 context");
+  if (restore && (size_t)slot >= c->ckpt.size())
+    return fail(c, WC_ERR_STATE, "no checkpoint in slot " + std::to_string(slot));
+  WC_TRY(c, {
+    if (restore)
+      wc::checkpoint_restore(c, slot);
+    else
+      wc::checkpoint_save(c, slot);
+    WC_HIP(hipStreamSynchronize(c->stream));
+  })
+  return WC_OK;
+}
+
+wc_status wc_substep_adjoint(wc_ctx* c, double dt, double tol, int32_t maxit, double* xb,
+                             double* vb, double* Cb, double* gb, double* residual_out,
+                             int32_t* iters_out) {
+  if (!ready(c) || maxit < 0) return fail(c, WC_ERR_ARG, "bad argument");
+  if (!c->adj.ready)
+    return fail(c, WC_ERR_STATE,
+                "wc_substep_adjoint must follow wc_substep run with its decisions recorded or "
+                "replayed (wc_set_frozen mode 1 or 2)");
+  bool ok = true;
+  double res = 0, g = 0;
+  int it = 0;
+  WC_TRY(c, {
+    upload_grad(c, xb, vb, Cb);
+    ok = wc::adjoint_substep(c, dt, tol, maxit, &g, &res, &it);
+    download_grad(c, xb, vb, Cb);
+  })
+  if (gb) *gb += g;
+  if (residual_out) *residual_out = res;
+  if (iters_out) *iters_out = it;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_rollout_forward(wc_ctx* c, int32_t steps, double dt, double tol, int32_t maxit,
+                             int32_t every, int32_t replay, int32_t* iters_out) {
+  if (!ready(c) || steps < 0 || steps >= kMaxSlots || every <= 0 || maxit < 0)
+    return fail(c, WC_ERR_ARG, "bad argument");
+  if (c->mixed) return fail(c, WC_ERR_STATE, "the rollout needs a WC_FP64 context");
+  if (replay && steps != c->roll_steps)
+    return fail(c, WC_ERR_STATE, "a replay must have the recorded rollout's length");
+  bool ok = true;
+  int total = 0;
+  wc_status st = WC_OK;
+  WC_TRY(c, {
+    if (!replay) {
+      c->roll_steps = 0;  // valid again once the run is recorded
+      c->roll_every = every;
+    }
+    if (c->frz_dep.p)
+      WC_HIP(hipMemsetAsync(c->frz_dep.p, 0, sizeof(unsigned long long), c->stream));
+    for (int k = 0; k < steps && st == WC_OK; ++k) {
+      if (!replay && k % every == 0) wc::checkpoint_save(c, k / every);
+      st = set_frozen(c, replay ? 2 : 1, k);
+      if (st != WC_OK) break;
+      double r = 0;
+      int i = 0;
+      if (!substep(c, dt, tol, maxit, &r, &i)) ok = false;
+      total += i;
+    }
+    c->frz_mode = 0;
+    if (!replay && st == WC_OK) c->roll_steps = steps;
+  })
+  if (st != WC_OK) return st;
+  if (iters_out) *iters_out = total;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
+}
+
+wc_status wc_rollout_backward(wc_ctx* c, double dt, double tol, int32_t maxit, double* xb,
+                              double* vb, double* Cb, double* gb, int32_t* iters_out) {
+  if (!ready(c) || maxit < 0) return fail(c, WC_ERR_ARG, "bad argument");
+  const int steps = c->roll_steps, every = c->roll_every;
+  if (steps <= 0) return fail(c, WC_ERR_STATE, "no recorded rollout: call wc_rollout_forward");
+  bool ok = true;
+  int total = 0;
+  double g = 0;
+  wc_status st = WC_OK;
+  WC_TRY(c, {
+    const int nseg = (steps + every - 1) / every;
+    wc::checkpoint_restore(c, 0);
+    upload_grad(c, xb, vb, Cb);
+    for (int seg = nseg - 1; seg >= 0 && st == WC_OK; --seg) {
+      const int c0 = seg * every, c1 = std::min(steps, c0 + every);
+      // the segment's start states, from its checkpoint
+      wc::checkpoint_restore(c, seg);
+      for (int k = c0; k < c1 && st == WC_OK; ++k) {
+        wc::checkpoint_save(c, nseg + (k - c0));
+        if (k + 1 == c1) break;
+        st = set_frozen(c, 2, k);
+        double r = 0;
+        int i = 0;
+        if (st == WC_OK) substep(c, dt, tol, maxit, &r, &i);
+      }
+      // each substep again from its start state, then its adjoint
+      for (int k = c1 - 1; k >= c0 && st == WC_OK; --k) {
+        wc::checkpoint_restore(c, nseg + (k - c0));
+        st = set_frozen(c, 2, k);
+        if (st != WC_OK) break;
+        double r = 0;
+        int i = 0;
+        substep(c, dt, tol, maxit, &r, &i);
+        if (!wc::adjoint_substep(c, dt, tol, maxit, &g, &r, &i)) ok = false;
+        total += i;
+      }
+    }
+    c->frz_mode = 0;
+    if (st == WC_OK) {
+      download_grad(c, xb, vb, Cb);
+      wc::checkpoint_restore(c, 0);
+      WC_HIP(hipStreamSynchronize(c->stream));
+    }
+  })
+  if (st != WC_OK) return st;
+  if (gb) *gb += g;
+  if (iters_out) *iters_out = total;
+  return ok ? WC_OK : WC_NOT_CONVERGED;
 }
 
 wc_status wc_grid_info(const wc_ctx* c, int32_t nx[3], int64_t* nnode,

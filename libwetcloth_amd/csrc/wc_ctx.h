@@ -236,6 +236,39 @@ struct PhaseEvents {
   bool have = false;
 };
 
+// The adjoint (wc_adjoint.hip; the reference has no counterpart).  The
+// decisions of one recorded substep (FrozenView): act per bucket; per node
+// liquid, flags, and arg = the caller index of the particle whose distance
+// gives the cell's liquid phi (-1: the default three buckets, -2: the solid
+// override); proj per caller index.
+struct FrozenRecord {
+  DevBuf<uint8_t> act, liquid, flags, proj;
+  DevBuf<int32_t> arg;
+  bool recorded = false;
+  int64_t np = 0;  // particles of the recorded substep
+  void release();
+};
+// a particle state in the caller's order (x[d*n+p], C[t*n+p])
+struct ParticleCheckpoint {
+  DevBuf<double> x, v, C;
+  int64_t n = 0;
+  void release();
+};
+// device state of the adjoint substep
+struct AdjointWork {
+  DevBuf<uint32_t> slot_of, argmin;       // caller index -> slot; per node scratch
+  DevBuf<double> x0, v0, C0;              // the substep's start state, slot order
+  DevBuf<double> gx, gv, gC;              // dL/d(particle state), caller order
+  DevBuf<double> ax;                      // dL/dx0 of the G2P part, slot order
+  DevBuf<double> ub[3], ub2[3], ub0[3];   // face gradients
+  DevBuf<uint8_t> code[3], cnt[3];        // extrapolation layers per face
+  DevBuf<double> phib, pb, bb, partial;   // per cell, per DOF, reduction partials
+  int64_t n = 0;                          // particles of the gradient state
+  bool have_start = false;                // x0.. belong to the last substep
+  bool ready = false;                     // the last call ran a whole substep in mode 1 or 2
+  void release();
+};
+
 }  // namespace wc
 
 struct wc_ctx {
@@ -293,6 +326,16 @@ struct wc_ctx {
   wc::DevBuf<int32_t> dof_ijk;
   double phase_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   wc::PhaseEvents events;
+
+  // the adjoint: decisions recorded (frz_mode 1) into or replayed (2) from
+  // frz[frz_slot]; particle checkpoints; the adjoint substep's state; the
+  // last rollout's length and checkpoint spacing
+  int frz_mode = 0, frz_slot = 0;
+  std::vector<wc::FrozenRecord> frz;
+  wc::DevBuf<unsigned long long> frz_dep;  // departures of the replayed substeps
+  std::vector<wc::ParticleCheckpoint> ckpt;
+  wc::AdjointWork adj;
+  int roll_steps = 0, roll_every = 0;
 };
 
 namespace wc {
@@ -324,6 +367,29 @@ void particles_set_soa(wc_ctx* ctx, int64_t n, const double* x, const double* v,
                        const double* C, bool aos_ref_layout);
 void particles_get_soa(wc_ctx* ctx, double* x, double* v, double* C,
                        bool aos_ref_layout);
+
+// ---- wc_adjoint.hip ----
+// the decisions of the current mode and record (mode 0: all null)
+FrozenView frozen_view(wc_ctx* ctx);
+// the record of the current slot, sized for the grid and particles
+FrozenRecord& frozen_record(wc_ctx* ctx);
+// mode 2: the liquid phi of every cell from its recorded particle
+void frozen_replay_lphi(wc_ctx* ctx);
+// mode 1: the particle whose distance gave each cell's liquid phi
+void frozen_record_lphi_arg(wc_ctx* ctx);
+// modes 1 and 2, after the substep's sort: the slot of every caller index and
+// a copy of the start state
+void frozen_after_sort(wc_ctx* ctx);
+// the adjoint of the substep just run (mode 1 or 2) on the device gradient
+// state: gx, gv, gC go from dL/d(state after) to dL/d(state before); *gb
+// accumulates dL/d|g| for gravity (0, -|g|, 0).  Returns whether the
+// adjoint pressure solve converged.
+bool adjoint_substep(wc_ctx* ctx, double dt, double tol, int maxit, double* gb,
+                     double* residual_out, int* iters_out);
+// device copies of the particle state in the caller's order (FP64 contexts)
+void checkpoint_save(wc_ctx* ctx, int slot);
+void checkpoint_restore(wc_ctx* ctx, int slot);
+void adjoint_release(wc_ctx* ctx);
 
 // ---- wc_grid.hip ----
 void grid_activate(wc_ctx* ctx);

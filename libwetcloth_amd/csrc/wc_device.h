@@ -258,6 +258,61 @@ __host__ __device__ inline R container_phi_r(const Grid& g, R p0, R p1, R p2) {
   return -phi;
 }
 
+// The discrete decisions of a substep that the adjoint holds fixed, as the
+// kernels see them: mode 0 computes every decision, 1 computes and records
+// them, 2 replays recorded ones.  liquid: per node, the cell is liquid
+// (phi < 0); flags: per node, bit d = face d has mass (M > 1e-20), bit 3 + d
+// = theta of face d clamped at the theta criterion; proj: per caller index,
+// the solid projection moved the particle; pid: the caller index of each
+// particle slot; dep (mode 2): a count of the decisions the run would have
+// taken differently from the record.
+struct FrozenView {
+  int mode = 0;
+  uint8_t* liquid = nullptr;
+  uint8_t* flags = nullptr;
+  uint8_t* proj = nullptr;
+  const uint32_t* pid = nullptr;
+  unsigned long long* dep = nullptr;
+};
+__device__ inline void fz_depart(const FrozenView& F) {
+  if (F.dep) atomicAdd(F.dep, 1ull);
+}
+__device__ inline bool fz_liquid(const FrozenView& F, const double* lphi, int64_t n) {
+  return F.mode == 2 ? F.liquid[n] != 0 : lphi[n] < 0.0;
+}
+// A replayed face with mass that has lost all of it (a departure) is taken
+// as massless rather than dividing by zero.
+__device__ inline bool fz_mass(const FrozenView& F, int d, int64_t n, double M) {
+  return F.mode == 2 ? ((F.flags[n] >> d) & 1) != 0 && M > 0.0 : M > 1e-20;
+}
+constexpr double kThetaCriterion = 0.01;  // Pressure.cpp:20
+// theta of a face from the two cells' phis and liquid flags (fraction_inside,
+// MathUtilities.h:286-293, clamped at the criterion); mode 2 takes the clamp
+// decision from flags (and counts a departure if count is set), mode 1
+// records it there
+__device__ inline double fz_theta(const FrozenView& F, int d, int64_t fn, bool ll, bool lr,
+                                  double phil, double phir, uint8_t* clamp_bits,
+                                  bool count = false) {
+  double th;
+  if (ll && lr)
+    th = 1.0;
+  else if (ll)
+    th = phil / (phil - phir);
+  else if (lr)
+    th = phir / (phir - phil);
+  else
+    th = 0.0;
+  bool clamp;
+  if (F.mode == 2) {
+    clamp = ((F.flags[fn] >> (3 + d)) & 1) != 0;
+    if (count && clamp != (th < kThetaCriterion)) fz_depart(F);
+  } else {
+    clamp = th < kThetaCriterion;
+  }
+  if (clamp_bits && clamp) *clamp_bits |= (uint8_t)(8u << d);
+  return clamp ? kThetaCriterion : th;
+}
+
 // Counter-based RNG (splitmix64 finaliser chained over the counter words).
 __host__ __device__ inline uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;

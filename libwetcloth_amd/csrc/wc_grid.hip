@@ -10,8 +10,6 @@
 namespace wc {
 namespace {
 
-constexpr double kThetaCriterion = 0.01;  // Pressure.cpp:20
-
 inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // expandFluidNodesMarked(1) (TwoDScene.cpp:5021-5064): a bucket is active if
@@ -55,6 +53,7 @@ struct NodeArgs {
   double* w[3];
   const double* mass[3];
   double* mom[3];  // momentum in, velocity out
+  FrozenView F;    // mode 2: lphi is already the replayed one
 };
 
 // updateSolidWeights (TwoDScene.cpp:1871-1962), updateLiquidPhi's solid
@@ -65,10 +64,14 @@ __global__ void k_grid_nodes(NodeArgs A) {
   if (n >= g.nnode) return;
   const double bucket_size = kBucket * g.dx;
   if (!A.act[n >> 6]) {
-    A.lphi[n] = 3.0 * bucket_size;
+    if (A.F.mode != 2) A.lphi[n] = 3.0 * bucket_size;
     for (int d = 0; d < 3; ++d) {
       A.w[d][n] = 0.0;
       A.mom[d][n] = 0.0;
+    }
+    if (A.F.mode == 1) {
+      A.F.liquid[n] = 0;
+      A.F.flags[n] = 0;
     }
     return;
   }
@@ -87,18 +90,28 @@ __global__ void k_grid_nodes(NodeArgs A) {
                                               corner_phi_w(g, act, i, j + 1, k),
                                               corner_phi_w(g, act, i + 1, j, k),
                                               corner_phi_w(g, act, i + 1, j + 1, k)));
-  const unsigned long long key = A.phikey[n];
-  double phi = key == 0xFFFFFFFFFFFFFFFFull ? 3.0 * bucket_size : phi_from_key(key);
-  // the stencil minimum never exceeds the default (phi < 3 buckets there)
-  if (phi > 3.0 * bucket_size) phi = 3.0 * bucket_size;
-  if (container_phi(g, node_pos(g, i, 0.5), node_pos(g, j, 0.5),
-                    node_pos(g, k, 0.5)) < 0.0)
-    phi = -0.5 * g.dx;
-  A.lphi[n] = phi;
+  if (A.F.mode != 2) {
+    const unsigned long long key = A.phikey[n];
+    double phi = key == 0xFFFFFFFFFFFFFFFFull ? 3.0 * bucket_size : phi_from_key(key);
+    // the stencil minimum never exceeds the default (phi < 3 buckets there)
+    if (phi > 3.0 * bucket_size) phi = 3.0 * bucket_size;
+    if (container_phi(g, node_pos(g, i, 0.5), node_pos(g, j, 0.5),
+                      node_pos(g, k, 0.5)) < 0.0)
+      phi = -0.5 * g.dx;
+    A.lphi[n] = phi;
+    if (A.F.mode == 1) A.F.liquid[n] = phi < 0.0;
+  } else if ((A.F.liquid[n] != 0) != (A.lphi[n] < 0.0)) {
+    fz_depart(A.F);
+  }
+  uint8_t has = 0;
   for (int d = 0; d < 3; ++d) {
     const double M = A.mass[d][n];
-    A.mom[d][n] = M > 1e-20 ? A.mom[d][n] / M : 0.0;
+    const bool hm = fz_mass(A.F, d, n, M);
+    if (A.F.mode == 2 && (((A.F.flags[n] >> d) & 1) != 0) != (M > 1e-20)) fz_depart(A.F);
+    has |= (uint8_t)(hm ? 1u << d : 0u);
+    A.mom[d][n] = hm ? A.mom[d][n] / M : 0.0;
   }
+  if (A.F.mode == 1) A.F.flags[n] = has;  // the theta clamps are added later
 }
 
 struct StepArgs {
@@ -111,6 +124,7 @@ struct StepArgs {
   double* ustar[3];
   double gravity[3];
   double dt;
+  FrozenView F;
 };
 
 // stepVelocity for liquid without solids (LinearizedImplicitEuler.cpp:
@@ -134,8 +148,9 @@ __global__ void k_step_velocity(StepArgs A) {
     A.rhs[d][n] = rhs;
     const double Q = 1.0, Ms = 0.0;
     const double Cc = M + Q * Ms;
-    A.invC[d][n] = Cc > 1e-20 ? 1.0 / Cc : 1.0;
-    A.ustar[d][n] = M > 1e-20 ? rhs / M : rhs;
+    const bool hm = fz_mass(A.F, d, n, M);  // M > 1e-20, and so Cc
+    A.invC[d][n] = hm ? 1.0 / Cc : 1.0;
+    A.ustar[d][n] = hm ? rhs / M : rhs;
   }
 }
 
@@ -152,6 +167,7 @@ struct DofArgs {
   const double* lphi;
   const double* w[3];
   uint32_t* isdof;
+  FrozenView F;
 };
 
 // Pressure.cpp:348-392: a cell is a DOF if phi < 0 and one of its existing
@@ -161,7 +177,7 @@ __global__ void k_dof_flags(DofArgs A) {
   const Grid& g = A.g;
   if (n >= g.nnode) return;
   uint32_t f = 0;
-  if (A.act[n >> 6] && A.lphi[n] < 0) {
+  if (A.act[n >> 6] && fz_liquid(A.F, A.lphi, n)) {
     int i, j, k;
     node_coords(g, n, i, j, k);
     for (int face = 0; face < 6 && !f; ++face) {
@@ -193,6 +209,7 @@ struct AsmArgs {
   int32_t* level;
   int32_t level_min;
   double dt;
+  FrozenView F;
 };
 
 __device__ inline void insert_sorted(uint32_t* cols, double* vals, int& len,
@@ -260,14 +277,15 @@ __global__ void k_assemble(AsmArgs A) {
       if (!node_active(g, A.act, nb[0], nb[1], nb[2])) continue;
       const int64_t nn = node_index(g, nb[0], nb[1], nb[2]);
       const double nphi = A.lphi[nn];
-      if (nphi < 0.0) {
+      if (fz_liquid(A.F, A.lphi, nn)) {
         if (A.isdof[nn]) {
           insert_sorted(cols, vals, len, me, term);
           insert_sorted(cols, vals, len, A.dofscan[nn], -term);
         }
       } else {
-        double theta = fraction_inside2(center_phi, nphi);
-        if (theta < kThetaCriterion) theta = kThetaCriterion;
+        // the face between this (liquid) cell and the air cell nn
+        const double theta = hi ? fz_theta(A.F, axis, fn, true, false, center_phi, nphi, nullptr)
+                                : fz_theta(A.F, axis, fn, false, true, nphi, center_phi, nullptr);
         insert_sorted(cols, vals, len, me, term / theta);
       }
     }
@@ -301,6 +319,7 @@ struct ApplyArgs {
   double* u[3];
   uint8_t* valid[3];
   double dt;
+  FrozenView F;
 };
 
 // applyPressureGradsFluidRHS (Pressure.cpp:1137-1314) after addSolidDragRHS
@@ -319,6 +338,7 @@ __global__ void k_apply_pressure(ApplyArgs A) {
   int i, j, k;
   node_coords(g, n, i, j, k);
   const double dx = g.dx;
+  uint8_t clamps = 0;
   for (int d = 0; d < 3; ++d) {
     const double Q = 1.0, Ms = 0.0, us = 0.0, Vs = 0.0;
     double rhs = A.rhs[d][n];
@@ -331,12 +351,11 @@ __global__ void k_apply_pressure(ApplyArgs A) {
       const int64_t ln = lex ? node_index(g, l[0], l[1], l[2]) : -1;
       const double left_phi = lex ? A.lphi[ln] : 0.0;
       const double right_phi = A.lphi[n];
-      if (left_phi < 0.0 || right_phi < 0.0) {
+      const bool lliq = lex && fz_liquid(A.F, A.lphi, ln), rliq = fz_liquid(A.F, A.lphi, n);
+      if (lliq || rliq) {
         double theta = 1.;
-        if (right_phi >= 0.0 || left_phi >= 0.0) {
-          theta = fraction_inside2(left_phi, right_phi);
-          if (theta < kThetaCriterion) theta = kThetaCriterion;
-        }
+        if (!lliq || !rliq)
+          theta = fz_theta(A.F, d, n, lliq, rliq, left_phi, right_phi, &clamps, true);
         double pg = 0;
         if (lex) pg -= A.pressure[ln];
         pg += A.pressure[n];
@@ -349,6 +368,7 @@ __global__ void k_apply_pressure(ApplyArgs A) {
     A.u[d][n] = rhs * A.invC[d][n];
     A.valid[d][n] = valid;
   }
+  if (A.F.mode == 1) A.F.flags[n] = (uint8_t)((A.F.flags[n] & 7u) | clamps);
 }
 
 struct ExtrapArgs {
@@ -412,13 +432,23 @@ __global__ void k_extrapolate(ExtrapArgs A) {
 
 void grid_activate(wc_ctx* ctx) {
   const Grid& g = ctx->g;
+  if (ctx->frz_mode == 2) {  // the recorded activation
+    WC_HIP(hipMemcpyAsync(ctx->act.p, frozen_record(ctx).act.p, g.nbucket,
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    return;
+  }
   hipLaunchKernelGGL(k_activate, dim3(blocks(g.nbucket, 256)), dim3(256), 0,
                      ctx->stream, g, ctx->touched.p, ctx->act.p);
   WC_LAUNCH_CHECK();
+  if (ctx->frz_mode == 1)
+    WC_HIP(hipMemcpyAsync(frozen_record(ctx).act.p, ctx->act.p, g.nbucket,
+                          hipMemcpyDeviceToDevice, ctx->stream));
 }
 
 void grid_nodes(wc_ctx* ctx) {
+  if (ctx->frz_mode == 2) frozen_replay_lphi(ctx);
   NodeArgs A;
+  A.F = frozen_view(ctx);
   A.g = ctx->g;
   A.act = ctx->act.p;
   A.phikey = ctx->phikey.p;
@@ -431,6 +461,7 @@ void grid_nodes(wc_ctx* ctx) {
   hipLaunchKernelGGL(k_grid_nodes, dim3(blocks(ctx->g.nnode, 256)), dim3(256),
                      0, ctx->stream, A);
   WC_LAUNCH_CHECK();
+  if (ctx->frz_mode == 1) frozen_record_lphi_arg(ctx);
 }
 
 void grid_step_velocity(wc_ctx* ctx, double dt) {
@@ -446,6 +477,7 @@ void grid_step_velocity(wc_ctx* ctx, double dt) {
     A.gravity[d] = ctx->desc.gravity[d];
   }
   A.dt = dt;
+  A.F = frozen_view(ctx);
   hipLaunchKernelGGL(k_step_velocity, dim3(blocks(ctx->g.nnode, 256)),
                      dim3(256), 0, ctx->stream, A);
   WC_LAUNCH_CHECK();
@@ -462,6 +494,7 @@ int64_t grid_assemble(wc_ctx* ctx, double dt) {
   D.lphi = ctx->lphi.p;
   for (int d = 0; d < 3; ++d) D.w[d] = ctx->w[d].p;
   D.isdof = ctx->isdof.p;
+  D.F = frozen_view(ctx);
   hipLaunchKernelGGL(k_dof_flags, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, D);
   WC_LAUNCH_CHECK();
   WC_HIP(hipMemsetAsync(ctx->isdof.p + g.nnode, 0, sizeof(uint32_t), st));
@@ -498,6 +531,7 @@ int64_t grid_assemble(wc_ctx* ctx, double dt) {
     A.level = ctx->dof_level.p;
     A.level_min = ctx->level_min;
     A.dt = dt;
+    A.F = frozen_view(ctx);
     hipLaunchKernelGGL(k_assemble, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, A);
     WC_LAUNCH_CHECK();
   }
@@ -554,6 +588,7 @@ void grid_apply_pressure(wc_ctx* ctx, double dt) {
     A.valid[d] = ctx->valid[d].p;
   }
   A.dt = dt;
+  A.F = frozen_view(ctx);
   hipLaunchKernelGGL(k_apply_pressure, dim3(blocks(g.nnode, 256)), dim3(256), 0,
                      st, A);
   WC_LAUNCH_CHECK();

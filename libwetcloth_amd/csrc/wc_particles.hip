@@ -303,6 +303,7 @@ struct G2PArgs {
   const uint32_t* bstart;  // bucket ranges of the sorted particles (k_g2p_bucket)
   R dt, flip, invD;
   int solid_projection;
+  FrozenView F;  // mode 1 records, mode 2 replays which particles the projection moves
 };
 
 // mapNodeParticlesAPIC fluid branch (TwoDScene.cpp:6958-7019), advectScene
@@ -323,9 +324,10 @@ struct G2PArgs {
 // fv dt; nx is moved out of the solid when phi_now < 0.
 template <class R>
 __device__ __attribute__((always_inline)) inline void solid_project(const G2PArgs<R>& A,
-                                                                    const R px[3], R nx[3],
-                                                                    const R fv[3]) {
+                                                                    int64_t p, const R px[3],
+                                                                    R nx[3], const R fv[3]) {
   const Grid& g = A.g;
+  uint8_t* proj = A.F.mode ? &A.F.proj[A.F.pid[p]] : nullptr;
   // solidProjection moves the particle only if phi_now < 0.  The box distance
   // is 1-Lipschitz, so every stencil node's phi (or the inactive value 3 dx)
   // is at least min(3 dx, phi(px) - 2.6 dx), phi_ori, their convex
@@ -335,8 +337,11 @@ __device__ __attribute__((always_inline)) inline void solid_project(const G2PArg
   {
     const R step = (fabs(fv[0]) + fabs(fv[1]) + fabs(fv[2])) * A.dt;
     const R cdx = (R)g.dx;
-    if (container_phi_r<R>(g, px[0], px[1], px[2]) > (R)4 * cdx + step && step < (R)2 * cdx)
+    if (A.F.mode != 2 && container_phi_r<R>(g, px[0], px[1], px[2]) > (R)4 * cdx + step &&
+        step < (R)2 * cdx) {
+      if (proj) *proj = 0;
       return;
+    }
   }
   int base[3];
   R w[3][3];
@@ -367,7 +372,10 @@ __device__ __attribute__((always_inline)) inline void solid_project(const G2PArg
   }
   const R dp0 = fv[0] * A.dt, dp1 = fv[1] * A.dt, dp2 = fv[2] * A.dt;
   const R phi_now = phi_ori + (gr[0] * dp0 + gr[1] * dp1 + gr[2] * dp2);
-  if (phi_now < (R)0) {
+  const bool move = A.F.mode == 2 ? *proj != 0 : phi_now < (R)0;
+  if (A.F.mode == 1) *proj = move;
+  if (A.F.mode == 2 && move != (phi_now < (R)0)) fz_depart(A.F);
+  if (move) {
     nx[0] -= phi_now * gr[0];
     nx[1] -= phi_now * gr[1];
     nx[2] -= phi_now * gr[2];
@@ -430,7 +438,7 @@ __device__ __attribute__((always_inline)) inline void g2p_particle_at(const G2PA
   if constexpr (ADVECT) {
     R nx[3];
     for (int d = 0; d < 3; ++d) nx[d] = px[d] + fv[d] * A.dt;
-    if (A.solid_projection) solid_project<R>(A, px, nx, fv);
+    if (A.solid_projection) solid_project<R>(A, p, px, nx, fv);
     for (int d = 0; d < 3; ++d) A.x[d * A.cap + p] = nx[d];
   }
 }
@@ -548,7 +556,7 @@ __global__ __launch_bounds__(256) void k_solid_projection(G2PArgs<R> A, const R*
   const R px[3] = {xs[p], xs[A.cap + p], xs[2 * A.cap + p]};
   R nx[3] = {A.x[p], A.x[A.cap + p], A.x[2 * A.cap + p]};
   const R fv[3] = {A.v[p], A.v[A.cap + p], A.v[2 * A.cap + p]};
-  solid_project<R>(A, px, nx, fv);
+  solid_project<R>(A, p, px, nx, fv);
   for (int d = 0; d < 3; ++d) A.x[d * A.cap + p] = nx[d];
 }
 
@@ -814,6 +822,7 @@ void sort_impl(wc_ctx* ctx, Planes<R> P) {
   const Grid& g = ctx->g;
   hipStream_t st = ctx->stream;
   const int64_t n = ctx->np, cap = ctx->pcap;
+  ctx->adj.ready = false;
   ctx->bcount.ensure(g.nbucket + 1);
   ctx->bcursor.ensure(g.nbucket + 1);
   WC_HIP(hipMemsetAsync(ctx->bcount.p, 0, sizeof(uint32_t) * (g.nbucket + 1), st));
@@ -839,6 +848,7 @@ void sort_impl(wc_ctx* ctx, Planes<R> P) {
   std::swap(ctx->pid, ctx->pid2);
   ctx->sorted = true;
   ctx->xw_valid = false;
+  if (ctx->frz_mode) frozen_after_sort(ctx);
 }
 
 template <class R>
@@ -886,6 +896,7 @@ G2PArgs<R> g2p_args(wc_ctx* ctx, Planes<R> P, double dt) {
   A.invD = (R)(4.0 / (ctx->g.dx * ctx->g.dx));
   A.solid_projection = ctx->desc.solid_projection;
   A.bstart = ctx->bstart.p;
+  A.F = frozen_view(ctx);
   return A;
 }
 
@@ -950,6 +961,8 @@ double max_vel_launch(wc_ctx* ctx, Planes<R> P) {
 void particles_set_soa(wc_ctx* ctx, int64_t n, const double* x, const double* v,
                        const double* C, bool ref_layout) {
   ensure_particles(ctx, n);
+  ctx->adj.ready = false;
+  ctx->adj.have_start = false;
   const int64_t cap = ctx->pcap;
   std::vector<double> hx(3 * (size_t)cap, 0.0), hv(3 * (size_t)cap, 0.0),
       hC(9 * (size_t)cap, 0.0);
@@ -1056,6 +1069,8 @@ int64_t particles_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3]
                         hipMemcpyDeviceToDevice, st));
   const uint32_t n = exclusive_scan(ctx, scan.p, ncand);
   ensure_particles(ctx, n);
+  ctx->adj.ready = false;
+  ctx->adj.have_start = false;
   // Seeds are generated in FP64; the mixed form rounds them once to FP32.
   const int64_t cap = ctx->pcap;
   DevBuf<double> tx, tv, tC;

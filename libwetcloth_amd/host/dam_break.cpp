@@ -2,7 +2,8 @@
 // robertbridson::PCGSolver<double> solve on a matrix assembled with
 // SparseMatrix::add_to_element, AMGPCGSolveSparse on a 3-D one with the
 // DOFs' cells, then liquid substeps through WetClothCore (MIC(0), then two
-// with the AMG solver).
+// with the AMG solver), then a SubstepRollout forward/backward pair whose
+// gradient is compared with a central difference of replayed runs.
 // Prints one JSON line.  Usage: dam_break <n> <steps> [outdir]
 #include <chrono>
 #include <cmath>
@@ -146,6 +147,53 @@ int main(int argc, char** argv) {
     bool amg_steps_ok = true;
     for (int s = 0; s < 2; ++s) amg_steps_ok &= core.stepSystem(1.0 / 480);
     const int amg_step_iterations = stepper.lastIterations();
+    // 4. the rollout as a differentiable function: dL/dv0 along a random
+    // direction against a central difference of two replays, and dL/d|g|
+    double rollout_dir_err = 1.0, rollout_g_err = 1.0;
+    bool rollout_ok = false;
+    {
+      wcamd::TwoDScene rs(d);
+      const double rlo[3] = {0.0, 0.0, 0.0}, rhi[3] = {0.5, 0.5, 0.5};
+      rs.seedBlock(rlo, rhi, 3, 0.5);
+      std::vector<double> x0, v0, C0, x, v, C;
+      rs.getParticlesSoA(x0, v0, C0);
+      const int rsteps = 6;
+      const double rdt = 1.0 / 480;
+      wcamd::SubstepRollout::Context rc;
+      rollout_ok = wcamd::SubstepRollout::forward(rc, rs, rsteps, rdt, x0, v0, C0, x, v, C,
+                                                  1e-12, 5000, 4);
+      std::vector<double> gx(x.size());
+      for (size_t i = 0; i < x.size(); ++i) gx[i] = 2.0 * (x[i] - x0[i]);  // L = |x - x0|^2
+      const wcamd::SubstepRollout::Grads gr = wcamd::SubstepRollout::backward(rc, gx, {}, {});
+      rollout_ok = rollout_ok && gr.converged;
+      auto loss = [&](const std::vector<double>& vv) {
+        std::vector<double> xe, ve, Ce;
+        wcamd::SubstepRollout::replay(rc, x0, vv, C0, xe, ve, Ce);
+        double l = 0.0;
+        for (size_t i = 0; i < xe.size(); ++i) l += (xe[i] - x0[i]) * (xe[i] - x0[i]);
+        return l;
+      };
+      const double eps = 1e-4;
+      std::vector<double> dir(v0.size()), vp(v0), vm(v0);
+      double ad = 0.0;
+      for (size_t i = 0; i < dir.size(); ++i) {
+        dir[i] = nd(rng);
+        vp[i] += eps * dir[i];
+        vm[i] -= eps * dir[i];
+        ad += gr.v[i] * dir[i];
+      }
+      const double fd = (loss(vp) - loss(vm)) / (2 * eps);
+      rollout_dir_err = std::fabs(fd - ad) / std::fabs(fd);
+      const double gp[3] = {0.0, -(9.81 + eps), 0.0}, gm[3] = {0.0, -(9.81 - eps), 0.0},
+                   g0[3] = {0.0, -9.81, 0.0};
+      rs.setGravity(gp);
+      const double lp = loss(v0);
+      rs.setGravity(gm);
+      const double lm = loss(v0);
+      rs.setGravity(g0);
+      const double fdg = (lp - lm) / (2 * eps);
+      rollout_g_err = std::fabs(fdg - gr.gravity) / std::fabs(fdg);
+    }
     std::printf(
         "{\"pcg_ok\": %s, \"pcg_iterations\": %d, \"pcg_residual\": %.17g, "
         "\"pcg_max_error\": %.17g, \"amg_ok\": %s, \"amg_iterations\": %d, "
@@ -153,12 +201,14 @@ int main(int argc, char** argv) {
         "\"last_iterations\": %d, \"mean_vy\": %.17g, \"seconds\": %.6f, "
         "\"amg_steps_ok\": %s, \"amg_step_iterations\": %d, "
         "\"split_fused_bitwise\": %s, \"vmax\": %.17g, \"big_step_ok\": %s, "
-        "\"big_step_substeps\": %d}\n",
+        "\"big_step_substeps\": %d, \"rollout_ok\": %s, \"rollout_dir_err\": %.6g, "
+        "\"rollout_g_err\": %.6g}\n",
         ok ? "true" : "false", iterations, residual, err / nrm, amg_ok ? "true" : "false",
         amg_it, amg_err / amg_nrm, (long long)np,
         all_ok ? "true" : "false", last_iterations, vy / (double)np, secs,
         amg_steps_ok ? "true" : "false", amg_step_iterations,
-        split_fused_bitwise ? "true" : "false", vmax, big_ok ? "true" : "false", big_substeps);
+        split_fused_bitwise ? "true" : "false", vmax, big_ok ? "true" : "false", big_substeps,
+        rollout_ok ? "true" : "false", rollout_dir_err, rollout_g_err);
   } catch (const wcamd::Error& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return 1;

@@ -12,6 +12,8 @@
 //    particles and grids stay resident on the GPU.
 //  * wcamd::TwoDSceneSerializer: the reference's particle files
 //    (App/TwoDSceneSerializer.cpp) written from that scene.
+//  * wcamd::SubstepRollout: substeps as a differentiable function with a
+//    forward/backward pair and checkpoints (no reference counterpart).
 //
 // Errors of the library (bad arguments, HIP failures) throw wcamd::Error; the
 // reference has no exceptions because it has no failure modes of this kind.
@@ -68,6 +70,24 @@ class TwoDScene {
     fB.assign(9 * n, 0.0);
     check(wc_get_particles(ctx_, x4.data(), fluid_v4.data(), fB.data()), ctx_);
   }
+  // structure of arrays in the caller's order: x[d*n+p], v[d*n+p],
+  // C[(r*3+c)*n+p] (C may be empty: zero)
+  void setParticlesSoA(const std::vector<double>& x, const std::vector<double>& v,
+                       const std::vector<double>& C) {
+    const int64_t n = (int64_t)x.size() / 3;
+    check(wc_set_particles_soa(ctx_, n, x.data(), v.empty() ? nullptr : v.data(),
+                               C.empty() ? nullptr : C.data()),
+          ctx_);
+  }
+  void getParticlesSoA(std::vector<double>& x, std::vector<double>& v,
+                       std::vector<double>& C) const {
+    const int64_t n = getNumParticles();
+    x.assign(3 * n, 0.0);
+    v.assign(3 * n, 0.0);
+    C.assign(9 * n, 0.0);
+    check(wc_get_particles_soa(ctx_, x.data(), v.data(), C.data()), ctx_);
+  }
+  void setGravity(const double g[3]) { check(wc_set_gravity(ctx_, g), ctx_); }
   int64_t getNumParticles() const {
     int64_t n = 0;
     check(wc_num_particles(ctx_, &n), ctx_);
@@ -208,6 +228,81 @@ class WetClothCore {
   LinearizedImplicitEuler& stepper_;
   bool fused_;
   int last_substeps_ = 0;
+};
+
+// ------------------------------------------------------------- the adjoint
+
+// `steps` liquid substeps of dt as a differentiable function of the initial
+// particle state and the gravity magnitude (no reference counterpart: the
+// reference has no differentiation), shaped like a torch.autograd.Function:
+// forward(ctx, ...) runs the substeps on the GPU, recording each one's
+// discrete decisions and checkpointing the particles before every
+// `every`-th; backward(ctx, ...) takes dL/d(final x, v, C) to dL/d(initial
+// x, v, C) and dL/d|g| (gravity (0, -|g|, 0)), recomputing each segment from
+// its checkpoint.  replay(ctx, ...) runs the recorded decisions again from
+// other initial particles (the same smooth map, for central differences).
+// FP64 scenes only; vectors are structure-of-arrays in the caller's order.
+struct SubstepRollout {
+  struct Context {
+    TwoDScene* scene = nullptr;
+    int steps = 0;
+    double dt = 0.0, tol = 1e-12;
+    int maxit = 5000, every = 10;
+  };
+  struct Grads {
+    std::vector<double> x, v, C;
+    double gravity = 0.0;
+    bool converged = true;
+  };
+
+  static bool forward(Context& ctx, TwoDScene& scene, int steps, double dt,
+                      const std::vector<double>& x0, const std::vector<double>& v0,
+                      const std::vector<double>& C0, std::vector<double>& x,
+                      std::vector<double>& v, std::vector<double>& C, double tol = 1e-12,
+                      int maxit = 5000, int every = 10) {
+    ctx.scene = &scene;
+    ctx.steps = steps;
+    ctx.dt = dt;
+    ctx.tol = tol;
+    ctx.maxit = maxit;
+    ctx.every = every;
+    scene.setParticlesSoA(x0, v0, C0);
+    int32_t it = 0;
+    const wc_status st = wc_rollout_forward(scene.handle(), steps, dt, tol, maxit, every, 0, &it);
+    check(st, scene.handle());
+    scene.getParticlesSoA(x, v, C);
+    return st == WC_OK;
+  }
+  static bool replay(const Context& ctx, const std::vector<double>& x0,
+                     const std::vector<double>& v0, const std::vector<double>& C0,
+                     std::vector<double>& x, std::vector<double>& v, std::vector<double>& C) {
+    TwoDScene& scene = *ctx.scene;
+    scene.setParticlesSoA(x0, v0, C0);
+    int32_t it = 0;
+    const wc_status st =
+        wc_rollout_forward(scene.handle(), ctx.steps, ctx.dt, ctx.tol, ctx.maxit, ctx.every, 1, &it);
+    check(st, scene.handle());
+    scene.getParticlesSoA(x, v, C);
+    return st == WC_OK;
+  }
+  static Grads backward(const Context& ctx, const std::vector<double>& grad_x,
+                        const std::vector<double>& grad_v, const std::vector<double>& grad_C) {
+    TwoDScene& scene = *ctx.scene;
+    const int64_t n = scene.getNumParticles();
+    Grads g;
+    g.x = grad_x;
+    g.v = grad_v;
+    g.C = grad_C;
+    g.x.resize(3 * n, 0.0);
+    g.v.resize(3 * n, 0.0);
+    g.C.resize(9 * n, 0.0);
+    int32_t it = 0;
+    const wc_status st = wc_rollout_backward(scene.handle(), ctx.dt, ctx.tol, ctx.maxit,
+                                             g.x.data(), g.v.data(), g.C.data(), &g.gravity, &it);
+    check(st, scene.handle());
+    g.converged = st == WC_OK;
+    return g;
+  }
 };
 
 }  // namespace wcamd

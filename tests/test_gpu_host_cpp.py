@@ -1,7 +1,8 @@
 """The C++ host mirror (libwetcloth_amd/host/wetcloth_amd.hpp) driven from a
 compiled program: PCGSolver<double>::solve and AMGPCGSolveSparse on
-manufactured systems and WetClothCore substeps of the dam break, checked
-against the oracle."""
+manufactured systems, WetClothCore substeps of the dam break, checked
+against the oracle, and the SubstepRollout forward/backward pair against
+central differences."""
 import json
 import os
 import subprocess
@@ -32,6 +33,9 @@ def test_cpp_host_mirror(oracle, tmp_path):
     # stepSystem(0.1) splits into ceil(0.1 / min(3 dx / v_max, 1/30)) substeps
     exp = max(1, int(np.ceil(0.1 / min((1.0 / n) / max(1e-63, r["vmax"]) * 3.0, 1.0 / 30.0))))
     assert r["big_step_ok"] and r["big_step_substeps"] == exp >= 3
+    # wcamd::SubstepRollout: the backward's dL/dv0 along a random direction
+    # and dL/d|g| against central differences of replayed forwards
+    assert r["rollout_ok"] and r["rollout_dir_err"] < 1e-6 and r["rollout_g_err"] < 1e-6
     g, x, v = oracle.dam_break(n)
     assert r["particles"] == x.shape[1]
     s = oracle.OracleScene((n, n, n), 1.0 / n, x, v)
