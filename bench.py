@@ -89,22 +89,29 @@ def measured_traffic(kernel: str, workload: str):
 
 # particle-state words a transfer reads and writes per particle: P2G reads
 # x, v and C (3 + 3 + 9); G2P reads x and writes x, v and C (3 + 3 + 3 + 9).
-# The grid's bytes are not counted, so the rates are lower bounds.
+# Grid words per active node (FP64 grid in both forms): P2G writes mass,
+# momentum and volume of three faces (3 x 3), G2P reads three face
+# velocities (3).
 P2G_WORDS, G2P_WORDS = 15, 18
+P2G_GRID_WORDS, G2P_GRID_WORDS = 9, 3
 
 
-def transfers(stats: dict, word: int) -> dict:
+def transfers(stats: dict, word: int, active_nodes: int) -> dict:
     """Achieved HBM rate of the P2G and G2P kernels from the library's
-    HIP-event times (every launch of the timed substeps)."""
+    HIP-event times (every launch of the timed substeps), against the
+    particle state they move plus the grid they write or read."""
     out = {}
-    for name, words in (("p2g", P2G_WORDS), ("g2p", G2P_WORDS)):
+    for name, words, gwords in (("p2g", P2G_WORDS, P2G_GRID_WORDS),
+                                ("g2p", G2P_WORDS, G2P_GRID_WORDS)):
         ms, nl, units = stats[name]
         if nl == 0:
             continue
-        b = units / nl * words * word
+        pb = units / nl * words * word
+        b = pb + active_nodes * gwords * 8
         gbs = b / (ms / nl * 1e-3) / 1e9
         out[name] = {"avg_ms": ms / nl, "launches": nl, "particles": units // nl,
-                     "particle_bytes_per_launch": b, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+                     "active_nodes": active_nodes, "particle_bytes_per_launch": pb,
+                     "bytes_per_launch": b, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
     return out
 
 
@@ -122,7 +129,8 @@ def roofline(stats: dict, word: int, workload: str) -> dict:
         bytes_per_launch = units / nl * words * word
         per[name] = {"avg_ms": ms / nl, "launches": nl, "dofs": units // nl,
                      "bytes_per_launch": bytes_per_launch,
-                     "GBs": bytes_per_launch / avg_s / 1e9}
+                     "GBs": bytes_per_launch / avg_s / 1e9,
+                     "frac": bytes_per_launch / avg_s / 1e9 / HBM_PEAK_GBS}
     if not per:
         return None
     dom = max(("sweep_fwd", "sweep_bwd"), key=lambda k: per.get(k, {}).get("avg_ms", -1))
@@ -170,6 +178,7 @@ def cpu_baseline(n_particles_gpu: int, steps: int) -> dict:
 
 
 def run(W, precision, n, dt, a, solver=0):
+    import numpy as np
     """Seed the workload in one precision, step it warmup + steps times and
     time the last steps (the library's stream is synchronized by every step:
     the solver reads its residual back each iteration).  solver 1 solves the
@@ -194,7 +203,9 @@ def run(W, precision, n, dt, a, solver=0):
         for k, v in scene.phase_times().items():
             phase_tot[k] += v
     wall = time.perf_counter() - t0
+    active_nodes = int(scene.field("act").astype(np.int64).sum()) * 64
     return scene, {
+        "active_nodes": active_nodes,
         "particles": npart,
         "value": a.steps / wall,
         "ms_per_step": 1e3 * wall / a.steps,
@@ -259,8 +270,8 @@ def main():
     }
     out["roofline"] = roofline(m["kernel_stats"], 4, workload)
     out["fp64"]["roofline"] = roofline(f["kernel_stats"], 8, workload)
-    out["transfers"] = transfers(m["kernel_stats"], 4)
-    out["fp64"]["transfers"] = transfers(f["kernel_stats"], 8)
+    out["transfers"] = transfers(m["kernel_stats"], 4, m["active_nodes"])
+    out["fp64"]["transfers"] = transfers(f["kernel_stats"], 8, f["active_nodes"])
     if not a.no_amg:
         amg, g = run(W, W.WC_FP64, n, dt, a, solver=1)
         amg.close()

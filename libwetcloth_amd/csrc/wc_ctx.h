@@ -121,6 +121,7 @@ struct SweepPlan {
   DevBuf<uint32_t> order, code;     // ticket order; per-cell coupling bits
   DevBuf<uint32_t> scode;           // per-cell sweep code (orders applied)
   DevBuf<uint32_t> opf, opb;        // per-cell LDS operand offsets of each sweep
+  DevBuf<uint16_t> sflag;           // per-cell SpMV flags (M_* in wc_sweep.hip)
   DevBuf<double> cf, cb;            // 32-byte factor records (forward / backward)
   DevBuf<double> coef;              // 7 matrix entries per cell
   DevBuf<double> fin;               // 96-byte factor-pass inputs per cell

@@ -51,13 +51,16 @@ struct NodeArgs {
   const unsigned long long* phikey;
   double* lphi;
   double* w[3];
-  const double* mass[3];
-  double* mom[3];  // momentum in, velocity out
+  double* mass[3];  // P2G's weight sums in, masses out
+  double* vol[3];   // volumes out
+  double* mom[3];   // momentum in, velocity out
+  double fm, fvol;  // the particle mass and volume P2G weighted with
   FrozenView F;    // mode 2: lphi is already the replayed one
 };
 
 // updateSolidWeights (TwoDScene.cpp:1871-1962), updateLiquidPhi's solid
-// override and default (:3297-3348), and u = P/M of P2G (:6700-6713).
+// override and default (:3297-3348), and of P2G (:6700-6713) M = fm sum w,
+// V = fvol sum w and u = P/M.
 __global__ void k_grid_nodes(NodeArgs A) {
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const Grid& g = A.g;
@@ -68,6 +71,8 @@ __global__ void k_grid_nodes(NodeArgs A) {
     for (int d = 0; d < 3; ++d) {
       A.w[d][n] = 0.0;
       A.mom[d][n] = 0.0;
+      A.mass[d][n] = 0.0;
+      A.vol[d][n] = 0.0;
     }
     if (A.F.mode == 1) {
       A.F.liquid[n] = 0;
@@ -105,7 +110,10 @@ __global__ void k_grid_nodes(NodeArgs A) {
   }
   uint8_t has = 0;
   for (int d = 0; d < 3; ++d) {
-    const double M = A.mass[d][n];
+    const double ws = A.mass[d][n];
+    const double M = A.fm * ws;
+    A.mass[d][n] = M;
+    A.vol[d][n] = A.fvol * ws;
     const bool hm = fz_mass(A.F, d, n, M);
     if (A.F.mode == 2 && (((A.F.flags[n] >> d) & 1) != 0) != (M > 1e-20)) fz_depart(A.F);
     has |= (uint8_t)(hm ? 1u << d : 0u);
@@ -456,8 +464,12 @@ void grid_nodes(wc_ctx* ctx) {
   for (int d = 0; d < 3; ++d) {
     A.w[d] = ctx->w[d].p;
     A.mass[d] = ctx->mass[d].p;
+    A.vol[d] = ctx->vol[d].p;
     A.mom[d] = ctx->mom[d].p;
   }
+  // the mixed form's P2G weights with the FP32 particle mass and volume
+  A.fm = ctx->mixed ? (double)(float)ctx->desc.particle_mass : ctx->desc.particle_mass;
+  A.fvol = ctx->mixed ? (double)(float)ctx->desc.particle_volume : ctx->desc.particle_volume;
   hipLaunchKernelGGL(k_grid_nodes, dim3(blocks(ctx->g.nnode, 256)), dim3(256),
                      0, ctx->stream, A);
   WC_LAUNCH_CHECK();

@@ -128,9 +128,8 @@ struct P2GArgs {
   const R* C;
   int64_t cap;
   const uint32_t* bstart;
-  double* mass[3];
+  double* wsum[3];  // sum of the weights (the node mass and volume are fm and fvol times it)
   double* mom[3];
-  double* vol[3];
   unsigned long long* phikey;
   uint8_t* touched;
   R fm, fvol, r0;
@@ -160,7 +159,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   // [axis][sum of weights, sum of weight * velocity], in FP64 for either R.
   // Particle mass and volume are uniform, so a node's mass and volume are
   // fm and fvol times its weight sum (TwoDScene.cpp:6700-6713 sums fm w and
-  // fvol w): two LDS atomics per node instead of three.
+  // fvol w): two sums per node instead of three, in LDS and in the flush
+  // (the grid kernel that follows forms the mass and the volume).
   __shared__ double sm[3][2][kLdsNodes];
   __shared__ unsigned long long sphi[kLdsNodes];
   __shared__ int stouch[27];
@@ -215,9 +215,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
               atomicAdd(&sm[d][1][l], (double)(vel * ww));
             } else {
               const int64_t n = node_index(g, i, j, k);
-              unsafeAtomicAdd(&A.mass[d][n], (double)A.fm * (double)ww);
+              unsafeAtomicAdd(&A.wsum[d][n], (double)ww);
               unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * (double)(vel * ww));
-              unsafeAtomicAdd(&A.vol[d][n], (double)A.fvol * (double)ww);
             }
           }
     }
@@ -273,9 +272,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
     for (int d = 0; d < 3; ++d) {
       const double sw = sm[d][0][l];
       if (sw != 0.0) {
-        unsafeAtomicAdd(&A.mass[d][n], (double)A.fm * sw);
+        unsafeAtomicAdd(&A.wsum[d][n], sw);
         unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * sm[d][1][l]);
-        unsafeAtomicAdd(&A.vol[d][n], (double)A.fvol * sw);
       }
     }
     const unsigned long long key = sphi[l];
@@ -862,9 +860,8 @@ void p2g_launch(wc_ctx* ctx, Planes<R> P) {
   A.cap = ctx->pcap;
   A.bstart = ctx->bstart.p;
   for (int d = 0; d < 3; ++d) {
-    A.mass[d] = ctx->mass[d].p;
+    A.wsum[d] = ctx->mass[d].p;  // weight sums; grid_nodes forms the masses
     A.mom[d] = ctx->mom[d].p;
-    A.vol[d] = ctx->vol[d].p;
   }
   A.phikey = ctx->phikey.p;
   A.touched = ctx->touched.p;
@@ -1139,7 +1136,6 @@ void particles_p2g(wc_ctx* ctx) {
   for (int d = 0; d < 3; ++d) {
     WC_HIP(hipMemsetAsync(ctx->mass[d].p, 0, sizeof(double) * nn, st));
     WC_HIP(hipMemsetAsync(ctx->mom[d].p, 0, sizeof(double) * nn, st));
-    WC_HIP(hipMemsetAsync(ctx->vol[d].p, 0, sizeof(double) * nn, st));
   }
   WC_HIP(hipMemsetAsync(ctx->phikey.p, 0xFF, sizeof(unsigned long long) * nn, st));
   WC_HIP(hipMemsetAsync(ctx->touched.p, 0, g.nbucket, st));

@@ -52,6 +52,14 @@ constexpr uint32_t C_DIAG = 1u << 13;  // the row has a diagonal entry
 constexpr int C_ALO = 14;    // bits 14..16: A(me, -x/-y/-z) present
 constexpr int C_AUP = 17;    // bits 17..19: A(me, +x/+y/+z) present
 
+// SpMV flags per slot (16 bits): the row is a DOF, which of its entries
+// below the diagonal exist (A(me, k_d) = A(k_d, me), read from the
+// neighbour's +e_d plane), and the orders of its lower and upper terms
+constexpr uint16_t M_DOF = 1u;
+constexpr int M_LO = 1;     // bits 1..3: A(me, -x/-y/-z) present
+constexpr int M_LORD = 4;   // bits 4..6: order of the lower terms
+constexpr int M_UORD = 7;   // bits 7..9: order of the upper terms
+
 // Sweep code word per slot, with each sweep's neighbour order applied: term
 // q of a sweep is the q-th neighbour in ascending DOF order (the order the
 // column loops of the reference apply them), present or not.
@@ -170,6 +178,7 @@ struct BoxArgs {
   uint32_t* scode;  // sweep code (S_*)
   uint32_t* opf;    // operand offsets of the forward / backward sweep
   uint32_t* opb;
+  uint16_t* sflag;  // SpMV flags (M_*)
   Cell* cf;      // forward sweep (lower terms), zeroed here, filled by the factor pass
   Cell* cb;      // backward sweep (upper terms), likewise
   double* coef;  // kCoef planes of A entries: coef[e * nslot + cell]
@@ -455,6 +464,11 @@ __global__ void k_fill_box(BoxArgs A, const RowSum* S) {
   A.pdof[t] = me;
   A.code[t] = code;
   A.scode[t] = sc;
+  uint16_t mf = 0;
+  if (code & C_DOF)
+    mf = (uint16_t)(M_DOF | (((code >> C_ALO) & 7u) << M_LO) |
+                    (((code >> C_LORD) & 7u) << M_LORD) | (((code >> C_UORD) & 7u) << M_UORD));
+  A.sflag[t] = mf;
   A.opf[t] = operand_word(sc, S_FPRES, S_FSRC, li, lj);
   A.opb[t] = operand_word(sc, S_BPRES, S_BSRC, kT - 1 - li, kT - 1 - lj);
   fo.fcode = fco;
@@ -560,23 +574,89 @@ __device__ inline double box_row(const Skew& K, uint32_t t, uint32_t c,
 // FP64 block partials of x . y, with the thread-to-slot mapping and the
 // reduction tree of the solver's dot-product kernel, so the sum is the one
 // that kernel would compute from the stored y.
-template <class V, bool DOT>
-__global__ __launch_bounds__(256) void k_box_spmv(uint32_t nslot, Skew K,
-                                                  const uint32_t* __restrict__ code,
-                                                  const V* __restrict__ coef,
-                                                  const V* __restrict__ x, V* __restrict__ y,
-                                                  double* partial) {
-  __shared__ double sh[256];
+// The same row from four planes: the diagonal and the +x/+y/+z entries; each
+// entry below the diagonal, A(me, k_d), is the lower neighbour's +e_d entry
+// A(k_d, me) (the matrix is symmetric bit for bit: both rows take the term of
+// their shared face).  The products are added in the same order, so y is the
+// seven-plane row's.  The loads do not branch: a slot that is not a DOF, or
+// an entry below the diagonal that does not exist, reads slot 0 instead (one
+// cached line for the whole launch) and its product is replaced by +0, which
+// leaves the sum as skipping it would (a sum that starts at +0 never
+// becomes -0); a slot that is not a DOF gives +0.  xself: x at the slot (+0
+// where it is not a DOF, as the vectors hold there), for the dot.
+template <class V, class CV>
+__device__ inline double box_row_sym(const Skew& K, uint32_t t, uint32_t f,
+                                     const CV* __restrict__ coef, uint32_t nslot,
+                                     const V* __restrict__ x, double& xself) {
+  const bool dof = (f & M_DOF) != 0;
+  const uint32_t lane = t & (kThreads - 1);
+  const uint32_t li = lane & (kT - 1), lj = lane / kT;
+  const uint32_t dx = (uint32_t)(int32_t)K.dx, dy = (uint32_t)(int32_t)K.dy;
+  const uint32_t last = nslot - 1;
+  auto clampn = [&](uint32_t n) { return n < last ? n : last; };
+  const uint32_t self = dof ? t : 0u;
+  // lower neighbours: where they exist (flags) they are DOFs inside the box
+  const uint32_t lo_raw[3] = {li > 0 ? t - kThreads - 1 : t + dx,
+                              lj > 0 ? t - kThreads - kT : t + dy, t - kThreads};
+  const uint32_t up_raw[3] = {li < kT - 1 ? t + kThreads + 1 : t - dx,
+                              lj < kT - 1 ? t + kThreads + kT : t - dy, t + kThreads};
+  double alo[3], xlo[3], aup[3], xup[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const bool lo = dof && ((f >> (M_LO + d)) & 1u);
+    const uint32_t nl = lo ? lo_raw[d] : 0u;
+    const uint32_t nu = dof ? clampn(up_raw[d]) : 0u;
+    const double cl = (double)coef[(uint32_t)(4 + d) * nslot + nl];
+    const double vl = (double)x[nl];
+    alo[d] = lo ? cl : 0.0;
+    xlo[d] = lo ? vl : 0.0;
+    aup[d] = (double)coef[(uint32_t)(4 + d) * nslot + self];
+    xup[d] = (double)x[nu];
+  }
+  const double a0 = (double)coef[self], x0 = (double)x[self];
+  xself = dof ? x0 : 0.0;
+  const int lor = (int)((f >> M_LORD) & 7), upr = (int)((f >> M_UORD) & 7);
   double acc = 0.0;
-  for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < nslot; t += gridDim.x * 256u) {
-    const V v = (V)box_row(K, t, code[t], coef, nslot, x);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(lor, q);
+    const double ca = ax == 0 ? alo[0] : (ax == 1 ? alo[1] : alo[2]);
+    const double cx = ax == 0 ? xlo[0] : (ax == 1 ? xlo[1] : xlo[2]);
+    acc += ca * cx;
+  }
+  acc += a0 * x0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(upr, q);
+    const double ca = ax == 0 ? aup[0] : (ax == 1 ? aup[1] : aup[2]);
+    const double cx = ax == 0 ? xup[0] : (ax == 1 ? xup[1] : xup[2]);
+    acc += ca * cx;
+  }
+  return dof ? acc : 0.0;
+}
+
+// kSpmvThreads per block (twice the reduction kernels' 256: the row's loads
+// are latency-bound, so more waves in flight), one dot partial per block.
+constexpr int kSpmvThreads = 512;
+template <class V, bool DOT>
+__global__ __launch_bounds__(kSpmvThreads) void k_box_spmv_sym(uint32_t nslot, Skew K,
+                                                      const uint16_t* __restrict__ sflag,
+                                                      const V* __restrict__ coef,
+                                                      const V* __restrict__ x,
+                                                      V* __restrict__ y, double* partial) {
+  __shared__ double sh[kSpmvThreads];
+  double acc = 0.0;
+  for (uint32_t t = blockIdx.x * (uint32_t)kSpmvThreads + threadIdx.x; t < nslot;
+       t += gridDim.x * (uint32_t)kSpmvThreads) {
+    double xs;
+    const V v = (V)box_row_sym(K, t, (uint32_t)sflag[t], coef, nslot, x, xs);
+    if (DOT) acc += xs * (double)v;
     y[t] = v;
-    if (DOT) acc += (double)x[t] * (double)v;
   }
   if (DOT) {
     sh[threadIdx.x] = acc;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = kSpmvThreads / 2; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
       __syncthreads();
     }
@@ -1255,6 +1335,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.scode.ensure(P.ncell);
   P.opf.ensure(P.ncell);
   P.opb.ensure(P.ncell);
+  P.sflag.ensure(P.ncell);
   P.cf.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.cb.ensure(P.ncell * sizeof(Cell) / sizeof(double));
   P.coef.ensure(P.ncell * kCoef);
@@ -1297,6 +1378,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   A.scode = P.scode.p;
   A.opf = P.opf.p;
   A.opb = P.opb.p;
+  A.sflag = P.sflag.p;
   A.cf = reinterpret_cast<Cell*>(P.cf.p);
   A.cb = reinterpret_cast<Cell*>(P.cb.p);
   A.coef = P.coef.p;
@@ -1430,26 +1512,26 @@ void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
 }
 
 static unsigned spmv_blocks(const SweepPlan& P) {
-  return (unsigned)std::min<int64_t>(1024, blocks(P.ncell, 256));
+  return (unsigned)std::min<int64_t>(1024, blocks(P.ncell, kSpmvThreads));
 }
 
 void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y) {
-  hipLaunchKernelGGL((k_box_spmv<double, false>), dim3(spmv_blocks(P)), dim3(256), 0, st,
-                     (uint32_t)P.ncell, skew_of(P), P.code.p, P.coef.p, x, y, nullptr);
+  hipLaunchKernelGGL((k_box_spmv_sym<double, false>), dim3(spmv_blocks(P)), dim3(kSpmvThreads), 0, st,
+                     (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef.p, x, y, nullptr);
   WC_LAUNCH_CHECK();
 }
 
 void box_spmv_dot(hipStream_t st, const SweepPlan& P, const double* x, double* y,
                   double* partial, int nblocks) {
-  hipLaunchKernelGGL((k_box_spmv<double, true>), dim3(nblocks), dim3(256), 0, st,
-                     (uint32_t)P.ncell, skew_of(P), P.code.p, P.coef.p, x, y, partial);
+  hipLaunchKernelGGL((k_box_spmv_sym<double, true>), dim3(nblocks), dim3(kSpmvThreads), 0, st,
+                     (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef.p, x, y, partial);
   WC_LAUNCH_CHECK();
 }
 
 void box_spmv32_dot(hipStream_t st, const SweepPlan& P, const float* x, float* y,
                     double* partial, int nblocks) {
-  hipLaunchKernelGGL((k_box_spmv<float, true>), dim3(nblocks), dim3(256), 0, st,
-                     (uint32_t)P.ncell, skew_of(P), P.code.p, P.coef32.p, x, y, partial);
+  hipLaunchKernelGGL((k_box_spmv_sym<float, true>), dim3(nblocks), dim3(kSpmvThreads), 0, st,
+                     (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef32.p, x, y, partial);
   WC_LAUNCH_CHECK();
 }
 
@@ -1489,6 +1571,7 @@ void SweepPlan::release() {
   scode.release();
   opf.release();
   opb.release();
+  sflag.release();
   cf.release();
   cb.release();
   coef.release();

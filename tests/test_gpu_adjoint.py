@@ -15,8 +15,9 @@
   with eps = 1e-4, the decisions of the recorded run replayed (and no
   replay departing from them: a uniform change of |g| by 1e-4 moves every
   particle, and the difference is taken again at 1e-5 or 1e-6 where that
-  departs), within 1e-4 relative.  Measured: velocity directions worst
-  2.6e-6 (median 6e-8), |g| 5e-8.
+  departs), within 1e-4 relative.  The difference of the two losses is
+  formed per particle, (x+ - x-) . (x+ + x- - 2 x_t), so that it does not
+  cancel between two sums of ~50.
 """
 import numpy as np
 import pytest
@@ -115,20 +116,26 @@ def test_config3_gradients_against_central_differences(wc):
     loss0 = float(((xf - xt) ** 2).sum())
     print(f"  forward/backward done: loss {loss0:.6e}", flush=True)
 
-    def loss(v, g=9.81):
-        """The replayed run's loss and its departures from the recorded
-        decisions."""
+    def final(v, g=9.81):
+        """The replayed run's final positions and its departures from the
+        recorded decisions."""
         s.set_gravity((0.0, -g, 0.0))
         s.set_particles(x0, v)
         ok, _ = s.rollout_forward(steps, dt, tol=TOL, replay=True)
         dep = s.frozen_departures()
         assert ok or dep > 0
-        return float(((s.particles()[0] - xt) ** 2).sum()), dep
+        return s.particles()[0], dep
 
-    l1, dep1 = loss(v0)
-    l2, _ = loss(v0)
+    def central(xp, xm, h):
+        # (L(+) - L(-)) / 2h with the difference formed per particle,
+        # (x+ - x-) . (x+ + x- - 2 x_t), instead of from the two sums of ~50
+        return float(((xp - xm) * (xp + xm - 2.0 * xt)).sum()) / (2 * h)
+
+    x1, dep1 = final(v0)
+    x2, _ = final(v0)
+    l1 = float(((x1 - xt) ** 2).sum())
     assert dep1 == 0 and abs(l1 - loss0) <= 1e-12 * loss0  # the replay is the recorded run
-    noise = abs(l1 - l2)
+    noise = float(np.abs(x1 - x2).max())
     # unit directions: the perturbation eps d moves each particle by ~1e-8 m
     # over the run, so the replays stay in the record's smooth neighbourhood
     # (a direction whose replay departs from a recorded decision -- a face
@@ -139,11 +146,11 @@ def test_config3_gradients_against_central_differences(wc):
     while len(errs) < ndir and skipped < ndir // 2:
         d = rng.standard_normal(v0.shape).astype(np.float32).astype(np.float64)
         d /= np.linalg.norm(d)
-        (lp, dp), (lm, dm) = loss(v0 + eps * d), loss(v0 - eps * d)
+        (xp, dp), (xm, dm) = final(v0 + eps * d), final(v0 - eps * d)
         if dp or dm:
             skipped += 1
             continue
-        fd = (lp - lm) / (2 * eps)
+        fd = central(xp, xm, eps)
         errs.append(abs(fd - float((gv * d).sum())) / abs(fd))
         if len(errs) % 8 == 0:
             print(f"  direction {len(errs)}/{ndir}: |dL/dv0 . d| {abs(fd):.3e}, worst relative "
@@ -151,15 +158,16 @@ def test_config3_gradients_against_central_differences(wc):
     # |g| moves every particle by ~0.5 eps t^2; where that departs from a
     # recorded decision, the difference is taken again with a smaller eps
     for eps_g in (eps, eps / 10, eps / 100):
-        (lp, dp), (lm, dm) = loss(v0, 9.81 + eps_g), loss(v0, 9.81 - eps_g)
-        fdg = (lp - lm) / (2 * eps_g)
+        (xp, dp), (xm, dm) = final(v0, 9.81 + eps_g), final(v0, 9.81 - eps_g)
+        fdg = central(xp, xm, eps_g)
         eg = abs(fdg - gg) / abs(fdg)
         print(f"  dL/d|g| with eps {eps_g:g}: {fdg:.8e} ({eg:.2e}), {dp + dm} departures",
               flush=True)
         if dp + dm == 0:
             break
     print(f"64^3, {x0.shape[1]} particles, {steps} substeps (checkpoints every {every}): "
-          f"replay noise {noise:.1e} in the loss; dL/dv0 along {len(errs)} unit directions vs "
+          f"replay noise {noise:.1e} m in the positions; dL/dv0 along {len(errs)} unit "
+          f"directions vs "
           f"central differences (eps {eps:g}): median {np.median(errs):.2e}, worst "
           f"{max(errs):.2e}, {skipped} directions drawn again; dL/d|g| {gg:.8e} vs "
           f"{fdg:.8e} ({eg:.2e}, eps {eps_g:g})")

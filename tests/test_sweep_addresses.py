@@ -126,3 +126,41 @@ def test_face_reads_hit_the_upstream_tile_cell():
                     # the upstream pencil (15, lj) computes its cell kk at step kk + 15 + lj
                     assert addr == skew_slot(P, a - 1, b, 15, lj, kk + 15 + lj)
                     assert kk + 15 + lj < nsteps
+
+
+def decode(P, t):
+    lane = t % THREADS
+    q = t // THREADS
+    w = q % P["nsteps"]
+    tile = q // P["nsteps"]
+    a, b = tile % P["nta"], tile // P["nta"]
+    li, lj = lane % KT, lane // KT
+    return a * KT + li, b * KT + lj, w - li - lj
+
+
+@pytest.mark.parametrize("B0,B1,Bz", [(16, 16, 8), (32, 48, 20), (64, 32, 37)])
+def test_spmv_lower_neighbour_slots(B0, B1, Bz):
+    """The symmetric SpMV (k_box_spmv_sym) reads A(k_d, me) and x at the slot
+    of the lower neighbour k_d of every DOF slot that has one: the previous
+    step of the lane one to the left (below, or the same lane for z), or
+    across the tile face by the face offset.  Wherever the neighbour cell is
+    inside the box that slot must be in range and hold exactly that cell."""
+    P = plan(B0, B1, Bz)
+    t = np.arange(P["ncell"], dtype=np.int64)
+    px, py, kk = decode(P, t)
+    inside = (kk >= 0) & (kk < Bz)
+    lane = t % THREADS
+    li, lj = lane % KT, lane // KT
+    forms = [np.where(li > 0, t - THREADS - 1, t + P["kdx"]),
+             np.where(lj > 0, t - THREADS - KT, t + P["kdy"]),
+             t - THREADS]
+    shifts = [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    for d in range(3):
+        sx, sy, sz = shifts[d]
+        has = inside & (px - sx >= 0) & (py - sy >= 0) & (kk - sz >= 0)
+        n = forms[d][has]
+        assert n.min() >= 0 and n.max() < P["ncell"]
+        qx, qy, qk = decode(P, n)
+        assert np.array_equal(qx, px[has] - sx)
+        assert np.array_equal(qy, py[has] - sy)
+        assert np.array_equal(qk, kk[has] - sz)
