@@ -49,7 +49,38 @@ class SceneStruct(C.Structure):
         ("frz_mode", C.c_int), ("lphi_arg", _i32p), ("frz_act", _u8p), ("frz_arg", _i32p),
         ("frz_liquid", _u8p), ("frz_dof", _i32p), ("frz_ndof", C.c_int64),
         ("frz_clamp", _u8p * 3), ("frz_mass", _u8p * 3), ("frz_proj", _u8p),
+        ("mix", C.c_void_p),
     ]
+
+
+class MixParams(C.Structure):
+    """wco_mix_params: the LiquidInfo fields of the mixture terms."""
+    _fields_ = [("use_drag", C.c_int), ("use_nonlinear_drag", C.c_int),
+                ("yarn_diameter", C.c_double), ("viscosity", C.c_double),
+                ("yazdchi_power", C.c_double), ("pore_radius", C.c_double),
+                ("surf_tension_coeff", C.c_double), ("rest_contact_angle", C.c_double)]
+
+
+def mix_params(**kw) -> MixParams:
+    """The mixture parameters of the cloth scenes in SI units: the reference's
+    LiquidInfo defaults (TwoDSceneXMLParser.cpp:1849-1904, given there in CGS)
+    converted; a rest volume fraction, where given, derives the pore radius
+    from the yarn diameter as the parser does (:1960-1963)."""
+    yarn = kw.get("yarn_diameter", 1e-4)
+    p = MixParams()
+    p.use_drag = int(kw.get("use_drag", 1))
+    p.use_nonlinear_drag = int(kw.get("use_nonlinear_drag", 1))
+    p.yarn_diameter = yarn
+    p.viscosity = kw.get("viscosity", 8.9e-4)
+    p.yazdchi_power = kw.get("yazdchi_power", 1.6)
+    if "rest_volume_fraction" in kw:
+        rvf = kw["rest_volume_fraction"]
+        p.pore_radius = yarn * 0.5 * np.sqrt(max(0.0, (1.0 - rvf) / rvf))
+    else:
+        p.pore_radius = kw.get("pore_radius", 5e-5)
+    p.surf_tension_coeff = kw.get("surf_tension_coeff", 0.0728)
+    p.rest_contact_angle = kw.get("rest_contact_angle", 40.8 / 180.0 * np.pi)
+    return p
 
 
 _lib = None
@@ -159,6 +190,19 @@ def _configure(L):
                                    C.POINTER(C.c_int)]
     for nm in ("wco_extrapolate", "wco_constrain_liquid_velocity"):
         getattr(L, nm).argtypes = [C.POINTER(SceneStruct)]
+    L.wco_set_elasto.argtypes = [C.POINTER(SceneStruct), C.c_int64, _dp, _dp, _dp, _dp, _dp,
+                                 _dp, _dp, _dp, C.POINTER(MixParams)]
+    L.wco_mix_field.restype = _dp
+    L.wco_mix_field.argtypes = [C.POINTER(SceneStruct), C.c_char_p, C.c_int]
+    L.wco_drag_coeff.restype = C.c_double
+    L.wco_drag_coeff.argtypes = [C.POINTER(MixParams), C.c_double, C.c_double, C.c_double,
+                                 C.c_double, _dp, C.c_double, C.c_int]
+    L.wco_rotated_drag.restype = C.c_double
+    L.wco_rotated_drag.argtypes = [_dp, _dp, C.c_int]
+    L.wco_rotated_pore_coeff.restype = C.c_double
+    L.wco_rotated_pore_coeff.argtypes = [_dp, C.c_double, C.c_int]
+    L.wco_capillary_pressure.restype = C.c_double
+    L.wco_capillary_pressure.argtypes = [C.POINTER(MixParams), C.c_double]
     L.wco_frozen_set.argtypes = [C.POINTER(SceneStruct), C.c_int]
     L.wco_substep_adjoint.restype = C.c_int
     L.wco_substep_adjoint.argtypes = [C.POINTER(SceneStruct), C.c_double, C.c_double, C.c_int,
@@ -182,6 +226,12 @@ def ref_lib():
     L.ref_blas_dot.argtypes = [_dp, _dp, C.c_long]
     L.ref_blas_abs_max.restype = C.c_double
     L.ref_blas_abs_max.argtypes = [_dp, C.c_long]
+    for nm in ("ref_rotated_drag", "ref_rotated_pore_coeff"):
+        if hasattr(L, nm):
+            getattr(L, nm).restype = C.c_double
+    if hasattr(L, "ref_rotated_drag"):
+        L.ref_rotated_drag.argtypes = [_dp, _dp, C.c_int]
+        L.ref_rotated_pore_coeff.argtypes = [_dp, C.c_double, C.c_int]
     return L
 
 
@@ -378,6 +428,21 @@ class OracleScene:
         ok = self.L.wco_substep_adjoint(self.s, dt, tol, maxit, ptr(x0), ptr(v0), ptr(C0),
                                         ptr(xb), ptr(vb), ptr(Cb), C.byref(gb))
         return bool(ok), xb, vb, Cb, gb.value
+
+    def set_elasto(self, x, m, fm, fvol, rest_vol, rest_vf, ori, sf, prm=None):
+        """The static porous solid: ne particles (SoA x and orientation)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64)
+                for a in (x, m, fm, fvol, rest_vol, rest_vf, ori, sf)]
+        self._elasto = arrs  # kept alive: the oracle copies them anyway
+        self._mixprm = prm if prm is not None else mix_params()
+        ne = arrs[0].shape[1] if arrs[0].ndim == 2 else arrs[0].size // 3
+        self.L.wco_set_elasto(self.s, ne, *(ptr(a) for a in arrs), C.byref(self._mixprm))
+
+    def mix_field(self, name, axis=0):
+        p = self.L.wco_mix_field(self.s, name.encode(), axis)
+        if not p:
+            raise KeyError(name)
+        return self._arr(p, self.st.g.nnode)
 
     def pre_solve(self, dt):
         """Phases up to and including assembly."""

@@ -26,6 +26,17 @@ double ref_blas_dot(const double* x, const double* y, long n) {
   std::vector<double> a(x, x + n), b(y, y + n);
   return robertbridson::BLAS::dot(a, b);
 }
+double ref_rotated_drag(const double* a, const double* d, int k) {
+  const Vector3s av(a[0], a[1], a[2]), dv(d[0], d[1], d[2]);
+  return k == 0 ? mathutils::get_rotated_drag_x(av, dv)
+                : (k == 1 ? mathutils::get_rotated_drag_y(av, dv) : mathutils::get_rotated_drag_z(av, dv));
+}
+double ref_rotated_pore_coeff(const double* a, double sf, int k) {
+  const Vector3s av(a[0], a[1], a[2]);
+  return k == 0 ? mathutils::get_rotated_pore_coeff_x(av, sf)
+                : (k == 1 ? mathutils::get_rotated_pore_coeff_y(av, sf)
+                          : mathutils::get_rotated_pore_coeff_z(av, sf));
+}
 double ref_blas_abs_max(const double* x, long n) {
   std::vector<double> a(x, x + n);
   return robertbridson::BLAS::abs_max(a);

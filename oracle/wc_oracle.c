@@ -26,10 +26,25 @@
  *   11 u* without the dt factor on gravity        12 liquid phi radius without the
  *                                                    0.883644 dx floor
  *   adjoint (wco_substep_adjoint) without:  13 the theta terms  14 the solid
- *   projection  15 the liquid phi's particle term  16 P2G's weight derivative */
+ *   projection  15 the liquid phi's particle term  16 P2G's weight derivative
+ *   mixture:  20 assembly without the (1 - psi) factor  21 pressure application
+ *   without the solid volume V_s Q  22 pore pressure gradient with its sign
+ *   flipped  23 Q formed from the liquid mass  24 the solid's P2G on the
+ *   cell-centre stencil for every axis */
 #ifndef WCO_PLANT
 #define WCO_PLANT 0
 #endif
+
+/* the static porous solid (the mixture section at the end of this file) */
+struct wco_mix;
+static void mix_touch(const wco_scene* s, uint8_t* touched);
+static void mix_p2g(wco_scene* s);
+static void mix_step_velocity(wco_scene* s, double dt);
+static void mix_free(struct wco_mix* m);
+/* a face's solid terms: volume fraction psi, solid volume Vs, drag ratios Q
+ * (mshdvm_hdvm) and P (mfhdvm_hdvm), psi_sf and solid mass */
+static void mix_face(const wco_scene* s, int d, int64_t n, double* psi, double* Vs, double* Q,
+                     double* P, double* psf, double* Ms);
 
 /* ------------------------------------------------------------------ grid */
 
@@ -454,6 +469,7 @@ void wco_scene_destroy(wco_scene* s) {
     free(s->frz_mass[d]);
   }
   free(s->frz_proj);
+  mix_free(s->mix);
   free_system(s);
   free(s);
 }
@@ -474,6 +490,7 @@ void wco_activate(wco_scene* s) {
                                     {0.5, 0.5, 0.0},
                                     {0.0, 0.0, 0.0},
                                     {0.5, 0.5, 0.5}};
+  if (s->mix) mix_touch(s, touched);
   for (int64_t p = 0; p < s->np; ++p) {
     const double px[3] = {s->x[p], s->x[s->np + p], s->x[2 * s->np + p]};
     for (int gi = 0; gi < 5; ++gi) {
@@ -691,6 +708,7 @@ void wco_p2g(wco_scene* s) {
     }
     free(mom[d]);
   }
+  if (s->mix) mix_p2g(s);
 }
 
 /* LinearizedImplicitEuler.cpp:224-337 stepVelocity for liquid without
@@ -702,6 +720,10 @@ void wco_p2g(wco_scene* s) {
  *   psi_fs = ((1-psi) invC + psi invCs P) M = invC M (LIE:1052-1107) */
 void wco_step_velocity(wco_scene* s, double dt) {
   const wco_grid* g = &s->g;
+  if (s->mix) {
+    mix_step_velocity(s, dt);
+    return;
+  }
   for (int d = 0; d < 3; ++d) {
     const double gd = s->gravity[d];
     for (int64_t n = 0; n < g->nnode; ++n) {
@@ -844,19 +866,22 @@ int64_t wco_assemble(wco_scene* s, double dt) {
         const int64_t fn = wco_node_index(g, q[0], q[1], q[2]);
         const double w = s->w[axis][fn];
         const double uf = s->ustar[axis][fn], pfs = s->psi_fs[axis][fn];
-        const double us = 0.0, psf = 0.0, usolid = 0.0;
+        double psi = 0.0, Vs = 0.0, Q = 1.0, Pm = 0.0, psf = 0.0, Ms = 0.0;
+        if (s->mix) mix_face(s, axis, fn, &psi, &Vs, &Q, &Pm, &psf, &Ms);
+        /* the solid's grid velocity m_node_v_plus: zero (not solved) */
+        const double us = 0.0, usolid = 0.0;
         const double flux = ((uf * pfs + us * psf) * w + usolid * (1.0 - w)) / dx;
         if (hi)
           rhs -= flux;
         else
           rhs += flux;
         if (w > 0.0) {
-          const double psi = 0.0, Vs = 0.0, Q = 1.0, invCs = 0.0;
+          const double invCs = 0.0; /* constructMsDVs runs only when the solid is solved */
           const double M = s->mass[axis][fn];
-          const double P = M > 1e-20 ? 0.0 / M : 1.0;
+          const double P = s->mix ? Pm : (M > 1e-20 ? 0.0 / M : 1.0);
           const double Vf = s->vol[axis][fn];
           double term = w * coeff;
-          term *= (1.0 - psi) * s->invC[axis][fn] * (Vf + Vs * Q) +
+          term *= (WCO_PLANT == 20 ? 1.0 : 1.0 - psi) * s->invC[axis][fn] * (Vf + Vs * Q) +
                   psi * invCs * (Vs + Vf * P);
           int nb[3] = {i, j, k};
           nb[axis] += hi ? 1 : -1;
@@ -1123,7 +1148,9 @@ void wco_apply_pressure(wco_scene* s, double dt) {
             s->valid[d][n] = 0;
             continue;
           }
-          const double Q = 1.0, Ms = 0.0, us = 0.0, Vs = 0.0;
+          double psi = 0.0, Vs = 0.0, Q = 1.0, Pm = 0.0, psf = 0.0, Ms = 0.0;
+          if (s->mix) mix_face(s, d, n, &psi, &Vs, &Q, &Pm, &psf, &Ms);
+          const double us = 0.0; /* the solid's P2G velocity: at rest */
           double rhs = s->rhs[d][n];
           rhs += Q * Ms * us;
           uint8_t valid = 0;
@@ -1143,7 +1170,7 @@ void wco_apply_pressure(wco_scene* s, double dt) {
               if (lex) pg -= s->pressure[ln];
               if (rex) pg += s->pressure[n];
               pg /= (dx * theta);
-              const double term = pg * dt * (s->vol[d][n] + Vs * Q);
+              const double term = pg * dt * (s->vol[d][n] + (WCO_PLANT == 21 ? 0.0 : Vs * Q));
               rhs -= term;
               valid = 1;
             }
@@ -1401,6 +1428,7 @@ static void theta_adjoint(const wco_scene* s, int d, int64_t fn, int64_t nl, int
 int wco_substep_adjoint(wco_scene* s, double dt, double tol, int maxit, const double* x0,
                         const double* v0, const double* C0, double* xb, double* vb, double* Cb,
                         double* gb) {
+  if (s->mix) return 0; /* the adjoint covers the liquid-only substep */
   const wco_grid* g = &s->g;
   const int64_t np = s->np, nn = g->nnode;
   const double dx = g->dx, invD = 4.0 / (dx * dx), fm = s->pmass;
@@ -1758,4 +1786,361 @@ int wco_substep_adjoint(wco_scene* s, double dt, double tol, int maxit, const do
   free(bb);
   free(phib);
   return ok;
+}
+
+/* ---------------------------------------- mixture: static porous solid */
+
+struct wco_mix {
+  wco_mix_params prm;
+  int64_t ne;
+  double *x, *m, *fm, *fvol, *rvol, *rvf, *ori, *sf; /* x, ori: 3 * ne SoA */
+  double *smass[3], *svol[3], *psi[3], *sat[3], *sfac[3], *ori_n[3]; /* ori_n: 3 * nnode */
+  double *Q[3], *P[3], *hdv[3], *psi_sf[3];
+  double *psi_p, *sat_p, *pp; /* cell centred */
+};
+
+static void mix_free(struct wco_mix* m) {
+  if (!m) return;
+  free(m->x);
+  free(m->m);
+  free(m->fm);
+  free(m->fvol);
+  free(m->rvol);
+  free(m->rvf);
+  free(m->ori);
+  free(m->sf);
+  for (int d = 0; d < 3; ++d) {
+    free(m->smass[d]);
+    free(m->svol[d]);
+    free(m->psi[d]);
+    free(m->sat[d]);
+    free(m->sfac[d]);
+    free(m->ori_n[d]);
+    free(m->Q[d]);
+    free(m->P[d]);
+    free(m->hdv[d]);
+    free(m->psi_sf[d]);
+  }
+  free(m->psi_p);
+  free(m->sat_p);
+  free(m->pp);
+  free(m);
+}
+
+void wco_set_elasto(wco_scene* s, int64_t ne, const double* x, const double* m,
+                    const double* fm, const double* fvol, const double* rest_vol,
+                    const double* rest_vf, const double* ori, const double* sf,
+                    const wco_mix_params* prm) {
+  mix_free(s->mix);
+  s->mix = NULL;
+  if (ne <= 0) return;
+  struct wco_mix* M = (struct wco_mix*)xcalloc(1, sizeof(struct wco_mix));
+  M->prm = *prm;
+  M->ne = ne;
+  M->x = (double*)xcalloc(3 * ne, sizeof(double));
+  M->ori = (double*)xcalloc(3 * ne, sizeof(double));
+  memcpy(M->x, x, sizeof(double) * 3 * ne);
+  memcpy(M->ori, ori, sizeof(double) * 3 * ne);
+  double** per[6] = {&M->m, &M->fm, &M->fvol, &M->rvol, &M->rvf, &M->sf};
+  const double* src[6] = {m, fm, fvol, rest_vol, rest_vf, sf};
+  for (int t = 0; t < 6; ++t) {
+    *per[t] = (double*)xcalloc(ne, sizeof(double));
+    memcpy(*per[t], src[t], sizeof(double) * ne);
+  }
+  const int64_t nn = s->g.nnode;
+  for (int d = 0; d < 3; ++d) {
+    M->smass[d] = (double*)xcalloc(nn, sizeof(double));
+    M->svol[d] = (double*)xcalloc(nn, sizeof(double));
+    M->psi[d] = (double*)xcalloc(nn, sizeof(double));
+    M->sat[d] = (double*)xcalloc(nn, sizeof(double));
+    M->sfac[d] = (double*)xcalloc(nn, sizeof(double));
+    M->ori_n[d] = (double*)xcalloc(3 * nn, sizeof(double));
+    M->Q[d] = (double*)xcalloc(nn, sizeof(double));
+    M->P[d] = (double*)xcalloc(nn, sizeof(double));
+    M->hdv[d] = (double*)xcalloc(nn, sizeof(double));
+    M->psi_sf[d] = (double*)xcalloc(nn, sizeof(double));
+  }
+  M->psi_p = (double*)xcalloc(nn, sizeof(double));
+  M->sat_p = (double*)xcalloc(nn, sizeof(double));
+  M->pp = (double*)xcalloc(nn, sizeof(double));
+  s->mix = M;
+}
+
+double* wco_mix_field(wco_scene* s, const char* name, int axis) {
+  struct wco_mix* M = s->mix;
+  if (!M || axis < 0) return NULL;
+  if (!strcmp(name, "ori")) return axis < 9 ? M->ori_n[axis / 3] + (int64_t)(axis % 3) * s->g.nnode : NULL;
+  if (!strcmp(name, "psi_p")) return M->psi_p;
+  if (!strcmp(name, "sat_p")) return M->sat_p;
+  if (!strcmp(name, "pp")) return M->pp;
+  if (axis > 2) return NULL;
+  if (!strcmp(name, "smass")) return M->smass[axis];
+  if (!strcmp(name, "svol")) return M->svol[axis];
+  if (!strcmp(name, "psi")) return M->psi[axis];
+  if (!strcmp(name, "sat")) return M->sat[axis];
+  if (!strcmp(name, "sf")) return M->sfac[axis];
+  if (!strcmp(name, "Q")) return M->Q[axis];
+  if (!strcmp(name, "P")) return M->P[axis];
+  if (!strcmp(name, "hdv")) return M->hdv[axis];
+  if (!strcmp(name, "psi_sf")) return M->psi_sf[axis];
+  return NULL;
+}
+
+static double clamp01(double a) { return a < 0.0 ? 0.0 : (a > 1.0 ? 1.0 : a); }
+
+/* The orientation terms: get_rotated_drag_x/y/z and get_rotated_pore_coeff_x/y/z
+ * (MathUtilities.h:792-878) are the component sums of one row of a tensor
+ * that is transversely isotropic about n = a / |a|, T = t I + (u - t) n n^T:
+ * row k sums to t + (u - t) n_k (n_x + n_y + n_z).  The drag tensor has
+ * t = d(0) = d(1) (c_xy: every caller passes c_xy twice) and u = d(2)
+ * (c_z); the pore tensor has t = 2 - sf and u = 1 + sf.  Both are zero for
+ * a = 0 (the reference divides a zero numerator by max(1e-63, |a|^2)). */
+static double iso_row_sum(const double a[3], double t, double u, int k) {
+  const double s2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+  if (s2 == 0.0) return 0.0;
+  const double r = sqrt(s2);
+  const double n[3] = {a[0] / r, a[1] / r, a[2] / r};
+  return t + (u - t) * n[k] * (n[0] + n[1] + n[2]);
+}
+
+double wco_rotated_drag(const double a[3], const double d[3], int index) {
+  return iso_row_sum(a, d[0], d[2], index);
+}
+
+double wco_rotated_pore_coeff(const double a[3], double sf, int index) {
+  return iso_row_sum(a, 2.0 - sf, 1.0 + sf, index);
+}
+
+/* TwoDScene.cpp:563-571 */
+double wco_capillary_pressure(const wco_mix_params* prm, double psi) {
+  if (1.0 - psi < 1e-20 || prm->pore_radius < 1e-20) return 0.0;
+  const double alpha = psi / (1. - psi);
+  const double contact_angle = cos(prm->rest_contact_angle);
+  return alpha * prm->surf_tension_coeff * contact_angle / prm->pore_radius;
+}
+
+/* TwoDScene.cpp:4042-4097, material 0 (the liquid: its viscosity and
+ * density): the permeabilities along and across the yarns ka, kb of volume
+ * fraction psi and yarn diameter, the coefficients mu / k plus the
+ * nonlinear (Ergun) term, mixed by the shape factor and rotated by the
+ * orientation. */
+double wco_drag_coeff(const wco_mix_params* prm, double rho, double psi, double sat, double dv,
+                      const double ori[3], double sf, int index) {
+  const double o2 = ori[0] * ori[0] + ori[1] * ori[1] + ori[2] * ori[2];
+  if (!prm->use_drag || psi == 0.0 || sat == 0.0 || o2 < 1e-20) return 0.0;
+  const double ergun_coeff = prm->use_nonlinear_drag ? 0.1428869017 : 0.0;
+  const double di = prm->yarn_diameter;
+  double ka = (-log(psi) - 1.476 + 2.0 * psi - 0.5 * psi * psi) / (16.0 * psi) * di * di;
+  if (ka < 1e-20) ka = 1e-20;
+  double kb = (-log(psi) - 1.476 + 2.0 * psi - 1.774 * psi * psi + 4.078 * pow(psi, 3.0)) /
+              (32.0 * psi) * di * di;
+  if (kb < 1e-20) kb = 1e-20;
+  const double mu = prm->viscosity;
+  const double pw = prm->yazdchi_power;
+  double ca = mu / ka + ergun_coeff * pow(di, pw - 1.0) * pow(mu, 1.0 - pw) /
+                            (pow(1.0 - psi, 1.5) * sqrt(ka)) * pow(rho * fabs(dv), pw);
+  if (ca > 1e+63) ca = 1e+63;
+  double cb = mu / kb + ergun_coeff * pow(di, pw - 1.0) * pow(mu, 1.0 - pw) /
+                            (pow(1.0 - psi, 1.5) * sqrt(kb)) * pow(rho * fabs(dv), pw);
+  if (cb > 1e+63) cb = 1e+63;
+  const double c_xy = ca * (1.0 - sf) + cb * sf;
+  const double c_z = ca * sf + cb * (1.0 - sf);
+  const double cv[3] = {c_xy, c_xy, c_z};
+  return wco_rotated_drag(ori, cv, index);
+}
+
+/* the five stencils of every solid particle touch their buckets (findNodes
+ * runs over all particles, TwoDScene.cpp:3683-3752) */
+static void mix_touch(const wco_scene* s, uint8_t* touched) {
+  const struct wco_mix* M = s->mix;
+  const wco_grid* g = &s->g;
+  static const double offs[5][3] = {
+      {0.0, 0.5, 0.5}, {0.5, 0.0, 0.5}, {0.5, 0.5, 0.0}, {0.0, 0.0, 0.0}, {0.5, 0.5, 0.5}};
+  for (int64_t p = 0; p < M->ne; ++p) {
+    const double px[3] = {M->x[p], M->x[M->ne + p], M->x[2 * M->ne + p]};
+    for (int gi = 0; gi < 5; ++gi) {
+      int blo[3], bhi[3];
+      for (int d = 0; d < 3; ++d) {
+        const int b = stencil_base(g, d, px[d], offs[gi][d]);
+        blo[d] = b >> 2;
+        bhi[d] = (b + 2) >> 2;
+      }
+      for (int bk = blo[2]; bk <= bhi[2]; ++bk)
+        for (int bj = blo[1]; bj <= bhi[1]; ++bj)
+          for (int bi = blo[0]; bi <= bhi[0]; ++bi) touched[wco_bucket_index(g, bi, bj, bk)] = 1;
+    }
+  }
+}
+
+/* The solid's branch of mapParticleNodesAPIC (TwoDScene.cpp:6686-6699,
+ * 6715-6736: the orientation sum normalised where its norm exceeds 1e-20) and mapParticleSaturationPsiNodes (:5952-5992) with
+ * updatePorePressureNodes (:5933-5947). */
+static void mix_p2g(wco_scene* s) {
+  struct wco_mix* M = s->mix;
+  const wco_grid* g = &s->g;
+  const int64_t nn = g->nnode, ne = M->ne;
+  const double dx = g->dx, dV = dx * dx * dx;
+  double* vsol = (double*)xcalloc(nn, sizeof(double));
+  double* vfe = (double*)xcalloc(nn, sizeof(double));
+  double* rw = (double*)xcalloc(nn, sizeof(double));
+  for (int d = 0; d < 3; ++d) {
+    memset(M->smass[d], 0, sizeof(double) * nn);
+    memset(M->sfac[d], 0, sizeof(double) * nn);
+    memset(M->ori_n[d], 0, sizeof(double) * 3 * nn);
+    memset(vsol, 0, sizeof(double) * nn);
+    memset(vfe, 0, sizeof(double) * nn);
+    memset(rw, 0, sizeof(double) * nn);
+    for (int64_t p = 0; p < ne; ++p) {
+      const double px[3] = {M->x[p], M->x[ne + p], M->x[2 * ne + p]};
+      int base[3];
+      double w[3][3];
+      for (int a = 0; a < 3; ++a) {
+        const double off = WCO_PLANT == 24 ? 0.5 : face_off[d][a];
+        base[a] = stencil_base(g, a, px[a], off);
+        stencil_weights(g, px[a], base[a], off, w[a]);
+      }
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            if (!node_active(s, i, j, k)) continue;
+            const int64_t n = wco_node_index(g, i, j, k);
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            M->smass[d][n] += (M->m[p] + M->fm[p]) * ww;
+            vsol[n] += M->rvol[p] * M->rvf[p] * ww;
+            vfe[n] += M->fvol[p] * ww;
+            M->sfac[d][n] += M->sf[p] * ww;
+            rw[n] += ww;
+            for (int c = 0; c < 3; ++c) M->ori_n[d][c * nn + n] += M->ori[c * ne + p] * ww;
+          }
+    }
+    for (int64_t n = 0; n < nn; ++n) {
+      if (rw[n] > 1e-20) M->sfac[d][n] /= rw[n];
+      double* o = M->ori_n[d];
+      const double lo = sqrt(o[n] * o[n] + o[nn + n] * o[nn + n] + o[2 * nn + n] * o[2 * nn + n]);
+      if (lo > 1e-20)
+        for (int c = 0; c < 3; ++c) o[c * nn + n] /= lo;
+      M->svol[d][n] = vsol[n] + vfe[n];
+      M->psi[d][n] = clamp01(vsol[n] / dV);
+      const double room = dV - vsol[n] > 1e-20 ? dV - vsol[n] : 1e-20;
+      M->sat[d][n] = clamp01((s->vol[d][n] + vfe[n]) / room);
+    }
+  }
+  /* cell centres: the liquid and solid volume of every particle's cell
+   * stencil (liquid particles carry no solid volume), then the pore pressure */
+  memset(vsol, 0, sizeof(double) * nn);
+  memset(vfe, 0, sizeof(double) * nn);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int64_t cnt = pass == 0 ? ne : s->np;
+    for (int64_t p = 0; p < cnt; ++p) {
+      const double* X = pass == 0 ? M->x : s->x;
+      const double px[3] = {X[p], X[cnt + p], X[2 * cnt + p]};
+      const double fv = pass == 0 ? M->fvol[p] : s->pvol;
+      const double sv = pass == 0 ? M->rvol[p] * M->rvf[p] : 0.0;
+      int base[3];
+      double w[3][3];
+      for (int a = 0; a < 3; ++a) {
+        base[a] = stencil_base(g, a, px[a], 0.5);
+        stencil_weights(g, px[a], base[a], 0.5, w[a]);
+      }
+      for (int c2 = 0; c2 < 3; ++c2)
+        for (int c1 = 0; c1 < 3; ++c1)
+          for (int c0 = 0; c0 < 3; ++c0) {
+            const int i = base[0] + c0, j = base[1] + c1, k = base[2] + c2;
+            if (!node_active(s, i, j, k)) continue;
+            const int64_t n = wco_node_index(g, i, j, k);
+            const double ww = w[0][c0] * w[1][c1] * w[2][c2];
+            vfe[n] += fv * ww;
+            vsol[n] += sv * ww;
+          }
+    }
+  }
+  for (int64_t n = 0; n < nn; ++n) {
+    M->psi_p[n] = clamp01(vsol[n] / dV);
+    const double room = dV - vsol[n] > 1e-20 ? dV - vsol[n] : 1e-20;
+    M->sat_p[n] = clamp01(vfe[n] / room);
+    M->pp[n] = wco_capillary_pressure(&M->prm, M->psi_p[n]) * (1.0 - M->sat_p[n]);
+  }
+  free(vsol);
+  free(vfe);
+  free(rw);
+}
+
+static void mix_face(const wco_scene* s, int d, int64_t n, double* psi, double* Vs, double* Q,
+                     double* P, double* psf, double* Ms) {
+  const struct wco_mix* M = s->mix;
+  *psi = M->psi[d][n];
+  *Vs = M->svol[d][n];
+  *Q = M->Q[d][n];
+  *P = M->P[d][n];
+  *psf = M->psi_sf[d][n];
+  *Ms = M->smass[d][n];
+}
+
+/* stepVelocity with the solid not solved (LinearizedImplicitEuler.cpp:
+ * 224-337): constructNodeForce (:548-659: gravity, then the pore pressure
+ * gradient of Pressure.cpp:26-142, then M u), constructHDV (:846-978, no
+ * drag by air: the damped mass is the solid mass), constructInvMDV
+ * (:1109-1148), performLocalSolve (:128-175) and constructPsiSF
+ * (:1052-1107; invCs stays zero: constructMsDVs runs only when the solid is
+ * solved). */
+static void mix_step_velocity(wco_scene* s, double dt) {
+  struct wco_mix* M = s->mix;
+  const wco_grid* g = &s->g;
+  const int64_t nn = g->nnode;
+  const double dx = g->dx;
+  for (int d = 0; d < 3; ++d) {
+    const double gd = s->gravity[d];
+    for (int k = 0; k < g->nx[2]; ++k)
+      for (int j = 0; j < g->nx[1]; ++j)
+        for (int i = 0; i < g->nx[0]; ++i) {
+          const int64_t n = wco_node_index(g, i, j, k);
+          if (!s->act[n >> 6]) {
+            s->rhs[d][n] = s->invC[d][n] = s->ustar[d][n] = s->psi_fs[d][n] = 0.0;
+            M->Q[d][n] = M->P[d][n] = M->hdv[d][n] = M->psi_sf[d][n] = 0.0;
+            continue;
+          }
+          const double Mf = s->mass[d][n];
+          double rhs = 0.0;
+          rhs -= Mf * gd * (-dt);
+          /* computePorePressureGrads: (p_right - p_left) / dx V_f dt sat w
+           * times the rotated pore coefficient, on faces with liquid volume
+           * and solid weight */
+          const double vf = s->vol[d][n], wv = s->w[d][n];
+          if (vf > 0.0 && wv > 0.0) {
+            int l[3] = {i, j, k};
+            l[d] -= 1;
+            const double pp0 = node_active(s, l[0], l[1], l[2])
+                                   ? M->pp[wco_node_index(g, l[0], l[1], l[2])]
+                                   : 0.0;
+            const double pp1 = M->pp[n];
+            const double grad = (WCO_PLANT == 22 ? pp0 - pp1 : pp1 - pp0) / dx;
+            const double a[3] = {M->ori_n[d][n], M->ori_n[d][nn + n], M->ori_n[d][2 * nn + n]};
+            rhs += grad * vf * dt * M->sat[d][n] * wv *
+                   wco_rotated_pore_coeff(a, M->sfac[d][n], d);
+          }
+          rhs += Mf * s->vel[d][n];
+          s->rhs[d][n] = rhs;
+          /* drag: the solid is at rest, so dv is the liquid's P2G velocity */
+          const double a[3] = {M->ori_n[d][n], M->ori_n[d][nn + n], M->ori_n[d][2 * nn + n]};
+          const double dc = wco_drag_coeff(&M->prm, s->rho, M->psi[d][n], M->sat[d][n],
+                                           s->vel[d][n] - 0.0, a, M->sfac[d][n], d);
+          const double Vm = M->svol[d][n] + vf;
+          const double hdV = dt * dc * Vm;
+          const double damped = M->smass[d][n];
+          const double mfhdvm = Mf + hdV, mshdvm = damped + hdV;
+          const double P = mfhdvm > 1e-20 ? hdV / mfhdvm : 1.0;
+          const double Q = WCO_PLANT == 23 ? P : (mshdvm > 1e-20 ? hdV / mshdvm : 1.0);
+          M->hdv[d][n] = hdV;
+          M->P[d][n] = P;
+          M->Q[d][n] = Q;
+          const double Cc = Mf + Q * damped;
+          const double invC = Cc > 1e-20 ? 1.0 / Cc : 1.0;
+          s->invC[d][n] = invC;
+          s->ustar[d][n] = Mf > 1e-20 ? rhs / Mf : rhs;
+          const double psi = M->psi[d][n], invCs = 0.0;
+          s->psi_fs[d][n] = ((1.0 - psi) * invC + psi * invCs * P) * Mf;
+          M->psi_sf[d][n] = (psi * invCs + (1.0 - psi) * invC * Q) * damped;
+        }
+  }
 }

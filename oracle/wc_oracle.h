@@ -150,6 +150,9 @@ typedef struct {
   uint8_t* frz_clamp[3];
   uint8_t* frz_mass[3];
   uint8_t* frz_proj;
+
+  /* the static porous solid of the mixture terms (NULL: liquid only) */
+  struct wco_mix* mix;
 } wco_scene;
 
 wco_scene* wco_scene_create(int ni, int nj, int nk, double dx, double rho,
@@ -193,6 +196,47 @@ void wco_frozen_set(wco_scene* s, int mode);
 int wco_substep_adjoint(wco_scene* s, double dt, double tol, int maxit, const double* x0,
                         const double* v0, const double* C0, double* xb, double* vb, double* Cb,
                         double* gb);
+
+/* ---- the mixture terms with a static porous solid (wc_oracle.c) ----
+ * Elasto particles (the cloth's vertices) that the liquid path sees with the
+ * solid not solved (LiquidInfo::solve_solid = 0) and at rest: their P2G
+ * (TwoDScene.cpp:6686-6699) gives each face the solid mass, volume, volume
+ * fraction psi, saturation, orientation and shape factor; the grid update
+ * takes the drag between the phases (constructHDV, constructInvMDV,
+ * constructPsiSF, LinearizedImplicitEuler.cpp:846-1107, with the drag of
+ * TwoDScene.cpp:4042-4097) and the pore pressure gradient
+ * (computePorePressureGrads, Pressure.cpp:26-142, from
+ * mapParticleSaturationPsiNodes / updatePorePressureNodes, TwoDScene.cpp:
+ * 5933-5992 and getCapillaryPressure :563-571); the assembly and the
+ * pressure application take psi, the solid volume and the drag ratio Q
+ * (Pressure.cpp:444-791, 1137-1314; LinearizedImplicitEuler.cpp:689-717).
+ * The solid velocity is zero on the grid (the solid's own P2G velocity of a
+ * resting solid, and m_node_v_plus, which stays zero when the solid is not
+ * solved), and the solid particles do not move.  LiquidInfo fields: */
+typedef struct {
+  int use_drag, use_nonlinear_drag;
+  double yarn_diameter, viscosity, yazdchi_power;
+  double pore_radius, surf_tension_coeff, rest_contact_angle;
+} wco_mix_params;
+/* ne solid particles: x (3*ne SoA), solid mass m, carried fluid mass fm and
+ * volume fvol, rest volume and rest volume fraction, orientation (3*ne SoA)
+ * and shape factor.  ne = 0 removes the solid. */
+void wco_set_elasto(wco_scene* s, int64_t ne, const double* x, const double* m,
+                    const double* fm, const double* fvol, const double* rest_vol,
+                    const double* rest_vf, const double* ori, const double* sf,
+                    const wco_mix_params* prm);
+/* a node field of the mixture (NULL without a solid): "smass", "svol",
+ * "psi", "sat", "sf", "Q", "P", "hdv", "psi_sf" per axis, "ori" (axis * 3 +
+ * component), cell fields "psi_p", "sat_p", "pp" (axis ignored) */
+double* wco_mix_field(wco_scene* s, const char* name, int axis);
+/* TwoDScene::getDragCoeffWithOrientation for the liquid (material 0) */
+double wco_drag_coeff(const wco_mix_params* prm, double rho, double psi, double sat, double dv,
+                      const double ori[3], double sf, int index);
+/* mathutils::get_rotated_drag_x/y/z (MathUtilities.h:819-878) for d = (c, c, e)
+ * and get_rotated_pore_coeff_x/y/z (:792-817) */
+double wco_rotated_drag(const double a[3], const double d[3], int index);
+double wco_rotated_pore_coeff(const double a[3], double sf, int index);
+double wco_capillary_pressure(const wco_mix_params* prm, double psi);
 
 /* ---- PCGSolver restatement on a sorted CSR matrix ---- */
 void wco_spmv(int64_t n, const uint32_t* rowstart, const uint32_t* col,
