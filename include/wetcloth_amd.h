@@ -87,6 +87,34 @@ wc_status wc_num_particles(const wc_ctx* ctx, int64_t* n);
 wc_status wc_seed_block(wc_ctx* ctx, const double lo[3], const double hi[3],
                         uint64_t seed, double sigma, int64_t* n_out);
 
+/* ---- a static porous solid in the liquid path (the mixture terms) ----
+ * The LiquidInfo fields the mixture terms read (TwoDScene.h:35-95; the
+ * parser's defaults, TwoDSceneXMLParser.cpp:1849-1904, are CGS values). */
+typedef struct wc_mix_params {
+  int32_t use_drag, use_nonlinear_drag;
+  double yarn_diameter, viscosity, yazdchi_power;
+  double pore_radius, surf_tension_coeff, rest_contact_angle;
+} wc_mix_params;
+/* The elasto particles (the cloth's vertices) of a scene whose solid is at
+ * rest and not solved (LiquidInfo::solve_solid = 0): replaces the elasto
+ * entries of TwoDScene's m_x, m_m, m_fluid_m, m_fluid_vol, m_rest_vol,
+ * m_rest_volume_fraction, m_orientation and m_shape_factor (TwoDScene.h:
+ * 932-960).  Arrays in structure-of-arrays form: x and orientation 3*ne
+ * (x[d*ne+p]), the others ne.  Every following substep then takes the
+ * solid's P2G (TwoDScene.cpp:6686-6736), the cell-centred saturation and
+ * volume fraction with the pore pressure (:5933-5992), the drag between the
+ * phases (LinearizedImplicitEuler.cpp:846-1107 with TwoDScene.cpp:
+ * 4042-4097), the pore pressure gradient (Pressure.cpp:26-142) and the
+ * volume-fraction-weighted assembly and pressure application
+ * (Pressure.cpp:444-791, 1137-1314).  The particles must lie in the
+ * container box.  ne = 0 removes the solid.  WC_ERR_STATE while decisions
+ * are recorded or replayed (the adjoint covers the liquid-only substep). */
+wc_status wc_set_elasto_particles(wc_ctx* ctx, int64_t ne, const double* x, const double* m,
+                                  const double* fluid_m, const double* fluid_vol,
+                                  const double* rest_vol, const double* rest_volume_fraction,
+                                  const double* orientation, const double* shape_factor,
+                                  const wc_mix_params* prm);
+
 /* ---- on-disk formats of the reference's serializer (App/TwoDSceneSerializer.cpp) ----
  * The particles are written in the caller's order (the order of
  * wc_set_particles), as doubles; the files are written before the call
@@ -239,6 +267,21 @@ wc_status wc_rollout_backward(wc_ctx* ctx, double dt, double tol, int32_t maxit,
 #define WC_FIELD_DOF 9       /* int32 per node: pressure DOF or -1 */
 #define WC_FIELD_PRESSURE 10 /* double per node */
 #define WC_FIELD_VALID 11    /* uint8 per node and axis */
+/* with a solid (wc_set_elasto_particles), double per node and axis: */
+#define WC_FIELD_PSI 12      /* solid volume fraction of the face */
+#define WC_FIELD_SAT 13      /* saturation of the face */
+#define WC_FIELD_SMASS 14    /* solid mass (elasto mass + carried liquid) */
+#define WC_FIELD_SVOL 15     /* solid volume (+ carried liquid volume) */
+#define WC_FIELD_SF 16       /* shape factor */
+#define WC_FIELD_ORI_X 17    /* orientation, normalised: x, y, z components */
+#define WC_FIELD_ORI_Y 18
+#define WC_FIELD_ORI_Z 19
+#define WC_FIELD_Q 20        /* drag ratio hdV / (M_s + hdV) */
+#define WC_FIELD_HDV 21      /* dt * drag coefficient * (V_s + V_f) */
+/* and per cell (axis ignored): */
+#define WC_FIELD_PSI_P 22    /* cell-centred volume fraction */
+#define WC_FIELD_SAT_P 23    /* cell-centred saturation */
+#define WC_FIELD_PORE_P 24   /* pore pressure */
 wc_status wc_grid_info(const wc_ctx* ctx, int32_t nx[3], int64_t* nnode,
                        int64_t* nbucket);
 wc_status wc_download_field(wc_ctx* ctx, int32_t field, int32_t axis,

@@ -24,6 +24,9 @@ WC_MIXED = 1
 FIELD = {
     "act": 0, "lphi": 1, "w": 2, "mass": 3, "vel": 4, "vol": 5, "rhs": 6,
     "ustar": 7, "unew": 8, "dof": 9, "pressure": 10, "valid": 11,
+    # with a solid (Scene.set_elasto)
+    "psi": 12, "sat": 13, "smass": 14, "svol": 15, "sf": 16, "ori_x": 17, "ori_y": 18,
+    "ori_z": 19, "Q": 20, "hdv": 21, "psi_p": 22, "sat_p": 23, "pp": 24,
 }
 _FIELD_DTYPE = {"act": np.uint8, "dof": np.int32, "valid": np.uint8}
 
@@ -51,6 +54,35 @@ class WcDesc(C.Structure):
         ("precision", C.c_int32),
         ("max_particles", C.c_int64),
     ]
+
+
+class MixParams(C.Structure):
+    """wc_mix_params: the LiquidInfo fields of the mixture terms."""
+    _fields_ = [("use_drag", C.c_int32), ("use_nonlinear_drag", C.c_int32),
+                ("yarn_diameter", C.c_double), ("viscosity", C.c_double),
+                ("yazdchi_power", C.c_double), ("pore_radius", C.c_double),
+                ("surf_tension_coeff", C.c_double), ("rest_contact_angle", C.c_double)]
+
+
+def mix_params(**kw) -> MixParams:
+    """LiquidInfo's mixture fields in SI units: the parser's defaults
+    (TwoDSceneXMLParser.cpp:1849-1904, CGS there) converted, any of them
+    overridden by keyword; a rest_volume_fraction keyword derives the pore
+    radius from the yarn diameter as the parser does (:1960-1963)."""
+    p = MixParams()
+    p.use_drag = int(kw.get("use_drag", 1))
+    p.use_nonlinear_drag = int(kw.get("use_nonlinear_drag", 1))
+    p.yarn_diameter = float(kw.get("yarn_diameter", 1e-4))
+    p.viscosity = float(kw.get("viscosity", 8.9e-4))
+    p.yazdchi_power = float(kw.get("yazdchi_power", 1.6))
+    if "rest_volume_fraction" in kw:
+        rvf = float(kw["rest_volume_fraction"])
+        p.pore_radius = p.yarn_diameter * 0.5 * float(np.sqrt(max(0.0, (1.0 - rvf) / rvf)))
+    else:
+        p.pore_radius = float(kw.get("pore_radius", 5e-5))
+    p.surf_tension_coeff = float(kw.get("surf_tension_coeff", 0.0728))
+    p.rest_contact_angle = float(kw.get("rest_contact_angle", 40.8 / 180.0 * np.pi))
+    return p
 
 
 class WetclothError(RuntimeError):
@@ -112,6 +144,8 @@ def lib():
                                        C.c_int32, _dp, _i32p]
         L.wc_pcg_solve_csr_noalloc.argtypes = L.wc_pcg_solve_csr.argtypes
         L.wc_set_gravity.argtypes = [vp, _dp]
+        L.wc_set_elasto_particles.argtypes = [vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                              _dp, C.POINTER(MixParams)]
         L.wc_set_frozen.argtypes = [vp, C.c_int32, C.c_int32]
         L.wc_checkpoint.argtypes = [vp, C.c_int32, C.c_int32]
         L.wc_frozen_departures.argtypes = [vp, _i64p]
@@ -322,6 +356,25 @@ class Scene:
         """The gravity vector of the following substeps."""
         g = np.ascontiguousarray(g, dtype=np.float64)
         _check(lib().wc_set_gravity(self.h, _ptr(g)), self.h)
+
+    def set_elasto(self, x, m, fm, fvol, rest_vol, rest_vf, ori, sf, prm=None):
+        """The resting, unsolved solid (the cloth's vertices): ne particles
+        with positions x (3, ne), mass m, carried liquid mass fm and volume
+        fvol, rest volume and rest volume fraction, orientation (3, ne) and
+        shape factor; prm = mix_params().  x = None removes the solid."""
+        if x is None:
+            _check(lib().wc_set_elasto_particles(self.h, 0, *([None] * 8), None), self.h)
+            return
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(3, -1)
+        ne = x.shape[1]
+        per = [np.ascontiguousarray(a, dtype=np.float64).reshape(ne)
+               for a in (m, fm, fvol, rest_vol, rest_vf)]
+        ori = np.ascontiguousarray(ori, dtype=np.float64).reshape(3, ne)
+        sf = np.ascontiguousarray(sf, dtype=np.float64).reshape(ne)
+        self._mixprm = mix_params() if prm is None else prm
+        _check(lib().wc_set_elasto_particles(self.h, ne, _ptr(x), *(_ptr(a) for a in per),
+                                             _ptr(ori), _ptr(sf), C.byref(self._mixprm)),
+               self.h)
 
     # ---- the adjoint (no reference counterpart) ----
     def set_frozen(self, mode, slot=0):

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libwcgpu.so")
 ARCH = os.environ.get("WC_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["wc_scan.hip", "wc_pcg.hip", "wc_sweep.hip", "wc_amg.hip", "wc_particles.hip",
-           "wc_grid.hip", "wc_adjoint.hip", "wc_capi.cpp"]
+           "wc_grid.hip", "wc_adjoint.hip", "wc_mix.hip", "wc_capi.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

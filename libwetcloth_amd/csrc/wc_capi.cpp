@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <charconv>
 #include <climits>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -139,6 +140,7 @@ void release_all(wc_ctx* c) {
   c->dof_ijk.release();
   c->scan_tmp.release();
   wc::adjoint_release(c);
+  c->mix.release();
 }
 
 wc::DevCsr system_csr(const wc_ctx* c) {
@@ -311,6 +313,53 @@ wc_status wc_seed_block(wc_ctx* c, const double lo[3], const double hi[3],
   return WC_OK;
 }
 
+wc_status wc_set_elasto_particles(wc_ctx* c, int64_t ne, const double* x, const double* m,
+                                  const double* fluid_m, const double* fluid_vol,
+                                  const double* rest_vol, const double* rest_volume_fraction,
+                                  const double* orientation, const double* shape_factor,
+                                  const wc_mix_params* prm) {
+  if (!ready(c) || ne < 0) return fail(c, WC_ERR_ARG, "bad argument");
+  if (c->frz_mode != 0)
+    return fail(c, WC_ERR_STATE,
+                "decisions are being recorded or replayed: the adjoint covers the liquid-only "
+                "substep");
+  if (ne > 0) {
+    if (!x || !m || !fluid_m || !fluid_vol || !rest_vol || !rest_volume_fraction ||
+        !orientation || !shape_factor || !prm)
+      return fail(c, WC_ERR_ARG, "every solid array and the parameters are needed");
+    const double vals[6] = {prm->yarn_diameter, prm->viscosity, prm->yazdchi_power,
+                            prm->pore_radius, prm->surf_tension_coeff, prm->rest_contact_angle};
+    for (double v : vals)
+      if (!std::isfinite(v)) return fail(c, WC_ERR_ARG, "the mixture parameters must be finite");
+    if (prm->use_drag && (prm->yarn_diameter <= 0 || prm->viscosity <= 0))
+      return fail(c, WC_ERR_ARG, "the drag needs a positive yarn diameter and viscosity");
+    for (int d = 0; d < 3; ++d) {
+      const double hi = c->g.n[d] * c->g.dx;
+      for (int64_t p = 0; p < ne; ++p) {
+        const double v = x[d * ne + p];
+        if (!(v >= 0.0 && v <= hi))
+          return fail(c, WC_ERR_ARG, "solid particle " + std::to_string(p) +
+                                         " lies outside the container box");
+      }
+    }
+    const double* per[6] = {m, fluid_m, fluid_vol, rest_vol, rest_volume_fraction, shape_factor};
+    for (const double* a : per)
+      for (int64_t p = 0; p < ne; ++p)
+        if (!std::isfinite(a[p]) || a[p] < 0.0)
+          return fail(c, WC_ERR_ARG, "solid particle " + std::to_string(p) +
+                                         ": masses, volumes, fractions and shape factors must "
+                                         "be finite and non-negative");
+    for (int64_t p = 0; p < 3 * ne; ++p)
+      if (!std::isfinite(orientation[p]))
+        return fail(c, WC_ERR_ARG, "the orientations must be finite");
+  }
+  WC_TRY(c, {
+    wc::mix_set(c, ne, x, m, fluid_m, fluid_vol, rest_vol, rest_volume_fraction, orientation,
+                shape_factor, ne > 0 ? *prm : wc_mix_params{});
+  })
+  return WC_OK;
+}
+
 wc_status wc_rebucketize(wc_ctx* c) {
   if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
   WC_TRY(c, {
@@ -324,8 +373,10 @@ wc_status wc_map_particle_nodes(wc_ctx* c) {
   if (!ready(c)) return fail(c, WC_ERR_ARG, "bad context");
   WC_TRY(c, {
     wc::particles_p2g(c);
+    if (c->mix.ne > 0) wc::mix_touch(c);
     wc::grid_activate(c);
     wc::grid_nodes(c);
+    if (c->mix.ne > 0) wc::mix_grid(c);
     WC_HIP(hipStreamSynchronize(c->stream));
   })
   return WC_OK;
@@ -429,8 +480,10 @@ bool substep(wc_ctx* c, double dt, double tol, int maxit, double* res, int* it) 
   wc::particles_sort(c);
   record(c, 1);
   wc::particles_p2g(c);
+  if (c->mix.ne > 0) wc::mix_touch(c);
   wc::grid_activate(c);
   wc::grid_nodes(c);
+  if (c->mix.ne > 0) wc::mix_grid(c);
   record(c, 2);
   wc::grid_step_velocity(c, dt);
   record(c, 3);
@@ -515,6 +568,10 @@ wc_status set_frozen(wc_ctx* c, int mode, int slot) {
     return fail(c, WC_ERR_ARG, "mode must be 0, 1 or 2 and slot in [0, 2^20)");
   if (mode != 0 && c->mixed)
     return fail(c, WC_ERR_STATE, "recorded decisions and the adjoint need a WC_FP64 context");
+  if (mode != 0 && c->mix.ne > 0)
+    return fail(c, WC_ERR_STATE,
+                "the adjoint covers the liquid-only substep: remove the solid first "
+                "(wc_set_elasto_particles with ne = 0)");
   if (mode == 2 && ((size_t)slot >= c->frz.size() || !c->frz[(size_t)slot].recorded ||
                     c->frz[(size_t)slot].np != c->np))
     return fail(c, WC_ERR_STATE,
@@ -766,6 +823,42 @@ wc_status wc_download_field(wc_ctx* c, int32_t field, int32_t axis, void* host) 
       case WC_FIELD_VALID:
         WC_HIP(hipMemcpy(host, c->valid[axis].p, nn, hipMemcpyDeviceToHost));
         break;
+      case WC_FIELD_PSI:
+      case WC_FIELD_SAT:
+      case WC_FIELD_SMASS:
+      case WC_FIELD_SVOL:
+      case WC_FIELD_SF:
+      case WC_FIELD_ORI_X:
+      case WC_FIELD_ORI_Y:
+      case WC_FIELD_ORI_Z:
+      case WC_FIELD_Q:
+      case WC_FIELD_HDV:
+      case WC_FIELD_PSI_P:
+      case WC_FIELD_SAT_P:
+      case WC_FIELD_PORE_P: {
+        const wc::MixState& M = c->mix;
+        if (M.ne == 0) return fail(c, WC_ERR_STATE, "no solid (wc_set_elasto_particles)");
+        const double* src = nullptr;
+        switch (field) {
+          case WC_FIELD_PSI: src = M.psi[axis].p; break;
+          case WC_FIELD_SAT: src = M.sat[axis].p; break;
+          case WC_FIELD_SMASS: src = M.plane(c->g, axis, wc::kMixPlaneMass); break;
+          case WC_FIELD_SVOL: src = M.plane(c->g, axis, wc::kMixPlaneSvol); break;
+          case WC_FIELD_SF: src = M.plane(c->g, axis, wc::kMixPlaneSf); break;
+          case WC_FIELD_ORI_X:
+          case WC_FIELD_ORI_Y:
+          case WC_FIELD_ORI_Z:
+            src = M.plane(c->g, axis, wc::kMixPlaneOri + (field - WC_FIELD_ORI_X));
+            break;
+          case WC_FIELD_Q: src = M.Q[axis].p; break;
+          case WC_FIELD_HDV: src = M.hdv[axis].p; break;
+          case WC_FIELD_PSI_P: src = M.psi_p.p; break;
+          case WC_FIELD_SAT_P: src = M.sat_p.p; break;
+          default: src = M.pp.p; break;
+        }
+        WC_HIP(hipMemcpy(host, src, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        break;
+      }
       case WC_FIELD_DOF: {
         std::vector<uint32_t> f(nn), s(nn);
         WC_HIP(hipMemcpy(f.data(), c->isdof.p, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost));

@@ -270,6 +270,25 @@ struct AdjointWork {
   void release();
 };
 
+// The static porous solid (wc_mix.hip): its particles, LiquidInfo's
+// mixture fields, the solid's face sums (taken once: the solid rests) and
+// the per-substep mixture fields.
+struct MixState {
+  int64_t ne = 0;  // solid particles; 0 = liquid only
+  wc_mix_params prm{};
+  DevBuf<double> x, m, fm, fvol, rvol, rvf, ori, sf;  // x, ori: 3 planes of ne
+  // per face axis d, 8 planes of nnode: solid mass, solid volume (rest_vol
+  // rest_vf), carried liquid volume, shape factor, solid volume incl. the
+  // carried liquid, orientation x, y, z (plane())
+  DevBuf<double> acc;
+  DevBuf<double> cvs, cvf, cvl;  // cell sums: solid volume, its liquid, the liquid particles'
+  DevBuf<uint8_t> touch, near;   // per bucket: touched by the solid's stencils; next to one
+  DevBuf<double> psi[3], sat[3], Q[3], hdv[3];
+  DevBuf<double> psi_p, sat_p, pp;  // per cell
+  double* plane(const Grid& g, int d, int k) const;
+  void release();
+};
+
 }  // namespace wc
 
 struct wc_ctx {
@@ -337,6 +356,9 @@ struct wc_ctx {
   std::vector<wc::ParticleCheckpoint> ckpt;
   wc::AdjointWork adj;
   int roll_steps = 0, roll_every = 0;
+
+  // a static porous solid in the liquid path (wc_set_elasto_particles)
+  wc::MixState mix;
 };
 
 namespace wc {
@@ -391,6 +413,21 @@ bool adjoint_substep(wc_ctx* ctx, double dt, double tol, int maxit, double* gb,
 void checkpoint_save(wc_ctx* ctx, int slot);
 void checkpoint_restore(wc_ctx* ctx, int slot);
 void adjoint_release(wc_ctx* ctx);
+
+// ---- wc_mix.hip ----
+// the solid's particles and parameters; its face and cell sums (ne = 0
+// removes the solid)
+void mix_set(wc_ctx* ctx, int64_t ne, const double* x, const double* m, const double* fm,
+             const double* fvol, const double* rvol, const double* rvf, const double* ori,
+             const double* sf, const wc_mix_params& prm);
+// after the liquid's P2G: the buckets the solid's stencils touch
+void mix_touch(wc_ctx* ctx);
+// after grid_nodes: saturations, the cell-centred fields and the pore pressure
+void mix_grid(wc_ctx* ctx);
+// the grid update with the pore pressure gradient and the drag
+void mix_step_velocity(wc_ctx* ctx, double dt);
+// the plane indices of MixState::acc
+constexpr int kMixPlaneMass = 0, kMixPlaneSf = 3, kMixPlaneSvol = 4, kMixPlaneOri = 5;
 
 // ---- wc_grid.hip ----
 void grid_activate(wc_ctx* ctx);

@@ -168,6 +168,11 @@ __device__ inline double psi_fs(double invC, double M) {
   const double P = M > 1e-20 ? 0.0 / M : 1.0;
   return ((1.0 - psi) * invC + psi * invCs * P) * M;
 }
+// the same with a solid: invCs = 0, so the finite P = hdV / (M + hdV) adds 0
+__device__ inline double psi_fs_mix(double invC, double M, double psi) {
+  const double invCs = 0.0, P = 0.0;
+  return ((1.0 - psi) * invC + psi * invCs * P) * M;
+}
 
 struct DofArgs {
   Grid g;
@@ -218,6 +223,11 @@ struct AsmArgs {
   int32_t level_min;
   double dt;
   FrozenView F;
+  // with a solid (MIX): volume fraction, solid volume and drag ratio Q of
+  // every face
+  const double* psi[3];
+  const double* svol[3];
+  const double* Q[3];
 };
 
 __device__ inline void insert_sorted(uint32_t* cols, double* vals, int& len,
@@ -244,6 +254,10 @@ __device__ inline void insert_sorted(uint32_t* cols, double* vals, int& len,
 // near, far; rhs += or -= ((u* psi_fs + u_s psi_sf) w + u_solid (1-w)) / dx;
 // open faces add term = w dt/dx^2 [(1-psi) invC (Vf + Vs Q) + ...] to the
 // diagonal and -term to a liquid neighbour, or term/theta for an air one.
+// The resting solid's velocity u_s is zero and invCs = 0 (constructMsDVs
+// runs only when the solid is solved), so psi_sf and the psi invCs terms
+// add zeros.  Without a solid psi = 0, Vs = 0, Q = 1.
+template <bool MIX>
 __global__ void k_assemble(AsmArgs A) {
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const Grid& g = A.g;
@@ -267,7 +281,8 @@ __global__ void k_assemble(AsmArgs A) {
     const double w = A.w[axis][fn];
     const double M = A.mass[axis][fn];
     const double invC = A.invC[axis][fn];
-    const double uf = A.ustar[axis][fn], pfs = psi_fs(invC, M);
+    const double psi = MIX ? A.psi[axis][fn] : 0.0;
+    const double uf = A.ustar[axis][fn], pfs = MIX ? psi_fs_mix(invC, M, psi) : psi_fs(invC, M);
     const double us = 0.0, psf = 0.0, usolid = 0.0;
     const double flux = ((uf * pfs + us * psf) * w + usolid * (1.0 - w)) / dx;
     if (hi)
@@ -275,8 +290,10 @@ __global__ void k_assemble(AsmArgs A) {
     else
       rhs += flux;
     if (w > 0.0) {
-      const double psi = 0.0, Vs = 0.0, Q = 1.0, invCs = 0.0;
-      const double P = M > 1e-20 ? 0.0 / M : 1.0;
+      const double Vs = MIX ? A.svol[axis][fn] : 0.0, Q = MIX ? A.Q[axis][fn] : 1.0;
+      const double invCs = 0.0;
+      // P only meets invCs = 0: any finite value gives the same term
+      const double P = MIX ? 0.0 : (M > 1e-20 ? 0.0 / M : 1.0);
       const double Vf = A.vol[axis][fn];
       double term = w * coeff;
       term *= (1.0 - psi) * invC * (Vf + Vs * Q) + psi * invCs * (Vs + Vf * P);
@@ -328,10 +345,14 @@ struct ApplyArgs {
   uint8_t* valid[3];
   double dt;
   FrozenView F;
+  const double* svol[3];  // MIX: solid volume and drag ratio Q of every face
+  const double* Q[3];
 };
 
 // applyPressureGradsFluidRHS (Pressure.cpp:1137-1314) after addSolidDragRHS
-// (+Q Ms us = 0, LIE:689-717), then performInvLocalSolve u = rhs invC.
+// (+Q Ms us = 0: the solid rests, LIE:689-717), then performInvLocalSolve
+// u = rhs invC.  Without a solid Vs = 0 and Q = 1.
+template <bool MIX>
 __global__ void k_apply_pressure(ApplyArgs A) {
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const Grid& g = A.g;
@@ -348,9 +369,9 @@ __global__ void k_apply_pressure(ApplyArgs A) {
   const double dx = g.dx;
   uint8_t clamps = 0;
   for (int d = 0; d < 3; ++d) {
-    const double Q = 1.0, Ms = 0.0, us = 0.0, Vs = 0.0;
+    const double Q = MIX ? A.Q[d][n] : 1.0, us = 0.0, Vs = MIX ? A.svol[d][n] : 0.0;
     double rhs = A.rhs[d][n];
-    rhs += Q * Ms * us;
+    rhs += Q * 0.0 * us;  // Q M_s u_s with u_s = 0
     uint8_t valid = 0;
     if (A.w[d][n] > 0.) {
       int l[3] = {i, j, k};
@@ -477,6 +498,10 @@ void grid_nodes(wc_ctx* ctx) {
 }
 
 void grid_step_velocity(wc_ctx* ctx, double dt) {
+  if (ctx->mix.ne > 0) {
+    mix_step_velocity(ctx, dt);
+    return;
+  }
   StepArgs A;
   A.g = ctx->g;
   A.act = ctx->act.p;
@@ -544,7 +569,16 @@ int64_t grid_assemble(wc_ctx* ctx, double dt) {
     A.level_min = ctx->level_min;
     A.dt = dt;
     A.F = frozen_view(ctx);
-    hipLaunchKernelGGL(k_assemble, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, A);
+    const bool mix = ctx->mix.ne > 0;
+    for (int d = 0; d < 3; ++d) {
+      A.psi[d] = mix ? ctx->mix.psi[d].p : nullptr;
+      A.svol[d] = mix ? ctx->mix.plane(g, d, kMixPlaneSvol) : nullptr;
+      A.Q[d] = mix ? ctx->mix.Q[d].p : nullptr;
+    }
+    if (mix)
+      hipLaunchKernelGGL(k_assemble<true>, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, A);
+    else
+      hipLaunchKernelGGL(k_assemble<false>, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, A);
     WC_LAUNCH_CHECK();
   }
   return ndof;
@@ -601,8 +635,16 @@ void grid_apply_pressure(wc_ctx* ctx, double dt) {
   }
   A.dt = dt;
   A.F = frozen_view(ctx);
-  hipLaunchKernelGGL(k_apply_pressure, dim3(blocks(g.nnode, 256)), dim3(256), 0,
-                     st, A);
+  const bool mix = ctx->mix.ne > 0;
+  for (int d = 0; d < 3; ++d) {
+    A.svol[d] = mix ? ctx->mix.plane(g, d, kMixPlaneSvol) : nullptr;
+    A.Q[d] = mix ? ctx->mix.Q[d].p : nullptr;
+  }
+  if (mix)
+    hipLaunchKernelGGL(k_apply_pressure<true>, dim3(blocks(g.nnode, 256)), dim3(256), 0, st, A);
+  else
+    hipLaunchKernelGGL(k_apply_pressure<false>, dim3(blocks(g.nnode, 256)), dim3(256), 0, st,
+                       A);
   WC_LAUNCH_CHECK();
   // layer 1: unew -> utmp; layer 2 (+ constraint): utmp -> unew
   for (int layer = 0; layer < 2; ++layer) {

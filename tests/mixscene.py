@@ -23,8 +23,10 @@ def sheet(nv, y, lo=(0.0, 0.0), hi=(1.0, 1.0), thickness=2e-3, areal_density=0.2
     gi, gk = gi.ravel(), gk.ravel()
     ne = gi.size
     x = np.stack([lo[0] + gi * hx, np.full(ne, float(y)), lo[1] + gk * hz])
-    if jitter > 0:
+    if jitter > 0:  # kept inside the sheet's square (and so the container)
         x += rng.uniform(-jitter, jitter, x.shape) * np.array([[hx], [hx], [hz]])
+        x[0] = np.clip(x[0], lo[0], hi[0])
+        x[2] = np.clip(x[2], lo[1], hi[1])
     # each vertex's share of the area: halved on an edge, quartered at a corner
     share = np.where((gi == 0) | (gi == nv - 1), 0.5, 1.0) * \
         np.where((gk == 0) | (gk == nv - 1), 0.5, 1.0)
@@ -42,6 +44,22 @@ def sheet(nv, y, lo=(0.0, 0.0), hi=(1.0, 1.0), thickness=2e-3, areal_density=0.2
     if np.ndim(sf) == 0 and sf < 0:
         out["sf"] = rng.uniform(0.0, 1.0, ne)
     return out
+
+
+def wet_sheet(seed=4):
+    """A 33 x 33 sheet at y = 0.3 over the unit square, 2 cm thick (psi
+    reaches ~0.15 in 6 cm cells), jittered by 0.3 spacings, with seeded
+    orientations and shape factors and a fifth of its pores filled."""
+    return sheet(33, 0.3, thickness=0.02, ori=None, sf=-1, jitter=0.3, fluid_fraction=0.2,
+                 seed=seed)
+
+
+def config2_sheet():
+    """The cloth of the 128^3 wet-cloth scene: 257 x 257 vertices over the
+    unit square at y = 0.3, 2 mm thick, 0.2 kg/m^2, porosity uniform in
+    [0.3, 0.7] (seed 2), orientation along the sheet normal, shape factor
+    0.25."""
+    return sheet(257, 0.3, seed=2, ori=(0.0, 1.0, 0.0), sf=0.25)
 
 
 def lattice(n, dx, rest_vf, m=1e-3):

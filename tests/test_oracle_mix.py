@@ -206,10 +206,7 @@ def make_scene(oracle, n, liquid_lo, liquid_hi, solid, prm=None, sigma=0.0, seed
     return s
 
 
-def wet_sheet(seed=4):
-    # 2 cm thick, so that psi reaches ~0.15 at this grid's 6 cm cells
-    return mixscene.sheet(33, 0.3, thickness=0.02, ori=None, sf=-1, jitter=0.3,
-                          fluid_fraction=0.2, seed=seed)
+wet_sheet = mixscene.wet_sheet
 
 
 def test_solid_p2g_matches_brute_force(oracle):
