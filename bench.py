@@ -13,6 +13,12 @@ with the ratio and the positions' difference after the same steps).  A third
 run solves the pressure with FP64 AMG-PCG instead, the reference's as-shipped
 solver, for comparison (fp64_amg; --no-amg skips it).
 
+A fourth leg (wet_cloth_128; --no-wet-cloth skips it) times the 128^3
+wet-cloth scene in the mixed form: the 128^3 liquid column (0.4 x 0.6 x 1 m,
+~4.0 M particles) starting at y = 0.35 over a resting 257 x 257-vertex
+porous cloth at y = 0.3, dt = 1/960 -- the liquid path with the mixture
+terms (volume fraction, drag, pore pressure).
+
 Prints ONE JSON line (see the keys below).  --gpus N > 1 exits with a message:
 the library runs the substep on a single GPU.
 """
@@ -43,6 +49,8 @@ def parse():
     ap.add_argument("--no-amg", action="store_true",
                     help="skip the FP64 AMG-PCG run of the workload (the reference's "
                          "as-shipped pressure solver, reported under fp64_amg)")
+    ap.add_argument("--no-wet-cloth", action="store_true",
+                    help="skip the 128^3 wet-cloth leg (reported under wet_cloth_128)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (the known-size "
@@ -216,6 +224,34 @@ def run(W, precision, n, dt, a, solver=0):
     }
 
 
+def wet_cloth(W, a, n=128, dt=1.0 / 960):
+    """The 128^3 wet-cloth scene in the mixed form: warmup + steps substeps,
+    the last steps timed."""
+    from libwetcloth_amd import scenes
+    scene = W.Scene((n, n, n), 1.0 / n, precision=W.WC_MIXED)
+    npart = scene.seed_block((0.0, 0.35, 0.0), (0.4, 0.95, 1.0), seed=2)
+    cloth = scenes.config2_cloth()
+    scene.set_elasto(*scenes.elasto_args(cloth), prm=W.mix_params())
+    for _ in range(a.warmup):
+        scene.substep(dt, a.tol, a.maxit)
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ok, it, _ = scene.substep(dt, a.tol, a.maxit)
+        if not ok:
+            raise RuntimeError(f"wet cloth: pressure solve did not converge ({it} iterations)")
+        iters.append(it)
+    wall = time.perf_counter() - t0
+    scene.close()
+    return {"workload": (f"wet cloth {n}^3: block [0,0.4]x[0.35,0.95]x[0,1] at 8 ppc over a "
+                         "resting 257x257-vertex cloth at y = 0.3 (1 m square, 2 mm, "
+                         f"0.2 kg/m^2, porosity U[0.3,0.7]), dt {dt:.6g}, mixed, "
+                         f"MIC(0)-PCG tol {a.tol:g}"),
+            "particles": npart, "cloth_vertices": int(cloth["x"].shape[1]),
+            "value": a.steps / wall, "unit": "substeps/s", "ms_per_step": 1e3 * wall / a.steps,
+            "pcg_iterations": iters}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -277,6 +313,8 @@ def main():
         amg.close()
         out["fp64_amg"] = {k: g[k] for k in ("value", "ms_per_step", "pcg_iterations",
                                              "phase_ms_per_step")}
+    if not a.no_wet_cloth:
+        out["wet_cloth_128"] = wet_cloth(W, a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(npart, a.cpu_steps)
     print(json.dumps(out))

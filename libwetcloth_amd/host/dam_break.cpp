@@ -194,6 +194,47 @@ int main(int argc, char** argv) {
       const double fdg = (lp - lm) / (2 * eps);
       rollout_g_err = std::fabs(fdg - gr.gravity) / std::fabs(fdg);
     }
+    // 5. the block over a resting porous sheet (wc_set_elasto_particles):
+    // 25 x 25 vertices at y = 0.3, 2 mm thick, 0.2 kg/m^2, rest volume
+    // fraction 0.5, orientation along the normal, shape factor 0.25; three
+    // substeps, the same again without the sheet
+    bool mix_ok = false;
+    double mix_vy = 0.0, free_vy = 0.0;
+    {
+      const int nv = 25;
+      const int64_t ne = (int64_t)nv * nv;
+      std::vector<double> ex(3 * ne), em(ne), efm(ne, 0.0), efv(ne, 0.0), erv(ne), erf(ne, 0.5),
+          eo(3 * ne, 0.0), esf(ne, 0.25);
+      const double h = 1.0 / (nv - 1);
+      for (int i = 0; i < nv; ++i)
+        for (int k = 0; k < nv; ++k) {
+          const int64_t p = (int64_t)i * nv + k;
+          ex[p] = i * h;
+          ex[ne + p] = 0.3;
+          ex[2 * ne + p] = k * h;
+          const double share = (i == 0 || i == nv - 1 ? 0.5 : 1.0) * (k == 0 || k == nv - 1 ? 0.5 : 1.0);
+          em[p] = 0.2 * share * h * h;
+          erv[p] = 2e-3 * share * h * h;
+          eo[ne + p] = 1.0;
+        }
+      const wc_mix_params prm = {1, 1, 1e-4, 8.9e-4, 1.6, 5e-5, 0.0728, 40.8 / 180.0 * M_PI};
+      auto run = [&](bool sheet, double& mean_vy) {
+        wcamd::TwoDScene ms(d);
+        const int64_t mn = ms.seedBlock(lo, hi, 0, 0.0);
+        if (sheet) ms.setElastoParticles(ex, em, efm, efv, erv, erf, eo, esf, prm);
+        wcamd::LinearizedImplicitEuler mst(1e-8, 1000);
+        wcamd::WetClothCore mc(ms, mst);
+        bool okm = true;
+        for (int s = 0; s < 3; ++s) okm &= mc.stepSystem(1.0 / 480);
+        std::vector<double> mx, mv, mC;
+        ms.getParticlesSoA(mx, mv, mC);
+        mean_vy = 0.0;
+        for (int64_t p = 0; p < mn; ++p) mean_vy += mv[mn + p];
+        mean_vy /= (double)mn;
+        return okm;
+      };
+      mix_ok = run(true, mix_vy) && run(false, free_vy);
+    }
     std::printf(
         "{\"pcg_ok\": %s, \"pcg_iterations\": %d, \"pcg_residual\": %.17g, "
         "\"pcg_max_error\": %.17g, \"amg_ok\": %s, \"amg_iterations\": %d, "
@@ -202,13 +243,14 @@ int main(int argc, char** argv) {
         "\"amg_steps_ok\": %s, \"amg_step_iterations\": %d, "
         "\"split_fused_bitwise\": %s, \"vmax\": %.17g, \"big_step_ok\": %s, "
         "\"big_step_substeps\": %d, \"rollout_ok\": %s, \"rollout_dir_err\": %.6g, "
-        "\"rollout_g_err\": %.6g}\n",
+        "\"rollout_g_err\": %.6g, \"mix_ok\": %s, \"mix_vy\": %.17g, \"free_vy\": %.17g}\n",
         ok ? "true" : "false", iterations, residual, err / nrm, amg_ok ? "true" : "false",
         amg_it, amg_err / amg_nrm, (long long)np,
         all_ok ? "true" : "false", last_iterations, vy / (double)np, secs,
         amg_steps_ok ? "true" : "false", amg_step_iterations,
         split_fused_bitwise ? "true" : "false", vmax, big_ok ? "true" : "false", big_substeps,
-        rollout_ok ? "true" : "false", rollout_dir_err, rollout_g_err);
+        rollout_ok ? "true" : "false", rollout_dir_err, rollout_g_err, mix_ok ? "true" : "false",
+        mix_vy, free_vy);
   } catch (const wcamd::Error& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return 1;

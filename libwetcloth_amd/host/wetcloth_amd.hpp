@@ -88,6 +88,27 @@ class TwoDScene {
     check(wc_get_particles_soa(ctx_, x.data(), v.data(), C.data()), ctx_);
   }
   void setGravity(const double g[3]) { check(wc_set_gravity(ctx_, g), ctx_); }
+  // the elasto particles of a resting, unsolved solid (the cloth's
+  // vertices, LiquidInfo::solve_solid = 0): positions and orientations
+  // 3*ne (x[d*ne+p]), the rest ne each; an empty x removes the solid
+  void setElastoParticles(const std::vector<double>& x, const std::vector<double>& m,
+                          const std::vector<double>& fluid_m,
+                          const std::vector<double>& fluid_vol,
+                          const std::vector<double>& rest_vol,
+                          const std::vector<double>& rest_volume_fraction,
+                          const std::vector<double>& orientation,
+                          const std::vector<double>& shape_factor, const wc_mix_params& prm) {
+    const int64_t ne = (int64_t)x.size() / 3;
+    const size_t sizes[7] = {m.size(), fluid_m.size(), fluid_vol.size(), rest_vol.size(),
+                             rest_volume_fraction.size(), shape_factor.size(),
+                             orientation.size() / 3};
+    for (size_t k : sizes)
+      if ((int64_t)k != ne) throw Error(WC_ERR_ARG, "wetcloth_amd: solid array sizes differ");
+    check(wc_set_elasto_particles(ctx_, ne, x.data(), m.data(), fluid_m.data(), fluid_vol.data(),
+                                  rest_vol.data(), rest_volume_fraction.data(),
+                                  orientation.data(), shape_factor.data(), &prm),
+          ctx_);
+  }
   int64_t getNumParticles() const {
     int64_t n = 0;
     check(wc_num_particles(ctx_, &n), ctx_);
@@ -195,7 +216,7 @@ class WetClothCore {
   WetClothCore(TwoDScene& scene, LinearizedImplicitEuler& stepper, bool fused = false)
       : scene_(scene), stepper_(stepper), fused_(fused) {}
   bool stepSystem(double dt) {
-    const double max_elasto_vel = 0.0;  // no elastic particles
+    const double max_elasto_vel = 0.0;  // no elastic particles, or a resting solid
     const double max_fluid_vel = scene_.getMaxFluidVelocity();
     const double dx = scene_.getCellSize();
     const double max_elasto_dt = std::min(dx / std::max(1e-63, max_elasto_vel) / 3.0, 1.0 / 30.0);

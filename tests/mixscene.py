@@ -8,42 +8,10 @@ elasto particles, TwoDScene.cpp:6686-6699)."""
 import numpy as np
 
 
-def sheet(nv, y, lo=(0.0, 0.0), hi=(1.0, 1.0), thickness=2e-3, areal_density=0.2,
-          porosity=(0.3, 0.7), seed=2, ori=(0.0, 1.0, 0.0), sf=0.25, jitter=0.0,
-          fluid_fraction=0.0):
-    """An nv x nv vertex sheet over [lo, hi] in x and z at height y.  `ori`
-    is one orientation for every vertex, or None for seeded unit vectors;
-    `jitter` moves each vertex by up to that fraction of the spacing;
-    `fluid_fraction` fills that fraction of each vertex's pore volume with
-    liquid (its fluid volume and mass at 1000 kg/m^3)."""
-    rng = np.random.default_rng(seed)
-    hx = (hi[0] - lo[0]) / (nv - 1)
-    hz = (hi[1] - lo[1]) / (nv - 1)
-    gi, gk = np.meshgrid(np.arange(nv), np.arange(nv), indexing="ij")
-    gi, gk = gi.ravel(), gk.ravel()
-    ne = gi.size
-    x = np.stack([lo[0] + gi * hx, np.full(ne, float(y)), lo[1] + gk * hz])
-    if jitter > 0:  # kept inside the sheet's square (and so the container)
-        x += rng.uniform(-jitter, jitter, x.shape) * np.array([[hx], [hx], [hz]])
-        x[0] = np.clip(x[0], lo[0], hi[0])
-        x[2] = np.clip(x[2], lo[1], hi[1])
-    # each vertex's share of the area: halved on an edge, quartered at a corner
-    share = np.where((gi == 0) | (gi == nv - 1), 0.5, 1.0) * \
-        np.where((gk == 0) | (gk == nv - 1), 0.5, 1.0)
-    area = share * hx * hz
-    rvol = area * thickness
-    rvf = 1.0 - rng.uniform(porosity[0], porosity[1], ne)
-    fvol = fluid_fraction * rvol * (1.0 - rvf)
-    out = dict(x=x, m=area * areal_density, fm=1000.0 * fvol, fvol=fvol, rest_vol=rvol,
-               rest_vf=rvf, sf=np.full(ne, float(sf)))
-    if ori is None:
-        o = rng.standard_normal((3, ne))
-        out["ori"] = o / np.linalg.norm(o, axis=0)
-    else:
-        out["ori"] = np.repeat(np.asarray(ori, dtype=np.float64)[:, None], ne, axis=1)
-    if np.ndim(sf) == 0 and sf < 0:
-        out["sf"] = rng.uniform(0.0, 1.0, ne)
-    return out
+def sheet(nv, y, **kw):
+    """libwetcloth_amd.scenes.cloth_sheet (a host-side builder, no GPU)."""
+    from libwetcloth_amd.scenes import cloth_sheet
+    return cloth_sheet(nv, y, **kw)
 
 
 def wet_sheet(seed=4):
@@ -55,11 +23,9 @@ def wet_sheet(seed=4):
 
 
 def config2_sheet():
-    """The cloth of the 128^3 wet-cloth scene: 257 x 257 vertices over the
-    unit square at y = 0.3, 2 mm thick, 0.2 kg/m^2, porosity uniform in
-    [0.3, 0.7] (seed 2), orientation along the sheet normal, shape factor
-    0.25."""
-    return sheet(257, 0.3, seed=2, ori=(0.0, 1.0, 0.0), sf=0.25)
+    """The cloth of the 128^3 wet-cloth scene (scenes.config2_cloth)."""
+    from libwetcloth_amd.scenes import config2_cloth
+    return config2_cloth()
 
 
 def lattice(n, dx, rest_vf, m=1e-3):

@@ -1,8 +1,8 @@
 """The C++ host mirror (libwetcloth_amd/host/wetcloth_amd.hpp) driven from a
 compiled program: PCGSolver<double>::solve and AMGPCGSolveSparse on
 manufactured systems, WetClothCore substeps of the dam break, checked
-against the oracle, and the SubstepRollout forward/backward pair against
-central differences."""
+against the oracle, the SubstepRollout forward/backward pair against
+central differences, and a scene with a resting porous sheet."""
 import json
 import os
 import subprocess
@@ -36,6 +36,9 @@ def test_cpp_host_mirror(oracle, tmp_path):
     # wcamd::SubstepRollout: the backward's dL/dv0 along a random direction
     # and dL/d|g| against central differences of replayed forwards
     assert r["rollout_ok"] and r["rollout_dir_err"] < 1e-6 and r["rollout_g_err"] < 1e-6
+    # TwoDScene::setElastoParticles: the block over a resting porous sheet
+    # that cuts through it; the sheet's drag slows the fall
+    assert r["mix_ok"] and r["free_vy"] < r["mix_vy"] < 0.0
     g, x, v = oracle.dam_break(n)
     assert r["particles"] == x.shape[1]
     s = oracle.OracleScene((n, n, n), 1.0 / n, x, v)
