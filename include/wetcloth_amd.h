@@ -324,7 +324,16 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
  *   the reference's liquid solver at Pressure.cpp:797), on the DOFs' cells
  *   bucket handle * 4 + node in the bucket on the context's bucket grid
  *   (Pressure.cpp:316-318, 433-436).  With 1, wc_apply_preconditioner has
- *   no factor and fails. */
+ *   no factor and fails.
+ *   "deterministic" (default 1 for WC_MIXED contexts, 0 for WC_FP64): 1
+ *   sums P2G's contributions in 64-bit fixed
+ *   point (units of 2^-56 for weights and 2^-48 for weight x velocity, each
+ *   contribution rounded to nearest; speeds below 1000 m/s), which does not
+ *   depend on the order the particles reach a node in, so the substep
+ *   repeats bit for bit; 0 sums them with FP64 atomics, whose order (and
+ *   last bits) vary from run to run.  Every other reduction of the liquid
+ *   substep runs in a fixed order either way.  The mixture's solid and cell
+ *   sums (wc_set_elasto_particles) stay FP64 atomic sums. */
 wc_status wc_set_option(wc_ctx* ctx, const char* name, int64_t value);
 
 /* Refinement passes of the context's last mixed-precision solve (0 for a

@@ -308,6 +308,9 @@ struct wc_ctx {
   wc::DevBuf<uint32_t> pid, pid2, pkey;
   wc::DevBuf<uint32_t> bcount, bstart, bcursor;  // per bucket (+1)
   bool sorted = false;  // bstart describes the current particle arrays
+  // P2G sums in fixed point (order independent), so that a run repeats
+  // bit for bit (wc_set_option "deterministic")
+  bool deterministic = false;
   // positions before the last separate advection pass (wc_advect_scene), the
   // ones the transfer weights belong to; solidProjection weighs at them
   wc::DevBuf<double> xw;

@@ -56,7 +56,15 @@ struct NodeArgs {
   double* mom[3];   // momentum in, velocity out
   double fm, fvol;  // the particle mass and volume P2G weighted with
   FrozenView F;    // mode 2: lphi is already the replayed one
+  int det;         // the sums are fixed point (the deterministic P2G)
 };
+
+// a fixed-point sum of the deterministic P2G (64-bit integer bits in a
+// double's storage, in units of 2^-56 for weights, 2^-48 for momentum) as
+// its value
+__device__ inline double from_fix(double bits, double unit) {
+  return (double)__double_as_longlong(bits) * unit;
+}
 
 // updateSolidWeights (TwoDScene.cpp:1871-1962), updateLiquidPhi's solid
 // override and default (:3297-3348), and of P2G (:6700-6713) M = fm sum w,
@@ -110,14 +118,15 @@ __global__ void k_grid_nodes(NodeArgs A) {
   }
   uint8_t has = 0;
   for (int d = 0; d < 3; ++d) {
-    const double ws = A.mass[d][n];
+    const double ws = A.det ? from_fix(A.mass[d][n], 0x1p-56) : A.mass[d][n];
+    const double P = A.det ? A.fm * from_fix(A.mom[d][n], 0x1p-48) : A.mom[d][n];
     const double M = A.fm * ws;
     A.mass[d][n] = M;
     A.vol[d][n] = A.fvol * ws;
     const bool hm = fz_mass(A.F, d, n, M);
     if (A.F.mode == 2 && (((A.F.flags[n] >> d) & 1) != 0) != (M > 1e-20)) fz_depart(A.F);
     has |= (uint8_t)(hm ? 1u << d : 0u);
-    A.mom[d][n] = hm ? A.mom[d][n] / M : 0.0;
+    A.mom[d][n] = hm ? P / M : 0.0;
   }
   if (A.F.mode == 1) A.F.flags[n] = has;  // the theta clamps are added later
 }
@@ -491,6 +500,7 @@ void grid_nodes(wc_ctx* ctx) {
   // the mixed form's P2G weights with the FP32 particle mass and volume
   A.fm = ctx->mixed ? (double)(float)ctx->desc.particle_mass : ctx->desc.particle_mass;
   A.fvol = ctx->mixed ? (double)(float)ctx->desc.particle_volume : ctx->desc.particle_volume;
+  A.det = ctx->deterministic ? 1 : 0;
   hipLaunchKernelGGL(k_grid_nodes, dim3(blocks(ctx->g.nnode, 256)), dim3(256),
                      0, ctx->stream, A);
   WC_LAUNCH_CHECK();

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include <vector>
 
@@ -142,10 +143,24 @@ __device__ inline double grid5_off(int gi, int a) {
   return gi == 3 ? 0.0 : 0.5;
 }
 
+// The deterministic form's fixed-point sums: each contribution rounded to
+// a multiple of 2^-56 (weights) or 2^-48 (weight x velocity) and added as a
+// 64-bit integer, so a node's sum does not depend on the order the
+// particles reach it in (integer addition is associative).  A node's
+// weight sum is about the particles per cell (the B-spline weights of a
+// particle sum to one over its stencil), far below the 2^7 the format
+// holds; its momentum sum stays below the 2^15 it holds for speeds below
+// 1000 m/s.
+constexpr double kFixW = 0x1p56, kFixP = 0x1p48;
+__device__ inline unsigned long long to_fix(double v, double scale) {
+  return (unsigned long long)__double2ll_rn(v * scale);
+}
+
 // R is the particle arithmetic (double, or float in the mixed-precision
 // form); each particle's contributions are summed in FP64 (LDS, then the
-// FP64 grid).
-template <class R>
+// FP64 grid), or with DET in fixed point (LDS, then the grid's arrays
+// holding 64-bit integers until k_grid_nodes converts them).
+template <class R, bool DET>
 __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   const int64_t b = blockIdx.x;
   const uint32_t pbeg = A.bstart[b], pend = A.bstart[b + 1];
@@ -161,13 +176,14 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
   // fm and fvol times its weight sum (TwoDScene.cpp:6700-6713 sums fm w and
   // fvol w): two sums per node instead of three, in LDS and in the flush
   // (the grid kernel that follows forms the mass and the volume).
-  __shared__ double sm[3][2][kLdsNodes];
+  using Acc = typename std::conditional<DET, unsigned long long, double>::type;
+  __shared__ Acc sm[3][2][kLdsNodes];
   __shared__ unsigned long long sphi[kLdsNodes];
   __shared__ int stouch[27];
   for (int i = threadIdx.x; i < kLdsNodes; i += kP2GThreads) {
     for (int d = 0; d < 3; ++d) {
-      sm[d][0][i] = 0.0;
-      sm[d][1][i] = 0.0;
+      sm[d][0][i] = (Acc)0;
+      sm[d][1][i] = (Acc)0;
     }
     sphi[i] = kPhiUntouched;
   }
@@ -211,12 +227,24 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
             if (li >= 0 && lj >= 0 && lk >= 0 && li < kTile && lj < kTile &&
                 lk < kTile) {
               const int l = p2g_lds(li, lj, lk);
-              atomicAdd(&sm[d][0][l], (double)ww);
-              atomicAdd(&sm[d][1][l], (double)(vel * ww));
+              if constexpr (DET) {
+                atomicAdd(&sm[d][0][l], to_fix((double)ww, kFixW));
+                atomicAdd(&sm[d][1][l], to_fix((double)(vel * ww), kFixP));
+              } else {
+                atomicAdd(&sm[d][0][l], (double)ww);
+                atomicAdd(&sm[d][1][l], (double)(vel * ww));
+              }
             } else {
               const int64_t n = node_index(g, i, j, k);
-              unsafeAtomicAdd(&A.wsum[d][n], (double)ww);
-              unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * (double)(vel * ww));
+              if constexpr (DET) {  // the particle mass is applied by k_grid_nodes
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.wsum[d][n]),
+                          to_fix((double)ww, kFixW));
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.mom[d][n]),
+                          to_fix((double)(vel * ww), kFixP));
+              } else {
+                unsafeAtomicAdd(&A.wsum[d][n], (double)ww);
+                unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * (double)(vel * ww));
+              }
             }
           }
     }
@@ -270,10 +298,15 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs<R> A) {
     const int64_t n = node_index(g, i, j, k);
     const int l = p2g_lds(li, lj, lk);
     for (int d = 0; d < 3; ++d) {
-      const double sw = sm[d][0][l];
-      if (sw != 0.0) {
-        unsafeAtomicAdd(&A.wsum[d][n], sw);
-        unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * sm[d][1][l]);
+      const Acc sw = sm[d][0][l];
+      if (sw != (Acc)0) {
+        if constexpr (DET) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(&A.wsum[d][n]), sw);
+          atomicAdd(reinterpret_cast<unsigned long long*>(&A.mom[d][n]), sm[d][1][l]);
+        } else {
+          unsafeAtomicAdd(&A.wsum[d][n], sw);
+          unsafeAtomicAdd(&A.mom[d][n], (double)A.fm * sm[d][1][l]);
+        }
       }
     }
     const unsigned long long key = sphi[l];
@@ -871,8 +904,12 @@ void p2g_launch(wc_ctx* ctx, Planes<R> P) {
                                                           : ctx->desc.particle_radius);
   KTimer& tm = ctx->pcg.timer;
   hipEvent_t e0 = tm.on ? tm.mark(ctx->stream) : nullptr;
-  hipLaunchKernelGGL(k_p2g<R>, dim3((unsigned)g.nbucket), dim3(kP2GThreads), 0,
-                     ctx->stream, A);
+  if (ctx->deterministic)
+    hipLaunchKernelGGL((k_p2g<R, true>), dim3((unsigned)g.nbucket), dim3(kP2GThreads), 0,
+                       ctx->stream, A);
+  else
+    hipLaunchKernelGGL((k_p2g<R, false>), dim3((unsigned)g.nbucket), dim3(kP2GThreads), 0,
+                       ctx->stream, A);
   WC_LAUNCH_CHECK();
   if (e0) tm.add(KTimer::P2G, ctx->np, e0, tm.mark(ctx->stream));
 }
