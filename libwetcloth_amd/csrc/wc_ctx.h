@@ -452,15 +452,17 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
 void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr);
 // MIC(0) apply on box vectors: r (box) -> P.zb; the backward sweep also
 // leaves the per-tile partials of z . r in P.tdot (P.ntiles of them)
+// cleared: the forward output already holds the sentinel (the PCG update
+// that precedes an apply in the loop sets it), so the apply skips its fill
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
-                 int64_t ndof);
+                 int64_t ndof, bool cleared = false);
 void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y);
 // mixed-precision solve: FP32 records/entries from the FP64 factor and A,
 // FP32 sweeps (result in P.zb32), FP32 SpMV with FP64 row sums, and the FP64
 // refinement residual r = b - A x rounded to FP32 with |r|_inf partials
 void sweep_narrow(hipStream_t st, SweepPlan& P);
 void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
-                   int64_t ndof);
+                   int64_t ndof, bool cleared = false);
 // y = A x with the block partials of x . y (nblocks blocks, the solver's
 // reduction layout)
 void box_spmv_dot(hipStream_t st, const SweepPlan& P, const double* x, double* y,

@@ -351,11 +351,19 @@ __global__ void k_update_s(V* s, const V* z, int64_t n, const double* scal) {
 
 // x += s alpha, r -= z alpha (FP64 r and z, or FP32 formed in FP64), |r|_inf
 // block partials into rpart and the sweeps' abort flag into flag_out (both
-// in mapped host memory, which the host reads once the kernel is complete).
+// in mapped host memory, which the host reads once the kernel is complete);
+// and the next forward sweep's output set to the all-ones sentinel (reset,
+// the same slots), which saves the sweep's own fill launch.
+__device__ inline void set_all_ones(double* p) {
+  *reinterpret_cast<unsigned long long*>(p) = ~0ull;
+}
+__device__ inline void set_all_ones(float* p) { *reinterpret_cast<unsigned int*>(p) = ~0u; }
+
 template <class V>
 __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
     double* x, V* r, const V* s, const V* z, int64_t n, const double* dpart, int nb,
-    const double* scal, int rho_slot, double* rpart, const int* flag, int* flag_out) {
+    const double* scal, int rho_slot, double* rpart, const int* flag, int* flag_out,
+    V* reset) {
   __shared__ double sh[kRedThreads];
   double acc = 0;
   for (int i = threadIdx.x; i < nb; i += kRedThreads) acc += dpart[i];
@@ -377,6 +385,7 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
       r[i] = ri;
       m = fmax(m, (double)fabsf(ri));
     }
+    set_all_ones(&reset[i]);
   }
   const double mm = block_max(m, sh);
   if (threadIdx.x == 0) {
@@ -750,11 +759,11 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
     hipLaunchKernelGGL(k_update_xr_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
                        z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
-                       P.progress.p + 2, w.host_flag_dev);
+                       P.progress.p + 2, w.host_flag_dev, P.zf.p);
     WC_LAUNCH_CHECK();
     mark_residual(st, ev);
     if (it + 1 < maxit) {
-      sweep_apply(st, P, r, ti, ndof);
+      sweep_apply(st, P, r, ti, ndof, true);  // the update has set zf to the sentinel
       const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
       hipLaunchKernelGGL(k_update_s_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
                          nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);
@@ -847,12 +856,12 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
       if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
       hipLaunchKernelGGL(k_update_xr_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, x, r,
                          s, z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
-                         P.progress.p + 2, w.host_flag_dev);
+                         P.progress.p + 2, w.host_flag_dev, P.zf32.p);
       WC_LAUNCH_CHECK();
       mark_residual(st, w.res_event);
       ++total;
       if (total < maxit) {
-        sweep_apply32(st, P, r, ti, ndof);
+        sweep_apply32(st, P, r, ti, ndof, true);  // the update has set zf32 to the sentinel
         const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
         hipLaunchKernelGGL(k_update_s_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
                            nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);

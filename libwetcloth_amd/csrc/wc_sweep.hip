@@ -1457,7 +1457,7 @@ __global__ void k_fill_sentinel(uint4* p, int64_t n) {
 
 template <class V>
 static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, const V* cb,
-                       V* zf, V* zb, KTimer* timer, int64_t ndof) {
+                       V* zf, V* zb, KTimer* timer, int64_t ndof, bool cleared) {
   // Two ticket counters; the outputs start as the sentinel.  Each pass zeroes
   // the other pass's counter, and the forward pass's rhs loader sets every
   // slot of zb to the sentinel as it goes, so an apply clears only zf (just
@@ -1468,8 +1468,10 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   if (!P.ticks_zero) WC_HIP(hipMemsetAsync(tick, 0, sizeof(int) * 2, st));
   static_assert(kThreads % 16 == 0, "slots come in whole 16-byte words");
   const int64_t words = (int64_t)sizeof(V) * P.ncell / 16;
-  hipLaunchKernelGGL(k_fill_sentinel, dim3((unsigned)std::min<int64_t>((words + 255) / 256, 4096)),
-                     dim3(256), 0, st, reinterpret_cast<uint4*>(zf), words);
+  if (!cleared)  // else the PCG update before this apply has filled zf
+    hipLaunchKernelGGL(k_fill_sentinel,
+                       dim3((unsigned)std::min<int64_t>((words + 255) / 256, 4096)), dim3(256), 0,
+                       st, reinterpret_cast<uint4*>(zf), words);
   SweepArgs<V> S = sweep_args<V>(P);
   S.cells = reinterpret_cast<const CellT<V>*>(cf);
   S.in = r;
@@ -1502,13 +1504,13 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
 }
 
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
-                 int64_t ndof) {
-  apply_impl<double>(st, P, r, P.cf.p, P.cb.p, P.zf.p, P.zb.p, timer, ndof);
+                 int64_t ndof, bool cleared) {
+  apply_impl<double>(st, P, r, P.cf.p, P.cb.p, P.zf.p, P.zb.p, timer, ndof, cleared);
 }
 
 void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
-                   int64_t ndof) {
-  apply_impl<float>(st, P, r, P.cf32.p, P.cb32.p, P.zf32.p, P.zb32.p, timer, ndof);
+                   int64_t ndof, bool cleared) {
+  apply_impl<float>(st, P, r, P.cf32.p, P.cb32.p, P.zf32.p, P.zb32.p, timer, ndof, cleared);
 }
 
 static unsigned spmv_blocks(const SweepPlan& P) {

@@ -9,7 +9,8 @@ wc_set_elasto_particles) against the FP64 oracle, through the C ABI.
   projected velocities, the particles after G2P.  The solid's and the
   cell-centred sums are atomic sums on the GPU (another order than the
   oracle's), so FP64 fields agree to 1e-12 relative.
-* Trajectories of 20 substeps at 24^3 in FP64 and in the mixed form.
+* Trajectories of 20 substeps at 24^3 in FP64 and in the mixed form, and
+  5 substeps solved with AMG-PCG.
 * The input checks, and that the adjoint refuses a scene with a solid.
 * The 128^3 wet-cloth scene: the 128^3 liquid column (4.0 M particles,
   block 0.4 x 0.6 x 1 m) starting just above a 257 x 257-vertex cloth at
@@ -117,6 +118,22 @@ def test_mix_trajectory_24(oracle, wc, mixed):
     print(f"{'mixed' if mixed else 'FP64'}: {steps} substeps at {n}^3 with the solid: position "
           f"error {err:.2e} dx, velocity {rel(vg, vc):.2e} relative")
     assert err < (1e-4 if mixed else 1e-8)
+
+
+def test_mix_amg_solver_matches_oracle(oracle, wc):
+    """The mixture's system solved with AMG-PCG ("solver" 1) instead of
+    MIC(0)-PCG: both converge to the same pressure, so 5 substeps at 16^3
+    stay within 1e-8 dx of the oracle at tol 1e-12."""
+    n = 16
+    cpu, gpu = pair(oracle, wc, n, mixscene.wet_sheet(), (0.2, 0.32, 0.2), (0.8, 0.6, 0.8))
+    gpu.set_option("solver", 1)
+    for _ in range(5):
+        assert cpu.substep(DT, 1e-12, 2000)
+        ok, it, _ = gpu.substep(DT, 1e-12, 2000)
+        assert ok and 0 < it < 60
+    err = np.abs(gpu.particles()[0] - cpu.particles()[0]).max() * n
+    print(f"AMG-PCG with the solid: 5 substeps within {err:.2e} dx of the oracle")
+    assert err < 1e-8
 
 
 def test_mix_input_checks(wc):
