@@ -17,7 +17,9 @@ wc_set_elasto_particles) against the FP64 oracle, through the C ABI.
   y = 0.3 (1 m square, 2 mm, 0.2 kg/m^2, porosity U[0.3, 0.7]), 150
   substeps of dt = 1/960 in the mixed form: every solve converges, the
   particles stay finite and inside the container, and the cloth holds
-  liquid back (fewer particles below it than in the same run without it).
+  liquid back (fewer particles below it than in the same run without it);
+  and one substep of each form from the same FP32 state once the column
+  has reached the cloth agrees within 1e-4 dx.
 """
 import numpy as np
 import pytest
@@ -163,6 +165,41 @@ def test_mix_input_checks(wc):
     with pytest.raises(wc.WetclothError, match="adjoint"):
         s.set_elasto(*mixscene.args(sh))
     s.set_frozen(0, 0)
+
+
+def test_config2_mixed_one_substep_against_fp64(wc):
+    """The 128^3 wet-cloth scene after 110 mixed substeps (the column has
+    reached the cloth), rounded to FP32, then one substep in each form from
+    that state: positions within 1e-4 dx, and both pressures meet the
+    solve's FP64 residual criterion."""
+    n, dt, tol = 128, 1.0 / 960, 1e-6
+    s = wc.Scene((n, n, n), 1.0 / n, precision=wc.WC_MIXED)
+    s.seed_block((0.0, 0.35, 0.0), (0.4, 0.95, 1.0), seed=2)
+    cloth = mixscene.config2_sheet()
+    s.set_elasto(*mixscene.args(cloth))
+    for _ in range(110):
+        assert s.substep(dt, tol, 2000)[0]
+    x, v, Cm = s.particles()
+    s.close()
+    wet = float((x[1] < 0.3 + 2.0 / n).mean())
+    out = {}
+    for prec in (wc.WC_FP64, wc.WC_MIXED):
+        t = wc.Scene((n, n, n), 1.0 / n, precision=prec)
+        t.set_particles(x, v, Cm)  # FP32 values, so both forms start alike
+        t.set_elasto(*mixscene.args(cloth))
+        ok, it, res = t.substep(dt, tol, 2000)
+        assert ok
+        rs, col, val, b, p = t.system()
+        import scipy.sparse as sp
+        A = sp.csr_matrix((val, col, rs), shape=(len(b), len(b)))
+        assert np.abs(b - A @ p).max() <= tol * np.abs(b).max()
+        out[prec] = t.particles()[0], it
+        t.close()
+    (xf, itf), (xm, itm) = out[wc.WC_FP64], out[wc.WC_MIXED]
+    dxe = float(np.abs(xm - xf).max() * n)
+    print(f"128^3 wet cloth, one substep after 110: {wet:.1%} of the liquid at or below "
+          f"the cloth; |x_mixed - x_fp64| {dxe:.2e} dx; iterations {itm} mixed, {itf} FP64")
+    assert wet > 0.01 and dxe < 1e-4
 
 
 def below_sheet(wc, cloth, steps, n=128, dt=1.0 / 960):
