@@ -998,6 +998,10 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
     c->pcg.use_sweeps = value != 0;
     return WC_OK;
   }
+  if (n == "sweep_subtiles") {
+    c->pcg.sweep.sub = value != 0;
+    return WC_OK;
+  }
   if (n == "mixed_digits") {
     if (value < 1 || value > 15) return fail(c, WC_ERR_ARG, "mixed_digits must be 1..15");
     c->pcg.mixed_digits = (int)value;

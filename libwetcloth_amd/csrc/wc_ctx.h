@@ -133,7 +133,12 @@ struct SweepPlan {
   DevBuf<float> cf32, cb32, coef32, zf32, zb32, rb32, sb32;
   DevBuf<double> bb;
   DevBuf<int> progress, flags;
-  DevBuf<double> tdot;              // per-tile z . r partials of the last apply
+  DevBuf<double> tdot;              // per-unit z . r partials of the last apply
+  int ndot = 0;                     // how many (tiles, or sub-tiles)
+  // Sub-tile sweeps (one wave per 8x8 quarter of a tile, k_sweep_sub) or
+  // the tile sweeps (k_sweep); both give the same values bit for bit
+  bool sub = true;
+  DevBuf<uint32_t> order_sub;       // sub-tile ticket order
   // The forward sweep's ticket counter is zero: the sweeps zero each
   // other's counter, so only the first apply after the plan needs a clear.
   bool ticks_zero = false;
@@ -451,7 +456,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
 // MIC(0) factor of the prepared box (pipelined), into the sweep records
 void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr);
 // MIC(0) apply on box vectors: r (box) -> P.zb; the backward sweep also
-// leaves the per-tile partials of z . r in P.tdot (P.ntiles of them)
+// leaves the per-unit partials of z . r in P.tdot (P.ndot of them)
 // cleared: the forward output already holds the sentinel (the PCG update
 // that precedes an apply in the loop sets it), so the apply skips its fill
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,

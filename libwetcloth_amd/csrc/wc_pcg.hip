@@ -739,7 +739,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   box_gather(st, P, b, r);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
   sweep_apply(st, P, r, tm ? tm->sample() : nullptr, ndof);  // z and the z . r tile partials
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ntiles,
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ndot,
                      w.scal.p, (int)R_SET_RHO);
   WC_LAUNCH_CHECK();
   const double rho = read_scal(st, w, S_RHO);
@@ -766,7 +766,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
       sweep_apply(st, P, r, ti, ndof, true);  // the update has set zf to the sentinel
       const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
       hipLaunchKernelGGL(k_update_s_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
-                         nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);
+                         nc, P.tdot.p, P.ndot, w.scal.p, rho_slot, rho_next);
       WC_LAUNCH_CHECK();
       rho_slot = rho_next;
     }
@@ -835,7 +835,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     ++w.passes;
     const double inner_tol = std::max(tolr, eta * res);
     sweep_apply32(st, P, r, tm ? tm->sample() : nullptr, ndof);  // z and the z . r partials
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ntiles,
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, P.tdot.p, P.ndot,
                        w.scal.p, (int)R_SET_RHO);
     WC_LAUNCH_CHECK();
     const double rho = read_scal(st, w, S_RHO);
@@ -864,7 +864,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
         sweep_apply32(st, P, r, ti, ndof, true);  // the update has set zf32 to the sentinel
         const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
         hipLaunchKernelGGL(k_update_s_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
-                           nc, P.tdot.p, P.ntiles, w.scal.p, rho_slot, rho_next);
+                           nc, P.tdot.p, P.ndot, w.scal.p, rho_slot, rho_next);
         WC_LAUNCH_CHECK();
         rho_slot = rho_next;
       }

@@ -29,6 +29,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "wc_ctx.h"
@@ -1283,6 +1284,349 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
   }
 }
 
+// ---- Sub-tile sweeps: one wave per 8x8 quarter of a tile ----
+//
+// The same passes over the same skewed layout, records and operand words,
+// with the work unit cut to an 8x8 quarter (sub-tile) of a 16x16 tile, so
+// that one wave holds every pencil of its unit.  Its lanes then exchange
+// each step's values through LDS without a workgroup barrier (the LDS
+// processes one wave's operations in order, so a read issued after a write
+// sees it), and the only barriers are those that hand over a batch of kNB
+// steps of records and face values from the loader and face waves.  The
+// hop between two units costs the unit's width in steps (8 instead of 16)
+// plus the face round trip, and there are twice as many hops; the step
+// itself is what the change buys.
+//
+// A sub-tile (sa, sb) of tile (a, b) runs every step w of the tile, 0 ..
+// nsteps - 1, for its 64 pencils (li, lj) = (8 sa + ui, 8 sb + uj): the
+// steps outside its pencils' cells are padding, which it writes as +0 like
+// every other slot, so each slot of the box is still written once per pass.
+// Units are claimed by ticket in (A + B, A) order over the sub-tile grid
+// (A, B) = (2 a + sa, 2 b + sb), and a unit waits only for the units at
+// (A - 1, B) and (A, B - 1): inside a tile through the slot offsets -257 and
+// -272 (one step back and one pencil over), across tiles through K.dx and
+// K.dy, mirrored in the backward pass.
+constexpr int kS = 8;             // sub-tile edge (pencils)
+constexpr int kNB = 8;            // steps per batch
+constexpr int kSubPlanes = 16;    // exchange planes (step % 16), > 2 kNB - 1
+constexpr int kSubThreads = 192;  // compute, face and loader waves
+
+template <class V>
+struct SubRing {
+  // ring depth (batches, a power of two) and batches loaded ahead (fewer),
+  // as deep as the load counter and the LDS allow
+  static constexpr int kBatches = sizeof(V) == 4 ? 8 : 4;
+  static constexpr int kAhead = sizeof(V) == 4 ? 5 : 3;
+  static constexpr int kCellOps = kNB * 64 * (int)sizeof(CellT<V>) / 1024;
+  static constexpr int kRhsOps = kNB * 64 * (int)sizeof(V) / 1024;
+  static constexpr int kOpOps = kNB * 64 * 4 / 1024;
+  static constexpr int kFwdLoads = kCellOps + kRhsOps + kOpOps;
+  static constexpr int kBwdLoads = kFwdLoads + kRhsOps;  // and r for z . r
+  static_assert(kAhead < kBatches, "the ring holds the batches in flight and the one in use");
+  static_assert((kAhead - 1) * kFwdLoads + kAhead * kRhsOps <= 63 &&
+                    (kAhead - 1) * kBwdLoads <= 63,
+                "the loader's loads and stores in flight fit its 6-bit counter");
+};
+
+// s_waitcnt lgkmcnt(0) alone (gfx9 encoding)
+__device__ inline void wait_lgkmcnt0() {
+  __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (3 << 14));
+}
+
+template <int KIND, class V>
+__global__ __launch_bounds__(kSubThreads) void k_sweep_sub(SweepArgs<V> S) {
+  static_assert(KIND == K_FWD || KIND == K_BWD, "sweeps only");
+  constexpr bool BWD = KIND == K_BWD;
+  using Ring = SubRing<V>;
+  constexpr int kRB = Ring::kBatches;
+  constexpr int kLA = Ring::kAhead;
+  // exchange planes in the tile's 17 x 17 geometry (the operand words'),
+  // each with a +0 at [0][0] that no unit writes
+  __shared__ V pl[kSubPlanes * kPlane];
+  // record rings [batch % kRB][step in batch][physical position in the unit]
+  __shared__ CellT<V> rc[kRB][kNB][64];
+  __shared__ V rr[kRB][kNB][64];
+  __shared__ uint32_t rk[kRB][kNB][64];
+  __shared__ V rq[BWD ? kRB : 1][kNB][64];
+  __shared__ int s_unit;
+  if (threadIdx.x < kSubPlanes) pl[threadIdx.x * kPlane] = (V)0;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int Bz = S.K.Bz;
+  const int nsteps = S.K.nsteps;
+  const int nb = (nsteps + kNB - 1) / kNB;
+  const int nsa = 2 * S.K.ntile_a;
+  auto PL = [](int w) -> uint32_t { return (uint32_t)(w & (kSubPlanes - 1)) * kPlane; };
+  const int sgn = BWD ? -kThreads : kThreads;
+  // The next unit (A, B), counted from the sweep's start corner, or -1.
+  // Each role runs its own loop over the units (so the compute wave's code
+  // holds none of the loader's direct-to-LDS loads, and the compiler adds
+  // no wait for them there); the loops meet at the same barriers.
+  auto claim = [&]() -> int {
+    if (threadIdx.x == 0) {
+      const int t = atomicAdd(S.ticket, 1);
+      s_unit = t < S.ntiles ? (int)S.order[t] : -1;
+      if (t == 0) *S.next_ticket = 0;  // the other pass's counter
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(s_unit);
+  };
+  // a unit's tile and quarter in the sweep's frame and the physical ones
+  struct Unit {
+    int A, B, sa, sb, a, b, psa, psb;
+  };
+  auto decode = [&](int unit) -> Unit {
+    Unit u;
+    u.A = unit & 0xFFFF;
+    u.B = unit >> 16;
+    u.sa = u.A & 1;
+    u.sb = u.B & 1;
+    u.a = BWD ? S.K.ntile_a - 1 - (u.A >> 1) : u.A >> 1;
+    u.b = BWD ? S.K.ntile_b - 1 - (u.B >> 1) : u.B >> 1;
+    u.psa = BWD ? 1 - u.sa : u.sa;
+    u.psb = BWD ? 1 - u.sb : u.sb;
+    return u;
+  };
+  auto base_of = [&](const Unit& u, int li, int lj) -> uint32_t {
+    const int lip = BWD ? kT - 1 - li : li, ljp = BWD ? kT - 1 - lj : lj;
+    return (uint32_t)skew_slot(S.K, u.a, u.b, lip, ljp, BWD ? nsteps - 1 : 0);
+  };
+  if (wave == 2) {
+    // loader: the unit's 64 slots of step w are 8 rows of 8 consecutive
+    // slots (physical order p = row * 8 + column); steps past the end
+    // reload the last one, so every batch issues the same operations and
+    // the counter waits stay exact
+    typedef __attribute__((address_space(3))) void* LdsPtr;
+    typedef unsigned int U4 __attribute__((ext_vector_type(4)));
+    U4 ones = {~0u, ~0u, ~0u, ~0u};
+    asm volatile("" : "+v"(ones));
+    for (;;) {
+      const int unit = claim();
+      if (unit < 0) return;
+      const Unit u = decode(unit);
+      const uint32_t tile0 = (uint32_t)skew_slot(S.K, u.a, u.b, 0, 0, 0);
+      auto slot = [&](int w, int p) -> uint32_t {
+        const int wc = w < nsteps ? w : nsteps - 1;
+        const int wp = BWD ? nsteps - 1 - wc : wc;
+        return tile0 + (uint32_t)wp * kThreads +
+               (uint32_t)((kS * u.psb + (p >> 3)) * kT + kS * u.psa + (p & 7));
+      };
+      // 16 bytes per lane from 'src' (element size E) into 'dst' for the
+      // kNB steps of batch n: each wave-wide load covers 1024 / (64 E) steps
+      auto stream = [&](const void* src, void* dst, int n, auto E) {
+        constexpr int per = 1024 / (64 * (int)decltype(E)::value);  // steps per load
+        constexpr int lanes = 64 / per;                              // lanes per step
+#pragma unroll
+        for (int j = 0; j < kNB / per; ++j) {
+          const int st = j * per + lane / lanes;
+          const int p = (lane % lanes) * (16 / (int)decltype(E)::value);
+          const char* g = reinterpret_cast<const char*>(src) +
+                          (size_t)slot(n * kNB + st, p) * decltype(E)::value;
+          __builtin_amdgcn_global_load_lds(g, (LdsPtr)(reinterpret_cast<char*>(dst) + j * 1024),
+                                           16, 0, 0);
+        }
+      };
+      using EV = std::integral_constant<int, (int)sizeof(V)>;
+      using E4 = std::integral_constant<int, 4>;
+      auto issue = [&](int n) {
+        const int r = n & (kRB - 1);
+        // records: 16 bytes per lane, sizeof(CellT) / 16 pieces per slot
+        constexpr int kPieces = (int)sizeof(CellT<V>) / 16;
+#pragma unroll
+        for (int st = 0; st < kNB; ++st)
+#pragma unroll
+          for (int h = 0; h < kPieces; ++h) {
+            const int p = h * (64 / kPieces) + lane / kPieces;
+            const char* g = reinterpret_cast<const char*>(S.cells) +
+                            (size_t)slot(n * kNB + st, p) * sizeof(CellT<V>) +
+                            (lane % kPieces) * 16;
+            __builtin_amdgcn_global_load_lds(
+                g, (LdsPtr)(reinterpret_cast<char*>(&rc[r][st][0]) + h * 1024), 16, 0, 0);
+          }
+        stream(S.in, &rr[r][0][0], n, EV{});
+        stream(S.code, &rk[r][0][0], n, E4{});
+        if constexpr (BWD) stream(S.dot_r, &rq[r][0][0], n, EV{});
+      };
+      // forward pass: the backward output's slots of batch n to the sentinel
+      auto clear = [&](int n) {
+        constexpr int per = 1024 / (64 * (int)sizeof(V));
+        constexpr int lanes = 64 / per;
+#pragma unroll
+        for (int j = 0; j < kNB / per; ++j) {
+          const int st = j * per + lane / lanes;
+          const int p = (lane % lanes) * (16 / (int)sizeof(V));
+          *reinterpret_cast<U4*>(reinterpret_cast<char*>(S.reset) +
+                                 (size_t)slot(n * kNB + st, p) * sizeof(V)) = ones;
+        }
+      };
+      for (int n = 0; n < kLA; ++n) issue(n);
+      for (int n = 0; n < nb; ++n) {
+        // batch n complete; the loads of the kLA - 1 batches after it (and
+        // in the forward pass the clears of the kLA batches before it,
+        // issued after its loads) may still be in flight
+        if (BWD)
+          wait_vmcnt<(kLA - 1) * Ring::kBwdLoads>();
+        else if (n < kLA)
+          wait_vmcnt<(kLA - 1) * Ring::kFwdLoads>();
+        else
+          wait_vmcnt<(kLA - 1) * Ring::kFwdLoads + kLA * Ring::kRhsOps>();
+        __builtin_amdgcn_s_barrier();
+        issue(n + kLA);
+        if constexpr (!BWD) clear(n);
+      }
+      wait_vmcnt<0>();
+      __builtin_amdgcn_s_barrier();
+    }
+  }
+  if (wave == 1) {
+    // face wave: lane (s, m) = (lane / 8, lane % 8) holds step n kNB + s
+    // of the x face of row m (pencil (8 sa, 8 sb + m)) and of the y face
+    // of column m (pencil (8 sa + m, 8 sb)); upstream values come from
+    // the unit before in x (y): the other quarter of this tile, one step
+    // back and one pencil over, or the tile before.  Batch n is read
+    // during batch n - 2 and published during batch n - 1.
+    const int s = lane >> 3, m = lane & 7;
+    for (;;) {
+      const int unit = claim();
+      if (unit < 0) return;
+      const Unit u = decode(unit);
+      const bool hx = u.A > 0, hy = u.B > 0;
+      const int xi = kS * u.sa, xj = kS * u.sb + m, yi = kS * u.sa + m, yj = kS * u.sb;
+      const uint32_t xbase = base_of(u, xi, xj), ybase = base_of(u, yi, yj);
+      const int64_t dxs =
+          u.sa ? (BWD ? kThreads + 1 : -(kThreads + 1)) : (BWD ? -S.K.dx : S.K.dx);
+      const int64_t dys =
+          u.sb ? (BWD ? kThreads + kT : -(kThreads + kT)) : (BWD ? -S.K.dy : S.K.dy);
+      const uint32_t xd = hx ? (uint32_t)dxs : 0u, yd = hy ? (uint32_t)dys : 0u;
+      // the pencil's cell of step w (clamped into the pencil, so the
+      // address is always a slot of the box) and the upstream slot beside it
+      auto addr = [&](uint32_t base, int off, int w, uint32_t d) -> uint32_t {
+        const int kk = w - off;
+        const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
+        return base + (uint32_t)(sgn * (kc + off)) + d;
+      };
+      auto need = [&](bool h, int off, int w) -> bool {
+        const int kk = w - off;
+        return h && w < nsteps && kk >= 0 && kk < Bz;
+      };
+      V fx = 0, fy = 0;
+      auto fetch = [&](int n) {
+        const int w = n * kNB + s;
+        fx = load_coherent(at(S.out, addr(xbase, xi + xj, w, xd)));
+        fy = load_coherent(at(S.out, addr(ybase, yi + yj, w, yd)));
+      };
+      // batch n's values into the halos of the planes its steps read;
+      // a value still the sentinel is read again (bounded wait)
+      auto publish = [&](int n) {
+        const int w = n * kNB + s;
+        const bool nx = need(hx, xi + xj, w), ny = need(hy, yi + yj, w);
+        // (the first check stays outside the retry loop, whose reloads
+        // would otherwise make it wait for the other sets' reads as well)
+        if (__any((nx && is_sentinel(fx)) || (ny && is_sentinel(fy)))) {
+          long long spins = 0;
+          for (;;) {
+            if (++spins > kSpinLimit || load_flag(S.abort_flag)) {
+              store_flag(S.abort_flag, 1);
+              fx = fy = (V)0;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            if (nx && is_sentinel(fx)) fx = load_coherent(at(S.out, addr(xbase, xi + xj, w, xd)));
+            if (ny && is_sentinel(fy)) fy = load_coherent(at(S.out, addr(ybase, yi + yj, w, yd)));
+            if (!__any((nx && is_sentinel(fx)) || (ny && is_sentinel(fy)))) break;
+          }
+        }
+        if (w < nsteps) {
+          pl[PL(w - 1) + (uint32_t)((xj + 1) * (kT + 1) + xi)] = nx ? fx : (V)0;
+          pl[PL(w - 1) + (uint32_t)(yj * (kT + 1) + yi + 1)] = ny ? fy : (V)0;
+        }
+      };
+      // backward pass: the z . r partial of compute lane 'lane', summed
+      // here so the compute wave's steps stay short: r of batch n is copied
+      // from the ring during batch n, and z of batch n read from the
+      // exchange planes during batch n + 1 (the planes keep a step for 16
+      // steps), in step order, as the compute lane would sum them
+      const int cli = kS * u.sa + (lane & 7), clj = kS * u.sb + (lane >> 3);
+      const uint32_t cown = (uint32_t)((clj + 1) * (kT + 1) + cli + 1);
+      const int cpos = BWD ? 63 - lane : lane;
+      double dot = 0.0;
+      V rsave[kNB];
+      auto take_r = [&](int n) {
+#pragma unroll
+        for (int st = 0; st < kNB; ++st) rsave[st] = rq[n & (kRB - 1)][st][cpos];
+      };
+      auto add_z = [&](int n) {
+#pragma unroll
+        for (int st = 0; st < kNB; ++st) {
+          const int w = n * kNB + st;
+          if (w < nsteps) dot += (double)pl[PL(w) + cown] * (double)rsave[st];
+        }
+      };
+      fetch(0);
+      publish(0);
+      if (nb > 1) fetch(1);
+      for (int n = 0; n < nb; ++n) {
+        __syncthreads();  // batch n handed over
+        if (n + 1 < nb) {
+          publish(n + 1);
+          if (n + 2 < nb) fetch(n + 2);
+        }
+        if constexpr (BWD) {
+          if (n > 0) add_z(n - 1);
+          take_r(n);
+        }
+      }
+      __syncthreads();
+      if constexpr (BWD) {
+        add_z(nb - 1);
+        // the unit's partial: a fixed tree over its 64 lanes, stored at the
+        // physical quarter's index
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
+        if (lane == 0) S.dot_out[(2 * u.b + u.psb) * nsa + 2 * u.a + u.psa] = dot;
+      }
+    }
+  }
+  // compute wave: lane (ui, uj) = (lane % 8, lane / 8) is pencil
+  // (li, lj) = (8 sa + ui, 8 sb + uj) of the tile
+  const int pos = BWD ? 63 - lane : lane;
+  for (;;) {
+    const int unit = claim();
+    if (unit < 0) return;
+    const Unit u = decode(unit);
+    const int li = kS * u.sa + (lane & 7), lj = kS * u.sb + (lane >> 3);
+    const uint32_t base = base_of(u, li, lj);
+    const uint32_t own = (uint32_t)((lj + 1) * (kT + 1) + li + 1);
+    for (int n = 0; n < nb; ++n) {
+      // this wave's plane writes are complete (the face wave sums z . r
+      // from them), and batch n's records and face values are in place
+      wait_lgkmcnt0();
+      __syncthreads();
+      const int r = n & (kRB - 1);
+#pragma unroll
+      for (int st = 0; st < kNB; ++st) {
+        const int w = n * kNB + st;
+        if (w >= nsteps) break;
+        const CellT<V> c = rc[r][st][pos];
+        const V rhs = rr[r][st][pos];
+        const uint32_t op = rk[r][st][pos];
+        const uint32_t at0 = PL(w - 1) + own;
+        const V v0 = pl[at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 0, 10)];
+        const V v1 = pl[at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 10, 10)];
+        const V v2 = pl[at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 20, 10)];
+        CellIn<V> ci;
+        ci.c = c;
+        ci.rhs = rhs;
+        const V z = sweep_cell_ordered<V>(ci, v0, v1, v2);
+        store_coherent(at(S.out, base + (uint32_t)(sgn * w)), z);
+        pl[PL(w) + own] = z;
+      }
+    }
+    wait_lgkmcnt0();
+    __syncthreads();
+  }
+}
+
 inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 Skew skew_of(const SweepPlan& P) {
@@ -1356,8 +1700,20 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   P.order.ensure(P.ntiles);
   WC_HIP(hipMemcpyAsync(P.order.p, order.data(), sizeof(uint32_t) * P.ntiles,
                         hipMemcpyHostToDevice, st));
+  // sub-tile tickets in (A+B, A) order over the 2 ntile_a x 2 ntile_b units
+  std::vector<uint32_t> order_sub;
+  const int nua = 2 * P.ntile_a, nub = 2 * P.ntile_b;
+  order_sub.reserve(4 * P.ntiles);
+  for (int s = 0; s <= nua + nub - 2; ++s)
+    for (int A = 0; A < nua; ++A) {
+      const int B = s - A;
+      if (B >= 0 && B < nub) order_sub.push_back((uint32_t)A | ((uint32_t)B << 16));
+    }
+  P.order_sub.ensure(4 * P.ntiles);
+  WC_HIP(hipMemcpyAsync(P.order_sub.p, order_sub.data(), sizeof(uint32_t) * 4 * P.ntiles,
+                        hipMemcpyHostToDevice, st));
   P.progress.ensure(8);
-  P.tdot.ensure(P.ntiles);
+  P.tdot.ensure(4 * P.ntiles);
   // tickets and the abort flag start at zero (the abort flag is cleared
   // again by each factor, not by the applies: once set it stays set)
   WC_HIP(hipMemsetAsync(P.progress.p, 0, sizeof(int) * 8, st));
@@ -1393,7 +1749,7 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   hipLaunchKernelGGL(k_fill_box, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, A,
                      (const RowSum*)rs);
   WC_LAUNCH_CHECK();
-  WC_HIP(hipStreamSynchronize(st));  // 'order' must outlive its copy
+  WC_HIP(hipStreamSynchronize(st));  // 'order' and 'order_sub' must outlive their copies
   P.valid = true;
 }
 
@@ -1480,8 +1836,17 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.code = P.opf.p;  // the sweeps read operand words, not the code
   S.ticket = tick;
   S.next_ticket = tick + 1;
+  const bool sub = P.sub;
+  if (sub) {
+    S.ntiles = 4 * P.ntiles;  // tickets: the sub-tile units
+    S.order = P.order_sub.p;
+  }
+  P.ndot = S.ntiles;
   hipEvent_t e0 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_FWD>::value), 0, st, S);
+  if (sub)
+    hipLaunchKernelGGL((k_sweep_sub<K_FWD, V>), dim3(S.ntiles), dim3(kSubThreads), 0, st, S);
+  else
+    hipLaunchKernelGGL((k_sweep<K_FWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_FWD>::value), 0, st, S);
   WC_LAUNCH_CHECK();
   hipEvent_t e1 = timer ? timer->mark(st) : nullptr;
   S.cells = reinterpret_cast<const CellT<V>*>(cb);
@@ -1494,7 +1859,10 @@ static void apply_impl(hipStream_t st, SweepPlan& P, const V* r, const V* cf, co
   S.ticket = tick + 1;
   S.next_ticket = tick;
   hipEvent_t e2 = timer ? timer->mark(st) : nullptr;
-  hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_BWD>::value), 0, st, S);
+  if (sub)
+    hipLaunchKernelGGL((k_sweep_sub<K_BWD, V>), dim3(S.ntiles), dim3(kSubThreads), 0, st, S);
+  else
+    hipLaunchKernelGGL((k_sweep<K_BWD, V>), dim3(P.ntiles), dim3(SweepBlock<K_BWD>::value), 0, st, S);
   WC_LAUNCH_CHECK();
   P.ticks_zero = true;
   if (timer) {
@@ -1593,6 +1961,7 @@ void SweepPlan::release() {
   sb32.release();
   bb.release();
   order.release();
+  order_sub.release();
   progress.release();
   tdot.release();
   flags.release();

@@ -51,7 +51,8 @@ def test_pipelined_matches_level_scheduled(oracle, wc, n, block_hi):
                                          (64, (0.4, 0.6, 1.0))])
 def test_box_apply_and_multiply_bitwise(oracle, wc, n, block_hi):
     """apply_preconditioner and multiply on the context's system, through the
-    pipelined sweeps and the box stencil, equal the oracle's bit for bit."""
+    pipelined sweeps (one wave per 8x8 quarter tile, and one workgroup per
+    16x16 tile) and the box stencil, equal the oracle's bit for bit."""
     s = prepared(wc, oracle, n, block_hi=block_hi)
     ok, it, res, p, val, b = solve(s, True)
     assert ok
@@ -59,12 +60,42 @@ def test_box_apply_and_multiply_bitwise(oracle, wc, n, block_hi):
     rng = np.random.default_rng(n)
     r = rng.standard_normal(len(b))
     invdiag, lv = oracle.mic0(rs, col, val)
-    assert np.array_equal(s.apply_preconditioner(r), oracle.apply_precond(rs, col, invdiag, lv, r))
+    want = oracle.apply_precond(rs, col, invdiag, lv, r)
+    for sub in (1, 0, 1):
+        s.set_option("sweep_subtiles", sub)
+        assert np.array_equal(s.apply_preconditioner(r), want), sub
     assert np.array_equal(s.multiply(r), oracle.spmv(rs, col, val, r))
     # and the level-scheduled kernels agree with both
     solve(s, False)
     assert np.array_equal(s.apply_preconditioner(r), oracle.apply_precond(rs, col, invdiag, lv, r))
     assert np.array_equal(s.multiply(r), oracle.spmv(rs, col, val, r))
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("n,block_hi", [(32, (0.4, 0.6, 1.0)), (48, (0.9, 0.3, 0.55))])
+def test_subtile_and_tile_solves_agree(oracle, wc, mixed, n, block_hi):
+    """Whole solves with either sweep kernel: the applies are the same
+    values, and only the z . r partials are summed over other units (a
+    quarter tile instead of a tile), so the iterations agree to within one
+    and the pressures to rounding."""
+    g, x, v = oracle.dam_break(n, block_hi=block_hi, seed=3, sigma=0.5)
+    res = {}
+    for sub in (1, 0):
+        s = wc.Scene((n, n, n), 1.0 / n, precision=wc.WC_MIXED if mixed else wc.WC_FP64)
+        s.set_option("sweep_subtiles", sub)
+        s.set_particles(x, v)
+        s.rebucketizeParticles()
+        s.mapParticleNodesAPIC()
+        s.stepVelocity(1.0 / 480)
+        ok, it, r = s.projectFine(1.0 / 480, 1e-8, 2000)
+        assert ok, sub
+        res[sub] = (it, s.system()[4])
+        s.close()
+    (i1, p1), (i0, p0) = res[1], res[0]
+    print(f"{n}^3 {'mixed' if mixed else 'FP64'}: iterations {i1} (quarter tiles) / {i0} (tiles), "
+          f"pressure difference {np.abs(p1 - p0).max() / np.abs(p0).max():.2e}")
+    assert abs(i1 - i0) <= 1
+    assert np.abs(p1 - p0).max() <= (1e-6 if mixed else 1e-9) * np.abs(p0).max()
 
 
 def test_pipelined_solve_matches_oracle(oracle, wc):
