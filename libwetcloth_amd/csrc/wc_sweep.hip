@@ -1307,25 +1307,28 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
 // -272 (one step back and one pencil over), across tiles through K.dx and
 // K.dy, mirrored in the backward pass.
 constexpr int kS = 8;             // sub-tile edge (pencils)
-constexpr int kNB = 8;            // steps per batch
+constexpr int kNB = 4;            // steps per batch
 constexpr int kSubPlanes = 16;    // exchange planes (step % 16), > 2 kNB - 1
-constexpr int kSubThreads = 192;  // compute, face and loader waves
+constexpr int kFaceWaves = 1;     // face waves (one read in flight each)
+constexpr int kSubThreads = 64 * (2 + kFaceWaves);  // compute, face(s), loader
 
 template <class V>
 struct SubRing {
-  // ring depth (batches, a power of two) and batches loaded ahead (fewer),
-  // as deep as the load counter and the LDS allow
-  static constexpr int kBatches = sizeof(V) == 4 ? 8 : 4;
-  static constexpr int kAhead = sizeof(V) == 4 ? 5 : 3;
   static constexpr int kCellOps = kNB * 64 * (int)sizeof(CellT<V>) / 1024;
   static constexpr int kRhsOps = kNB * 64 * (int)sizeof(V) / 1024;
   static constexpr int kOpOps = kNB * 64 * 4 / 1024;
   static constexpr int kFwdLoads = kCellOps + kRhsOps + kOpOps;
   static constexpr int kBwdLoads = kFwdLoads + kRhsOps;  // and r for z . r
-  static_assert(kAhead < kBatches, "the ring holds the batches in flight and the one in use");
-  static_assert((kAhead - 1) * kFwdLoads + kAhead * kRhsOps <= 63 &&
-                    (kAhead - 1) * kBwdLoads <= 63,
-                "the loader's loads and stores in flight fit its 6-bit counter");
+  // ring depth (batches, a power of two): 64 KB of factor records
+  static constexpr int kBatches = 64 * 1024 / (kNB * 64 * (int)sizeof(CellT<V>));
+  // batches loaded ahead: as many as the 6-bit load counter and the ring allow
+  static constexpr int ahead() {
+    int a = kBatches - 1;
+    while ((a - 1) * kFwdLoads + a * kRhsOps > 63 || (a - 1) * kBwdLoads > 63) --a;
+    return a;
+  }
+  static constexpr int kAhead = ahead();
+  static_assert(kAhead >= 2 && kAhead < kBatches, "the ring holds the batches in flight");
 };
 
 // s_waitcnt lgkmcnt(0) alone (gfx9 encoding)
@@ -1478,67 +1481,101 @@ __global__ __launch_bounds__(kSubThreads) void k_sweep_sub(SweepArgs<V> S) {
       __builtin_amdgcn_s_barrier();
     }
   }
-  if (wave == 1) {
-    // face wave: lane (s, m) = (lane / 8, lane % 8) holds step n kNB + s
-    // of the x face of row m (pencil (8 sa, 8 sb + m)) and of the y face
-    // of column m (pencil (8 sa + m, 8 sb)); upstream values come from
-    // the unit before in x (y): the other quarter of this tile, one step
-    // back and one pencil over, or the tile before.  Batch n is read
-    // during batch n - 2 and published during batch n - 1.
-    const int s = lane >> 3, m = lane & 7;
+  if (wave == 1 || wave >= 3) {
+    // face waves (wave 1, and waves 3 .. with kFaceWaves > 1): face wave p
+    // publishes the batches n = p (mod kFaceWaves), each read with one
+    // read in flight on the wave's own counter, issued when the wave
+    // publishes its previous batch, so a read has kFaceWaves batches to
+    // return.  Lane (s, m) = (lane / 8, lane % 8) holds step n kNB + s of
+    // the x face of row m (pencil (8 sa, 8 sb + m)) and of the y face of
+    // column m (pencil (8 sa + m, 8 sb)); upstream values come from the
+    // unit before in x (y): the other quarter of this tile, one step back
+    // and one pencil over, or the tile before.
+    const int par = wave == 1 ? 0 : wave - 2;
+    // item k of a lane is value g = 64 k + lane of the batch: face g /
+    // (kNB kS) (x, then y), step (g / kS) % kNB, row or column g % kS
+    constexpr int kPer = kNB * kS;
+    constexpr int kItems = 2 * kPer / 64;
+    static_assert(2 * kPer % 64 == 0, "a batch's face values fill whole waves");
     for (;;) {
       const int unit = claim();
       if (unit < 0) return;
       const Unit u = decode(unit);
-      const bool hx = u.A > 0, hy = u.B > 0;
-      const int xi = kS * u.sa, xj = kS * u.sb + m, yi = kS * u.sa + m, yj = kS * u.sb;
-      const uint32_t xbase = base_of(u, xi, xj), ybase = base_of(u, yi, yj);
       const int64_t dxs =
           u.sa ? (BWD ? kThreads + 1 : -(kThreads + 1)) : (BWD ? -S.K.dx : S.K.dx);
       const int64_t dys =
           u.sb ? (BWD ? kThreads + kT : -(kThreads + kT)) : (BWD ? -S.K.dy : S.K.dy);
-      const uint32_t xd = hx ? (uint32_t)dxs : 0u, yd = hy ? (uint32_t)dys : 0u;
+      uint32_t ibase[kItems], idel[kItems], ihalo[kItems];
+      int ioff[kItems], istep[kItems];
+      bool iup[kItems];
+#pragma unroll
+      for (int k = 0; k < kItems; ++k) {
+        const int g = 64 * k + lane;
+        const bool xf = g < kPer;
+        const int m = g % kS;
+        istep[k] = (g / kS) % kNB;
+        const int li = xf ? kS * u.sa : kS * u.sa + m, lj = xf ? kS * u.sb + m : kS * u.sb;
+        iup[k] = xf ? u.A > 0 : u.B > 0;
+        ibase[k] = base_of(u, li, lj);
+        ioff[k] = li + lj;
+        idel[k] = iup[k] ? (uint32_t)(xf ? dxs : dys) : 0u;
+        // the halo cell beside the pencil in the exchange plane
+        ihalo[k] = (uint32_t)(xf ? (lj + 1) * (kT + 1) + li : lj * (kT + 1) + li + 1);
+      }
       // the pencil's cell of step w (clamped into the pencil, so the
       // address is always a slot of the box) and the upstream slot beside it
-      auto addr = [&](uint32_t base, int off, int w, uint32_t d) -> uint32_t {
-        const int kk = w - off;
+      auto addr = [&](int k, int w) -> uint32_t {
+        const int kk = w - ioff[k];
         const int kc = kk < 0 ? 0 : (kk >= Bz ? Bz - 1 : kk);
-        return base + (uint32_t)(sgn * (kc + off)) + d;
+        return ibase[k] + (uint32_t)(sgn * (kc + ioff[k])) + idel[k];
       };
-      auto need = [&](bool h, int off, int w) -> bool {
-        const int kk = w - off;
-        return h && w < nsteps && kk >= 0 && kk < Bz;
+      auto need = [&](int k, int w) -> bool {
+        const int kk = w - ioff[k];
+        return iup[k] && w < nsteps && kk >= 0 && kk < Bz;
       };
-      V fx = 0, fy = 0;
+      V fv[kItems];
+#pragma unroll
+      for (int k = 0; k < kItems; ++k) fv[k] = (V)0;
       auto fetch = [&](int n) {
-        const int w = n * kNB + s;
-        fx = load_coherent(at(S.out, addr(xbase, xi + xj, w, xd)));
-        fy = load_coherent(at(S.out, addr(ybase, yi + yj, w, yd)));
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) fv[k] = load_coherent(at(S.out, addr(k, n * kNB + istep[k])));
       };
       // batch n's values into the halos of the planes its steps read;
       // a value still the sentinel is read again (bounded wait)
       auto publish = [&](int n) {
-        const int w = n * kNB + s;
-        const bool nx = need(hx, xi + xj, w), ny = need(hy, yi + yj, w);
+        bool nd[kItems];
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+          nd[k] = need(k, n * kNB + istep[k]);
+          bad = bad || (nd[k] && is_sentinel(fv[k]));
+        }
         // (the first check stays outside the retry loop, whose reloads
-        // would otherwise make it wait for the other sets' reads as well)
-        if (__any((nx && is_sentinel(fx)) || (ny && is_sentinel(fy)))) {
+        // would otherwise make it wait for the other reads in flight too)
+        if (__any(bad)) {
           long long spins = 0;
           for (;;) {
             if (++spins > kSpinLimit || load_flag(S.abort_flag)) {
               store_flag(S.abort_flag, 1);
-              fx = fy = (V)0;
+#pragma unroll
+              for (int k = 0; k < kItems; ++k) fv[k] = (V)0;
               break;
             }
             __builtin_amdgcn_s_sleep(1);
-            if (nx && is_sentinel(fx)) fx = load_coherent(at(S.out, addr(xbase, xi + xj, w, xd)));
-            if (ny && is_sentinel(fy)) fy = load_coherent(at(S.out, addr(ybase, yi + yj, w, yd)));
-            if (!__any((nx && is_sentinel(fx)) || (ny && is_sentinel(fy)))) break;
+            bool again = false;
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+              if (nd[k] && is_sentinel(fv[k]))
+                fv[k] = load_coherent(at(S.out, addr(k, n * kNB + istep[k])));
+              again = again || (nd[k] && is_sentinel(fv[k]));
+            }
+            if (!__any(again)) break;
           }
         }
-        if (w < nsteps) {
-          pl[PL(w - 1) + (uint32_t)((xj + 1) * (kT + 1) + xi)] = nx ? fx : (V)0;
-          pl[PL(w - 1) + (uint32_t)(yj * (kT + 1) + yi + 1)] = ny ? fy : (V)0;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+          const int w = n * kNB + istep[k];
+          if (w < nsteps) pl[PL(w - 1) + ihalo[k]] = nd[k] ? fv[k] : (V)0;
         }
       };
       // backward pass: the z . r partial of compute lane 'lane', summed
@@ -1562,22 +1599,32 @@ __global__ __launch_bounds__(kSubThreads) void k_sweep_sub(SweepArgs<V> S) {
           if (w < nsteps) dot += (double)pl[PL(w) + cown] * (double)rsave[st];
         }
       };
-      fetch(0);
-      publish(0);
-      if (nb > 1) fetch(1);
+      // batch 0 is due at the first barrier: face wave 0 reads and
+      // publishes it now; then each face wave has its next batch in flight
+      if (par == 0) {
+        fetch(0);
+        publish(0);
+      }
+      {
+        const int first = par == 0 ? kFaceWaves : par;
+        if (first < nb) fetch(first);
+      }
       for (int n = 0; n < nb; ++n) {
         __syncthreads();  // batch n handed over
-        if (n + 1 < nb) {
-          publish(n + 1);
-          if (n + 2 < nb) fetch(n + 2);
+        const int due = n + 1;  // published during batch n
+        if (due < nb && due % kFaceWaves == par) {
+          publish(due);
+          if (due + kFaceWaves < nb) fetch(due + kFaceWaves);
         }
         if constexpr (BWD) {
-          if (n > 0) add_z(n - 1);
-          take_r(n);
+          if (par == 0) {
+            if (n > 0) add_z(n - 1);
+            take_r(n);
+          }
         }
       }
       __syncthreads();
-      if constexpr (BWD) {
+      if (BWD && par == 0) {
         add_z(nb - 1);
         // the unit's partial: a fixed tree over its 64 lanes, stored at the
         // physical quarter's index
@@ -1603,21 +1650,26 @@ __global__ __launch_bounds__(kSubThreads) void k_sweep_sub(SweepArgs<V> S) {
       wait_lgkmcnt0();
       __syncthreads();
       const int r = n & (kRB - 1);
+      // the batch's records and operand addresses first (the ring is
+      // complete for the whole batch), so a step waits on one LDS round
+      // trip only: its operands, written by the step before
+      CellIn<V> ci[kNB];
+      uint32_t src[kNB][3];
+#pragma unroll
+      for (int st = 0; st < kNB; ++st) {
+        ci[st].c = rc[r][st][pos];
+        ci[st].rhs = rr[r][st][pos];
+        const uint32_t op = rk[r][st][pos];
+        const uint32_t at0 = PL(n * kNB + st - 1) + own;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          src[st][q] = at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 10 * q, 10);
+      }
 #pragma unroll
       for (int st = 0; st < kNB; ++st) {
         const int w = n * kNB + st;
         if (w >= nsteps) break;
-        const CellT<V> c = rc[r][st][pos];
-        const V rhs = rr[r][st][pos];
-        const uint32_t op = rk[r][st][pos];
-        const uint32_t at0 = PL(w - 1) + own;
-        const V v0 = pl[at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 0, 10)];
-        const V v1 = pl[at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 10, 10)];
-        const V v2 = pl[at0 + (uint32_t)__builtin_amdgcn_sbfe((int)op, 20, 10)];
-        CellIn<V> ci;
-        ci.c = c;
-        ci.rhs = rhs;
-        const V z = sweep_cell_ordered<V>(ci, v0, v1, v2);
+        const V z = sweep_cell_ordered<V>(ci[st], pl[src[st][0]], pl[src[st][1]], pl[src[st][2]]);
         store_coherent(at(S.out, base + (uint32_t)(sgn * w)), z);
         pl[PL(w) + own] = z;
       }

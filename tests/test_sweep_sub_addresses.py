@@ -23,8 +23,8 @@ import pytest
 KT = 16
 THREADS = KT * KT
 KS = 8
-KNB = 8
-KLA = 3
+KNB = 4
+KLA = 9
 ROW = KT + 1  # exchange-plane row stride
 
 
