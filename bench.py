@@ -145,7 +145,7 @@ def roofline(stats: dict, word: int, workload: str) -> dict:
     d = per[dom]
     # the kernels' names as rocprofv3 reports them
     vt = "double" if word == 8 else "float"
-    kname = {"sweep_fwd": f"k_sweep<0, {vt}>", "sweep_bwd": f"k_sweep<1, {vt}>"}[dom]
+    kname = {"sweep_fwd": f"k_sweep_sub<0, {vt}>", "sweep_bwd": f"k_sweep_sub<1, {vt}>"}[dom]
     return {
         "bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(kname, workload),

@@ -308,6 +308,10 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
  *   "pipelined_sweeps" (default 1): apply MIC(0) with the persistent tiled
  *   wavefront kernels; 0 selects one launch per i+j+k level.  Both give
  *   bitwise the same result.
+ *   "sweep_subtiles" (default 1): the persistent kernels work on 8x8
+ *   quarters of the 16x16 tiles, one wave each; 0 selects one workgroup per
+ *   tile.  Both give bitwise the same apply; the PCG's z . r is summed per
+ *   quarter or per tile, so whole solves agree to rounding.
  *   "kernel_timing" (default 0): bracket the solver's sweep and SpMV launches
  *   with HIP events on the context's stream; n > 1 brackets those of every
  *   n-th PCG iteration only (each pair of events costs a little time on the
