@@ -1536,9 +1536,13 @@ __global__ __launch_bounds__(kSubThreads) void k_sweep_sub(SweepArgs<V> S) {
       V fv[kItems];
 #pragma unroll
       for (int k = 0; k < kItems; ++k) fv[k] = (V)0;
+      // (only where the value is needed: a unit's face reads are a fair
+      // share of the sweep's coherent traffic)
       auto fetch = [&](int n) {
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) fv[k] = load_coherent(at(S.out, addr(k, n * kNB + istep[k])));
+        for (int k = 0; k < kItems; ++k)
+          fv[k] = need(k, n * kNB + istep[k]) ? load_coherent(at(S.out, addr(k, n * kNB + istep[k])))
+                                              : (V)0;
       };
       // batch n's values into the halos of the planes its steps read;
       // a value still the sentinel is read again (bounded wait)
