@@ -1292,10 +1292,10 @@ __global__ __launch_bounds__(SweepBlock<KIND>::value) void k_sweep(SweepArgs<V> 
 // each step's values through LDS without a workgroup barrier (the LDS
 // processes one wave's operations in order, so a read issued after a write
 // sees it), and the only barriers are those that hand over a batch of kNB
-// steps of records and face values from the loader and face waves.  The
-// hop between two units costs the unit's width in steps (8 instead of 16)
-// plus the face round trip, and there are twice as many hops; the step
-// itself is what the change buys.
+// steps of records and face values from the loader and face waves.  There
+// are twice as many hops between units as between tiles, each costing the
+// face read-ahead and a round trip; the shorter step is what the change
+// buys.
 //
 // A sub-tile (sa, sb) of tile (a, b) runs every step w of the tile, 0 ..
 // nsteps - 1, for its 64 pencils (li, lj) = (8 sa + ui, 8 sb + uj): the
@@ -1486,11 +1486,11 @@ __global__ __launch_bounds__(kSubThreads) void k_sweep_sub(SweepArgs<V> S) {
     // publishes the batches n = p (mod kFaceWaves), each read with one
     // read in flight on the wave's own counter, issued when the wave
     // publishes its previous batch, so a read has kFaceWaves batches to
-    // return.  Lane (s, m) = (lane / 8, lane % 8) holds step n kNB + s of
-    // the x face of row m (pencil (8 sa, 8 sb + m)) and of the y face of
-    // column m (pencil (8 sa + m, 8 sb)); upstream values come from the
-    // unit before in x (y): the other quarter of this tile, one step back
-    // and one pencil over, or the tile before.
+    // return.  A batch's face values are the x face of rows m (pencils
+    // (8 sa, 8 sb + m)) and the y face of columns m (pencils (8 sa + m,
+    // 8 sb)) at each of its steps; upstream values come from the unit
+    // before in x (y): the other quarter of this tile, one step back and
+    // one pencil over, or the tile before.
     const int par = wave == 1 ? 0 : wave - 2;
     // item k of a lane is value g = 64 k + lane of the batch: face g /
     // (kNB kS) (x, then y), step (g / kS) % kNB, row or column g % kS
@@ -1554,8 +1554,8 @@ __global__ __launch_bounds__(kSubThreads) void k_sweep_sub(SweepArgs<V> S) {
           nd[k] = need(k, n * kNB + istep[k]);
           bad = bad || (nd[k] && is_sentinel(fv[k]));
         }
-        // (the first check stays outside the retry loop, whose reloads
-        // would otherwise make it wait for the other reads in flight too)
+        // (the first check stays outside the retry loop: with the loop's
+        // reloads on its path the compiler waits for every read in flight)
         if (__any(bad)) {
           long long spins = 0;
           for (;;) {
