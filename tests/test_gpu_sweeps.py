@@ -48,7 +48,8 @@ def test_pipelined_matches_level_scheduled(oracle, wc, n, block_hi):
 
 
 @pytest.mark.parametrize("n,block_hi", [(16, (0.4, 0.6, 1.0)), (40, (0.9, 0.3, 0.55)),
-                                         (64, (0.4, 0.6, 1.0))])
+                                         (64, (0.4, 0.6, 1.0)),
+                                         (24, (1.0, 0.15, 0.2))])  # one tile row, few layers
 def test_box_apply_and_multiply_bitwise(oracle, wc, n, block_hi):
     """apply_preconditioner and multiply on the context's system, through the
     pipelined sweeps (one wave per 8x8 quarter tile, and one workgroup per
