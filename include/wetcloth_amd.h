@@ -312,6 +312,11 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
  *   quarters of the 16x16 tiles, one wave each; 0 selects one workgroup per
  *   tile.  Both give bitwise the same apply; the PCG's z . r is summed per
  *   quarter or per tile, so whole solves agree to rounding.
+ *   "spmv_window" (default 1): the box SpMV walks each tile in runs of
+ *   consecutive skewed steps with its neighbours in LDS; 0 selects one row
+ *   per thread.  Both give bitwise the same product; the PCG's s . A s is
+ *   summed per run or per block, so whole solves agree to rounding.
+ *   "spmv_run" (default 12, 1..4096): the steps of one run.
  *   "kernel_timing" (default 0): bracket the solver's sweep and SpMV launches
  *   with HIP events on the context's stream; n > 1 brackets those of every
  *   n-th PCG iteration only (each pair of events costs a little time on the

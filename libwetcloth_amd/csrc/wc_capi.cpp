@@ -1002,6 +1002,15 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
     c->pcg.sweep.sub = value != 0;
     return WC_OK;
   }
+  if (n == "spmv_window") {
+    c->pcg.sweep.spmv_win = value != 0;
+    return WC_OK;
+  }
+  if (n == "spmv_run") {
+    if (value < 1 || value > 4096) return fail(c, WC_ERR_ARG, "spmv_run must be in 1..4096");
+    c->pcg.sweep.spmv_run = (int)value;
+    return WC_OK;
+  }
   if (n == "mixed_digits") {
     if (value < 1 || value > 15) return fail(c, WC_ERR_ARG, "mixed_digits must be 1..15");
     c->pcg.mixed_digits = (int)value;

@@ -139,6 +139,12 @@ struct SweepPlan {
   // the tile sweeps (k_sweep); both give the same values bit for bit
   bool sub = true;
   DevBuf<uint32_t> order_sub;       // sub-tile ticket order
+  // The windowed SpMV (k_box_spmv_win: one block per tile and run of
+  // spmv_run steps) or the slot-parallel one; both give the same y bit for
+  // bit.  spart: the windowed product's x . y partials, one per block.
+  bool spmv_win = true;
+  int spmv_run = 12;
+  DevBuf<double> spart;
   // The forward sweep's ticket counter is zero: the sweeps zero each
   // other's counter, so only the first apply after the plan needs a clear.
   bool ticks_zero = false;
@@ -461,19 +467,24 @@ void sweep_factor(hipStream_t st, SweepPlan& P, double tau, double mdr);
 // that precedes an apply in the loop sets it), so the apply skips its fill
 void sweep_apply(hipStream_t st, SweepPlan& P, const double* r, KTimer* timer,
                  int64_t ndof, bool cleared = false);
-void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y);
+void box_spmv(hipStream_t st, SweepPlan& P, const double* x, double* y);
 // mixed-precision solve: FP32 records/entries from the FP64 factor and A,
 // FP32 sweeps (result in P.zb32), FP32 SpMV with FP64 row sums, and the FP64
 // refinement residual r = b - A x rounded to FP32 with |r|_inf partials
 void sweep_narrow(hipStream_t st, SweepPlan& P);
 void sweep_apply32(hipStream_t st, SweepPlan& P, const float* r, KTimer* timer,
                    int64_t ndof, bool cleared = false);
-// y = A x with the block partials of x . y (nblocks blocks, the solver's
-// reduction layout)
-void box_spmv_dot(hipStream_t st, const SweepPlan& P, const double* x, double* y,
-                  double* partial, int nblocks);
-void box_spmv32_dot(hipStream_t st, const SweepPlan& P, const float* x, float* y,
-                    double* partial, int nblocks);
+// y = A x and the partials of x . y: with the windowed product one per
+// block in P.spart, else nblocks of them in `partial` (the solver's
+// reduction layout).  Returns where they are and how many.
+struct DotParts {
+  const double* p;
+  int n;
+};
+DotParts box_spmv_dot(hipStream_t st, SweepPlan& P, const double* x, double* y,
+                      double* partial, int nblocks);
+DotParts box_spmv32_dot(hipStream_t st, SweepPlan& P, const float* x, float* y,
+                        double* partial, int nblocks);
 void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const double* x,
                   float* r, double* partial, int nblocks);
 void box_gather(hipStream_t st, const SweepPlan& P, const double* dof, double* box);

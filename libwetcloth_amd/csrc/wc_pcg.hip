@@ -798,10 +798,11 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   for (it = 0; it < maxit; ++it) {
     KTimer* ti = tm ? tm->sample() : nullptr;
     hipEvent_t e0 = ti ? ti->mark(st) : nullptr;
-    box_spmv_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
+    // z = A s and the partials of s . z
+    const DotParts dp = box_spmv_dot(st, P, s, z, w.partial.p, nb);
     if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
     hipLaunchKernelGGL(k_update_xr_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
-                       z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
+                       z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
                        P.progress.p + 2, w.host_flag_dev, P.zf.p);
     WC_LAUNCH_CHECK();
     mark_residual(st, ev);
@@ -895,10 +896,11 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     while (total < maxit) {
       KTimer* ti = tm ? tm->sample() : nullptr;
       hipEvent_t e0 = ti ? ti->mark(st) : nullptr;
-      box_spmv32_dot(st, P, s, z, w.partial.p, nb);  // z = A s and the partials of s . z
+      // z = A s and the partials of s . z
+      const DotParts dp = box_spmv32_dot(st, P, s, z, w.partial.p, nb);
       if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
       hipLaunchKernelGGL(k_update_xr_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, x, r,
-                         s, z, nc, w.partial.p, nb, w.scal.p, rho_slot, w.host_part_dev,
+                         s, z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
                          P.progress.p + 2, w.host_flag_dev, P.zf32.p);
       WC_LAUNCH_CHECK();
       mark_residual(st, w.res_event);

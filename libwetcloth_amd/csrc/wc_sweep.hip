@@ -571,6 +571,35 @@ __device__ inline double box_row(const Skew& K, uint32_t t, uint32_t c,
   return acc;
 }
 
+// A row's products in ascending column order: the lower terms in the row's
+// lower order, the diagonal, the upper terms in its upper order (FP64);
+// +0 for a slot that is not a DOF.  (Entries and values by axis, as scalars:
+// arrays picked by a computed axis end up in scratch memory.)
+struct RowTerms {
+  double al0, al1, al2, xl0, xl1, xl2;  // -x, -y, -z neighbours
+  double a0, x0;                        // diagonal
+  double au0, au1, au2, xu0, xu1, xu2;  // +x, +y, +z neighbours
+};
+__device__ inline double pick3(int ax, double v0, double v1, double v2) {
+  return ax == 0 ? v0 : (ax == 1 ? v1 : v2);
+}
+__device__ inline double row_sum(uint32_t f, const RowTerms& R) {
+  const int lor = (int)((f >> M_LORD) & 7), upr = (int)((f >> M_UORD) & 7);
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(lor, q);
+    acc += pick3(ax, R.al0, R.al1, R.al2) * pick3(ax, R.xl0, R.xl1, R.xl2);
+  }
+  acc += R.a0 * R.x0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ax = perm_axis(upr, q);
+    acc += pick3(ax, R.au0, R.au1, R.au2) * pick3(ax, R.xu0, R.xu1, R.xu2);
+  }
+  return (f & M_DOF) ? acc : 0.0;
+}
+
 // y = A x (FP64, or FP32 storage with FP64 row sums).  With DOT, also the
 // FP64 block partials of x . y, with the thread-to-slot mapping and the
 // reduction tree of the solver's dot-product kernel, so the sum is the one
@@ -616,24 +645,9 @@ __device__ inline double box_row_sym(const Skew& K, uint32_t t, uint32_t f,
   }
   const double a0 = (double)coef[self], x0 = (double)x[self];
   xself = dof ? x0 : 0.0;
-  const int lor = (int)((f >> M_LORD) & 7), upr = (int)((f >> M_UORD) & 7);
-  double acc = 0.0;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int ax = perm_axis(lor, q);
-    const double ca = ax == 0 ? alo[0] : (ax == 1 ? alo[1] : alo[2]);
-    const double cx = ax == 0 ? xlo[0] : (ax == 1 ? xlo[1] : xlo[2]);
-    acc += ca * cx;
-  }
-  acc += a0 * x0;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int ax = perm_axis(upr, q);
-    const double ca = ax == 0 ? aup[0] : (ax == 1 ? aup[1] : aup[2]);
-    const double cx = ax == 0 ? xup[0] : (ax == 1 ? xup[1] : xup[2]);
-    acc += ca * cx;
-  }
-  return dof ? acc : 0.0;
+  const RowTerms T{alo[0], alo[1], alo[2], xlo[0], xlo[1], xlo[2], a0, x0,
+                   aup[0], aup[1], aup[2], xup[0], xup[1], xup[2]};
+  return row_sum(f, T);
 }
 
 // kSpmvThreads per block (twice the reduction kernels' 256: the row's loads
@@ -659,6 +673,117 @@ __global__ __launch_bounds__(kSpmvThreads) void k_box_spmv_sym(uint32_t nslot, S
     __syncthreads();
     for (int s = kSpmvThreads / 2; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+  }
+}
+
+// The same product through a window: one 256-thread block per (tile, run
+// of consecutive steps), walking the run's steps.  Every neighbour of a slot
+// in the tile lies in the step before or after it (lane -1 / -16 / same, or
+// +1 / +16 / same), so the block keeps x of steps w - 1 .. w + 1 and the
+// three upper coefficient planes of step w - 1 (which hold the lower
+// couplings of step w) in LDS rings, and reads from global memory only its
+// own slots of each step and, on the tile's faces, the neighbours in the
+// next tile.  Each slot's values are read once per run instead of by up to
+// seven rows.  A step's loads are issued one step ahead, into registers.
+// The products are row_sum's, so y is k_box_spmv_sym's bit for bit; with
+// DOT, one partial of x . y per block (blocks in (tile, run) order).
+template <class V>
+struct WinRow {
+  uint16_t f;
+  V a0, up[3], xn;          // diagonal, upper planes; x of the next step
+  V cl[2], xl[2], xu[2];    // face neighbours in the next tile (x, y)
+};
+
+template <class V, bool DOT>
+__global__ __launch_bounds__(256) void k_box_spmv_win(uint32_t nslot, Skew K, int run, int nrun,
+                                                     const uint16_t* __restrict__ sflag,
+                                                     const V* __restrict__ coef,
+                                                     const V* __restrict__ x,
+                                                     V* __restrict__ y, double* partial) {
+  __shared__ V xs[4][kThreads];     // x of step w at [w & 3]
+  __shared__ V us[3][3][kThreads];  // upper planes of step w at [w % 3]
+  __shared__ double sh[kThreads];
+  const int lane = threadIdx.x, li = lane & (kT - 1), lj = lane / kT;
+  const int nsteps = K.nsteps;
+  const int tile = blockIdx.x / nrun;
+  const int w0 = (blockIdx.x % nrun) * run;
+  const int w1 = w0 + run < nsteps ? w0 + run : nsteps;
+  const uint32_t tb = (uint32_t)tile * (uint32_t)nsteps * kThreads + (uint32_t)lane;
+  const uint32_t dx = (uint32_t)(int32_t)K.dx, dy = (uint32_t)(int32_t)K.dy;
+  const uint32_t last = nslot - 1;
+  auto clampn = [&](uint32_t n) { return n < last ? n : last; };
+  auto ldx = [&](int w) -> V { return w >= 0 && w < nsteps ? x[tb + (uint32_t)w * kThreads] : (V)0; };
+  // the loads of step w (w0 <= w < w1); the face slots are clamped into the
+  // box and used only where the neighbour exists (then they are in it)
+  auto fetch = [&](int w, WinRow<V>& R) {
+    const uint32_t t = tb + (uint32_t)w * kThreads;
+    R.f = sflag[t];
+    R.a0 = coef[t];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) R.up[d] = coef[(uint32_t)(4 + d) * nslot + t];
+    R.xn = ldx(w + 1);
+    if (li == 0) {
+      const uint32_t n = clampn(t + dx);
+      R.cl[0] = coef[4u * nslot + n];
+      R.xl[0] = x[n];
+    }
+    if (lj == 0) {
+      const uint32_t n = clampn(t + dy);
+      R.cl[1] = coef[5u * nslot + n];
+      R.xl[1] = x[n];
+    }
+    if (li == kT - 1) R.xu[0] = x[clampn(t - dx)];
+    if (lj == kT - 1) R.xu[1] = x[clampn(t - dy)];
+  };
+  xs[(w0 + 3) & 3][lane] = ldx(w0 - 1);
+  xs[w0 & 3][lane] = ldx(w0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    us[(w0 + 2) % 3][d][lane] =
+        w0 > 0 ? coef[(uint32_t)(4 + d) * nslot + tb + (uint32_t)(w0 - 1) * kThreads] : (V)0;
+  WinRow<V> R;
+  R.cl[0] = R.cl[1] = R.xl[0] = R.xl[1] = R.xu[0] = R.xu[1] = (V)0;
+  fetch(w0, R);
+  double acc = 0.0;
+  for (int w = w0; w < w1; ++w) {
+    const WinRow<V> C = R;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) us[w % 3][d][lane] = C.up[d];
+    xs[(w + 1) & 3][lane] = C.xn;
+    if (w + 1 < w1) fetch(w + 1, R);
+    __syncthreads();  // step w + 1's x and step w's planes are in place
+    const uint32_t f = C.f;
+    const bool dof = (f & M_DOF) != 0;
+    const int pm = (w + 2) % 3, xm = (w + 3) & 3, xp = (w + 1) & 3;
+    const bool lox = dof && ((f >> M_LO) & 1u), loy = dof && ((f >> (M_LO + 1)) & 1u),
+               loz = dof && ((f >> (M_LO + 2)) & 1u);
+    RowTerms T;
+    T.al0 = lox ? (double)(li > 0 ? us[pm][0][lane - 1] : C.cl[0]) : 0.0;
+    T.xl0 = lox ? (double)(li > 0 ? xs[xm][lane - 1] : C.xl[0]) : 0.0;
+    T.al1 = loy ? (double)(lj > 0 ? us[pm][1][lane - kT] : C.cl[1]) : 0.0;
+    T.xl1 = loy ? (double)(lj > 0 ? xs[xm][lane - kT] : C.xl[1]) : 0.0;
+    T.al2 = loz ? (double)us[pm][2][lane] : 0.0;
+    T.xl2 = loz ? (double)xs[xm][lane] : 0.0;
+    T.a0 = (double)C.a0;
+    T.x0 = (double)xs[w & 3][lane];
+    T.au0 = (double)C.up[0];
+    T.au1 = (double)C.up[1];
+    T.au2 = (double)C.up[2];
+    T.xu0 = (double)(li < kT - 1 ? xs[xp][lane + 1] : C.xu[0]);
+    T.xu1 = (double)(lj < kT - 1 ? xs[xp][lane + kT] : C.xu[1]);
+    T.xu2 = (double)xs[xp][lane];
+    const V v = (V)row_sum(f, T);
+    if (DOT) acc += (dof ? T.x0 : 0.0) * (double)v;
+    y[tb + (uint32_t)w * kThreads] = v;
+  }
+  if (DOT) {
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s2 = kThreads / 2; s2 > 0; s2 >>= 1) {
+      if ((int)threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
       __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
@@ -1941,24 +2066,44 @@ static unsigned spmv_blocks(const SweepPlan& P) {
   return (unsigned)std::min<int64_t>(1024, blocks(P.ncell, kSpmvThreads));
 }
 
-void box_spmv(hipStream_t st, const SweepPlan& P, const double* x, double* y) {
+template <class V, bool DOT>
+static DotParts spmv_win(hipStream_t st, SweepPlan& P, const V* coef, const V* x, V* y) {
+  const int nrun = (P.nsteps + P.spmv_run - 1) / P.spmv_run;
+  const int nb = P.ntiles * nrun;
+  if (DOT) P.spart.ensure(nb);
+  hipLaunchKernelGGL((k_box_spmv_win<V, DOT>), dim3(nb), dim3(kThreads), 0, st,
+                     (uint32_t)P.ncell, skew_of(P), P.spmv_run, nrun, P.sflag.p, coef, x, y,
+                     DOT ? P.spart.p : nullptr);
+  WC_LAUNCH_CHECK();
+  return DotParts{P.spart.p, nb};
+}
+
+void box_spmv(hipStream_t st, SweepPlan& P, const double* x, double* y) {
+  if (P.spmv_win) {
+    spmv_win<double, false>(st, P, P.coef.p, x, y);
+    return;
+  }
   hipLaunchKernelGGL((k_box_spmv_sym<double, false>), dim3(spmv_blocks(P)), dim3(kSpmvThreads), 0, st,
                      (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef.p, x, y, nullptr);
   WC_LAUNCH_CHECK();
 }
 
-void box_spmv_dot(hipStream_t st, const SweepPlan& P, const double* x, double* y,
-                  double* partial, int nblocks) {
+DotParts box_spmv_dot(hipStream_t st, SweepPlan& P, const double* x, double* y,
+                      double* partial, int nblocks) {
+  if (P.spmv_win) return spmv_win<double, true>(st, P, P.coef.p, x, y);
   hipLaunchKernelGGL((k_box_spmv_sym<double, true>), dim3(nblocks), dim3(kSpmvThreads), 0, st,
                      (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef.p, x, y, partial);
   WC_LAUNCH_CHECK();
+  return DotParts{partial, nblocks};
 }
 
-void box_spmv32_dot(hipStream_t st, const SweepPlan& P, const float* x, float* y,
-                    double* partial, int nblocks) {
+DotParts box_spmv32_dot(hipStream_t st, SweepPlan& P, const float* x, float* y,
+                        double* partial, int nblocks) {
+  if (P.spmv_win) return spmv_win<float, true>(st, P, P.coef32.p, x, y);
   hipLaunchKernelGGL((k_box_spmv_sym<float, true>), dim3(nblocks), dim3(kSpmvThreads), 0, st,
                      (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef32.p, x, y, partial);
   WC_LAUNCH_CHECK();
+  return DotParts{partial, nblocks};
 }
 
 void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const double* x,
@@ -2018,6 +2163,7 @@ void SweepPlan::release() {
   bb.release();
   order.release();
   order_sub.release();
+  spart.release();
   progress.release();
   tdot.release();
   flags.release();

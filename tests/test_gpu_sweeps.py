@@ -1,7 +1,7 @@
 """The pipelined MIC(0) sweeps and the box-resident PCG of a grid system.
 
-The sweeps and the box SpMV apply the reference's operations in the
-reference's order, so on the context's own system they are bitwise equal to
+The sweeps and the box SpMV (windowed or one row per thread) apply the
+reference's operations in the reference's order, so on the context's own system they are bitwise equal to
 the oracle's apply_preconditioner and multiply.  The box PCG sums its dot
 products in box order rather than DOF order, so whole solves agree with the
 level-scheduled (DOF-ordered) solve to rounding: same convergence, iteration
@@ -65,7 +65,12 @@ def test_box_apply_and_multiply_bitwise(oracle, wc, n, block_hi):
     for sub in (1, 0, 1):
         s.set_option("sweep_subtiles", sub)
         assert np.array_equal(s.apply_preconditioner(r), want), sub
-    assert np.array_equal(s.multiply(r), oracle.spmv(rs, col, val, r))
+    y = oracle.spmv(rs, col, val, r)
+    # the windowed product (runs of 1, 7 and 64 steps) and the slot-parallel one
+    for win, run in ((1, 1), (1, 7), (0, 64), (1, 64)):
+        s.set_option("spmv_window", win)
+        s.set_option("spmv_run", run)
+        assert np.array_equal(s.multiply(r), y), (win, run)
     # and the level-scheduled kernels agree with both
     solve(s, False)
     assert np.array_equal(s.apply_preconditioner(r), oracle.apply_precond(rs, col, invdiag, lv, r))
@@ -97,6 +102,38 @@ def test_subtile_and_tile_solves_agree(oracle, wc, mixed, n, block_hi):
           f"pressure difference {np.abs(p1 - p0).max() / np.abs(p0).max():.2e}")
     assert abs(i1 - i0) <= 1
     assert np.abs(p1 - p0).max() <= (1e-6 if mixed else 1e-9) * np.abs(p0).max()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_windowed_and_slot_spmv_solves_agree(oracle, wc, mixed):
+    """Whole solves with either SpMV: the products are the same values, and
+    only the s . A s partials are summed over other blocks, so the
+    iterations agree to within one and the pressures to rounding."""
+    n = 40
+    g, x, v = oracle.dam_break(n, block_hi=(0.9, 0.3, 0.55), seed=5, sigma=0.5)
+    res = {}
+    for win in (1, 0):
+        s = wc.Scene((n, n, n), 1.0 / n, precision=wc.WC_MIXED if mixed else wc.WC_FP64)
+        s.set_option("spmv_window", win)
+        s.set_particles(x, v)
+        s.rebucketizeParticles()
+        s.mapParticleNodesAPIC()
+        s.stepVelocity(1.0 / 480)
+        ok, it, r = s.projectFine(1.0 / 480, 1e-8, 2000)
+        assert ok, win
+        res[win] = (it, s.system()[4])
+        s.close()
+    (i1, p1), (i0, p0) = res[1], res[0]
+    assert abs(i1 - i0) <= 1
+    assert np.abs(p1 - p0).max() <= (1e-6 if mixed else 1e-9) * np.abs(p0).max()
+
+
+def test_spmv_run_option_rejects_bad_values(wc):
+    s = wc.Scene((16, 16, 16), 1.0 / 16)
+    for bad in (0, -3, 5000):
+        with pytest.raises(Exception):
+            s.set_option("spmv_run", bad)
+    s.close()
 
 
 def test_pipelined_solve_matches_oracle(oracle, wc):
