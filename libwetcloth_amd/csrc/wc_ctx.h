@@ -145,6 +145,11 @@ struct SweepPlan {
   bool spmv_win = true;
   int spmv_run = 12;
   DevBuf<double> spart;
+  // its x faces as compact columns (ncell / 16 each): li = 15 and li = 0 of
+  // the operand (written by the s update, else extracted), and li = 15 of
+  // the -x coupling plane (once per plan)
+  DevBuf<double> cf15, xf15, xf0;
+  DevBuf<float> cf15_32, xf15_32, xf0_32;
   // The forward sweep's ticket counter is zero: the sweeps zero each
   // other's counter, so only the first apply after the plan needs a clear.
   bool ticks_zero = false;
@@ -481,10 +486,12 @@ struct DotParts {
   const double* p;
   int n;
 };
+// faces_ready: the windowed product's x-face columns of x (P.xf15 / xf0,
+// or the FP32 ones) are current, as after the s update that wrote them.
 DotParts box_spmv_dot(hipStream_t st, SweepPlan& P, const double* x, double* y,
-                      double* partial, int nblocks);
+                      double* partial, int nblocks, bool faces_ready = false);
 DotParts box_spmv32_dot(hipStream_t st, SweepPlan& P, const float* x, float* y,
-                        double* partial, int nblocks);
+                        double* partial, int nblocks, bool faces_ready = false);
 void box_residual(hipStream_t st, const SweepPlan& P, const double* b, const double* x,
                   float* r, double* partial, int nblocks);
 void box_gather(hipStream_t st, const SweepPlan& P, const double* dof, double* box);

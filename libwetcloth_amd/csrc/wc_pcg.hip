@@ -427,11 +427,13 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
 }
 
 // beta = (z . r) / rho_old (PCGSolver.h:301), s = z + s beta; the new rho is
-// written to the other rho slot, which no block reads here.
+// written to the other rho slot, which no block reads here.  With f15 / f0,
+// also the x-face columns of s that the windowed SpMV reads: s at the slots
+// with (t & 15) == 15 to f15[t >> 4], with (t & 15) == 0 to f0[t >> 4].
 template <class V>
 __global__ __launch_bounds__(kRedThreads) void k_update_s_fused(
     V* s, const V* z, int64_t n, const double* dpart, int nb, double* scal, int rho_old,
-    int rho_new) {
+    int rho_new, V* f15, V* f0) {
   __shared__ double sh[kRedThreads];
   double acc = 0;
   for (int i = threadIdx.x; i < nb; i += kRedThreads) acc += dpart[i];
@@ -448,10 +450,22 @@ __global__ __launch_bounds__(kRedThreads) void k_update_s_fused(
 #pragma unroll
     for (int k = 0; k < N; ++k) sv.v[k] = (V)((double)zv.v[k] + (double)sv.v[k] * beta);
     reinterpret_cast<PV*>(s)[i] = sv;
+    if (f15) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const int64_t t = i * N + k;
+        if ((t & 15) == 15) f15[t >> 4] = sv.v[k];
+        if ((t & 15) == 0) f0[t >> 4] = sv.v[k];
+      }
+    }
   }
   for (int64_t i = nv * N + blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kRedThreads)
-    s[i] = (V)((double)z[i] + (double)s[i] * beta);
+       i += (int64_t)gridDim.x * kRedThreads) {
+    const V v = (V)((double)z[i] + (double)s[i] * beta);
+    s[i] = v;
+    if (f15 && (i & 15) == 15) f15[i >> 4] = v;
+    if (f15 && (i & 15) == 0) f0[i >> 4] = v;
+  }
 }
 
 inline unsigned blocks_for(int64_t n, int t) {
@@ -799,7 +813,8 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     KTimer* ti = tm ? tm->sample() : nullptr;
     hipEvent_t e0 = ti ? ti->mark(st) : nullptr;
     // z = A s and the partials of s . z
-    const DotParts dp = box_spmv_dot(st, P, s, z, w.partial.p, nb);
+    // from the second iteration on, the s update has written s's face columns
+    const DotParts dp = box_spmv_dot(st, P, s, z, w.partial.p, nb, it > 0);
     if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
     hipLaunchKernelGGL(k_update_xr_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
                        z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
@@ -810,7 +825,8 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
       sweep_apply(st, P, r, ti, ndof, true);  // the update has set zf to the sentinel
       const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
       hipLaunchKernelGGL(k_update_s_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
-                         nc, P.tdot.p, P.ndot, w.scal.p, rho_slot, rho_next);
+                         nc, P.tdot.p, P.ndot, w.scal.p, rho_slot, rho_next,
+                         P.spmv_win ? P.xf15.p : nullptr, P.xf0.p);
       WC_LAUNCH_CHECK();
       rho_slot = rho_next;
     }
@@ -893,11 +909,13 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     // the residual (as in pcg_loop_box: it touches only s and the sweep
     // outputs, which the next refinement pass recomputes)
     int rho_slot = S_RHO;
+    bool faces = false;  // s's face columns, written by the s update
     while (total < maxit) {
       KTimer* ti = tm ? tm->sample() : nullptr;
       hipEvent_t e0 = ti ? ti->mark(st) : nullptr;
       // z = A s and the partials of s . z
-      const DotParts dp = box_spmv32_dot(st, P, s, z, w.partial.p, nb);
+      const DotParts dp = box_spmv32_dot(st, P, s, z, w.partial.p, nb, faces);
+      faces = true;
       if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
       hipLaunchKernelGGL(k_update_xr_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, x, r,
                          s, z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
@@ -909,7 +927,8 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
         sweep_apply32(st, P, r, ti, ndof, true);  // the update has set zf32 to the sentinel
         const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
         hipLaunchKernelGGL(k_update_s_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, s, z,
-                           nc, P.tdot.p, P.ndot, w.scal.p, rho_slot, rho_next);
+                           nc, P.tdot.p, P.ndot, w.scal.p, rho_slot, rho_next,
+                           P.spmv_win ? P.xf15_32.p : nullptr, P.xf0_32.p);
         WC_LAUNCH_CHECK();
         rho_slot = rho_next;
       }

@@ -690,6 +690,31 @@ __global__ __launch_bounds__(kSpmvThreads) void k_box_spmv_sym(uint32_t nslot, S
 // seven rows.  A step's loads are issued one step ahead, into registers.
 // The products are row_sum's, so y is k_box_spmv_sym's bit for bit; with
 // DOT, one partial of x . y per block (blocks in (tile, run) order).
+// A tile's x faces as compact columns: f15[t >> 4] = x[t] at the slots
+// with li = 15 and f0[t >> 4] = x[t] at li = 0, a step's 16 values of each
+// column side by side.  The -x neighbour of lane (li = 0, lj) at step w is
+// slot (tile - 1, w + 15, li = 15, lj), so its column entry is (t + dx) >> 4
+// (the clamp keeps li = 15); the +x neighbour of (li = 15, lj) is
+// (tile + 1, w - 15, li = 0, lj), entry (t - dx) >> 4.  A column is spread
+// over every cache line of its step; the compact copy is one line.
+template <class V>
+__global__ void k_face_cols(int64_t n16, const V* __restrict__ x, V* __restrict__ f15,
+                            V* __restrict__ f0) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    f15[i] = x[i * 16 + 15];
+    if (f0) f0[i] = x[i * 16];
+  }
+}
+
+template <class V>
+static void face_cols(hipStream_t st, int64_t ncell, const V* x, V* f15, V* f0) {
+  const int64_t n16 = ncell / 16;
+  const int64_t nb = std::min<int64_t>(4096, (n16 + 255) / 256);
+  hipLaunchKernelGGL(k_face_cols<V>, dim3((unsigned)nb), dim3(256), 0, st, n16, x, f15, f0);
+  WC_LAUNCH_CHECK();
+}
+
 template <class V>
 struct WinRow {
   uint16_t f;
@@ -702,6 +727,9 @@ __global__ __launch_bounds__(256) void k_box_spmv_win(uint32_t nslot, Skew K, in
                                                      const uint16_t* __restrict__ sflag,
                                                      const V* __restrict__ coef,
                                                      const V* __restrict__ x,
+                                                     const V* __restrict__ f15,
+                                                     const V* __restrict__ f0,
+                                                     const V* __restrict__ c15,
                                                      V* __restrict__ y, double* partial) {
   __shared__ V xs[4][kThreads];     // x of step w at [w & 3]
   __shared__ V us[3][3][kThreads];  // upper planes of step w at [w % 3]
@@ -717,7 +745,8 @@ __global__ __launch_bounds__(256) void k_box_spmv_win(uint32_t nslot, Skew K, in
   auto clampn = [&](uint32_t n) { return n < last ? n : last; };
   auto ldx = [&](int w) -> V { return w >= 0 && w < nsteps ? x[tb + (uint32_t)w * kThreads] : (V)0; };
   // the loads of step w (w0 <= w < w1); the face slots are clamped into the
-  // box and used only where the neighbour exists (then they are in it)
+  // box and used only where the neighbour exists (then they are in it); the
+  // x faces come from the compact columns of x and of the -x coupling plane
   auto fetch = [&](int w, WinRow<V>& R) {
     const uint32_t t = tb + (uint32_t)w * kThreads;
     R.f = sflag[t];
@@ -726,16 +755,16 @@ __global__ __launch_bounds__(256) void k_box_spmv_win(uint32_t nslot, Skew K, in
     for (int d = 0; d < 3; ++d) R.up[d] = coef[(uint32_t)(4 + d) * nslot + t];
     R.xn = ldx(w + 1);
     if (li == 0) {
-      const uint32_t n = clampn(t + dx);
-      R.cl[0] = coef[4u * nslot + n];
-      R.xl[0] = x[n];
+      const uint32_t n = clampn(t + dx) >> 4;
+      R.cl[0] = c15[n];
+      R.xl[0] = f15[n];
     }
     if (lj == 0) {
       const uint32_t n = clampn(t + dy);
       R.cl[1] = coef[5u * nslot + n];
       R.xl[1] = x[n];
     }
-    if (li == kT - 1) R.xu[0] = x[clampn(t - dx)];
+    if (li == kT - 1) R.xu[0] = f0[clampn(t - dx) >> 4];
     if (lj == kT - 1) R.xu[1] = x[clampn(t - dy)];
   };
   xs[(w0 + 3) & 3][lane] = ldx(w0 - 1);
@@ -1930,6 +1959,12 @@ void sweep_prepare(hipStream_t st, SweepPlan& P, const Grid& g, const uint8_t* a
   hipLaunchKernelGGL(k_fill_box, dim3(blocks(P.ncell, 256)), dim3(256), 0, st, A,
                      (const RowSum*)rs);
   WC_LAUNCH_CHECK();
+  // the windowed SpMV's x-face columns: of the -x coupling plane once, of
+  // its operand with every operand
+  P.cf15.ensure(P.ncell / 16);
+  P.xf15.ensure(P.ncell / 16);
+  P.xf0.ensure(P.ncell / 16);
+  face_cols<double>(st, P.ncell, P.coef.p + 4 * P.ncell, P.cf15.p, nullptr);
   WC_HIP(hipStreamSynchronize(st));  // 'order' and 'order_sub' must outlive their copies
   P.valid = true;
 }
@@ -1982,6 +2017,10 @@ void sweep_narrow(hipStream_t st, SweepPlan& P) {
   hipLaunchKernelGGL(k_narrow_box, dim3(blocks(P.ncell * kCoef, 256)), dim3(256), 0, st,
                      P.coef.p, P.coef32.p, P.ncell * kCoef);
   WC_LAUNCH_CHECK();
+  P.cf15_32.ensure(P.ncell / 16);
+  P.xf15_32.ensure(P.ncell / 16);
+  P.xf0_32.ensure(P.ncell / 16);
+  face_cols<float>(st, P.ncell, P.coef32.p + 4 * P.ncell, P.cf15_32.p, nullptr);
 }
 
 // all-ones (the sentinel) over n 16-byte words
@@ -2066,21 +2105,25 @@ static unsigned spmv_blocks(const SweepPlan& P) {
   return (unsigned)std::min<int64_t>(1024, blocks(P.ncell, kSpmvThreads));
 }
 
+// faces_ready: the columns of x are already in f15 / f0 (the s update
+// writes them); else they are extracted first
 template <class V, bool DOT>
-static DotParts spmv_win(hipStream_t st, SweepPlan& P, const V* coef, const V* x, V* y) {
+static DotParts spmv_win(hipStream_t st, SweepPlan& P, const V* coef, const V* c15, V* f15,
+                         V* f0, const V* x, V* y, bool faces_ready) {
   const int nrun = (P.nsteps + P.spmv_run - 1) / P.spmv_run;
   const int nb = P.ntiles * nrun;
   if (DOT) P.spart.ensure(nb);
+  if (!faces_ready) face_cols<V>(st, P.ncell, x, f15, f0);
   hipLaunchKernelGGL((k_box_spmv_win<V, DOT>), dim3(nb), dim3(kThreads), 0, st,
-                     (uint32_t)P.ncell, skew_of(P), P.spmv_run, nrun, P.sflag.p, coef, x, y,
-                     DOT ? P.spart.p : nullptr);
+                     (uint32_t)P.ncell, skew_of(P), P.spmv_run, nrun, P.sflag.p, coef, x, f15,
+                     f0, c15, y, DOT ? P.spart.p : nullptr);
   WC_LAUNCH_CHECK();
   return DotParts{P.spart.p, nb};
 }
 
 void box_spmv(hipStream_t st, SweepPlan& P, const double* x, double* y) {
   if (P.spmv_win) {
-    spmv_win<double, false>(st, P, P.coef.p, x, y);
+    spmv_win<double, false>(st, P, P.coef.p, P.cf15.p, P.xf15.p, P.xf0.p, x, y, false);
     return;
   }
   hipLaunchKernelGGL((k_box_spmv_sym<double, false>), dim3(spmv_blocks(P)), dim3(kSpmvThreads), 0, st,
@@ -2089,8 +2132,10 @@ void box_spmv(hipStream_t st, SweepPlan& P, const double* x, double* y) {
 }
 
 DotParts box_spmv_dot(hipStream_t st, SweepPlan& P, const double* x, double* y,
-                      double* partial, int nblocks) {
-  if (P.spmv_win) return spmv_win<double, true>(st, P, P.coef.p, x, y);
+                      double* partial, int nblocks, bool faces_ready) {
+  if (P.spmv_win)
+    return spmv_win<double, true>(st, P, P.coef.p, P.cf15.p, P.xf15.p, P.xf0.p, x, y,
+                                  faces_ready);
   hipLaunchKernelGGL((k_box_spmv_sym<double, true>), dim3(nblocks), dim3(kSpmvThreads), 0, st,
                      (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef.p, x, y, partial);
   WC_LAUNCH_CHECK();
@@ -2098,8 +2143,10 @@ DotParts box_spmv_dot(hipStream_t st, SweepPlan& P, const double* x, double* y,
 }
 
 DotParts box_spmv32_dot(hipStream_t st, SweepPlan& P, const float* x, float* y,
-                        double* partial, int nblocks) {
-  if (P.spmv_win) return spmv_win<float, true>(st, P, P.coef32.p, x, y);
+                        double* partial, int nblocks, bool faces_ready) {
+  if (P.spmv_win)
+    return spmv_win<float, true>(st, P, P.coef32.p, P.cf15_32.p, P.xf15_32.p, P.xf0_32.p, x, y,
+                                 faces_ready);
   hipLaunchKernelGGL((k_box_spmv_sym<float, true>), dim3(nblocks), dim3(kSpmvThreads), 0, st,
                      (uint32_t)P.ncell, skew_of(P), P.sflag.p, P.coef32.p, x, y, partial);
   WC_LAUNCH_CHECK();
@@ -2164,6 +2211,12 @@ void SweepPlan::release() {
   order.release();
   order_sub.release();
   spart.release();
+  cf15.release();
+  xf15.release();
+  xf0.release();
+  cf15_32.release();
+  xf15_32.release();
+  xf0_32.release();
   progress.release();
   tdot.release();
   flags.release();

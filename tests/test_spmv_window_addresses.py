@@ -11,7 +11,8 @@ many box shapes and run lengths:
   - for every cell of the box and each of its six neighbours inside the box,
     the ring entry or face slot the term reads is that neighbour's slot;
   - the ring indices of the values a step reads are never the ones the next
-    step overwrites before its barrier (x ring of 4, planes ring of 3).
+    step overwrites before its barrier (x ring of 4, planes ring of 3);
+  - the compact x-face columns it reads hold the face slots' values.
 """
 import numpy as np
 import pytest
@@ -128,3 +129,31 @@ def test_ring_entries_live_until_read():
         assert ((w + 1) % 3) not in reads_u
         # the planes step w reads are the ones step w - 1 wrote
         assert (w + 2) % 3 == (w - 1) % 3
+
+
+@pytest.mark.parametrize("B0,B1,Bz", SHAPES)
+def test_face_columns_hold_the_face_slots(B0, B1, Bz):
+    """The x faces come from compact columns: f15[i] = x[16 i + 15] and
+    f0[i] = x[16 i].  A lane with li = 0 reads f15[clampn(t + dx) >> 4]: the
+    clamped slot has li = 15, so that entry is exactly x at the slot the
+    row needs.  A lane with li = 15 reads f0[clampn(t - dx) >> 4]: where
+    the slot is inside the box it has li = 0, so the entry is x there (a
+    clamped slot belongs to no neighbour, and its entry is still a value of
+    x).  Every index is inside the columns."""
+    P = plan(B0, B1, Bz)
+    n16 = P["ncell"] // 16
+    t = np.arange(P["ncell"], dtype=np.int64)
+    li = (t % THREADS) % KT
+    lo = clampn(P, t[li == 0] + P["kdx"]).astype(np.int64)
+    assert np.all(lo % 16 == 15)
+    assert np.array_equal((lo >> 4) * 16 + 15, lo)
+    assert (lo >> 4).max() < n16
+    raw = t[li == KT - 1] - P["kdx"]
+    up = clampn(P, raw).astype(np.int64)
+    inside = (raw >= 0) & (raw < P["ncell"])
+    assert np.all(up[inside] % 16 == 0)
+    assert np.array_equal((up[inside] >> 4) * 16, up[inside])
+    assert (up >> 4).max() < n16
+    # the s update writes f15 / f0 at exactly the slots with t & 15 == 15 / 0
+    assert np.array_equal(np.unique(t[(t & 15) == 15] >> 4), np.arange(n16))
+    assert np.array_equal(np.unique(t[(t & 15) == 0] >> 4), np.arange(n16))
