@@ -198,7 +198,6 @@ struct PcgWork {
   double* host_part = nullptr;  // mapped host memory, red_blocks |r| partials
   double* host_part_dev = nullptr;  // its device address
   int* host_flag_dev = nullptr;     // device address of host_flag (mapped)
-  hipEvent_t res_event = nullptr;  // marks the update kernel of the box loops
   void release();
 };
 

@@ -13,6 +13,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <chrono>
 
 #include "wc_ctx.h"
 
@@ -421,8 +422,16 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
   }
   const double mm = block_max(m, sh);
   if (threadIdx.x == 0) {
-    rpart[blockIdx.x] = mm;
-    if (blockIdx.x == 0) *flag_out = *flag;
+    // the host polls rpart, whose entries it set to -1: each block stores
+    // its |r| max (>= 0), block 0 stores -2 instead when the sweeps' abort
+    // flag is set (one store, so nothing has to be ordered before it)
+    double v = mm;
+    if (blockIdx.x == 0) {
+      const int fl = *flag;
+      *flag_out = fl;
+      if (fl) v = -2.0;
+    }
+    rpart[blockIdx.x] = v;
   }
 }
 
@@ -559,8 +568,6 @@ void PcgWork::release() {
   if (host_part) (void)hipHostFree(host_part);
   host_part = nullptr;
   host_part_dev = nullptr;
-  if (res_event) (void)hipEventDestroy(res_event);
-  res_event = nullptr;
 }
 
 static size_t slot_count(const DevCsr& A) { return (size_t)A.nnz_slots; }
@@ -598,7 +605,6 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
                          hipHostMallocMapped | hipHostMallocCoherent));
     WC_HIP(hipHostGetDevicePointer((void**)&w.host_part_dev, w.host_part, 0));
   }
-  if (!w.res_event) WC_HIP(hipEventCreateWithFlags(&w.res_event, hipEventDisableTiming));
   if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
   if (!w.host_flag) {
     WC_HIP(hipHostMalloc((void**)&w.host_flag, sizeof(int),
@@ -753,18 +759,49 @@ static double read_scal(hipStream_t st, PcgWork& w, int slot) {
 // the update kernel wrote into mapped host memory with the sweeps' abort
 // flag, once the event recorded after it has completed (work enqueued after
 // the event may still be running).
-static double box_residual_max(PcgWork& w, const SweepPlan& P, hipEvent_t ev) {
-  WC_HIP(hipEventSynchronize(ev));
-  if (*w.host_flag)
+// The update kernel's |r|_inf partials arrive in mapped host memory, each
+// block's over the -1 the host left there: a max (>= 0), or -2 from block 0
+// when the sweeps' abort flag is set.  Polling them instead of waiting on
+// an event keeps the event's marker packet, ~5.6 us of idle GPU per
+// iteration, out of the stream.  After 10 ms (an iteration takes well
+// under one) the stream is queried as well, which enqueues a marker of its
+// own: a stream that failed, or finished without the partials, ends the
+// wait with an error.
+static void clear_residual(PcgWork& w) {
+  for (int i = 0; i < w.red_blocks; ++i) w.host_part[i] = -1.0;
+}
+
+static double box_residual_max(hipStream_t st, PcgWork& w) {
+  volatile const double* hp = w.host_part;
+  const int nb = w.red_blocks;
+  int i = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (long spin = 1;; ++spin) {
+    while (i < nb && hp[i] != -1.0) ++i;
+    if (i == nb) break;
+    if ((spin & 4095) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(10)) {
+      const hipError_t e = hipStreamQuery(st);
+      if (e == hipSuccess) {  // idle: every partial is written by now
+        while (i < nb && hp[i] != -1.0) ++i;
+        if (i == nb) break;
+        throw HipError{hipErrorUnknown, "PCG update: residual partials missing", __FILE__,
+                       __LINE__};
+      }
+      if (e != hipErrorNotReady) WC_HIP(e);
+    }
+    __builtin_ia32_pause();
+  }
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (hp[0] == -2.0)
     throw HipError{hipErrorLaunchTimeOut,
                    "pipelined MIC(0) sweep: a tile wait exceeded its bound", __FILE__,
                    __LINE__};
   double m = 0;
-  for (int i = 0; i < w.red_blocks; ++i) m = fmax(m, w.host_part[i]);
+  for (int k = 0; k < nb; ++k) m = fmax(m, hp[k]);
+  clear_residual(w);  // for the next update, which the host enqueues after this
   return m;
 }
-
-static void mark_residual(hipStream_t st, hipEvent_t ev) { WC_HIP(hipEventRecord(ev, st)); }
 
 // Harvest the kernel timer's completed launches without waiting (the rest
 // are harvested later, at the latest when the statistics are read), so the
@@ -805,7 +842,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     return false;
   }
   WC_HIP(hipMemcpyAsync(s, z, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
-  hipEvent_t ev = w.res_event;
+  clear_residual(w);
   int rho_slot = S_RHO;
   int it;
   bool ok = false;
@@ -820,7 +857,6 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
                        z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
                        P.progress.p + 2, w.host_flag_dev, P.zf.p);
     WC_LAUNCH_CHECK();
-    mark_residual(st, ev);
     if (it + 1 < maxit) {
       sweep_apply(st, P, r, ti, ndof, true);  // the update has set zf to the sentinel
       const int rho_next = rho_slot == S_RHO ? S_RHO2 : S_RHO;
@@ -830,7 +866,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
       WC_LAUNCH_CHECK();
       rho_slot = rho_next;
     }
-    const double res = box_residual_max(w, P, ev);
+    const double res = box_residual_max(st, w);
     *residual_out = res;
     if (res <= tolr) {
       ++it;
@@ -908,6 +944,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
     // inner PCG iterations, with the next apply enqueued before the host reads
     // the residual (as in pcg_loop_box: it touches only s and the sweep
     // outputs, which the next refinement pass recomputes)
+    clear_residual(w);
     int rho_slot = S_RHO;
     bool faces = false;  // s's face columns, written by the s update
     while (total < maxit) {
@@ -921,7 +958,6 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
                          s, z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
                          P.progress.p + 2, w.host_flag_dev, P.zf32.p);
       WC_LAUNCH_CHECK();
-      mark_residual(st, w.res_event);
       ++total;
       if (total < maxit) {
         sweep_apply32(st, P, r, ti, ndof, true);  // the update has set zf32 to the sentinel
@@ -932,7 +968,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
         WC_LAUNCH_CHECK();
         rho_slot = rho_next;
       }
-      const double r32 = box_residual_max(w, P, w.res_event);
+      const double r32 = box_residual_max(st, w);
       if (r32 <= inner_tol || r32 != r32) break;
     }
   }
