@@ -34,7 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "substeps/s at 256³ grid, 40M particles: mixed-prec vs FP64 GPU at 1e-6 rel res"
-TIMING_STRIDE = 8
+TIMING_STRIDE = 32
 
 
 def parse():
@@ -198,8 +198,9 @@ def run(W, precision, n, dt, a, solver=0):
     for _ in range(a.warmup):
         ok, it, res = scene.step(dt, a.tol, a.maxit)
     phase_tot = dict.fromkeys(W.PHASES, 0.0)
-    # HIP events around the sweep and SpMV launches of every 8th PCG iteration
-    # (timing all of them costs ~3 % of the substep); also clears the counters
+    # HIP events around the sweep and SpMV launches of every 32nd PCG
+    # iteration (each event is a marker packet that idles the GPU ~5 us;
+    # timing every launch costs ~3 % of the substep); also clears the counters
     scene.set_option("kernel_timing", TIMING_STRIDE)
     t0 = time.perf_counter()
     for _ in range(a.steps):
