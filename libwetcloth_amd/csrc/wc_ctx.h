@@ -194,10 +194,8 @@ struct PcgWork {
   double* host_scal = nullptr;  // pinned, or host_scal_own when no_alloc
   bool no_alloc = false;        // every buffer lent (DevBuf::lend); no pinned memory either
   double host_scal_own[8] = {};
-  int* host_flag = nullptr;     // mapped host memory: the sweeps' abort flag
   double* host_part = nullptr;  // mapped host memory, red_blocks |r| partials
   double* host_part_dev = nullptr;  // its device address
-  int* host_flag_dev = nullptr;     // device address of host_flag (mapped)
   void release();
 };
 

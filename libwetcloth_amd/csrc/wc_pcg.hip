@@ -351,10 +351,10 @@ __global__ void k_update_s(V* s, const V* z, int64_t n, const double* scal) {
 // value k_dot_final would (rho_old / s . z, and z . r / rho_old).
 
 // x += s alpha, r -= z alpha (FP64 r and z, or FP32 formed in FP64), |r|_inf
-// block partials into rpart and the sweeps' abort flag into flag_out (both
-// in mapped host memory, which the host reads once the kernel is complete);
-// and the next forward sweep's output set to the all-ones sentinel (reset,
-// the same slots), which saves the sweep's own fill launch.
+// block partials into rpart (mapped host memory, which the host polls; the
+// sweeps' abort flag rides in block 0's entry); and the next forward
+// sweep's output set to the all-ones sentinel (reset, the same slots),
+// which saves the sweep's own fill launch.
 // N values of T, aligned to their size (N defaults to 16 bytes' worth):
 // one vector access per lane instead of N scalar ones
 template <class T, int NN = 16 / (int)sizeof(T)>
@@ -371,8 +371,7 @@ __device__ inline void set_all_ones(float* p) { *reinterpret_cast<unsigned int*>
 template <class V>
 __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
     double* x, V* r, const V* s, const V* z, int64_t n, const double* dpart, int nb,
-    const double* scal, int rho_slot, double* rpart, const int* flag, int* flag_out,
-    V* reset) {
+    const double* scal, int rho_slot, double* rpart, const int* flag, V* reset) {
   __shared__ double sh[kRedThreads];
   double acc = 0;
   for (int i = threadIdx.x; i < nb; i += kRedThreads) acc += dpart[i];
@@ -425,13 +424,7 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr_fused(
     // the host polls rpart, whose entries it set to -1: each block stores
     // its |r| max (>= 0), block 0 stores -2 instead when the sweeps' abort
     // flag is set (one store, so nothing has to be ordered before it)
-    double v = mm;
-    if (blockIdx.x == 0) {
-      const int fl = *flag;
-      *flag_out = fl;
-      if (fl) v = -2.0;
-    }
-    rpart[blockIdx.x] = v;
+    rpart[blockIdx.x] = blockIdx.x == 0 && *flag ? -2.0 : mm;
   }
 }
 
@@ -562,9 +555,6 @@ void PcgWork::release() {
   lvl_list.release();
   if (host_scal && host_scal != host_scal_own) (void)hipHostFree(host_scal);
   host_scal = nullptr;
-  if (host_flag) (void)hipHostFree(host_flag);
-  host_flag = nullptr;
-  host_flag_dev = nullptr;
   if (host_part) (void)hipHostFree(host_part);
   host_part = nullptr;
   host_part_dev = nullptr;
@@ -606,11 +596,6 @@ void pcg_prepare(hipStream_t st, PcgWork& w, const DevCsr& A,
     WC_HIP(hipHostGetDevicePointer((void**)&w.host_part_dev, w.host_part, 0));
   }
   if (!w.host_scal) WC_HIP(hipHostMalloc((void**)&w.host_scal, sizeof(double) * S_NSCAL));
-  if (!w.host_flag) {
-    WC_HIP(hipHostMalloc((void**)&w.host_flag, sizeof(int),
-                         hipHostMallocMapped | hipHostMallocCoherent));
-    WC_HIP(hipHostGetDevicePointer((void**)&w.host_flag_dev, w.host_flag, 0));
-  }
 }
 
 void pcg_prepare_vectors(hipStream_t st, PcgWork& w, int64_t n) {
@@ -855,7 +840,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
     if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
     hipLaunchKernelGGL(k_update_xr_fused<double>, dim3(nb), dim3(kRedThreads), 0, st, x, r, s,
                        z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
-                       P.progress.p + 2, w.host_flag_dev, P.zf.p);
+                       P.progress.p + 2, P.zf.p);
     WC_LAUNCH_CHECK();
     if (it + 1 < maxit) {
       sweep_apply(st, P, r, ti, ndof, true);  // the update has set zf to the sentinel
@@ -956,7 +941,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
       if (e0) ti->add(KTimer::SPMV, ndof, e0, ti->mark(st));
       hipLaunchKernelGGL(k_update_xr_fused<float>, dim3(nb), dim3(kRedThreads), 0, st, x, r,
                          s, z, nc, dp.p, dp.n, w.scal.p, rho_slot, w.host_part_dev,
-                         P.progress.p + 2, w.host_flag_dev, P.zf32.p);
+                         P.progress.p + 2, P.zf32.p);
       WC_LAUNCH_CHECK();
       ++total;
       if (total < maxit) {
