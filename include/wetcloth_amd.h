@@ -317,6 +317,10 @@ wc_status wc_phase_times(const wc_ctx* ctx, double ms[7]);
  *   per thread.  Both give bitwise the same product; the PCG's s . A s is
  *   summed per run or per block, so whole solves agree to rounding.
  *   "spmv_run" (default 12, 1..4096): the steps of one run.
+ *   "debug_abort_sweeps" (default 0; a test hook): the pipelined sweeps'
+ *   abort flag is set before each PCG loop, as a sweep whose bounded wait
+ *   ran out sets it; the solve then fails with WC_ERR_HIP and a message
+ *   naming the sweep wait, and the next solve without it is unaffected.
  *   "kernel_timing" (default 0): bracket the solver's sweep and SpMV launches
  *   with HIP events on the context's stream; n > 1 brackets those of every
  *   n-th PCG iteration only (each pair of events costs a little time on the

@@ -1002,6 +1002,10 @@ wc_status wc_set_option(wc_ctx* c, const char* name, int64_t value) {
     c->pcg.sweep.sub = value != 0;
     return WC_OK;
   }
+  if (n == "debug_abort_sweeps") {
+    c->pcg.sweep.debug_abort = value != 0;
+    return WC_OK;
+  }
   if (n == "spmv_window") {
     c->pcg.sweep.spmv_win = value != 0;
     return WC_OK;

@@ -142,6 +142,7 @@ struct SweepPlan {
   // The windowed SpMV (k_box_spmv_win: one block per tile and run of
   // spmv_run steps) or the slot-parallel one; both give the same y bit for
   // bit.  spart: the windowed product's x . y partials, one per block.
+  bool debug_abort = false;  // test hook: the sweeps' abort flag set before each PCG loop
   bool spmv_win = true;
   int spmv_run = 12;
   DevBuf<double> spart;

@@ -752,6 +752,13 @@ static double read_scal(hipStream_t st, PcgWork& w, int slot) {
 // under one) the stream is queried as well, which enqueues a marker of its
 // own: a stream that failed, or finished without the partials, ends the
 // wait with an error.
+// Test hook ("debug_abort_sweeps"): the sweeps' abort flag set before the
+// loop, as a sweep whose bounded wait ran out would set it, so the solve
+// takes its error path (every kernel still runs to its end).
+static void force_abort(hipStream_t st, SweepPlan& P) {
+  if (P.debug_abort) WC_HIP(hipMemsetAsync(P.progress.p + 2, 1, 1, st));
+}
+
 static void clear_residual(PcgWork& w) {
   for (int i = 0; i < w.red_blocks; ++i) w.host_part[i] = -1.0;
 }
@@ -815,6 +822,7 @@ static bool pcg_loop_box(hipStream_t st, PcgWork& w, int64_t ndof, const double*
   double* x = P.xb.p;
   double* z = P.zb.p;  // the preconditioned vector, then A s
   KTimer* tm = w.timer.on ? &w.timer : nullptr;
+  force_abort(st, P);
   box_gather(st, P, b, r);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
   sweep_apply(st, P, r, tm ? tm->sample() : nullptr, ndof);  // z and the z . r tile partials
@@ -887,6 +895,7 @@ static bool pcg_loop_box_mixed(hipStream_t st, PcgWork& w, int64_t ndof, const d
   float* s = P.sb32.p;
   float* z = P.zb32.p;  // the preconditioned vector, then A s
   KTimer* tm = w.timer.on ? &w.timer : nullptr;
+  force_abort(st, P);
   box_gather(st, P, b, P.bb.p);
   WC_HIP(hipMemsetAsync(x, 0, sizeof(double) * nc, st));
   double eta = 1.0;

@@ -128,6 +128,29 @@ def test_windowed_and_slot_spmv_solves_agree(oracle, wc, mixed):
     assert np.abs(p1 - p0).max() <= (1e-6 if mixed else 1e-9) * np.abs(p0).max()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_sweep_abort_reaches_the_host_and_the_next_solve_recovers(oracle, wc, mixed):
+    """The sweeps' abort flag (set by a sweep whose bounded wait ran out;
+    here by the "debug_abort_sweeps" hook before the loop) reaches the host
+    through the update kernel's first residual partial: the solve fails with
+    the sweep-wait error instead of waiting, and the next solve without the
+    hook converges as usual."""
+    n = 24
+    g, x, v = oracle.dam_break(n, seed=4, sigma=0.4)
+    s = wc.Scene((n, n, n), 1.0 / n, precision=wc.WC_MIXED if mixed else wc.WC_FP64)
+    s.set_particles(x, v)
+    s.rebucketizeParticles()
+    s.mapParticleNodesAPIC()
+    s.stepVelocity(1.0 / 480)
+    s.set_option("debug_abort_sweeps", 1)
+    with pytest.raises(wc.WetclothError, match="tile wait"):
+        s.projectFine(1.0 / 480, 1e-8, 2000)
+    s.set_option("debug_abort_sweeps", 0)
+    ok, it, res = s.projectFine(1.0 / 480, 1e-8, 2000)
+    assert ok and it > 0
+    s.close()
+
+
 def test_spmv_run_option_rejects_bad_values(wc):
     s = wc.Scene((16, 16, 16), 1.0 / 16)
     for bad in (0, -3, 5000):
