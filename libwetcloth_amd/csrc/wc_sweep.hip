@@ -679,17 +679,6 @@ __global__ __launch_bounds__(kSpmvThreads) void k_box_spmv_sym(uint32_t nslot, S
   }
 }
 
-// The same product through a window: one 256-thread block per (tile, run
-// of consecutive steps), walking the run's steps.  Every neighbour of a slot
-// in the tile lies in the step before or after it (lane -1 / -16 / same, or
-// +1 / +16 / same), so the block keeps x of steps w - 1 .. w + 1 and the
-// three upper coefficient planes of step w - 1 (which hold the lower
-// couplings of step w) in LDS rings, and reads from global memory only its
-// own slots of each step and, on the tile's faces, the neighbours in the
-// next tile.  Each slot's values are read once per run instead of by up to
-// seven rows.  A step's loads are issued one step ahead, into registers.
-// The products are row_sum's, so y is k_box_spmv_sym's bit for bit; with
-// DOT, one partial of x . y per block (blocks in (tile, run) order).
 // A tile's x faces as compact columns: f15[t >> 4] = x[t] at the slots
 // with li = 15 and f0[t >> 4] = x[t] at li = 0, a step's 16 values of each
 // column side by side.  The -x neighbour of lane (li = 0, lj) at step w is
@@ -715,6 +704,18 @@ static void face_cols(hipStream_t st, int64_t ncell, const V* x, V* f15, V* f0) 
   WC_LAUNCH_CHECK();
 }
 
+// The same product through a window: one 256-thread block per (tile, run
+// of consecutive steps), walking the run's steps.  Every neighbour of a slot
+// in the tile lies in the step before or after it (lane -1 / -16 / same, or
+// +1 / +16 / same), so the block keeps x of steps w - 1 .. w + 1 and the
+// three upper coefficient planes of step w - 1 (which hold the lower
+// couplings of step w) in LDS rings, and reads from global memory only its
+// own slots of each step and, on the tile's faces, the neighbours in the
+// next tile (the x faces from the compact columns above).  Each slot's
+// values are read once per run instead of by up to seven rows.  A step's
+// loads are issued one step ahead, into registers.
+// The products are row_sum's, so y is k_box_spmv_sym's bit for bit; with
+// DOT, one partial of x . y per block (blocks in (tile, run) order).
 template <class V>
 struct WinRow {
   uint16_t f;
