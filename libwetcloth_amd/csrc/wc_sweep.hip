@@ -715,7 +715,8 @@ static void face_cols(hipStream_t st, int64_t ncell, const V* x, V* f15, V* f0) 
 // values are read once per run instead of by up to seven rows.  A step's
 // loads are issued one step ahead, into registers.
 // The products are row_sum's, so y is k_box_spmv_sym's bit for bit; with
-// DOT, one partial of x . y per block (blocks in (tile, run) order).
+// DOT, one partial of x . y per block (blocks in (tile, run) order), summed
+// by wave shuffles.
 template <class V>
 struct WinRow {
   uint16_t f;
@@ -734,7 +735,8 @@ __global__ __launch_bounds__(256) void k_box_spmv_win(uint32_t nslot, Skew K, in
                                                      V* __restrict__ y, double* partial) {
   __shared__ V xs[4][kThreads];     // x of step w at [w & 3]
   __shared__ V us[3][3][kThreads];  // upper planes of step w at [w % 3]
-  __shared__ double sh[kThreads];
+  __shared__ double sh[kThreads / 64];  // the waves' x . y sums
+  static_assert(kThreads / 64 == 4, "four waves per block");
   const int lane = threadIdx.x, li = lane & (kT - 1), lj = lane / kT;
   const int nsteps = K.nsteps;
   const int tile = blockIdx.x / nrun;
@@ -810,13 +812,12 @@ __global__ __launch_bounds__(256) void k_box_spmv_win(uint32_t nslot, Skew K, in
     y[tb + (uint32_t)w * kThreads] = v;
   }
   if (DOT) {
-    sh[threadIdx.x] = acc;
+    // a butterfly over the wave's 64 lanes, then the 4 waves in order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((lane & 63) == 0) sh[lane >> 6] = acc;
     __syncthreads();
-    for (int s2 = kThreads / 2; s2 > 0; s2 >>= 1) {
-      if ((int)threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    if (lane == 0) partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
   }
 }
 
